@@ -1,0 +1,163 @@
+/*
+ * fdtd_mi355x.h — C ABI of the MI355X (gfx950) backend for the 3D staggered
+ * pressure-velocity FDTD step of strata-fdtd.
+ *
+ * All field, table and trace pointers are DEVICE pointers (the Python host
+ * passes torch.Tensor.data_ptr() of tensors it owns). Every entry point returns
+ * 0 on success and a negative code on failure; fdtd_last_error() returns the
+ * message of the last failure on the calling thread. Step entry points only
+ * enqueue work on the context's stream: they never allocate, never
+ * synchronise and are capturable into a hipGraph.
+ *
+ * Layout: C order (nx, ny, nz), k fastest, flat index (i*ny + j)*nz + k, as
+ * the reference's _kernels/include/fdtd_types.hpp:33-42. Staggering: vx[i]
+ * sits on the face between cells i and i+1 (core/solver.py:2009-2013).
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef FDTD_MI355X_H
+#define FDTD_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Cell-flag byte built on the device from the geometry mask (bit layout). */
+#define FDTD_FLAG_AIR    0x01u  /* cell is air (geometry True)                  */
+#define FDTD_FLAG_OPEN_X 0x02u  /* face (i|i+1) below the last plane, both air  */
+#define FDTD_FLAG_OPEN_Y 0x04u
+#define FDTD_FLAG_OPEN_Z 0x08u
+#define FDTD_FLAG_ADE    0x10u  /* cell carries a material with poles          */
+
+#define FDTD_MAX_SOURCES 8
+#define FDTD_MAX_PROBES  64
+#define FDTD_MAX_POLES   8
+
+typedef struct fdtd_ctx fdtd_ctx;
+
+/* ---- errors / info ------------------------------------------------------ */
+/* Replaces: RuntimeError messages of _kernels/kernels.cpp:22-65. */
+const char* fdtd_last_error(void);
+/* Replaces: _kernels.__version__ / has_openmp / get_num_threads (kernels.cpp:74-102). */
+const char* fdtd_version(void);
+int fdtd_device_count(int* out);
+
+/* ---- context (one solve, one stream) ------------------------------------ */
+/* Replaces: FDTDSolver's _use_native state (core/solver.py:1536-1574). stream
+ * is a hipStream_t (NULL = legacy default stream). */
+int fdtd_ctx_create(int device, void* stream, fdtd_ctx** out);
+int fdtd_ctx_destroy(fdtd_ctx* ctx);
+
+/* Grid and coefficients. Replaces: FDTDSolver.__init__ coefficient setup
+ * (core/solver.py:1582-1602) and create_nonuniform_grid_data (kernels.cpp:443-475).
+ * For a uniform grid pass nonuniform=0 and NULL tables; cv/cp are
+ * coeff_v/coeff_p. For a nonuniform grid cv/cp are coeff_v_base/coeff_p_base and
+ * inv_face[a] (n_a - 1 floats) / inv_cell[a] (n_a floats) are device tables. */
+int fdtd_ctx_set_grid(fdtd_ctx* ctx, int nx, int ny, int nz, int nonuniform, float cv, float cp,
+                      const float* inv_face_x, const float* inv_face_y, const float* inv_face_z,
+                      const float* inv_cell_x, const float* inv_cell_y, const float* inv_cell_z);
+
+/* Ping-pong field buffers: set[0] = {p, vx, vy, vz}, set[1] likewise. Step s
+ * reads set[s & 1] and writes set[(s & 1) ^ 1]. Replaces the in-place numpy
+ * arrays solver.p/vx/vy/vz (core/solver.py:1604-1608). */
+int fdtd_ctx_set_fields(fdtd_ctx* ctx, float* p0, float* vx0, float* vy0, float* vz0, float* p1,
+                        float* vx1, float* vy1, float* vz1);
+
+/* Cell flags (1 byte/cell, see FDTD_FLAG_*). Replaces precompute_boundary_cells
+ * + the bool geometry argument (boundaries.cpp:13-64, fdtd_step.cpp:84-91). */
+int fdtd_ctx_set_flags(fdtd_ctx* ctx, const uint8_t* flags);
+
+/* PML decay tables exp(-sigma dt) per axis (device, n_a floats; tables of
+ * 1.0f for an axis without PML). Replaces initialize_pml / PMLData
+ * (pml.cpp:13-45). */
+int fdtd_ctx_set_pml(fdtd_ctx* ctx, const float* decay_x, const float* decay_y,
+                     const float* decay_z);
+
+/* Sources: kind[q] 0 = point (a[q] = flat cell index), 1 = plane (a[q] = axis,
+ * b[q] = index). wave: device float64 table [n][wave_len], value at step s in
+ * column s. Host arrays for kind/a/b. Replaces _inject_sources
+ * (core/solver.py:2386-2433). */
+int fdtd_ctx_set_sources(fdtd_ctx* ctx, int n, const int32_t* kind, const int64_t* a,
+                         const int64_t* b, const double* wave, int64_t wave_len);
+
+/* Probes: host flat indices; trace: device float32 [trace_len][n]; step s
+ * writes row s. Replaces _record_probes (core/solver.py:2435-2439). */
+int fdtd_ctx_set_probes(fdtd_ctx* ctx, int n, const int64_t* flat_idx, float* trace,
+                        int64_t trace_len);
+
+/* Device int64 holding the index of the next step (read by every step kernel,
+ * advanced by fdtd_step_n) and a device uint32 health word (bit0: a probe
+ * sample was not finite). */
+int fdtd_ctx_set_counters(fdtd_ctx* ctx, int64_t* step_counter, uint32_t* health);
+
+/* ADE (frequency-dependent materials). Replaces ADEMaterialData + the
+ * update_ade_* / apply_ade_* kernels (ade.cpp:25-473). Poles are host arrays:
+ * mat_id, target (0 density, 1 modulus), lorentz (0/1), coefficients a,b,d and
+ * corr (density: -dt/rho_inf (times 1/dx uniform); modulus: -K_inf dt). The
+ * auxiliary fields live on a dense box [lo, hi) (the material bounding box):
+ * J and Jprev are device arrays [n_pole][box cells]. material: device uint8
+ * per grid cell. */
+int fdtd_ctx_set_ade(fdtd_ctx* ctx, int n_pole, const int32_t* mat_id, const int32_t* target,
+                     const int32_t* lorentz, const float* a, const float* b, const float* d,
+                     const float* corr, const uint8_t* material, const int32_t* box_lo,
+                     const int32_t* box_hi, float* J, float* Jprev, float inv_dx);
+
+/* Launch shape for the fused kernel: rows per wave (2, 4, 8 or 16) and planes
+ * per wave (>= 1). 0 picks the built-in default. */
+int fdtd_ctx_set_tiling(fdtd_ctx* ctx, int rows, int planes);
+
+/* Enqueue n fused time steps. Replaces FDTDSolver.step's native dispatch
+ * (core/solver.py:2028-2077: _step_native + boundary.apply_* + sources +
+ * probes). Step s (read from the device counter) reads buffer set s&1. */
+int fdtd_step_n(fdtd_ctx* ctx, int n_steps);
+
+/* Capture n fused steps (plus the counter advance) into a hipGraph owned by
+ * the context and replay it `replays` times. */
+int fdtd_graph_run(fdtd_ctx* ctx, int n_steps, int replays);
+
+/* Average device time of the fused step kernel over the last timed window,
+ * measured with HIP events on the context stream (milliseconds). */
+int fdtd_time_steps(fdtd_ctx* ctx, int n_steps, int use_graph, float* ms_total);
+
+/* ---- per-phase entry points (one kernel per reference function) --------- */
+/* All in place, on `stream`, float32, uniform unless noted. */
+/* Replaces fdtd::update_velocity (fdtd_step.cpp:15-82). */
+int fdtd_update_velocity(const float* p, float* vx, float* vy, float* vz, int nx, int ny, int nz,
+                         float cv, void* stream);
+/* Replaces fdtd::update_velocity_nonuniform (fdtd_step.cpp:235-307). */
+int fdtd_update_velocity_nonuniform(const float* p, float* vx, float* vy, float* vz, int nx,
+                                    int ny, int nz, float cv_base, const float* ifx,
+                                    const float* ify, const float* ifz, void* stream);
+/* Replaces fdtd::apply_rigid_boundaries (boundaries.cpp:66-89), driven by flags. */
+int fdtd_apply_rigid(float* vx, float* vy, float* vz, const uint8_t* flags, int nx, int ny,
+                     int nz, void* stream);
+/* Replaces fdtd::update_pressure (fdtd_step.cpp:84-212). */
+int fdtd_update_pressure(float* p, const float* vx, const float* vy, const float* vz,
+                         const uint8_t* flags, int nx, int ny, int nz, float cp, void* stream);
+/* Replaces fdtd::update_pressure_nonuniform (fdtd_step.cpp:309-439). */
+int fdtd_update_pressure_nonuniform(float* p, const float* vx, const float* vy, const float* vz,
+                                    const uint8_t* flags, int nx, int ny, int nz, float cp_base,
+                                    const float* icx, const float* icy, const float* icz,
+                                    void* stream);
+/* Replaces fdtd::apply_pml_velocity / apply_pml_pressure (pml.cpp:47-149).
+ * NULL table = axis without PML. */
+int fdtd_apply_pml_velocity(float* vx, float* vy, float* vz, const float* dx, const float* dy,
+                            const float* dz, int nx, int ny, int nz, void* stream);
+int fdtd_apply_pml_pressure(float* p, const float* dx, const float* dy, const float* dz, int nx,
+                            int ny, int nz, void* stream);
+/* Replaces fdtd::compute_divergence(_nonuniform) (ade.cpp:479-692); icx NULL
+ * = uniform with inv_dx. */
+int fdtd_compute_divergence(float* div, const float* vx, const float* vy, const float* vz, int nx,
+                            int ny, int nz, float inv_dx, const float* icx, const float* icy,
+                            const float* icz, void* stream);
+/* Geometry mask (uint8 0/1) + material ids (nullable) -> cell flags.
+ * Replaces precompute_boundary_cells (boundaries.cpp:13-64). */
+int fdtd_build_flags(const uint8_t* air, const uint8_t* material, uint8_t* flags, int nx, int ny,
+                     int nz, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD_MI355X_H */

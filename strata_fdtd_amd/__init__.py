@@ -1,0 +1,28 @@
+"""strata_fdtd_amd — MI355X (gfx950) backend for strata-fdtd's 3D staggered
+pressure-velocity FDTD time step.
+
+Drop-in surface (the reference's names and semantics, ``backend="mi355x"``):
+``FDTDSolver``, ``UniformGrid``, ``NonuniformGrid``, ``PML``, ``GaussianPulse``,
+SDF geometry (``Box``, ``Sphere``, ``Cylinder``, ``Union``, ``Intersection``,
+``Difference``, ``HelmholtzResonator``, ``MaterializedGeometry``) and ADE
+materials (``Pole``, ``PoleType``, ``SimpleMaterial``, ``AIR_20C``).
+Beyond the reference: ``serving`` (continuous batching of many solves) and
+``adjoint`` (gradients w.r.t. per-voxel density and bulk modulus).
+"""
+
+from .boundaries import PML, RigidBoundary
+from .geometry import (Box, Cylinder, Difference, HelmholtzResonator, Intersection,
+                       MaterializedGeometry, SDFPrimitive, Sphere, Union)
+from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
+from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, Pole, PoleType, SimpleMaterial
+from .solver import FDTDSolver, has_mi355x_backend
+from .sources import GaussianPulse, Probe
+
+__version__ = "0.1.0"
+
+__all__ = [
+    "FDTDSolver", "has_mi355x_backend", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
+    "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder",
+    "Union", "Intersection", "Difference", "HelmholtzResonator", "MaterializedGeometry", "Pole",
+    "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C",
+]

@@ -1,0 +1,92 @@
+"""ctypes binding of ``libfdtd_mi355x.so`` (the C ABI in include/fdtd_mi355x.h).
+
+The library is built in-tree by ``build.py`` (hipcc, gfx950). There is no
+fallback: if the library is missing or no MI355X is visible, using the backend
+raises.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libfdtd_mi355x.so")
+
+_vp = C.c_void_p
+_i = C.c_int
+_f = C.c_float
+_i64 = C.c_int64
+
+_SIGS = {
+    "fdtd_last_error": (C.c_char_p, []),
+    "fdtd_version": (C.c_char_p, []),
+    "fdtd_device_count": (_i, [C.POINTER(_i)]),
+    "fdtd_ctx_create": (_i, [_i, _vp, C.POINTER(_vp)]),
+    "fdtd_ctx_destroy": (_i, [_vp]),
+    "fdtd_ctx_set_grid": (_i, [_vp, _i, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "fdtd_ctx_set_fields": (_i, [_vp] + [_vp] * 8),
+    "fdtd_ctx_set_flags": (_i, [_vp, _vp]),
+    "fdtd_ctx_set_pml": (_i, [_vp, _vp, _vp, _vp]),
+    "fdtd_ctx_set_sources": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64]),
+    "fdtd_ctx_set_probes": (_i, [_vp, _i, _vp, _vp, _i64]),
+    "fdtd_ctx_set_counters": (_i, [_vp, _vp, _vp]),
+    "fdtd_ctx_set_ade": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f]),
+    "fdtd_ctx_set_tiling": (_i, [_vp, _i, _i]),
+    "fdtd_step_n": (_i, [_vp, _i]),
+    "fdtd_graph_run": (_i, [_vp, _i, _i]),
+    "fdtd_time_steps": (_i, [_vp, _i, _i, C.POINTER(_f)]),
+    "fdtd_update_velocity": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
+    "fdtd_update_velocity_nonuniform": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
+    "fdtd_apply_rigid": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "fdtd_update_pressure": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
+    "fdtd_update_pressure_nonuniform": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
+    "fdtd_apply_pml_velocity": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "fdtd_apply_pml_pressure": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
+    "fdtd_compute_divergence": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
+    "fdtd_build_flags": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
+}
+
+_lib: C.CDLL | None = None
+
+
+class NativeError(RuntimeError):
+    """A C-ABI call returned a negative status."""
+
+
+def declared_symbols() -> list[str]:
+    return list(_SIGS)
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load the library (once) and attach signatures. Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(
+            f"MI355X backend library not built: {path} is missing "
+            "(run `python -c 'import __graft_entry__ as g; g.build()'` or "
+            "`python -m strata_fdtd_amd.build`)")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name: str, *args) -> int:
+    """Call a status-returning entry point; raise NativeError on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.fdtd_last_error().decode(errors="replace")
+        raise NativeError(f"{name} failed ({rc}): {msg}")
+    return rc
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None passes NULL)."""
+    return None if t is None else t.data_ptr()

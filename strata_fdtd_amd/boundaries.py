@@ -1,0 +1,145 @@
+"""Boundary conditions: PML absorbing layers and the rigid-wall protocol.
+
+``PML`` keeps the reference's interface (boundaries/_boundaries.py:104-417:
+``PML(depth, axis, max_sigma, order)``, ``initialize(solver)``,
+``apply_velocity/apply_pressure(solver)``, ``reset()``,
+``get_interior_slice()``). On ``backend="mi355x"`` the damping itself runs
+inside the fused step kernel; this class only builds the three float32 decay
+tables ``exp(-sigma*dt)`` exactly as the reference's numpy path does
+(:340-377) and hands them to the solver. Interior entries are exactly 1.0f, so
+the kernel's multiply is an identity there: damping is confined to the slabs.
+"""
+
+from __future__ import annotations
+
+from typing import Literal
+
+import numpy as np
+from numpy.typing import NDArray
+
+
+class RigidBoundary:
+    """No-op boundary: rigid walls come from the geometry mask (boundaries/_boundaries.py:75-101)."""
+
+    def initialize(self, solver) -> None:
+        pass
+
+    def apply_velocity(self, solver) -> None:
+        pass
+
+    def apply_pressure(self, solver) -> None:
+        pass
+
+    def reset(self) -> None:
+        pass
+
+
+class PML:
+    """Polynomially graded absorbing layer (boundaries/_boundaries.py:104-417)."""
+
+    def __init__(self, depth: int = 10,
+                 axis: Literal["all", "x", "y", "z"] | tuple[str, ...] = "all",
+                 max_sigma: float | None = None, order: int = 3):
+        self.depth = depth
+        self.order = order
+        if axis == "all":
+            self.axes = ("x", "y", "z")
+        elif isinstance(axis, str):
+            self.axes = (axis,)
+        else:
+            self.axes = tuple(axis)
+        self._max_sigma = max_sigma
+        self._initialized = False
+        self._sigma_x: NDArray[np.float32] | None = None
+        self._sigma_y: NDArray[np.float32] | None = None
+        self._sigma_z: NDArray[np.float32] | None = None
+        self._solver = None
+
+    # ---- profile construction (host, float64 -> float32) -------------------
+    def initialize(self, solver) -> None:
+        """Build sigma profiles for the solver's grid (boundaries/_boundaries.py:166-226)."""
+        self._solver = solver
+        nx, ny, nz = solver.shape
+        grid = solver.grid
+        if self._max_sigma is None:
+            r_target = 1e-6
+            d = self.depth * grid.min_spacing
+            self._max_sigma = -(self.order + 1) * solver.c * np.log(r_target) / (2 * d)
+        if grid.is_uniform:
+            self._sigma_x = self._create_profile(nx, solver.dx) if "x" in self.axes else None
+            self._sigma_y = self._create_profile(ny, solver.dx) if "y" in self.axes else None
+            self._sigma_z = self._create_profile(nz, solver.dx) if "z" in self.axes else None
+        else:
+            self._sigma_x = (self._create_profile_nonuniform(grid.x_coords, grid.dx)
+                             if "x" in self.axes else None)
+            self._sigma_y = (self._create_profile_nonuniform(grid.y_coords, grid.dy)
+                             if "y" in self.axes else None)
+            self._sigma_z = (self._create_profile_nonuniform(grid.z_coords, grid.dz)
+                             if "z" in self.axes else None)
+        self._initialized = True
+
+    def _create_profile(self, n: int, dx: float) -> NDArray[np.float32]:
+        """sigma_max * (dist/d)**order on both slabs, float64 values stored as float32 (:228-255)."""
+        sigma = np.zeros(n, dtype=np.float32)
+        d = self.depth
+        for i in range(d):
+            sigma[i] = self._max_sigma * (((d - i) / d) ** self.order)
+        for i in range(n - d, n):
+            sigma[i] = self._max_sigma * (((i - (n - d - 1)) / d) ** self.order)
+        return sigma
+
+    def _create_profile_nonuniform(self, coords, cell_spacing) -> NDArray[np.float32]:
+        """Profile from physical distance to the PML interface (:257-315)."""
+        n = len(coords)
+        sigma = np.zeros(n, dtype=np.float32)
+        d = self.depth
+        if d >= n // 2:
+            d = max(1, n // 4)
+        left_thickness = float(np.sum(cell_spacing[:d]))
+        right_thickness = float(np.sum(cell_spacing[-d:]))
+        left_interface = coords[d] - cell_spacing[d] / 2 if d < n else coords[-1]
+        right_interface = (coords[n - d - 1] + cell_spacing[n - d - 1] / 2) if n - d > 0 else coords[0]
+        for i in range(d):
+            dist = left_interface - coords[i]
+            nd = min(1.0, dist / left_thickness) if left_thickness > 0 else 0.0
+            sigma[i] = self._max_sigma * (nd ** self.order)
+        for i in range(n - d, n):
+            dist = coords[i] - right_interface
+            nd = min(1.0, dist / right_thickness) if right_thickness > 0 else 0.0
+            sigma[i] = self._max_sigma * (nd ** self.order)
+        return sigma
+
+    def decay_tables(self, dt: float, shape) -> tuple[NDArray[np.float32], ...]:
+        """float32 exp(-sigma*dt) per axis; an axis without PML gets ones.
+
+        Formed as the reference's numpy path does (``np.exp(-sigma * dt)`` on a
+        float32 array with a python-float dt, boundaries/_boundaries.py:340-377),
+        so every operation runs in float32.
+        """
+        out = []
+        for sig, n in ((self._sigma_x, shape[0]), (self._sigma_y, shape[1]), (self._sigma_z, shape[2])):
+            out.append(np.ones(n, dtype=np.float32) if sig is None else np.exp(-sig * dt))
+        return tuple(out)
+
+    # ---- boundary protocol -------------------------------------------------
+    def apply_velocity(self, solver) -> None:
+        """Fused into the step kernel on the MI355X backend (nothing to do here)."""
+
+    def apply_pressure(self, solver) -> None:
+        """Fused into the step kernel on the MI355X backend (nothing to do here)."""
+
+    def reset(self) -> None:
+        """The PML holds no time-dependent state."""
+
+    def get_interior_slice(self) -> tuple[slice, slice, slice]:
+        if self._solver is None:
+            raise RuntimeError("PML not initialized")
+        d = self.depth
+        nx, ny, nz = self._solver.shape
+        return (slice(d, nx - d) if "x" in self.axes else slice(None),
+                slice(d, ny - d) if "y" in self.axes else slice(None),
+                slice(d, nz - d) if "z" in self.axes else slice(None))
+
+    @property
+    def is_initialized(self) -> bool:
+        return self._initialized

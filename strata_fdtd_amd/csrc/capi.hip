@@ -1,0 +1,452 @@
+// C ABI of the MI355X FDTD backend (declarations in include/fdtd_mi355x.h).
+// Host-side only: validates arguments, keeps the launch arguments of one solve
+// in a context, enqueues kernels on the context stream. No entry point that
+// steps the solver allocates device memory or synchronises.
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fdtd_common.hpp"
+
+namespace fdtd_amd {
+int fused_grid(FusedArgs& a, int rows, int planes);
+hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st);
+hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
+hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
+                          const float*, const float*, hipStream_t);
+hipError_t phase_rigid(float*, float*, float*, const uint8_t*, int, int, int, hipStream_t);
+hipError_t phase_pressure(float*, const float*, const float*, const float*, const uint8_t*, int, int,
+                          int, float, const float*, const float*, const float*, hipStream_t);
+hipError_t phase_pml_velocity(float*, float*, float*, const float*, const float*, const float*, int,
+                              int, int, hipStream_t);
+hipError_t phase_pml_pressure(float*, const float*, const float*, const float*, int, int, int,
+                              hipStream_t);
+hipError_t phase_divergence(float*, const float*, const float*, const float*, int, int, int, float,
+                            const float*, const float*, const float*, hipStream_t);
+hipError_t phase_build_flags(const uint8_t*, const uint8_t*, uint8_t*, int, int, int, hipStream_t);
+}  // namespace fdtd_amd
+
+using namespace fdtd_amd;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return 0;
+    return fail(-5, "%s: %s", what, hipGetErrorString(e));
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+}  // namespace
+
+struct fdtd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    FusedArgs a{};
+    bool nonuniform = false;
+    bool have_grid = false, have_fields = false, have_flags = false, have_pml = false;
+    bool have_counters = false;
+    int rows = 8, planes = 16;
+    int32_t* probe_dev = nullptr;     // owned: [n_probe][3]
+    int64_t* step_counter = nullptr;  // same as a.step_ptr, writable
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+extern "C" {
+
+const char* fdtd_last_error(void) { return g_err.c_str(); }
+
+const char* fdtd_version(void) { return "strata_fdtd_amd 0.1.0 (gfx950, HIP)"; }
+
+int fdtd_device_count(int* out) {
+    if (!out) return fail(-1, "fdtd_device_count: null out");
+    return check(hipGetDeviceCount(out), "hipGetDeviceCount");
+}
+
+int fdtd_ctx_create(int device, void* stream, fdtd_ctx** out) {
+    if (!out) return fail(-1, "fdtd_ctx_create: null out");
+    int n = 0;
+    if (int rc = check(hipGetDeviceCount(&n), "hipGetDeviceCount")) return rc;
+    if (device < 0 || device >= n) return fail(-1, "fdtd_ctx_create: device %d of %d", device, n);
+    auto* c = new fdtd_ctx();
+    c->device = device;
+    c->stream = static_cast<hipStream_t>(stream);
+    DeviceGuard g(device);
+    if (int rc = check(hipEventCreate(&c->ev0), "hipEventCreate")) { delete c; return rc; }
+    if (int rc = check(hipEventCreate(&c->ev1), "hipEventCreate")) { delete c; return rc; }
+    *out = c;
+    return 0;
+}
+
+int fdtd_ctx_destroy(fdtd_ctx* c) {
+    if (!c) return 0;
+    DeviceGuard g(c->device);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->probe_dev) (void)hipFree(c->probe_dev);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+    return 0;
+}
+
+static void drop_graph(fdtd_ctx* c) {
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+        c->graph_steps = 0;
+    }
+}
+
+int fdtd_ctx_set_grid(fdtd_ctx* c, int nx, int ny, int nz, int nonuniform, float cv, float cp,
+                      const float* ifx, const float* ify, const float* ifz, const float* icx,
+                      const float* icy, const float* icz) {
+    if (!c) return fail(-1, "null context");
+    if (nx < 2 || ny < 2 || nz < 2) return fail(-2, "grid %dx%dx%d: every axis needs >= 2 cells", nx, ny, nz);
+    if (static_cast<int64_t>(nx) * ny * nz >= (int64_t(1) << 40)) return fail(-2, "grid too large");
+    if (nonuniform && (!ifx || !ify || !ifz || !icx || !icy || !icz))
+        return fail(-2, "nonuniform grid needs all six spacing tables");
+    drop_graph(c);
+    c->a.nx = nx; c->a.ny = ny; c->a.nz = nz;
+    c->a.cv = cv; c->a.cp = cp;
+    c->nonuniform = nonuniform != 0;
+    c->a.ifc[0] = ifx; c->a.ifc[1] = ify; c->a.ifc[2] = ifz;
+    c->a.icl[0] = icx; c->a.icl[1] = icy; c->a.icl[2] = icz;
+    c->have_grid = true;
+    return 0;
+}
+
+int fdtd_ctx_set_fields(fdtd_ctx* c, float* p0, float* vx0, float* vy0, float* vz0, float* p1,
+                        float* vx1, float* vy1, float* vz1) {
+    if (!c) return fail(-1, "null context");
+    float* ptrs[8] = {p0, vx0, vy0, vz0, p1, vx1, vy1, vz1};
+    for (auto* q : ptrs)
+        if (!q) return fail(-2, "fdtd_ctx_set_fields: null field pointer");
+    drop_graph(c);
+    for (int s = 0; s < 2; ++s)
+        for (int f = 0; f < 4; ++f) c->a.f[s][f] = ptrs[4 * s + f];
+    c->have_fields = true;
+    return 0;
+}
+
+int fdtd_ctx_set_flags(fdtd_ctx* c, const uint8_t* flags) {
+    if (!c) return fail(-1, "null context");
+    if (!flags) return fail(-2, "null flags");
+    drop_graph(c);
+    c->a.flags = flags;
+    c->have_flags = true;
+    return 0;
+}
+
+int fdtd_ctx_set_pml(fdtd_ctx* c, const float* dx, const float* dy, const float* dz) {
+    if (!c) return fail(-1, "null context");
+    if (!dx || !dy || !dz) return fail(-2, "PML tables must all be set (use 1.0f tables for an axis without PML)");
+    drop_graph(c);
+    c->a.dec[0] = dx; c->a.dec[1] = dy; c->a.dec[2] = dz;
+    c->have_pml = true;
+    return 0;
+}
+
+int fdtd_ctx_set_sources(fdtd_ctx* c, int n, const int32_t* kind, const int64_t* a, const int64_t* b,
+                         const double* wave, int64_t wave_len) {
+    if (!c) return fail(-1, "null context");
+    if (!c->have_grid) return fail(-3, "set the grid before the sources");
+    if (n < 0 || n > FDTD_MAX_SOURCES) return fail(-2, "%d sources (max %d)", n, FDTD_MAX_SOURCES);
+    if (n > 0 && (!kind || !a || !b || !wave || wave_len < 1)) return fail(-2, "bad source arrays");
+    const int nx = c->a.nx, ny = c->a.ny, nz = c->a.nz;
+    for (int q = 0; q < n; ++q) {
+        if (kind[q] == 0) {
+            const int64_t f = a[q];
+            if (f < 0 || f >= static_cast<int64_t>(nx) * ny * nz)
+                return fail(-2, "point source %d: flat index %lld outside the grid", q, (long long)f);
+            c->a.src_kind[q] = 0;
+            c->a.src_k[q] = static_cast<int>(f % nz);
+            c->a.src_j[q] = static_cast<int>((f / nz) % ny);
+            c->a.src_i[q] = static_cast<int>(f / (static_cast<int64_t>(ny) * nz));
+        } else if (kind[q] == 1) {
+            const int ax = static_cast<int>(a[q]);
+            const int dims[3] = {nx, ny, nz};
+            if (ax < 0 || ax > 2 || b[q] < 0 || b[q] >= dims[ax])
+                return fail(-2, "plane source %d: axis %d index %lld outside the grid", q, ax, (long long)b[q]);
+            c->a.src_kind[q] = 1;
+            c->a.src_i[q] = ax;
+            c->a.src_j[q] = static_cast<int>(b[q]);
+            c->a.src_k[q] = 0;
+        } else {
+            return fail(-2, "source %d: unknown kind %d", q, kind[q]);
+        }
+    }
+    drop_graph(c);
+    c->a.n_src = n;
+    c->a.wave = wave;
+    c->a.wave_len = n > 0 ? wave_len : 0;
+    return 0;
+}
+
+int fdtd_ctx_set_probes(fdtd_ctx* c, int n, const int64_t* idx, float* trace, int64_t trace_len) {
+    if (!c) return fail(-1, "null context");
+    if (!c->have_grid) return fail(-3, "set the grid before the probes");
+    if (n < 0 || n > FDTD_MAX_PROBES) return fail(-2, "%d probes (max %d)", n, FDTD_MAX_PROBES);
+    if (n > 0 && (!idx || !trace || trace_len < 1)) return fail(-2, "bad probe arrays");
+    const int nx = c->a.nx, ny = c->a.ny, nz = c->a.nz;
+    int32_t host[3 * FDTD_MAX_PROBES];
+    for (int q = 0; q < n; ++q) {
+        const int64_t f = idx[q];
+        if (f < 0 || f >= static_cast<int64_t>(nx) * ny * nz)
+            return fail(-2, "probe %d: flat index %lld outside the grid", q, (long long)f);
+        host[3 * q + 2] = static_cast<int32_t>(f % nz);
+        host[3 * q + 1] = static_cast<int32_t>((f / nz) % ny);
+        host[3 * q + 0] = static_cast<int32_t>(f / (static_cast<int64_t>(ny) * nz));
+    }
+    DeviceGuard g(c->device);
+    drop_graph(c);
+    if (!c->probe_dev) {
+        // set-up allocation (never on a step path): the probe coordinate table
+        if (int rc = check(hipMalloc(&c->probe_dev, sizeof(int32_t) * 3 * FDTD_MAX_PROBES), "hipMalloc"))
+            return rc;
+    }
+    if (n > 0) {
+        if (int rc = check(hipMemcpy(c->probe_dev, host, sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice),
+                           "hipMemcpy"))
+            return rc;
+    }
+    c->a.n_probe = n;
+    c->a.probe_ijk = c->probe_dev;
+    c->a.trace = trace;
+    c->a.trace_len = n > 0 ? trace_len : 0;
+    return 0;
+}
+
+int fdtd_ctx_set_counters(fdtd_ctx* c, int64_t* step_counter, uint32_t* health) {
+    if (!c) return fail(-1, "null context");
+    if (!step_counter || !health) return fail(-2, "null counter");
+    drop_graph(c);
+    c->a.step_ptr = step_counter;
+    c->step_counter = step_counter;
+    c->a.health = health;
+    c->have_counters = true;
+    return 0;
+}
+
+int fdtd_ctx_set_ade(fdtd_ctx* c, int n_pole, const int32_t* mat_id, const int32_t* target,
+                     const int32_t* lorentz, const float* pa, const float* pb, const float* pd,
+                     const float* corr, const uint8_t* material, const int32_t* lo, const int32_t* hi,
+                     float* J, float* Jprev, float inv_dx) {
+    if (!c) return fail(-1, "null context");
+    if (n_pole < 0 || n_pole > FDTD_MAX_POLES) return fail(-2, "%d poles (max %d)", n_pole, FDTD_MAX_POLES);
+    drop_graph(c);
+    c->a.n_pole = n_pole;
+    if (n_pole == 0) return 0;
+    if (!mat_id || !target || !lorentz || !pa || !pb || !pd || !corr || !material || !lo || !hi || !J)
+        return fail(-2, "bad ADE arrays");
+    const int dims[3] = {c->a.nx, c->a.ny, c->a.nz};
+    for (int ax = 0; ax < 3; ++ax) {
+        if (lo[ax] < 0 || hi[ax] > dims[ax] || lo[ax] >= hi[ax])
+            return fail(-2, "ADE box axis %d [%d,%d) outside the grid", ax, lo[ax], hi[ax]);
+        c->a.box_lo[ax] = lo[ax];
+        c->a.box_hi[ax] = hi[ax];
+    }
+    for (int q = 0; q < n_pole; ++q) {
+        if (mat_id[q] < 1 || mat_id[q] > 255) return fail(-2, "pole %d: material id %d", q, mat_id[q]);
+        if (lorentz[q] && !Jprev) return fail(-2, "Lorentz pole without a history array");
+        c->a.pole_mat[q] = mat_id[q];
+        c->a.pole_target[q] = target[q];
+        c->a.pole_lorentz[q] = lorentz[q];
+        c->a.pole_a[q] = pa[q];
+        c->a.pole_b[q] = pb[q];
+        c->a.pole_d[q] = pd[q];
+        c->a.pole_corr[q] = corr[q];
+    }
+    c->a.material = material;
+    c->a.J = J;
+    c->a.Jprev = Jprev;
+    c->a.ade_inv_dx = inv_dx;
+    return 0;
+}
+
+int fdtd_ctx_set_tiling(fdtd_ctx* c, int rows, int planes) {
+    if (!c) return fail(-1, "null context");
+    if (rows == 0) rows = 8;
+    if (planes == 0) planes = 16;
+    if (rows != 2 && rows != 4 && rows != 8 && rows != 16) return fail(-2, "rows must be 2, 4, 8 or 16");
+    if (planes < 1) return fail(-2, "planes must be >= 1");
+    drop_graph(c);
+    c->rows = rows;
+    c->planes = planes;
+    return 0;
+}
+
+static int ready(fdtd_ctx* c) {
+    if (!c) return fail(-1, "null context");
+    if (!c->have_grid || !c->have_fields || !c->have_flags || !c->have_pml || !c->have_counters)
+        return fail(-3, "context incomplete: grid=%d fields=%d flags=%d pml=%d counters=%d", c->have_grid,
+                    c->have_fields, c->have_flags, c->have_pml, c->have_counters);
+    return 0;
+}
+
+static int enqueue_steps(fdtd_ctx* c, int n) {
+    FusedArgs a = c->a;
+    const int blocks = fused_grid(a, c->rows, c->planes);
+    for (int s = 0; s < n; ++s) {
+        a.koff = s;
+        if (int rc = check(launch_fused(a, c->rows, c->nonuniform, a.n_pole > 0, blocks, c->stream),
+                           "fused step launch"))
+            return rc;
+    }
+    return check(launch_advance(c->step_counter, n, c->stream), "advance launch");
+}
+
+int fdtd_step_n(fdtd_ctx* c, int n_steps) {
+    if (int rc = ready(c)) return rc;
+    if (n_steps < 0) return fail(-2, "negative step count");
+    if (n_steps == 0) return 0;
+    DeviceGuard g(c->device);
+    return enqueue_steps(c, n_steps);
+}
+
+int fdtd_graph_run(fdtd_ctx* c, int n_steps, int replays) {
+    if (int rc = ready(c)) return rc;
+    if (n_steps < 1 || replays < 0) return fail(-2, "bad graph size");
+    DeviceGuard g(c->device);
+    if (!c->graph || c->graph_steps != n_steps) {
+        drop_graph(c);
+        hipStream_t cap = c->stream;
+        bool own = false;
+        if (!cap) {
+            // the legacy default stream cannot be captured: capture on a private one
+            if (int rc = check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate")) return rc;
+            own = true;
+        }
+        hipGraph_t gr = nullptr;
+        if (int rc = check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture")) {
+            if (own) (void)hipStreamDestroy(cap);
+            return rc;
+        }
+        hipStream_t saved = c->stream;
+        c->stream = cap;
+        int rc = enqueue_steps(c, n_steps);
+        c->stream = saved;
+        hipError_t e = hipStreamEndCapture(cap, &gr);
+        if (own) (void)hipStreamDestroy(cap);
+        if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+        if (int r2 = check(e, "end capture")) return r2;
+        e = hipGraphInstantiate(&c->graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (int r3 = check(e, "graph instantiate")) return r3;
+        c->graph_steps = n_steps;
+    }
+    for (int r = 0; r < replays; ++r)
+        if (int rc = check(hipGraphLaunch(c->graph, c->stream), "graph launch")) return rc;
+    return 0;
+}
+
+int fdtd_time_steps(fdtd_ctx* c, int n_steps, int use_graph, float* ms_total) {
+    if (int rc = ready(c)) return rc;
+    if (!ms_total) return fail(-2, "null output");
+    DeviceGuard g(c->device);
+    if (int rc = check(hipEventRecord(c->ev0, c->stream), "event record")) return rc;
+    int rc = use_graph ? fdtd_graph_run(c, n_steps, 1) : fdtd_step_n(c, n_steps);
+    if (rc) return rc;
+    if (int r2 = check(hipEventRecord(c->ev1, c->stream), "event record")) return r2;
+    if (int r3 = check(hipEventSynchronize(c->ev1), "event sync")) return r3;
+    return check(hipEventElapsedTime(ms_total, c->ev0, c->ev1), "elapsed time");
+}
+
+// ---- per-phase entry points ---------------------------------------------------
+#define CHECK_DIMS()                                                   \
+    if (nx < 1 || ny < 1 || nz < 1) return fail(-2, "bad dimensions")
+
+int fdtd_update_velocity(const float* p, float* vx, float* vy, float* vz, int nx, int ny, int nz,
+                         float cv, void* stream) {
+    CHECK_DIMS();
+    return check(phase_velocity(p, vx, vy, vz, nx, ny, nz, cv, nullptr, nullptr, nullptr,
+                                static_cast<hipStream_t>(stream)), "update_velocity");
+}
+
+int fdtd_update_velocity_nonuniform(const float* p, float* vx, float* vy, float* vz, int nx, int ny,
+                                    int nz, float cv_base, const float* ifx, const float* ify,
+                                    const float* ifz, void* stream) {
+    CHECK_DIMS();
+    if (!ifx || !ify || !ifz) return fail(-2, "null spacing table");
+    return check(phase_velocity(p, vx, vy, vz, nx, ny, nz, cv_base, ifx, ify, ifz,
+                                static_cast<hipStream_t>(stream)), "update_velocity_nonuniform");
+}
+
+int fdtd_apply_rigid(float* vx, float* vy, float* vz, const uint8_t* flags, int nx, int ny, int nz,
+                     void* stream) {
+    CHECK_DIMS();
+    return check(phase_rigid(vx, vy, vz, flags, nx, ny, nz, static_cast<hipStream_t>(stream)),
+                 "apply_rigid");
+}
+
+int fdtd_update_pressure(float* p, const float* vx, const float* vy, const float* vz,
+                         const uint8_t* flags, int nx, int ny, int nz, float cp, void* stream) {
+    CHECK_DIMS();
+    return check(phase_pressure(p, vx, vy, vz, flags, nx, ny, nz, cp, nullptr, nullptr, nullptr,
+                                static_cast<hipStream_t>(stream)), "update_pressure");
+}
+
+int fdtd_update_pressure_nonuniform(float* p, const float* vx, const float* vy, const float* vz,
+                                    const uint8_t* flags, int nx, int ny, int nz, float cp_base,
+                                    const float* icx, const float* icy, const float* icz,
+                                    void* stream) {
+    CHECK_DIMS();
+    if (!icx || !icy || !icz) return fail(-2, "null spacing table");
+    return check(phase_pressure(p, vx, vy, vz, flags, nx, ny, nz, cp_base, icx, icy, icz,
+                                static_cast<hipStream_t>(stream)), "update_pressure_nonuniform");
+}
+
+int fdtd_apply_pml_velocity(float* vx, float* vy, float* vz, const float* dx, const float* dy,
+                            const float* dz, int nx, int ny, int nz, void* stream) {
+    CHECK_DIMS();
+    return check(phase_pml_velocity(vx, vy, vz, dx, dy, dz, nx, ny, nz, static_cast<hipStream_t>(stream)),
+                 "apply_pml_velocity");
+}
+
+int fdtd_apply_pml_pressure(float* p, const float* dx, const float* dy, const float* dz, int nx,
+                            int ny, int nz, void* stream) {
+    CHECK_DIMS();
+    return check(phase_pml_pressure(p, dx, dy, dz, nx, ny, nz, static_cast<hipStream_t>(stream)),
+                 "apply_pml_pressure");
+}
+
+int fdtd_compute_divergence(float* div, const float* vx, const float* vy, const float* vz, int nx,
+                            int ny, int nz, float inv_dx, const float* icx, const float* icy,
+                            const float* icz, void* stream) {
+    CHECK_DIMS();
+    return check(phase_divergence(div, vx, vy, vz, nx, ny, nz, inv_dx, icx, icy, icz,
+                                  static_cast<hipStream_t>(stream)), "compute_divergence");
+}
+
+int fdtd_build_flags(const uint8_t* air, const uint8_t* material, uint8_t* flags, int nx, int ny,
+                     int nz, void* stream) {
+    CHECK_DIMS();
+    if (!air || !flags) return fail(-2, "null mask");
+    return check(phase_build_flags(air, material, flags, nx, ny, nz, static_cast<hipStream_t>(stream)),
+                 "build_flags");
+}
+
+}  // extern "C"

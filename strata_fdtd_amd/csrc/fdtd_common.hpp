@@ -1,0 +1,71 @@
+// Shared device-side definitions for the MI355X FDTD backend (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fdtd_mi355x.h"
+
+namespace fdtd_amd {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kBlockWaves = 4;     // waves per workgroup of the fused kernel
+constexpr int kBlock = kWave * kBlockWaves;
+
+__host__ __device__ inline int64_t lin(int i, int j, int k, int ny, int nz) {
+    return (static_cast<int64_t>(i) * ny + j) * nz + k;
+}
+
+// Everything one fused step launch needs. Passed by value (kernarg segment).
+struct FusedArgs {
+    float* f[2][4];                 // ping-pong sets {p, vx, vy, vz}
+    const uint8_t* flags;           // FDTD_FLAG_* per cell
+    const float* dec[3];            // PML decay per axis (tables of 1.0f if absent)
+    const float* ifc[3];            // nonuniform inverse face spacings (n-1)
+    const float* icl[3];            // nonuniform inverse cell spacings (n)
+    int nx, ny, nz;
+    float cv, cp;                   // coeff_v/coeff_p (uniform) or *_base (nonuniform)
+
+    // launch tiling: wave w -> (segment, j-strip, i-chunk)
+    int planes;                     // planes per wave (i extent of a strip)
+    int nseg;                       // ceil(nz / 64)
+    int nstrip_j;                   // ceil(ny / R)
+    int nstrip_i;                   // ceil(nx / planes)
+    int nwaves;                     // nseg * nstrip_j * nstrip_i
+    int blocks_per_xcd;             // gridDim.x / 8
+
+    // time
+    const int64_t* step_ptr;        // device counter: index of step koff==0
+    int koff;                       // offset of this launch inside a chunk
+
+    // sources
+    int n_src;
+    int src_kind[FDTD_MAX_SOURCES]; // 0 point, 1 plane
+    int src_i[FDTD_MAX_SOURCES];    // point: i ; plane: axis
+    int src_j[FDTD_MAX_SOURCES];    // point: j ; plane: index
+    int src_k[FDTD_MAX_SOURCES];    // point: k
+    const double* wave;             // [n_src][wave_len]
+    int64_t wave_len;
+
+    // probes
+    int n_probe;
+    const int32_t* probe_ijk;       // device [n_probe][3]
+    float* trace;                   // [trace_len][n_probe]
+    int64_t trace_len;
+    uint32_t* health;               // bit0 non-finite probe, bit1 table overrun
+
+    // ADE (materials with poles)
+    int n_pole;
+    int pole_mat[FDTD_MAX_POLES];
+    int pole_target[FDTD_MAX_POLES];
+    int pole_lorentz[FDTD_MAX_POLES];
+    float pole_a[FDTD_MAX_POLES], pole_b[FDTD_MAX_POLES], pole_d[FDTD_MAX_POLES];
+    float pole_corr[FDTD_MAX_POLES];
+    const uint8_t* material;        // per cell
+    int box_lo[3], box_hi[3];       // ADE box [lo, hi)
+    float* J;                       // [n_pole][box cells]
+    float* Jprev;                   // [n_pole][box cells] (Lorentz)
+    float ade_inv_dx;               // uniform grids: 1/dx of the divergence source
+};
+
+}  // namespace fdtd_amd

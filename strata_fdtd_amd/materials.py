@@ -1,0 +1,137 @@
+"""Frequency-dependent materials: Debye/Lorentz poles (materials/base.py:38-555).
+
+Only what reaches the step kernel is kept: the pole parameters, their FDTD
+recurrence coefficients, the instantaneous density/modulus, and the material
+library constants the workloads use (materials/library.py:31-60).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from enum import Enum
+from typing import Literal
+
+import numpy as np
+
+
+class PoleType(Enum):
+    DEBYE = "debye"
+    LORENTZ = "lorentz"
+
+
+@dataclass
+class Pole:
+    """One dispersion pole targeting density or bulk modulus (materials/base.py:55-212)."""
+
+    pole_type: PoleType
+    delta_chi: float
+    target: Literal["density", "modulus"]
+    tau: float | None = None
+    omega_0: float | None = None
+    gamma: float | None = None
+
+    def __post_init__(self):
+        if self.pole_type == PoleType.DEBYE:
+            if self.tau is None:
+                raise ValueError("Debye poles require tau parameter")
+            if self.tau <= 0:
+                raise ValueError("tau must be positive")
+        elif self.pole_type == PoleType.LORENTZ:
+            if self.omega_0 is None or self.gamma is None:
+                raise ValueError("Lorentz poles require omega_0 and gamma parameters")
+            if self.omega_0 <= 0:
+                raise ValueError("omega_0 must be positive")
+            if self.gamma < 0:
+                raise ValueError("gamma must be non-negative")
+        if self.target not in ("density", "modulus"):
+            raise ValueError("target must be 'density' or 'modulus'")
+
+    def susceptibility(self, omega):
+        omega = np.asarray(omega)
+        if self.pole_type == PoleType.DEBYE:
+            return self.delta_chi / (1 + 1j * omega * self.tau)
+        return self.delta_chi * self.omega_0**2 / (self.omega_0**2 - omega**2 + 1j * self.gamma * omega)
+
+    def fdtd_coefficients(self, dt: float) -> tuple[float, ...]:
+        """Debye (alpha, beta); Lorentz (a, b, d) — materials/base.py:157-188."""
+        if self.pole_type == PoleType.DEBYE:
+            tau_dt = self.tau / dt
+            denom = 1 + tau_dt
+            return (tau_dt / denom, self.delta_chi / denom)
+        w0, g = self.omega_0, self.gamma
+        dt2 = dt * dt
+        c = 1 + g * dt / 2
+        a = (2 - w0**2 * dt2) / c
+        b = -(1 - g * dt / 2) / c
+        d = self.delta_chi * w0**2 * dt2 / c
+        return (a, b, d)
+
+    @property
+    def is_debye(self) -> bool:
+        return self.pole_type == PoleType.DEBYE
+
+    @property
+    def is_lorentz(self) -> bool:
+        return self.pole_type == PoleType.LORENTZ
+
+
+@dataclass
+class SimpleMaterial:
+    """Constant rho/c material with optional poles (materials/base.py:508-555)."""
+
+    name: str = "unnamed"
+    _rho: float = field(default=1.2, repr=False)
+    _c: float = field(default=343.0, repr=False)
+    _poles: list[Pole] = field(default_factory=list, repr=False)
+
+    @property
+    def rho_inf(self) -> float:
+        return self._rho
+
+    @property
+    def K_inf(self) -> float:
+        return self._rho * self._c**2
+
+    @property
+    def poles(self) -> list[Pole]:
+        return self._poles
+
+    @property
+    def density_poles(self) -> list[Pole]:
+        return [p for p in self.poles if p.target == "density"]
+
+    @property
+    def modulus_poles(self) -> list[Pole]:
+        return [p for p in self.poles if p.target == "modulus"]
+
+    @property
+    def c_inf(self) -> float:
+        return float(np.sqrt(self.K_inf / self.rho_inf))
+
+    def n_poles(self) -> int:
+        return len(self.poles)
+
+
+Material = SimpleMaterial
+
+AIR_20C = SimpleMaterial(name="air_20C", _rho=1.204, _c=343.0)
+AIR_25C = SimpleMaterial(name="air_25C", _rho=1.184, _c=346.3)
+AIR_15C = SimpleMaterial(name="air_15C", _rho=1.225, _c=340.3)
+WATER_20C = SimpleMaterial(name="water_20C", _rho=998.2, _c=1482.0)
+
+
+def density_correction_coeff(dt: float, rho_inf: float, inv_dx: float | None) -> np.float32:
+    """Velocity-correction coefficient as the native kernel forms it in float32:
+    ``-dt / rho_inf * inv_dx`` (_kernels/src/ade.cpp:247). On nonuniform grids
+    (inv_dx None) the base ``-dt / rho_inf`` is returned and the kernel
+    multiplies by the per-face inverse spacing (core/solver.py:3349-3385)."""
+    f = np.float32
+    if inv_dx is None:
+        return f(-dt / rho_inf)
+    return f((f(-f(dt)) / f(rho_inf)) * f(inv_dx))
+
+
+def modulus_correction_coeff(dt: float, K_inf: float) -> np.float32:
+    """Pressure-correction coefficient ``-K_inf * dt`` in float32 (_kernels/src/ade.cpp:422)."""
+    f = np.float32
+    return f(f(-f(K_inf)) * f(dt))

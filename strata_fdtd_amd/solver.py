@@ -1,0 +1,666 @@
+"""FDTDSolver with the MI355X backend.
+
+Same constructor, methods and semantics as the reference's ``FDTDSolver``
+(core/solver.py:1442-3443) for the staggered pressure-velocity step, with the
+compute on one MI355X: ``backend="mi355x"`` (also what ``"auto"`` selects).
+
+Device state (torch tensors on ``cuda:<device>``): two ping-pong sets of
+{p, vx, vy, vz} float32 fields, the cell-flag byte, 1D PML decay and spacing
+tables, the source waveform table (float64, one column per step), the probe
+trace buffer, a device step counter and a health word. ``step()``/``run()``
+enqueue the fused step kernel through the C ABI; nothing is copied back until
+the caller reads fields or probe data.
+
+Host views: ``solver.p`` (and vx, vy, vz) return numpy arrays synchronised
+from the device. If a caller writes into such an array, the write is uploaded
+before the next step (the arrays handed out are treated as possibly modified),
+so reference-style code like ``solver.p[50, 50, 50] = 1.0`` keeps working.
+"""
+
+from __future__ import annotations
+
+import warnings
+from typing import Callable, Literal
+
+import numpy as np
+from numpy.typing import NDArray
+
+from . import _native
+from .boundaries import PML
+from .grid import NonuniformGrid, UniformGrid
+from .materials import density_correction_coeff, modulus_correction_coeff
+from .sources import GaussianPulse, Probe, accumulated_times, waveform_table
+from .tuning import pick_tiling
+
+_FIELDS = ("p", "vx", "vy", "vz")
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def has_mi355x_backend() -> bool:
+    """True if the HIP library is built and a GPU is visible."""
+    try:
+        _native.load()
+        return _torch().cuda.is_available()
+    except (ImportError, OSError):
+        return False
+
+
+def _fast_waveform(source, times: NDArray[np.float64], dt: float) -> NDArray[np.float64]:
+    """Vectorised sampling when it reproduces the per-step calls exactly."""
+    if isinstance(source, GaussianPulse):
+        full = source.waveform(times, dt)
+        m = min(len(times), 257)
+        if np.array_equal(full[:m], waveform_table(source, times[:m], dt)):
+            return full
+    return waveform_table(source, times, dt)
+
+
+class FDTDSolver:
+    """3D acoustic FDTD solver (staggered Yee grid) on one MI355X GPU.
+
+    Args mirror core/solver.py:1507-1518; ``backend`` accepts "auto" or
+    "mi355x" ("native"/"gpu"/"python" are the reference's CPU/MPS backends and
+    are not provided by this package). ``device`` selects the GPU ordinal.
+    """
+
+    def __init__(self, shape=None, resolution=None, grid=None, c: float = 343.0, rho: float = 1.2,
+                 courant: float = 0.95,
+                 backend: Literal["auto", "mi355x"] = "auto",
+                 warn_energy_drift: bool = False, energy_drift_threshold: float = 0.01,
+                 device: int = 0, tiling: tuple[int, int] | None = None):
+        if grid is not None:
+            self._grid = grid
+            self.shape = tuple(grid.shape)
+        elif shape is not None and resolution is not None:
+            self._grid = UniformGrid(shape=tuple(shape), resolution=resolution)
+            self.shape = self._grid.shape
+        else:
+            raise ValueError("Must provide either 'grid' or both 'shape' and 'resolution'")
+        if backend not in ("auto", "mi355x"):
+            raise ValueError(f"backend={backend!r} is not provided by strata_fdtd_amd "
+                             "(use 'mi355x' or 'auto')")
+        torch = _torch()
+        _native.load()  # raise early if the library is missing
+        if not torch.cuda.is_available():
+            raise RuntimeError("backend='mi355x' needs a visible MI355X GPU (torch.cuda is unavailable)")
+
+        self.dx = self._grid.min_spacing
+        self.c = c
+        self.rho = rho
+        self.backend = "mi355x"
+        min_spacing = self._grid.min_spacing
+        self.dt = courant * (min_spacing / (self.c * np.sqrt(3)))
+        if self._grid.is_uniform:
+            self._coeff_p = -self.rho * self.c**2 * self.dt / self.dx
+            self._coeff_v = -self.dt / (self.rho * self.dx)
+        else:
+            self._coeff_p_base = -self.rho * self.c**2 * self.dt
+            self._coeff_v_base = -self.dt / self.rho
+            self._spacing_arrays = self._grid.get_spacing_arrays_for_stencil()
+
+        self._device = torch.device(f"cuda:{device}")
+        self._dev_index = device
+        nx, ny, nz = self.shape
+        dev = self._device
+        f32 = torch.float32
+        self._f = [[torch.zeros(self.shape, dtype=f32, device=dev) for _ in _FIELDS] for _ in range(2)]
+        self._flags = torch.empty(self.shape, dtype=torch.uint8, device=dev)
+        self._decay = [torch.ones(n, dtype=f32, device=dev) for n in self.shape]
+        self._step_dev = torch.zeros(1, dtype=torch.int64, device=dev)
+        self._health = torch.zeros(1, dtype=torch.int32, device=dev)
+        if not self._grid.is_uniform:
+            sa = self._spacing_arrays
+            self._tables = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in sa.items()}
+
+        self.geometry = np.ones(self.shape, dtype=bool)
+        self._material_id = np.zeros(self.shape, dtype=np.uint8)
+        self._materials: dict = {}
+        self._sources: list = []
+        self._probes: dict[str, Probe] = {}
+        self._probe_start: dict[str, int] = {}
+        self._boundaries: list = []
+        self._step_count = 0
+        self._time = 0.0
+        self._snapshots: list = []
+        self._velocity_snapshots: list = []
+        self._snapshot_interval: int | None = None
+        self._snapshot_velocity = False
+        self._warn_energy_drift = warn_energy_drift
+        self._energy_drift_threshold = energy_drift_threshold
+        self._energy_history: list[tuple[int, float, float]] = []
+        self._track_energy = False
+        self._energy_sample_interval = 1
+
+        # host mirrors of the current field set
+        self._mirror: list[np.ndarray | None] = [None] * 4
+        self._mirror_valid = False
+        self._exported: set[int] = set()
+
+        # device tables with growable capacity
+        self._cap = 0
+        self._wave = None
+        self._trace = None
+        self._probe_order: list[str] = []
+        self._dirty_geometry = True
+        self._dirty_sources = True
+        self._dirty_ade = True
+        self._ade_tensors: dict = {}
+
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        h = _native._vp()
+        _native.call("fdtd_ctx_create", device, stream, _native.C.byref(h))
+        self._ctx = h.value
+        self._stream = torch.cuda.current_stream(dev)
+        self._tiling = tiling
+        self._bind_static()
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx:
+            try:
+                _native.load().fdtd_ctx_destroy(ctx)
+            except Exception:
+                pass
+            self._ctx = None
+
+    # ------------------------------------------------------------------ props
+    time = property(lambda self: self._time)
+    step_count = property(lambda self: self._step_count)
+    grid = property(lambda self: self._grid)
+    using_native = property(lambda self: False)
+    using_gpu = property(lambda self: True)
+    has_materials = property(lambda self: len(self._materials) > 0)
+    material_count = property(lambda self: len(self._materials))
+
+    @property
+    def _cur(self) -> int:
+        return self._step_count & 1
+
+    def device_fields(self) -> dict:
+        """Current device tensors {p, vx, vy, vz} (read them after a sync)."""
+        return dict(zip(_FIELDS, self._f[self._cur]))
+
+    def _host_field(self, idx: int) -> np.ndarray:
+        if not self._mirror_valid:
+            torch = _torch()
+            torch.cuda.synchronize(self._device)
+            self._mirror = [t.cpu().numpy() for t in self._f[self._cur]]
+            self._mirror_valid = True
+        self._exported.add(idx)
+        return self._mirror[idx]
+
+    def _set_field(self, idx: int, value) -> None:
+        arr = np.ascontiguousarray(np.asarray(value, dtype=np.float32))
+        if arr.shape != self.shape:
+            raise ValueError(f"field shape {arr.shape} != solver shape {self.shape}")
+        self._flush_mirrors()
+        self._f[self._cur][idx].copy_(_torch().from_numpy(arr))
+        self._mirror_valid = False
+
+    p = property(lambda self: self._host_field(0), lambda self, v: self._set_field(0, v))
+    vx = property(lambda self: self._host_field(1), lambda self, v: self._set_field(1, v))
+    vy = property(lambda self: self._host_field(2), lambda self, v: self._set_field(2, v))
+    vz = property(lambda self: self._host_field(3), lambda self, v: self._set_field(3, v))
+
+    def _flush_mirrors(self) -> None:
+        """Upload host arrays handed out since the last device step."""
+        if self._exported and self._mirror_valid:
+            torch = _torch()
+            for idx in sorted(self._exported):
+                self._f[self._cur][idx].copy_(torch.from_numpy(self._mirror[idx]))
+        self._exported.clear()
+        self._mirror_valid = False
+
+    # ---------------------------------------------------------------- set-up
+    def _bind_static(self) -> None:
+        nx, ny, nz = self.shape
+        ctx = self._ctx
+        if self._grid.is_uniform:
+            _native.call("fdtd_ctx_set_grid", ctx, nx, ny, nz, 0, float(np.float32(self._coeff_v)),
+                         float(np.float32(self._coeff_p)), None, None, None, None, None, None)
+        else:
+            t = self._tables
+            _native.call("fdtd_ctx_set_grid", ctx, nx, ny, nz, 1, float(np.float32(self._coeff_v_base)),
+                         float(np.float32(self._coeff_p_base)),
+                         *[_native.ptr(t[f"inv_d{a}_face"]) for a in "xyz"],
+                         *[_native.ptr(t[f"inv_d{a}_cell"]) for a in "xyz"])
+        _native.call("fdtd_ctx_set_fields", ctx, *[_native.ptr(x) for x in self._f[0]],
+                     *[_native.ptr(x) for x in self._f[1]])
+        _native.call("fdtd_ctx_set_flags", ctx, _native.ptr(self._flags))
+        _native.call("fdtd_ctx_set_pml", ctx, *[_native.ptr(d) for d in self._decay])
+        _native.call("fdtd_ctx_set_counters", ctx, _native.ptr(self._step_dev), _native.ptr(self._health))
+        rows, planes = self._tiling or pick_tiling(self.shape, not self._grid.is_uniform, False)
+        _native.call("fdtd_ctx_set_tiling", ctx, rows, planes)
+
+    def set_tiling(self, rows: int, planes: int) -> None:
+        self._tiling = (rows, planes)
+        _native.call("fdtd_ctx_set_tiling", self._ctx, rows, planes)
+
+    def set_geometry(self, geometry) -> None:
+        """Bool mask (True = air), an SDF primitive, or a MaterializedGeometry
+        (core/solver.py:1682-1780)."""
+        from .geometry import MaterializedGeometry, SDFPrimitive
+        if isinstance(geometry, MaterializedGeometry):
+            mask, ids = geometry.voxelize_with_materials(self.grid)
+            if mask.shape != self.shape:
+                raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
+            self.geometry = mask
+            for mid, material in geometry.get_material_table().items():
+                if mid == 0:
+                    continue
+                if mid not in self._materials:
+                    self.register_material(material, material_id=mid)
+                region = ids == mid
+                if np.any(region):
+                    self.set_material_region(region, material_id=mid)
+        elif isinstance(geometry, SDFPrimitive):
+            mask = geometry.voxelize(self.grid)
+            if mask.shape != self.shape:
+                raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
+            self.geometry = mask
+            self._material_id.fill(0)
+            self._dirty_ade = True
+        else:
+            mask = np.asarray(geometry)
+            if mask.shape != self.shape:
+                raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
+            self.geometry = mask.astype(bool)
+        self._dirty_geometry = True
+
+    def _position_to_index(self, pos, what: str):
+        if isinstance(pos, tuple) and len(pos) == 3:
+            if any(isinstance(v, float) and v < max(self.shape) for v in pos):
+                gp = tuple(int(round(v / self.dx)) for v in pos)
+                for ax, (ix, dim) in enumerate(zip(gp, self.shape)):
+                    if not 0 <= ix < dim:
+                        raise ValueError(f"{what} {'xyz'[ax]} position {pos[ax]:.4f}m "
+                                         f"(index {ix}) is outside grid (0-{dim-1})")
+                return gp
+        return pos
+
+    def add_source(self, source) -> None:
+        """Point or plane source (core/solver.py:1782-1841)."""
+        if getattr(source, "source_type", "point") == "membrane":
+            raise NotImplementedError("membrane sources are not implemented on the mi355x backend")
+        if source.source_type == "point":
+            source.position = self._position_to_index(source.position, "Source")
+        self._sources.append(source)
+        self._dirty_sources = True
+
+    def add_probe(self, name: str, position) -> None:
+        """Pressure probe at a grid index or physical position (core/solver.py:1843-1885)."""
+        if name in self._probes:
+            raise ValueError(f"Probe '{name}' already exists")
+        position = self._position_to_index(position, "Probe")
+        self._probes[name] = Probe(name=name, position=tuple(int(v) for v in position))
+        self._probe_start[name] = self._step_count
+        self._dirty_sources = True
+
+    def add_microphone(self, *args, **kwargs):
+        raise NotImplementedError("microphones (trilinear gather) are not implemented on the mi355x backend")
+
+    def add_boundary(self, boundary) -> None:
+        boundary.initialize(self)
+        if isinstance(boundary, PML):
+            if any(isinstance(b, PML) for b in self._boundaries):
+                raise NotImplementedError("more than one PML per solver")
+            torch = _torch()
+            for ax, tab in enumerate(boundary.decay_tables(self.dt, self.shape)):
+                self._decay[ax].copy_(torch.from_numpy(np.ascontiguousarray(tab)))
+        elif type(boundary).__name__ != "RigidBoundary":
+            raise NotImplementedError(f"{type(boundary).__name__} is not implemented on the mi355x backend")
+        self._boundaries.append(boundary)
+
+    def enable_snapshots(self, interval: int, capture_velocity: bool = False) -> None:
+        self._snapshot_interval = interval
+        self._snapshot_velocity = capture_velocity
+
+    # ---- materials (core/solver.py:2810-2942) --------------------------------
+    def register_material(self, material, material_id: int | None = None) -> int:
+        if material_id is None:
+            used = set(self._materials)
+            for cand in range(1, 256):
+                if cand not in used:
+                    material_id = cand
+                    break
+            else:
+                raise ValueError("Maximum of 255 materials reached")
+        if not 1 <= material_id <= 255:
+            raise ValueError("material_id must be in range 1-255 (0 is reserved for air)")
+        if material_id in self._materials:
+            raise ValueError(f"Material ID {material_id} is already registered")
+        self._materials[material_id] = material
+        self._dirty_ade = True
+        return material_id
+
+    def set_material_region(self, mask, material_id: int) -> None:
+        if mask.shape != self.shape:
+            raise ValueError(f"Mask shape {mask.shape} doesn't match solver shape {self.shape}")
+        if material_id != 0 and material_id not in self._materials:
+            raise ValueError(f"Material ID {material_id} not registered. Use register_material() first.")
+        self._material_id[mask] = material_id
+        self._dirty_ade = True
+
+    def set_material_box(self, material_id, x_range, y_range, z_range) -> None:
+        mask = np.zeros(self.shape, dtype=bool)
+        mask[x_range[0]:x_range[1], y_range[0]:y_range[1], z_range[0]:z_range[1]] = True
+        self.set_material_region(mask, material_id)
+
+    def get_material_at(self, position):
+        mid = self._material_id[position]
+        return None if mid == 0 else self._materials.get(mid)
+
+    # ------------------------------------------------------ device preparation
+    def _upload_geometry(self) -> None:
+        torch = _torch()
+        air = torch.from_numpy(np.ascontiguousarray(self.geometry, dtype=np.uint8)).to(self._device)
+        mat = torch.from_numpy(np.ascontiguousarray(self._material_id)).to(self._device)
+        nx, ny, nz = self.shape
+        _native.call("fdtd_build_flags", _native.ptr(air), _native.ptr(mat), _native.ptr(self._flags),
+                     nx, ny, nz, self._stream.cuda_stream)
+        self._air_dev = air
+        self._mat_dev = mat
+        self._dirty_geometry = False
+
+    def _pole_list(self):
+        poles = []
+        for mid, material in self._materials.items():
+            for pole in material.poles:
+                poles.append((mid, material, pole))
+        return poles
+
+    def _setup_ade(self) -> None:
+        torch = _torch()
+        poles = self._pole_list()
+        if not poles:
+            _native.call("fdtd_ctx_set_ade", self._ctx, 0, *([None] * 12), 0.0)
+            self._ade_tensors = {}
+            self._dirty_ade = False
+            return
+        mids = sorted({m for m, _, _ in poles})
+        sel = np.isin(self._material_id, mids)
+        if not np.any(sel):
+            _native.call("fdtd_ctx_set_ade", self._ctx, 0, *([None] * 12), 0.0)
+            self._ade_tensors = {}
+            self._dirty_ade = False
+            return
+        idx = np.nonzero(sel)
+        lo = np.array([int(a.min()) for a in idx], dtype=np.int32)
+        hi = np.array([int(a.max()) + 1 for a in idx], dtype=np.int32)
+        box_n = int(np.prod(hi - lo))
+        n = len(poles)
+        inv_dx = (1.0 / self.dx) if self._grid.is_uniform else None
+        mat_id = np.array([m for m, _, _ in poles], dtype=np.int32)
+        target = np.array([0 if p.target == "density" else 1 for _, _, p in poles], dtype=np.int32)
+        lorentz = np.array([0 if p.is_debye else 1 for _, _, p in poles], dtype=np.int32)
+        a = np.zeros(n, np.float32)
+        b = np.zeros(n, np.float32)
+        d = np.zeros(n, np.float32)
+        corr = np.zeros(n, np.float32)
+        for q, (_m, material, pole) in enumerate(poles):
+            co = pole.fdtd_coefficients(self.dt)
+            a[q], b[q] = co[0], co[1]
+            if not pole.is_debye:
+                d[q] = co[2]
+            if pole.target == "density":
+                corr[q] = density_correction_coeff(self.dt, material.rho_inf, inv_dx)
+            else:
+                corr[q] = modulus_correction_coeff(self.dt, material.K_inf)
+        J = torch.zeros((n, box_n), dtype=torch.float32, device=self._device)
+        Jp = torch.zeros((n, box_n), dtype=torch.float32, device=self._device)
+        mat_dev = torch.from_numpy(np.ascontiguousarray(self._material_id)).to(self._device)
+        keep = dict(J=J, Jp=Jp, mat=mat_dev, mat_id=mat_id, target=target, lorentz=lorentz, a=a, b=b,
+                    d=d, corr=corr, lo=lo, hi=hi)
+        cp = lambda arr: arr.ctypes.data  # noqa: E731 - host arrays for the C ABI
+        _native.call("fdtd_ctx_set_ade", self._ctx, n, cp(mat_id), cp(target), cp(lorentz), cp(a), cp(b),
+                     cp(d), cp(corr), _native.ptr(mat_dev), cp(lo), cp(hi), _native.ptr(J), _native.ptr(Jp),
+                     float(np.float32(1.0 / self.dx)))
+        self._ade_tensors = keep
+        self._dirty_ade = False
+        rows, planes = self._tiling or pick_tiling(self.shape, not self._grid.is_uniform, True)
+        _native.call("fdtd_ctx_set_tiling", self._ctx, rows, planes)
+
+    def _ensure_capacity(self, total_steps: int) -> None:
+        """Waveform and trace tables must cover steps [0, total_steps)."""
+        torch = _torch()
+        if total_steps <= self._cap and not self._dirty_sources:
+            return
+        cap = self._cap
+        if total_steps > cap:
+            cap = max(total_steps, 2 * cap, 1024)
+        srcs = self._sources
+        if srcs:
+            times = accumulated_times(self.dt, cap)
+            wave = np.stack([_fast_waveform(s, times, self.dt) for s in srcs]).astype(np.float64)
+            self._wave = torch.from_numpy(np.ascontiguousarray(wave)).to(self._device)
+        else:
+            self._wave = None
+        names = list(self._probes)
+        new_trace = torch.zeros((cap, max(len(names), 1)), dtype=torch.float32, device=self._device)
+        if self._trace is not None and self._step_count:
+            n_old = min(self._step_count, self._trace.shape[0])
+            for q_new, name in enumerate(names):
+                if name in self._probe_order:
+                    q_old = self._probe_order.index(name)
+                    new_trace[:n_old, q_new] = self._trace[:n_old, q_old]
+        self._trace = new_trace
+        self._probe_order = names
+        kind, a, b = [], [], []
+        for s in srcs:
+            if s.source_type == "point":
+                i, j, k = (int(v) for v in s.position)
+                kind.append(0)
+                a.append(np.ravel_multi_index((i, j, k), self.shape))
+                b.append(0)
+            else:
+                kind.append(1)
+                a.append(int(s.position["axis"]))
+                b.append(int(s.position["index"]))
+        kind = np.array(kind or [0], dtype=np.int32)
+        a = np.array(a or [0], dtype=np.int64)
+        b = np.array(b or [0], dtype=np.int64)
+        _native.call("fdtd_ctx_set_sources", self._ctx, len(srcs), kind.ctypes.data, a.ctypes.data,
+                     b.ctypes.data, _native.ptr(self._wave), cap)
+        flat = np.array([np.ravel_multi_index(self._probes[n].position, self.shape) for n in names] or [0],
+                        dtype=np.int64)
+        _native.call("fdtd_ctx_set_probes", self._ctx, len(names), flat.ctypes.data,
+                     _native.ptr(self._trace), cap)
+        self._cap = cap
+        self._dirty_sources = False
+
+    def _prepare(self, n_steps: int) -> None:
+        if self._dirty_geometry:
+            self._upload_geometry()
+        if self._dirty_ade:
+            self._setup_ade()
+        self._ensure_capacity(self._step_count + n_steps)
+        self._flush_mirrors()
+
+    def _advance_host(self, n: int) -> None:
+        for _ in range(n):
+            self._time += self.dt
+        self._step_count += n
+        self._mirror_valid = False
+
+    # ------------------------------------------------------------------ steps
+    def step(self) -> None:
+        """Advance one time step (core/solver.py:2003-2077)."""
+        self._prepare(1)
+        n_before, t_before = self._step_count, self._time
+        _native.call("fdtd_step_n", self._ctx, 1)
+        self._advance_host(1)
+        self._after_step_hooks(n_before, t_before)
+
+    def _after_step_hooks(self, n_before: int, t_before: float) -> None:
+        if self._snapshot_interval and n_before % self._snapshot_interval == 0:
+            self._snapshots.append((t_before, self._host_field(0).copy()))
+            self._exported.discard(0)
+            if self._snapshot_velocity:
+                self._velocity_snapshots.append((t_before, *self._centred_velocity()))
+        if self._track_energy and self._step_count % self._energy_sample_interval == 0:
+            self._energy_history.append((self._step_count, self._time, self.compute_energy()))
+
+    def step_n(self, n: int, graph_chunk: int = 64) -> None:
+        """Advance n steps on the device without host round trips."""
+        if n <= 0:
+            return
+        self._prepare(n)
+        done = 0
+        if graph_chunk and n >= 2 * graph_chunk:
+            reps = n // graph_chunk
+            _native.call("fdtd_graph_run", self._ctx, graph_chunk, reps)
+            done = reps * graph_chunk
+        if n - done:
+            _native.call("fdtd_step_n", self._ctx, n - done)
+        self._advance_host(n)
+
+    def run(self, duration: float, progress: bool = False, track_energy: bool = False,
+            energy_sample_interval: int = 1, output_file: str | None = None,
+            script_content: str | None = None, callback: Callable[[int], None] | None = None,
+            snapshot_interval: int | None = None) -> None:
+        """Run ceil(duration/dt) steps (core/solver.py:2520-2606)."""
+        if output_file:
+            raise NotImplementedError("HDF5 output is not implemented on the mi355x backend")
+        n_steps = int(np.ceil(duration / self.dt))
+        self._track_energy = track_energy
+        self._energy_sample_interval = energy_sample_interval
+        if track_energy and not self._energy_history:
+            self._energy_history.append((self._step_count, self._time, self.compute_energy()))
+        per_step = bool(callback or track_energy or self._snapshot_interval or progress)
+        if per_step:
+            for _ in range(n_steps):
+                self.step()
+                if callback:
+                    callback(self._step_count - 1)
+        else:
+            self.step_n(n_steps)
+        self.check_health()
+        if self._warn_energy_drift and len(self._energy_history) >= 2:
+            rep = self.energy_report()
+            if abs(rep["energy_change_percent"]) > self._energy_drift_threshold * 100:
+                warnings.warn(f"Energy drift detected: {rep['energy_change_percent']:.2f}% change "
+                              f"(threshold: {self._energy_drift_threshold * 100:.1f}%). "
+                              f"Status: {rep['conservation_status']}", UserWarning, stacklevel=2)
+
+    def synchronize(self) -> None:
+        _torch().cuda.synchronize(self._device)
+
+    def check_health(self) -> int:
+        """Read the device health word: bit0 non-finite probe sample, bit1 table overrun."""
+        h = int(self._health.item())
+        if h & 2:
+            raise RuntimeError("device step ran past the source/probe tables (internal error)")
+        if h & 1:
+            warnings.warn("non-finite probe sample detected (simulation unstable?)", RuntimeWarning,
+                          stacklevel=2)
+        return h
+
+    # ---------------------------------------------------------------- results
+    def get_probe_data(self, name: str | None = None) -> dict[str, NDArray[np.floating]]:
+        if name is not None and name not in self._probes:
+            raise KeyError(f"Probe '{name}' not found")
+        names = [name] if name is not None else list(self._probes)
+        if self._trace is None or self._step_count == 0 or self._probe_order != list(self._probes):
+            return {n: np.zeros(0, dtype=np.float32) for n in names}
+        tr = self._trace[: self._step_count].cpu().numpy()
+        out = {}
+        for n in names:
+            q = self._probe_order.index(n)
+            out[n] = np.ascontiguousarray(tr[self._probe_start.get(n, 0):, q], dtype=np.float32)
+        return out
+
+    def get_snapshots(self):
+        return self._snapshots
+
+    def get_velocity_snapshots(self):
+        return self._velocity_snapshots
+
+    def _centred_velocity(self):
+        out = []
+        for ax, idx in ((0, 1), (1, 2), (2, 3)):
+            v = self._host_field(idx)
+            self._exported.discard(idx)
+            c = np.zeros(self.shape, dtype=np.float32)
+            sl_hi = [slice(None)] * 3
+            sl_lo = [slice(None)] * 3
+            sl_hi[ax] = slice(1, None)
+            sl_lo[ax] = slice(None, -1)
+            c[tuple(sl_hi)] = 0.5 * (v[tuple(sl_lo)] + v[tuple(sl_hi)])
+            first = [slice(None)] * 3
+            first[ax] = 0
+            c[tuple(first)] = 0.5 * v[tuple(first)]
+            out.append(c)
+        return out
+
+    def compute_energy(self) -> float:
+        """1/2 sum_air p^2/(rho c^2) dV + 1/2 rho sum_air |v|^2 dV (core/solver.py:2689-2706),
+        reduced on the device in float64."""
+        torch = _torch()
+        self._flush_mirrors()
+        if self._dirty_geometry:
+            self._upload_geometry()
+        p, vx, vy, vz = (t.double() for t in self._f[self._cur])
+        air = self._air_dev.bool()
+        dV = self.dx**3
+        pe = 0.5 * torch.sum(p[air] ** 2).item() / (self.rho * self.c**2) * dV
+        ke = 0.5 * self.rho * torch.sum((vx**2 + vy**2 + vz**2)[air]).item() * dV
+        return pe + ke
+
+    def get_energy_history(self):
+        return self._energy_history.copy()
+
+    def energy_report(self) -> dict:
+        if not self._energy_history:
+            raise ValueError("No energy history recorded. Call run() with track_energy=True first.")
+        e = np.array([x for _, _, x in self._energy_history])
+        e0, e1 = e[0], e[-1]
+        if e0 == 0:
+            pct = 0.0 if e1 == 0 else float("inf")
+        else:
+            pct = (e1 - e0) / e0 * 100
+        status = "stable" if abs(pct) <= 1.0 else ("growing" if pct > 0 else "decaying")
+        return {"initial_energy": float(e0), "final_energy": float(e1), "max_energy": float(e.max()),
+                "min_energy": float(e.min()), "energy_change_percent": pct,
+                "conservation_status": status, "n_samples": len(self._energy_history)}
+
+    def reset(self) -> None:
+        """Zero fields, clock, probes, energy history and ADE state (core/solver.py:2781-2800)."""
+        for s in self._f:
+            for t in s:
+                t.zero_()
+        self._step_dev.zero_()
+        self._health.zero_()
+        if self._trace is not None:
+            self._trace.zero_()
+        for key in ("J", "Jp"):
+            if key in self._ade_tensors:
+                self._ade_tensors[key].zero_()
+        self._step_count = 0
+        self._time = 0.0
+        self._snapshots.clear()
+        self._velocity_snapshots.clear()
+        self._energy_history.clear()
+        self._track_energy = False
+        self._exported.clear()
+        self._mirror_valid = False
+        for name in self._probe_start:
+            self._probe_start[name] = 0
+        for b in self._boundaries:
+            b.reset()
+
+    def get_sample_rate(self) -> float:
+        return 1.0 / self.dt
+
+    def get_frequency_response(self, probe_name: str, n_fft: int | None = None):
+        data = self.get_probe_data(probe_name)[probe_name]
+        if n_fft is None:
+            n_fft = int(2 ** np.ceil(np.log2(len(data))))
+        spectrum = np.fft.rfft(data, n=n_fft)
+        return np.fft.rfftfreq(n_fft, self.dt), np.abs(spectrum)
+
+
+__all__ = ["FDTDSolver", "has_mi355x_backend", "NonuniformGrid", "UniformGrid"]

@@ -1,0 +1,64 @@
+"""Launch-shape selection for the fused step kernel.
+
+A launch is described by (rows, planes): a wave owns a 64-wide k segment of
+``rows`` consecutive j rows and marches ``planes`` planes along i. The
+heuristic below gives each of the 256 CUs several resident waves while keeping
+the per-strip prologue (one extra plane) and the j-halo rows small. The
+autotuner (``strata_fdtd_amd.autotune``) measures candidates on the device and
+caches the winner per grid shape; ``pick_tiling`` consults that cache first.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+NUM_CU = 256
+ROW_CHOICES = (2, 4, 8, 16)
+_CACHE_ENV = "STRATA_FDTD_AMD_TUNE_CACHE"
+_DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "tune_cache.json")
+
+
+def cache_path() -> str:
+    return os.environ.get(_CACHE_ENV, _DEFAULT_CACHE)
+
+
+def _load_cache() -> dict:
+    try:
+        with open(cache_path()) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def shape_key(shape, nonuniform: bool, ade: bool) -> str:
+    return f"{shape[0]}x{shape[1]}x{shape[2]}:{'nu' if nonuniform else 'u'}:{'ade' if ade else 'plain'}"
+
+
+def heuristic_tiling(shape) -> tuple[int, int]:
+    nx, ny, nz = shape
+    rows = 4
+    segs = (nz + 63) // 64
+    strips_ji = segs * ((ny + rows - 1) // rows)
+    # aim for ~24 waves per CU, but never fewer than 4 planes per strip
+    target = 24 * NUM_CU
+    planes = max(4, min(nx, (nx * strips_ji + target - 1) // target))
+    return rows, planes
+
+
+def pick_tiling(shape, nonuniform: bool = False, ade: bool = False) -> tuple[int, int]:
+    hit = _load_cache().get(shape_key(shape, nonuniform, ade))
+    if hit:
+        return int(hit[0]), int(hit[1])
+    return heuristic_tiling(shape)
+
+
+def store_tiling(shape, nonuniform: bool, ade: bool, rows: int, planes: int, ms: float) -> None:
+    cache = _load_cache()
+    cache[shape_key(shape, nonuniform, ade)] = [rows, planes, ms]
+    path = cache_path()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump(cache, fh, indent=1, sort_keys=True)
+    os.replace(tmp, path)
