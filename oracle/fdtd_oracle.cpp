@@ -10,7 +10,7 @@
 //   rigid-face zeroing   _kernels/src/boundaries.cpp:13-89
 //   pressure update      _kernels/src/fdtd_step.cpp:84-212  (nonuniform :309-439)
 //   PML damping          _kernels/src/pml.cpp:47-149, decay tables as
-//                        boundaries/_boundaries.py:340-377 builds them
+//                        initialize_pml (pml.cpp:13-45) builds them
 //   ADE updates          _kernels/src/ade.cpp:25-222, corrections :228-473,
 //                        divergence :479-692
 //   step ordering        core/solver.py:2003-2077 (python ordering, which is the
@@ -250,6 +250,12 @@ void orc_divergence_f32(float* div, const float* vx, const float* vy, const floa
                         int nx, int ny, int nz, float inv_u,
                         const float* icx, const float* icy, const float* icz) {
     divergence<float>(div, vx, vy, vz, nx, ny, nz, inv_u, icx, icy, icz);
+}
+
+// PML decay as the native initialize_pml forms it: std::exp(-sigma * dt) with
+// float sigma and float dt (_kernels/src/pml.cpp:13-45).
+void orc_pml_decay(const float* sigma, int n, float dt, float* out) {
+    for (int i = 0; i < n; ++i) out[i] = std::exp(-sigma[i] * dt);
 }
 
 int orc_num_threads(void) {

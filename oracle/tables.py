@@ -80,14 +80,23 @@ def pml_profile_nonuniform(coords: np.ndarray, spacing: np.ndarray, depth: int, 
 
 
 def pml_decay(sigma: np.ndarray, dt: float) -> np.ndarray:
-    """exp(-sigma*dt) as the numpy fallback forms it: float32 throughout.
+    """float32 exp(-sigma*dt) as the native initialize_pml forms it
+    (_kernels/src/pml.cpp:13-45): float sigma times float dt, then std::exp on
+    float. Evaluated by the C++ oracle so the libm expf is the same one the
+    reference's build uses.
 
-    boundaries/_boundaries.py:340-351 evaluates ``np.exp(-self._sigma_x * dt)``
-    on a float32 array with a python-float dt (NEP 50 weak scalar), so negate,
-    multiply and exp all run in float32. That fallback is what runs as shipped
-    (the native decay precompute is unreachable).
+    (The reference's numpy fallback, boundaries/_boundaries.py:340-377, forms
+    the table in float64 because ``solver.dt`` is a numpy float64 scalar; the
+    native form is the one the parity target names.)
     """
-    return np.exp(-sigma * dt)
+    import ctypes as C
+    from . import cpu
+    sig = np.ascontiguousarray(sigma, dtype=np.float32)
+    out = np.empty_like(sig)
+    fp = C.POINTER(C.c_float)
+    cpu._lib("oracle").orc_pml_decay(sig.ctypes.data_as(fp), C.c_int(len(sig)), C.c_float(float(dt)),
+                                     out.ctypes.data_as(fp))
+    return out
 
 
 def stretched_coords(n: int, base_dx: float, stretch: float, center_fine: bool = True) -> np.ndarray:

@@ -5,17 +5,33 @@
 ``apply_velocity/apply_pressure(solver)``, ``reset()``,
 ``get_interior_slice()``). On ``backend="mi355x"`` the damping itself runs
 inside the fused step kernel; this class only builds the three float32 decay
-tables ``exp(-sigma*dt)`` exactly as the reference's numpy path does
-(:340-377) and hands them to the solver. Interior entries are exactly 1.0f, so
-the kernel's multiply is an identity there: damping is confined to the slabs.
+tables ``exp(-sigma*dt)`` exactly as the native ``initialize_pml`` does
+(_kernels/src/pml.cpp:13-45) and hands them to the solver. Interior entries are
+exactly 1.0f, so the kernel's multiply is an identity there: damping is
+confined to the slabs.
 """
 
 from __future__ import annotations
 
+import ctypes
+import ctypes.util
 from typing import Literal
 
 import numpy as np
 from numpy.typing import NDArray
+
+
+_libm = None
+
+
+def _expf(x: np.ndarray) -> np.ndarray:
+    """C's expf on each float32 value (the function std::exp(float) calls)."""
+    global _libm
+    if _libm is None:
+        _libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        _libm.expf.restype = ctypes.c_float
+        _libm.expf.argtypes = [ctypes.c_float]
+    return np.array([_libm.expf(float(v)) for v in np.asarray(x, dtype=np.float32)], dtype=np.float32)
 
 
 class RigidBoundary:
@@ -112,13 +128,17 @@ class PML:
     def decay_tables(self, dt: float, shape) -> tuple[NDArray[np.float32], ...]:
         """float32 exp(-sigma*dt) per axis; an axis without PML gets ones.
 
-        Formed as the reference's numpy path does (``np.exp(-sigma * dt)`` on a
-        float32 array with a python-float dt, boundaries/_boundaries.py:340-377),
-        so every operation runs in float32.
+        Formed as the native initialize_pml does (_kernels/src/pml.cpp:13-45):
+        float sigma times float dt, then std::exp on float (C's expf). Interior
+        cells have sigma = 0, so their decay is exactly 1.0f.
         """
         out = []
+        dt32 = np.float32(dt)
         for sig, n in ((self._sigma_x, shape[0]), (self._sigma_y, shape[1]), (self._sigma_z, shape[2])):
-            out.append(np.ones(n, dtype=np.float32) if sig is None else np.exp(-sig * dt))
+            if sig is None:
+                out.append(np.ones(n, dtype=np.float32))
+            else:
+                out.append(_expf((-sig) * dt32))
         return tuple(out)
 
     # ---- boundary protocol -------------------------------------------------

@@ -9,9 +9,18 @@ must match the reference bit for bit, so every formula uses the same float64
 numpy operations in the same order (norms via ``np.linalg.norm``, the cylinder
 projection via ``np.dot``): a point is air when ``sdf <= 0``.
 
-Voxelisation evaluates the cell centres in slabs of whole i-planes to bound
-host memory on large grids; every operation is row-wise, so the result does not
-depend on the slab size (checked by the tests against single-pass evaluation).
+Fast exact voxelisation. For the primitives and CSG nodes defined here the
+mask is computed as a boolean combination instead of a float min/max tree:
+``min(d1, d2) <= 0`` iff ``d1 <= 0 or d2 <= 0``; ``max(d1, d2) <= 0`` iff both;
+``max(b, -s) <= 0`` iff ``b <= 0 and not (s < 0)`` (and the strict forms dually).
+Box masks are separable per axis (all three |x - c| - s/2 <= 0, the same float64
+values the reference computes; differences of millimetre coordinates are never
+small enough to underflow the norm). Sphere and cylinder formulas are evaluated
+with the reference's expressions only on the cells inside their bounding box
+plus a two-cell margin; outside it the distance is positive by construction.
+Every value is therefore the reference's own float64 value, so the mask is
+identical; the tests compare it with the direct single-pass evaluation. Other
+SDF types fall back to the direct evaluation (in slabs of whole i-planes).
 """
 
 from __future__ import annotations
@@ -44,18 +53,80 @@ class SDFPrimitive(ABC):
 
     def voxelize(self, grid, slab_points: int | None = VOXEL_SLAB_POINTS) -> NDArray[np.bool_]:
         """Boolean mask, True where sdf(cell centre) <= 0 (geometry/sdf.py:99-127)."""
+        return _mask(self, grid, strict=False)
+
+    def voxelize_direct(self, grid, slab_points: int | None = VOXEL_SLAB_POINTS,
+                        strict: bool = False) -> NDArray[np.bool_]:
+        """The reference's evaluation: sdf at every cell centre, then <= 0
+        (< 0 when strict). ``slab_points=None`` evaluates all cells at once."""
         nx, ny, nz = grid.shape
-        xc, yc, zc = grid.x_coords, grid.y_coords, grid.z_coords
-        plane = ny * nz
-        step = nx if slab_points is None else max(1, slab_points // max(plane, 1))
-        mask = np.empty(grid.shape, dtype=bool)
-        for i0 in range(0, nx, step):
-            i1 = min(nx, i0 + step)
-            X, Y, Z = np.meshgrid(xc[i0:i1], yc, zc, indexing="ij")
-            points = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
-            d = self.sdf(points)
-            mask[i0:i1] = d.reshape((i1 - i0, ny, nz)) <= 0
-        return mask
+        return _direct(self, grid.x_coords, grid.y_coords, grid.z_coords, slab_points, strict)
+
+
+def _direct(node, xc, yc, zc, slab_points, strict) -> NDArray[np.bool_]:
+    nx, ny, nz = len(xc), len(yc), len(zc)
+    plane = ny * nz
+    step = nx if slab_points is None else max(1, slab_points // max(plane, 1))
+    mask = np.empty((nx, ny, nz), dtype=bool)
+    for i0 in range(0, nx, step):
+        i1 = min(nx, i0 + step)
+        X, Y, Z = np.meshgrid(xc[i0:i1], yc, zc, indexing="ij")
+        points = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+        d = node.sdf(points).reshape((i1 - i0, ny, nz))
+        mask[i0:i1] = (d < 0) if strict else (d <= 0)
+    return mask
+
+
+def _index_range(coords, lo, hi, margin):
+    a = int(np.searchsorted(coords, lo - margin, side="left"))
+    b = int(np.searchsorted(coords, hi + margin, side="right"))
+    return a, b
+
+
+def _mask(node, grid, strict: bool) -> NDArray[np.bool_]:
+    """Exact mask of ``sdf <= 0`` (``< 0`` when strict) for a CSG tree."""
+    xc, yc, zc = grid.x_coords, grid.y_coords, grid.z_coords
+    shape = (len(xc), len(yc), len(zc))
+    if isinstance(node, Box):
+        half = node.size / 2
+        ms = []
+        for ax, c in enumerate((xc, yc, zc)):
+            q = np.abs(c - node.center[ax]) - half[ax]
+            ms.append((q < 0) if strict else (q <= 0))
+        return ms[0][:, None, None] & ms[1][None, :, None] & ms[2][None, None, :]
+    if isinstance(node, (Sphere, Cylinder)):
+        if isinstance(node, Cylinder) and np.count_nonzero(node.axis_normalized) != 1:
+            return _direct(node, xc, yc, zc, None if np.prod(shape) <= (1 << 25) else VOXEL_SLAB_POINTS,
+                           strict)
+        lo, hi = node.bounding_box
+        margin = 2.0 * max(float(np.max(np.diff(c))) if len(c) > 1 else 0.0 for c in (xc, yc, zc))
+        out = np.zeros(shape, dtype=bool)
+        rng = [_index_range(c, lo[ax], hi[ax], margin) for ax, c in enumerate((xc, yc, zc))]
+        if any(b <= a for a, b in rng):
+            return out
+        (a0, b0), (a1, b1), (a2, b2) = rng
+        out[a0:b0, a1:b1, a2:b2] = _direct(node, xc[a0:b0], yc[a1:b1], zc[a2:b2], VOXEL_SLAB_POINTS, strict)
+        return out
+    if isinstance(node, Union):
+        out = np.zeros(shape, dtype=bool)
+        for ch in node.children:
+            out |= _mask(ch, grid, strict)
+        return out
+    if isinstance(node, Intersection):
+        out = np.ones(shape, dtype=bool)
+        for ch in node.children:
+            out &= _mask(ch, grid, strict)
+        return out
+    if isinstance(node, Difference):
+        out = _mask(node.base, grid, strict)
+        for sub in node.subtracted:
+            # max(b, -s) <= 0  <=>  b <= 0 and s >= 0  <=>  b <= 0 and not (s < 0)
+            # max(b, -s) <  0  <=>  b <  0 and s >  0  <=>  b <  0 and not (s <= 0)
+            out &= ~_mask(sub, grid, not strict)
+        return out
+    if isinstance(node, HelmholtzResonator):
+        return _mask(node._combined, grid, strict)
+    return _direct(node, xc, yc, zc, VOXEL_SLAB_POINTS, strict)
 
 
 def _points(points) -> NDArray[np.float64]:

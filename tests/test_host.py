@@ -1,0 +1,181 @@
+"""Host-side precompute of the product package vs the oracle (no GPU needed).
+
+The product builds every O(n) table the kernels read (time step, coefficients,
+PML decays, nonuniform spacings, waveform tables, pole coefficients) and the
+voxelised masks. Each is compared bit for bit with the oracle's independent
+restatement, and the voxeliser with the direct (reference-style) evaluation.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import tables as T
+from strata_fdtd_amd import (AIR_20C, PML, Box, Cylinder, Difference, GaussianPulse,
+                             HelmholtzResonator, Intersection, MaterializedGeometry, NonuniformGrid,
+                             Pole, PoleType, SimpleMaterial, Sphere, UniformGrid, Union)
+from strata_fdtd_amd.geometry import _mask
+from strata_fdtd_amd.materials import density_correction_coeff, modulus_correction_coeff
+from strata_fdtd_amd.sources import accumulated_times, waveform_table
+
+
+class _FakeSolver:
+    """The attributes PML.initialize reads, without a GPU."""
+
+    def __init__(self, grid, c=343.0, rho=1.204, courant=0.95):
+        self.grid = grid
+        self.shape = grid.shape
+        self.c = c
+        self.dx = grid.min_spacing
+        self.dt = courant * (grid.min_spacing / (c * np.sqrt(3)))
+
+
+def test_time_step_and_coefficients_match_oracle():
+    s = _FakeSolver(UniformGrid((10, 10, 10), 1e-3))
+    assert s.dt == T.time_step(1e-3, 343.0)
+    cv, cp = T.uniform_coefficients(s.dt, 1e-3, 1.204, 343.0)
+    assert np.float32(cv) == np.float32(-s.dt / (1.204 * 1e-3))
+    assert np.float32(cp) == np.float32(-1.204 * 343.0**2 * s.dt / 1e-3)
+
+
+@pytest.mark.parametrize("depth,axes", [(10, "all"), (4, "x"), (6, ("y", "z"))])
+def test_pml_decay_tables_match_oracle(oracle_lib, depth, axes):
+    grid = UniformGrid((40, 36, 44), 1e-3)
+    s = _FakeSolver(grid)
+    pml = PML(depth=depth, axis=axes)
+    pml.initialize(s)
+    tabs = pml.decay_tables(s.dt, grid.shape)
+    sm = T.pml_sigma_max(depth, 3, 343.0, 1e-3)
+    for ax, name in enumerate("xyz"):
+        if name in pml.axes:
+            want = T.pml_decay(T.pml_profile_uniform(grid.shape[ax], depth, 3, sm), s.dt)
+        else:
+            want = np.ones(grid.shape[ax], np.float32)
+        assert tabs[ax].dtype == np.float32
+        assert np.array_equal(tabs[ax], want), name
+        assert np.all(tabs[ax][depth:-depth] == 1.0)
+
+
+def test_nonuniform_tables_match_oracle(oracle_lib):
+    grid = NonuniformGrid.from_stretch((30, 34, 40), 1e-3, stretch_x=1.04, stretch_z=1.06)
+    want = T.stencil_arrays(T.stretched_coords(30, 1e-3, 1.04), T.stretched_coords(34, 1e-3, 1.0),
+                            T.stretched_coords(40, 1e-3, 1.06))
+    got = grid.get_spacing_arrays_for_stencil()
+    for k, v in want.items():
+        assert np.array_equal(got[k], v), k
+    s = _FakeSolver(grid)
+    pml = PML(depth=5)
+    pml.initialize(s)
+    tabs = pml.decay_tables(s.dt, grid.shape)
+    sm = T.pml_sigma_max(5, 3, 343.0, grid.min_spacing)
+    for ax, c in enumerate((grid.x_coords, grid.y_coords, grid.z_coords)):
+        prof = T.pml_profile_nonuniform(c, T.cell_spacing(c), 5, 3, sm)
+        assert np.array_equal(tabs[ax], T.pml_decay(prof, s.dt))
+
+
+def test_from_stretch_odd_count_keeps_reference_quirk():
+    """core/grid.py:386-398 yields n-1 centres for an odd n with center_fine."""
+    g = NonuniformGrid.from_stretch((21, 20, 20), 1e-3, stretch_x=1.05)
+    assert g.shape == (20, 20, 20)
+    assert np.array_equal(g.x_coords, T.stretched_coords(21, 1e-3, 1.05))
+
+
+def test_waveform_table_matches_oracle():
+    dt = T.time_step(1e-3, 343.0)
+    for f in (200.0, 1000.0, 4000.0):
+        src = GaussianPulse(position=(1, 1, 1), frequency=f)
+        t = accumulated_times(dt, 3000)
+        assert np.array_equal(t, T.accumulated_times(dt, 3000))
+        assert np.array_equal(waveform_table(src, t, dt), T.waveform_table(t, f))
+
+
+def test_pole_and_correction_coefficients_match_oracle():
+    dt = T.time_step(1e-3, 343.0)
+    p = Pole(PoleType.DEBYE, 0.3, "density", tau=2e-5)
+    assert p.fdtd_coefficients(dt) == T.debye_coefficients(2e-5, 0.3, dt)
+    q = Pole(PoleType.LORENTZ, 0.4, "modulus", omega_0=2e4, gamma=300.0)
+    assert q.fdtd_coefficients(dt) == T.lorentz_coefficients(2e4, 300.0, 0.4, dt)
+    assert float(density_correction_coeff(dt, 1.204, 1e3)) == T.native_density_correction(dt, 1.204, 1e-3)
+    assert float(modulus_correction_coeff(dt, 1.204 * 343.0**2)) == T.native_modulus_correction(
+        dt, 1.204 * 343.0**2)
+
+
+# ------------------------------------------------------------------ voxeliser
+def _random_tree(rng, depth=0):
+    def r(a, b, n=None, d=3):
+        return np.round(rng.uniform(a, b, n), d)
+    t = rng.integers(0, 6 if depth < 2 else 3)
+    c = r(0.01, 0.08, 3, 4)
+    if t == 0:
+        return Box(center=tuple(c), size=tuple(r(0.005, 0.05, 3)))
+    if t == 1:
+        return Sphere(center=tuple(c), radius=float(r(0.005, 0.03)))
+    if t == 2:
+        d = np.zeros(3)
+        d[rng.integers(0, 3)] = r(0.01, 0.05)
+        if rng.random() < 0.2:
+            d = r(-0.03, 0.03, 3)  # oblique axis: direct evaluation path
+        return Cylinder(p1=tuple(c), p2=tuple(c + d), radius=float(r(0.003, 0.02)))
+    kids = [_random_tree(rng, depth + 1) for _ in range(rng.integers(1, 4))]
+    return (Union, Intersection, lambda *k: Difference(k[0], *k[1:]))[t - 3](*kids)
+
+
+def test_exact_voxeliser_equals_direct_evaluation():
+    """Boolean CSG voxelisation == sdf <= 0 evaluated at every cell, including
+    cells exactly on a surface (grid-aligned coordinates create ties)."""
+    rng = np.random.default_rng(0)
+    grid = UniformGrid((64, 56, 48), 1e-3)
+    ties = 0
+    for _ in range(40):
+        node = _random_tree(rng)
+        for strict in (False, True):
+            assert np.array_equal(_mask(node, grid, strict), node.voxelize_direct(grid, None, strict))
+        ties += int((node.voxelize_direct(grid, None, False) != node.voxelize_direct(grid, None, True)).sum())
+    assert ties > 0
+
+
+def test_voxeliser_slabs_equal_single_pass():
+    node = Difference(Box(center=(0.03, 0.03, 0.03), size=(0.05, 0.04, 0.05)),
+                      Sphere(center=(0.03, 0.03, 0.03), radius=0.012))
+    grid = UniformGrid((48, 40, 44), 1e-3)
+    assert np.array_equal(node.voxelize_direct(grid, 2000), node.voxelize_direct(grid, None))
+
+
+def test_voxelize_known_answers():
+    """Spot checks from the reference tests (tests/test_sdf.py:128-167, 341-356)."""
+    grid = UniformGrid((100, 100, 100), 1e-3)
+    box = Box(center=(0.05, 0.05, 0.05), size=(0.02, 0.02, 0.02))
+    m = box.voxelize(grid)
+    assert m[50, 50, 50] and not m[0, 0, 0] and not m[99, 99, 99]
+    assert m.sum() == 20**3
+    cyl = Cylinder(p1=(0.05, 0.05, 0.04), p2=(0.05, 0.05, 0.06), radius=0.005)
+    mc = cyl.voxelize(grid)
+    assert mc[50, 50, 50] and not mc[0, 0, 0]
+    sph = Sphere(center=(0.05, 0.05, 0.05), radius=0.01)
+    ms = sph.voxelize(grid)
+    ii, jj, kk = np.nonzero(ms)
+    d = np.sqrt(((ii + 0.5) * 1e-3 - 0.05) ** 2 + ((jj + 0.5) * 1e-3 - 0.05) ** 2 + ((kk + 0.5) * 1e-3 - 0.05) ** 2)
+    assert d.max() <= 0.01 + 1e-12
+    nug = NonuniformGrid.from_stretch((50, 50, 50), 1e-3, stretch_z=1.05)
+    mn = box.voxelize(nug)
+    assert mn.any() and not mn.all()
+
+
+def test_materialized_geometry_known_answers():
+    """tests/test_material_geometry.py:86-157: regions only claim air, later ones win."""
+    grid = UniformGrid((10, 10, 10), 1e-2)
+    geo = MaterializedGeometry(Box(center=(0.05, 0.05, 0.05), size=(0.1, 0.1, 0.1)))
+    m1 = SimpleMaterial(name="m1", _rho=50.0, _c=300.0)
+    m2 = SimpleMaterial(name="m2", _rho=60.0, _c=310.0)
+    geo.set_material(Box(center=(0.03, 0.05, 0.05), size=(0.06, 0.1, 0.1)), m1)
+    geo.set_material(Box(center=(0.05, 0.05, 0.05), size=(0.04, 0.08, 0.08)), m2)
+    mask, ids = geo.voxelize_with_materials(grid)
+    assert mask.all()
+    assert ids[5, 5, 5] == 2 and ids[0, 5, 5] == 1 and ids[9, 0, 0] == 0
+    assert geo.get_material_table()[0] is AIR_20C
+
+
+def test_helmholtz_resonator_union():
+    res = HelmholtzResonator(position=(0.03, 0.03, 0.03), cavity_shape="cylinder",
+                             cavity_size=(0.01, 0.02), neck_length=0.005, neck_radius=0.003)
+    grid = UniformGrid((60, 60, 60), 1e-3)
+    assert np.array_equal(res.voxelize(grid), res.voxelize_direct(grid, None))
