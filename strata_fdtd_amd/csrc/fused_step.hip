@@ -59,13 +59,27 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     // an L2. gridDim.x is a multiple of 8.
     const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
     const int w = lb * kBlockWaves + (threadIdx.x >> 6);
-    if (w >= a.nwaves) return;
+    int seg, sj, si;
+    if (a.sub_mode) {
+        const int n0 = a.sub_hi[0] - a.sub_lo[0], n1 = a.sub_hi[1] - a.sub_lo[1],
+                  n2 = a.sub_hi[2] - a.sub_lo[2];
+        if (w >= n0 * n1 * n2) return;
+        seg = a.sub_lo[0] + w % n0;
+        const int t = w / n0;
+        sj = a.sub_lo[1] + t % n1;
+        si = a.sub_lo[2] + t / n1;
+    } else {
+        if (w >= a.nwaves) return;
+        seg = w % a.nseg;
+        const int t = w / a.nseg;
+        sj = t % a.nstrip_j;
+        si = t / a.nstrip_j;
+        if (a.skip_sub && seg >= a.sub_lo[0] && seg < a.sub_hi[0] && sj >= a.sub_lo[1] &&
+            sj < a.sub_hi[1] && si >= a.sub_lo[2] && si < a.sub_hi[2])
+            return;
+    }
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int seg = w % a.nseg;
-    const int t = w / a.nseg;
-    const int sj = t % a.nstrip_j;
-    const int si = t / a.nstrip_j;
     const int k0 = seg * kWave;
     const int k = k0 + lane;
     const bool kv = k < nz;
@@ -452,10 +466,51 @@ int fused_grid(FusedArgs& a, int rows, int planes) {
     a.nstrip_j = (a.ny + rows - 1) / rows;
     a.nstrip_i = (a.nx + planes - 1) / planes;
     a.nwaves = a.nseg * a.nstrip_j * a.nstrip_i;
+    a.sub_mode = 0;
+    a.skip_sub = 0;
     int blocks = (a.nwaves + kBlockWaves - 1) / kBlockWaves;
     blocks = (blocks + 7) & ~7;
     a.blocks_per_xcd = blocks / 8;
     return blocks;
+}
+
+template <int R>
+static hipError_t launch_rows(const FusedArgs& a, bool nu, bool ade, int blocks, hipStream_t st) {
+    if (!ade) return launch_variants<R>(a, nu, false, blocks, st);
+    // Strips that touch the ADE box run the ADE instantiation; every other strip
+    // runs the plain one (fewer registers, higher occupancy).
+    FusedArgs plain = a;
+    FusedArgs mat = a;
+    const int rows = R;
+    int lo[3] = {a.nseg, a.nstrip_j, a.nstrip_i}, hi[3] = {0, 0, 0};
+    for (int s = 0; s < a.nseg; ++s) {
+        const int k0 = s * kWave;
+        if (k0 + kWave > a.box_lo[2] && k0 - 1 < a.box_hi[2]) { lo[0] = std::min(lo[0], s); hi[0] = s + 1; }
+    }
+    for (int s = 0; s < a.nstrip_j; ++s) {
+        const int j0 = s * rows;
+        if (j0 + rows > a.box_lo[1] && j0 - 1 < a.box_hi[1]) { lo[1] = std::min(lo[1], s); hi[1] = s + 1; }
+    }
+    for (int s = 0; s < a.nstrip_i; ++s) {
+        const int i0 = s * a.planes, i1 = std::min(i0 + a.planes, a.nx);
+        if (i1 > a.box_lo[0] && i0 - 1 < a.box_hi[0]) { lo[2] = std::min(lo[2], s); hi[2] = s + 1; }
+    }
+    for (int d = 0; d < 3; ++d) {
+        if (hi[d] <= lo[d]) return launch_variants<R>(a, nu, false, blocks, st);
+        plain.sub_lo[d] = mat.sub_lo[d] = lo[d];
+        plain.sub_hi[d] = mat.sub_hi[d] = hi[d];
+    }
+    plain.sub_mode = 0;
+    plain.skip_sub = 1;
+    hipError_t e = launch_variants<R>(plain, nu, false, blocks, st);
+    if (e != hipSuccess) return e;
+    mat.sub_mode = 1;
+    mat.skip_sub = 0;
+    const int n_sub = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    int mb = (n_sub + kBlockWaves - 1) / kBlockWaves;
+    mb = (mb + 7) & ~7;
+    mat.blocks_per_xcd = mb / 8;
+    return launch_variants<R>(mat, nu, true, mb, st);
 }
 
 hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st) {
@@ -468,10 +523,10 @@ hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blo
         if (e != hipSuccess) return e;
     }
     switch (rows) {
-        case 2: return launch_variants<2>(a, nu, ade, blocks, st);
-        case 4: return launch_variants<4>(a, nu, ade, blocks, st);
-        case 8: return launch_variants<8>(a, nu, ade, blocks, st);
-        case 16: return launch_variants<16>(a, nu, ade, blocks, st);
+        case 2: return launch_rows<2>(a, nu, ade, blocks, st);
+        case 4: return launch_rows<4>(a, nu, ade, blocks, st);
+        case 8: return launch_rows<8>(a, nu, ade, blocks, st);
+        case 16: return launch_rows<16>(a, nu, ade, blocks, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -51,6 +51,7 @@ def oracle_problem(sc: Scenario, n_steps: int) -> cpu.Problem:
     min_dx = _spacing_min(sc)
     dt = T.time_step(min_dx, sc.c, sc.courant)
     if sc.nonuniform:
+        assert tuple(len(c) for c in sc.coords) == tuple(sc.shape), "coords do not match the shape"
         cv, cp = T.base_coefficients(dt, sc.rho, sc.c)
         st = T.stencil_arrays(*sc.coords)
         inv_face = (st["inv_dx_face"], st["inv_dy_face"], st["inv_dz_face"])
@@ -198,6 +199,9 @@ def seeded_scenario(seed: int, shape=(24, 20, 28), pml=4, geometry=True, ade=Fal
     coords = None
     if nonuniform:
         coords = tuple(T.stretched_coords(n, 1e-3, r) for n, r in zip(shape, (1.04, 1.0, 1.06)))
+        if tuple(len(c) for c in coords) != tuple(shape):
+            raise ValueError(f"from_stretch yields {tuple(len(c) for c in coords)} cells for {shape} "
+                             "(odd counts lose one cell); use even sizes on stretched axes")
     src_pos = tuple(int(v) for v in (nx // 2 + 1, ny // 2, nz // 2 - 1))
     sources = [("point", src_pos, 8000.0)]
     if plane:

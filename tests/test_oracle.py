@@ -56,7 +56,7 @@ needs_ref = pytest.mark.skipif(not cpu.have_ref(), reason="oracle/_ref not built
     (104, dict(geometry=False, pml=0)),
 ])
 def test_oracle_equals_reference_build(oracle_lib, seed, kw):
-    sc = seeded_scenario(seed, shape=(22, 19, 31), **kw)
+    sc = seeded_scenario(seed, shape=(22, 19, 32), **kw)
     pb = oracle_problem(sc, 35)
     a, ta = cpu.run(pb, 35)
     b, tb = cpu.run(pb, 35, impl="ref")
