@@ -113,6 +113,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--graph-chunk", type=int, default=50)
     ap.add_argument("--tiling", default=None, help="rows,planes override")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); multi-GPU slab "
@@ -142,6 +144,12 @@ def main() -> int:
     log(f"setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")
     solver.step_n(args.warmup, graph_chunk=args.graph_chunk)
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     stream = solver._stream
     ev0 = torch.cuda.Event(enable_timing=True)

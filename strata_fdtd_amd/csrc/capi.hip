@@ -12,7 +12,8 @@
 
 namespace fdtd_amd {
 int fused_grid(FusedArgs& a, int rows, int planes);
-hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st);
+hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st,
+                        hipStream_t side, hipEvent_t fork, hipEvent_t join);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
 hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
                           const float*, const float*, hipStream_t);
@@ -74,6 +75,8 @@ struct fdtd_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipStream_t side = nullptr;               // ADE strips run here, forked/joined per step
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 extern "C" {
@@ -98,6 +101,10 @@ int fdtd_ctx_create(int device, void* stream, fdtd_ctx** out) {
     DeviceGuard g(device);
     if (int rc = check(hipEventCreate(&c->ev0), "hipEventCreate")) { delete c; return rc; }
     if (int rc = check(hipEventCreate(&c->ev1), "hipEventCreate")) { delete c; return rc; }
+    // default-priority side stream (no priority variants) and timing-free events
+    if (int rc = check(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "hipStreamCreate")) { delete c; return rc; }
+    if (int rc = check(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming), "hipEventCreate")) { delete c; return rc; }
+    if (int rc = check(hipEventCreateWithFlags(&c->join, hipEventDisableTiming), "hipEventCreate")) { delete c; return rc; }
     *out = c;
     return 0;
 }
@@ -109,6 +116,9 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->probe_dev) (void)hipFree(c->probe_dev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
     return 0;
 }
@@ -312,7 +322,8 @@ static int enqueue_steps(fdtd_ctx* c, int n) {
     const int blocks = fused_grid(a, c->rows, c->planes);
     for (int s = 0; s < n; ++s) {
         a.koff = s;
-        if (int rc = check(launch_fused(a, c->rows, c->nonuniform, a.n_pole > 0, blocks, c->stream),
+        if (int rc = check(launch_fused(a, c->rows, c->nonuniform, a.n_pole > 0, blocks, c->stream, c->side,
+                                        c->fork, c->join),
                            "fused step launch"))
             return rc;
     }

@@ -474,8 +474,13 @@ int fused_grid(FusedArgs& a, int rows, int planes) {
     return blocks;
 }
 
+// With an ADE box the step is two launches over disjoint strips that read the
+// same inputs: the plain strips on `st` and the ADE strips on `side`, forked
+// after the density-pole update and joined before the step ends, so the small
+// ADE launch overlaps the large plain one (also inside a captured graph).
 template <int R>
-static hipError_t launch_rows(const FusedArgs& a, bool nu, bool ade, int blocks, hipStream_t st) {
+static hipError_t launch_rows(const FusedArgs& a, bool nu, bool ade, int blocks, hipStream_t st,
+                              hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     if (!ade) return launch_variants<R>(a, nu, false, blocks, st);
     // Strips that touch the ADE box run the ADE instantiation; every other strip
     // runs the plain one (fewer registers, higher occupancy).
@@ -502,18 +507,27 @@ static hipError_t launch_rows(const FusedArgs& a, bool nu, bool ade, int blocks,
     }
     plain.sub_mode = 0;
     plain.skip_sub = 1;
-    hipError_t e = launch_variants<R>(plain, nu, false, blocks, st);
-    if (e != hipSuccess) return e;
     mat.sub_mode = 1;
     mat.skip_sub = 0;
     const int n_sub = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
     int mb = (n_sub + kBlockWaves - 1) / kBlockWaves;
     mb = (mb + 7) & ~7;
     mat.blocks_per_xcd = mb / 8;
+    hipError_t e;
+    if (side) {
+        if ((e = hipEventRecord(fork, st)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return e;
+        if ((e = launch_variants<R>(mat, nu, true, mb, side)) != hipSuccess) return e;
+        if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
+        if ((e = launch_variants<R>(plain, nu, false, blocks, st)) != hipSuccess) return e;
+        return hipStreamWaitEvent(st, join, 0);
+    }
+    if ((e = launch_variants<R>(plain, nu, false, blocks, st)) != hipSuccess) return e;
     return launch_variants<R>(mat, nu, true, mb, st);
 }
 
-hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st) {
+hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st,
+                        hipStream_t side, hipEvent_t fork, hipEvent_t join) {
     if (ade) {
         const int64_t bn = static_cast<int64_t>(a.box_hi[0] - a.box_lo[0]) *
                            (a.box_hi[1] - a.box_lo[1]) * (a.box_hi[2] - a.box_lo[2]);
@@ -523,10 +537,10 @@ hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blo
         if (e != hipSuccess) return e;
     }
     switch (rows) {
-        case 2: return launch_rows<2>(a, nu, ade, blocks, st);
-        case 4: return launch_rows<4>(a, nu, ade, blocks, st);
-        case 8: return launch_rows<8>(a, nu, ade, blocks, st);
-        case 16: return launch_rows<16>(a, nu, ade, blocks, st);
+        case 2: return launch_rows<2>(a, nu, ade, blocks, st, side, fork, join);
+        case 4: return launch_rows<4>(a, nu, ade, blocks, st, side, fork, join);
+        case 8: return launch_rows<8>(a, nu, ade, blocks, st, side, fork, join);
+        case 16: return launch_rows<16>(a, nu, ade, blocks, st, side, fork, join);
         default: return hipErrorInvalidValue;
     }
 }
