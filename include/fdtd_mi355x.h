@@ -121,6 +121,22 @@ int fdtd_graph_run(fdtd_ctx* ctx, int n_steps, int replays);
  * measured with HIP events on the context stream (milliseconds). */
 int fdtd_time_steps(fdtd_ctx* ctx, int n_steps, int use_graph, float* ms_total);
 
+/* ---- batched serving (no counterpart in the reference) ------------------- */
+/* A batch advances many independent solves (uniform grids, no ADE) by one step
+ * per launch: waves are dealt out over the solves' strips, each solve keeps its
+ * own buffers, tables and counter, and a solve whose counter reached its
+ * limit is left untouched. Replaces running BatchedGPUFDTDSolver-style band
+ * batches (core/solver_gpu.py:978-1351) for a design sweep. */
+typedef struct fdtd_batch fdtd_batch;
+int fdtd_batch_create(int device, void* stream, int rows, fdtd_batch** out);
+int fdtd_batch_destroy(fdtd_batch* b);
+/* (Re)build the device table from n ready contexts; step_limits[q] is the
+ * total step count of solve q. Set-up call (may allocate, synchronises). */
+int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_limits, int n);
+/* Enqueue n batched steps (use_graph: capture once per size, then replay). */
+int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph);
+int fdtd_batch_info(fdtd_batch* b, int* n, int* total_waves);
+
 /* ---- per-phase entry points (one kernel per reference function) --------- */
 /* All in place, on `stream`, float32, uniform unless noted. */
 /* Replaces fdtd::update_velocity (fdtd_step.cpp:15-82). */

@@ -36,6 +36,11 @@ _SIGS = {
     "fdtd_step_n": (_i, [_vp, _i]),
     "fdtd_graph_run": (_i, [_vp, _i, _i]),
     "fdtd_time_steps": (_i, [_vp, _i, _i, C.POINTER(_f)]),
+    "fdtd_batch_create": (_i, [_i, _vp, _i, C.POINTER(_vp)]),
+    "fdtd_batch_destroy": (_i, [_vp]),
+    "fdtd_batch_set": (_i, [_vp, _vp, _vp, _i]),
+    "fdtd_batch_step_n": (_i, [_vp, _i, _i]),
+    "fdtd_batch_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "fdtd_update_velocity": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
     "fdtd_update_velocity_nonuniform": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "fdtd_apply_rigid": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),

@@ -1,0 +1,252 @@
+"""Probe-loss gradients with respect to per-voxel density and bulk modulus.
+
+``DesignProblem`` describes a solve whose design region (a box of cells)
+carries per-voxel density rho and bulk modulus K; outside the box the medium is
+the background air. ``probe_traces(rho, K, problem)`` is a
+``torch.autograd.Function``: its forward runs the per-voxel forward step on
+the GPU (checkpointing the state every ``checkpoint`` steps) and returns the
+probe traces; its backward runs the hand-written time-reversed adjoint kernel
+segment by segment — restoring a checkpoint, recomputing the segment's states,
+then stepping the adjoint backwards through them — and accumulates dL/drho and
+dL/dK with Kahan-compensated summation over time. Any loss built from the
+traces in torch gets its gradient through this Function.
+
+Model (no counterpart in the reference, which has scalar rho and c):
+face coefficient cv0 / mean(r_lo, r_hi) with r = rho/rho0, cell coefficient
+cp0 * K/K0; with rho = rho0 and K = K0 everywhere it is the reference step.
+Float64 mode (``dtype=torch.float64``) runs the same kernels in double for
+finite-difference checks.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .boundaries import PML
+from .grid import UniformGrid
+from .sources import GaussianPulse, accumulated_times
+
+_vp = C.c_void_p
+
+
+def _lib():
+    lib = _native.load()
+    if not getattr(lib, "_adjoint_sigs", False):
+        i, f, d, i64 = C.c_int, C.c_float, C.c_double, C.c_int64
+        lib.fdtd_param_step.restype = i
+        lib.fdtd_param_step.argtypes = [i] + [_vp] * 8 + [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                         d, d, i64, d, i, _vp, _vp, _vp]
+        lib.fdtd_adjoint_step.restype = i
+        lib.fdtd_adjoint_step.argtypes = [i] + [_vp] * 12 + [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                            _vp, d, d, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        lib._adjoint_sigs = True
+    return lib
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
+@dataclass
+class DesignProblem:
+    shape: tuple[int, int, int]
+    resolution: float = 1e-3
+    c: float = 343.0
+    rho: float = 1.204
+    pml_depth: int = 12
+    source: GaussianPulse | None = None
+    probes: list = field(default_factory=list)
+    steps: int = 2000
+    box_lo: tuple[int, int, int] = (0, 0, 0)
+    box_hi: tuple[int, int, int] = (0, 0, 0)
+    checkpoint: int = 100
+    geometry: np.ndarray | None = None        # bool air mask (None: all air)
+    courant: float = 0.95
+
+    @property
+    def box_shape(self) -> tuple[int, int, int]:
+        return tuple(h - l for l, h in zip(self.box_lo, self.box_hi))
+
+    @property
+    def K0(self) -> float:
+        return self.rho * self.c**2
+
+
+class _Device:
+    """Static device data of a DesignProblem (flags, decays, probes, waveform)."""
+
+    def __init__(self, pb: DesignProblem, dtype, device):
+        import torch
+        self.pb = pb
+        self.dtype = dtype
+        self.dev = device
+        nx, ny, nz = pb.shape
+        self.dt = pb.courant * (pb.resolution / (pb.c * np.sqrt(3)))
+        self.cv0 = -self.dt / (pb.rho * pb.resolution)
+        self.cp0 = -pb.rho * pb.c**2 * self.dt / pb.resolution
+        if dtype == torch.float32:
+            self.cv0, self.cp0 = float(np.float32(self.cv0)), float(np.float32(self.cp0))
+        air = np.ones(pb.shape, np.uint8) if pb.geometry is None else pb.geometry.astype(np.uint8)
+        air_d = torch.from_numpy(air).to(device)
+        self.flags = torch.empty(pb.shape, dtype=torch.uint8, device=device)
+        _native.call("fdtd_build_flags", _ptr(air_d), None, _ptr(self.flags), nx, ny, nz,
+                     torch.cuda.current_stream(device).cuda_stream)
+        self.air = air_d
+
+        class _S:  # PML.initialize reads these attributes
+            pass
+        s = _S()
+        s.grid = UniformGrid(pb.shape, pb.resolution)
+        s.shape, s.c, s.dx, s.dt = pb.shape, pb.c, pb.resolution, self.dt
+        pml = PML(depth=pb.pml_depth)
+        if pb.pml_depth:
+            pml.initialize(s)
+            tabs = pml.decay_tables(self.dt, pb.shape)
+        else:
+            tabs = [np.ones(n, np.float32) for n in pb.shape]
+        self.dec = [torch.from_numpy(np.ascontiguousarray(t)).to(device) for t in tabs]
+        self.probe_cells = torch.tensor([np.ravel_multi_index(p, pb.shape) for p in pb.probes] or [0],
+                                        dtype=torch.int64, device=device)
+        self.n_probe = len(pb.probes)
+        src = pb.source
+        self.src_cell = int(np.ravel_multi_index(tuple(src.position), pb.shape)) if src is not None else -1
+        times = accumulated_times(self.dt, pb.steps)
+        self.wave = src.waveform(times, self.dt) if src is not None else np.zeros(pb.steps)
+        self.lo = np.array(pb.box_lo, dtype=np.int32)
+        self.hi = np.array(pb.box_hi, dtype=np.int32)
+        self.code = 8 if dtype == torch.float64 else 4
+        self.stream = torch.cuda.current_stream(device).cuda_stream
+
+    def zeros_state(self):
+        import torch
+        return [torch.zeros(self.pb.shape, dtype=self.dtype, device=self.dev) for _ in range(4)]
+
+    def step(self, src_state, dst_state, n, r, kappa, trace_row):
+        pb = self.pb
+        rc = _lib().fdtd_param_step(self.code, *[_ptr(t) for t in src_state], *[_ptr(t) for t in dst_state],
+                                    *pb.shape, _ptr(self.flags), *[_ptr(d) for d in self.dec],
+                                    self.lo.ctypes.data, self.hi.ctypes.data, _ptr(r), _ptr(kappa),
+                                    self.cv0, self.cp0, self.src_cell, float(self.wave[n]), self.n_probe,
+                                    _ptr(self.probe_cells), _ptr(trace_row), self.stream)
+        if rc != 0:
+            raise _native.NativeError(f"fdtd_param_step failed ({rc})")
+
+    def adjoint(self, lam, lam_out, state, r, kappa, g_row, acc):
+        pb = self.pb
+        rc = _lib().fdtd_adjoint_step(self.code, *[_ptr(t) for t in lam], *[_ptr(t) for t in lam_out],
+                                      *[_ptr(t) for t in state], *pb.shape, _ptr(self.flags),
+                                      *[_ptr(d) for d in self.dec], self.lo.ctypes.data, self.hi.ctypes.data,
+                                      _ptr(r), _ptr(kappa), self.cv0, self.cp0, self.n_probe,
+                                      _ptr(self.probe_cells), _ptr(g_row), *[_ptr(a) for a in acc],
+                                      self.stream)
+        if rc != 0:
+            raise _native.NativeError(f"fdtd_adjoint_step failed ({rc})")
+
+
+def forward_traces(pb: DesignProblem, rho=None, K=None, dtype=None, device="cuda",
+                   keep_checkpoints: bool = False):
+    """Run the per-voxel forward solve; returns (traces [steps, n_probe], checkpoints)."""
+    import torch
+    dtype = dtype or torch.float32
+    dv = _Device(pb, dtype, torch.device(device))
+    r = kappa = None
+    if rho is not None:
+        r = (rho.detach().to(device=device, dtype=torch.float32) / pb.rho).contiguous()
+    if K is not None:
+        kappa = (K.detach().to(device=device, dtype=torch.float32) / pb.K0).contiguous()
+    a, b = dv.zeros_state(), dv.zeros_state()
+    traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=dtype, device=device)
+    ckpts = []
+    for n in range(pb.steps):
+        if keep_checkpoints and n % pb.checkpoint == 0:
+            ckpts.append([t.clone() for t in a])
+        dv.step(a, b, n, r, kappa, traces[n])
+        a, b = b, a
+    return traces[:, :dv.n_probe], (dv, r, kappa, ckpts)
+
+
+def adjoint_gradients(saved, grad_traces):
+    """Backward pass: returns (dL/dr, dL/dkappa) on the design box."""
+    import torch
+    dv, r, kappa, ckpts = saved
+    pb = dv.pb
+    K = pb.checkpoint
+    lam, lam_out = dv.zeros_state(), dv.zeros_state()
+    acc = [torch.zeros(pb.box_shape, dtype=dv.dtype, device=dv.dev) for _ in range(4)]
+    g = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=dv.dtype, device=dv.dev)
+    g[:, :dv.n_probe] = grad_traces.to(dv.dtype)
+    slots = [dv.zeros_state() for _ in range(K)]
+    scratch = torch.zeros(max(dv.n_probe, 1), dtype=dv.dtype, device=dv.dev)
+    n_seg = (pb.steps + K - 1) // K
+    for seg in reversed(range(n_seg)):
+        n0 = seg * K
+        n1 = min(pb.steps, n0 + K)
+        for t, c in zip(slots[0], ckpts[seg]):
+            t.copy_(c)
+        for m in range(n1 - n0 - 1):
+            dv.step(slots[m], slots[m + 1], n0 + m, r, kappa, scratch)
+        for n in range(n1 - 1, n0 - 1, -1):
+            dv.adjoint(lam, lam_out, slots[n - n0], r, kappa, g[n], acc)
+            lam, lam_out = lam_out, lam
+    # Kahan: the compensation holds minus the low-order part the sums lost
+    return acc[0] - acc[1], acc[2] - acc[3]
+
+
+class _ProbeTraces:
+    """Namespace holder so the Function is built lazily (torch import)."""
+
+    fn = None
+
+
+def probe_traces(rho, K, problem: DesignProblem):
+    """Differentiable probe traces [steps, n_probe] as a function of the design
+    region's density and bulk modulus (torch tensors of the box shape)."""
+    import torch
+
+    if _ProbeTraces.fn is None:
+        class ProbeTraces(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, rho_, K_, pb):
+                traces, saved = forward_traces(pb, rho_, K_, dtype=rho_.dtype if rho_.dtype == torch.float64
+                                               else torch.float32, device=rho_.device,
+                                               keep_checkpoints=True)
+                ctx.saved = saved
+                ctx.pb = pb
+                return traces
+
+            @staticmethod
+            def backward(ctx, grad_traces):
+                gr, gk = adjoint_gradients(ctx.saved, grad_traces.contiguous())
+                pb = ctx.pb
+                ctx.saved = None
+                return (gr / pb.rho).to(grad_traces.dtype), (gk / pb.K0).to(grad_traces.dtype), None
+
+        _ProbeTraces.fn = ProbeTraces
+    return _ProbeTraces.fn.apply(rho, K, problem)
+
+
+def design_problem_192(n: int = 192, steps: int = 2000, seed: int = 5, pml: int = 12,
+                       box: int = 128, checkpoint: int = 100) -> tuple[DesignProblem, object, object, object]:
+    """The adjoint workload: n^3 at 1 mm, air, PML 12, a box^3 central design
+    region of per-voxel rho and K = air * (1 + 0.05 N(0,1)) (seeded), 1 kHz
+    pulse, 4 probes, `steps` steps, L2 loss to seeded target traces.
+    Returns (problem, rho, K, target) as torch tensors on cuda."""
+    import torch
+    rng = np.random.default_rng(seed)
+    lo = ((n - box) // 2,) * 3
+    hi = tuple(v + box for v in lo)
+    c0 = n // 2
+    src = GaussianPulse(position=(c0 - box // 4, c0, c0), frequency=1000.0)
+    probes = [tuple(int(v) for v in rng.integers(lo[0] + 4, hi[0] - 4, 3)) for _ in range(4)]
+    pb = DesignProblem(shape=(n, n, n), pml_depth=pml, source=src, probes=probes, steps=steps,
+                       box_lo=lo, box_hi=hi, checkpoint=checkpoint)
+    rho = torch.tensor(pb.rho * (1 + 0.05 * rng.standard_normal((box,) * 3)), dtype=torch.float32,
+                       device="cuda")
+    K = torch.tensor(pb.K0 * (1 + 0.05 * rng.standard_normal((box,) * 3)), dtype=torch.float32,
+                     device="cuda")
+    target = torch.tensor(0.8 * rng.standard_normal((steps, 4)) * 1e-3, dtype=torch.float32, device="cuda")
+    return pb, rho, K, target

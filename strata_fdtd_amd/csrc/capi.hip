@@ -3,6 +3,8 @@
 // in a context, enqueues kernels on the context stream. No entry point that
 // steps the solver allocates device memory or synchronises.
 
+#include <algorithm>
+#include <climits>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -12,9 +14,10 @@
 
 namespace fdtd_amd {
 int fused_grid(FusedArgs& a, int rows, int planes);
-hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st,
-                        hipStream_t side, hipEvent_t fork, hipEvent_t join);
+hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
+hipError_t launch_batch(const FusedArgs* table, const int* offsets, int n, int total_waves, int rows,
+                        hipStream_t st);
 hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
                           const float*, const float*, hipStream_t);
 hipError_t phase_rigid(float*, float*, float*, const uint8_t*, int, int, int, hipStream_t);
@@ -75,8 +78,6 @@ struct fdtd_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipStream_t side = nullptr;               // ADE strips run here, forked/joined per step
-    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 extern "C" {
@@ -97,14 +98,11 @@ int fdtd_ctx_create(int device, void* stream, fdtd_ctx** out) {
     if (device < 0 || device >= n) return fail(-1, "fdtd_ctx_create: device %d of %d", device, n);
     auto* c = new fdtd_ctx();
     c->device = device;
+    c->a.step_limit = INT64_MAX;
     c->stream = static_cast<hipStream_t>(stream);
     DeviceGuard g(device);
     if (int rc = check(hipEventCreate(&c->ev0), "hipEventCreate")) { delete c; return rc; }
     if (int rc = check(hipEventCreate(&c->ev1), "hipEventCreate")) { delete c; return rc; }
-    // default-priority side stream (no priority variants) and timing-free events
-    if (int rc = check(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking), "hipStreamCreate")) { delete c; return rc; }
-    if (int rc = check(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming), "hipEventCreate")) { delete c; return rc; }
-    if (int rc = check(hipEventCreateWithFlags(&c->join, hipEventDisableTiming), "hipEventCreate")) { delete c; return rc; }
     *out = c;
     return 0;
 }
@@ -116,9 +114,6 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->probe_dev) (void)hipFree(c->probe_dev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->fork) (void)hipEventDestroy(c->fork);
-    if (c->join) (void)hipEventDestroy(c->join);
-    if (c->side) (void)hipStreamDestroy(c->side);
     delete c;
     return 0;
 }
@@ -322,8 +317,7 @@ static int enqueue_steps(fdtd_ctx* c, int n) {
     const int blocks = fused_grid(a, c->rows, c->planes);
     for (int s = 0; s < n; ++s) {
         a.koff = s;
-        if (int rc = check(launch_fused(a, c->rows, c->nonuniform, a.n_pole > 0, blocks, c->stream, c->side,
-                                        c->fork, c->join),
+        if (int rc = check(launch_fused(a, c->rows, c->nonuniform, a.n_pole > 0, blocks, c->stream),
                            "fused step launch"))
             return rc;
     }
@@ -458,6 +452,144 @@ int fdtd_build_flags(const uint8_t* air, const uint8_t* material, uint8_t* flags
     if (!air || !flags) return fail(-2, "null mask");
     return check(phase_build_flags(air, material, flags, nx, ny, nz, static_cast<hipStream_t>(stream)),
                  "build_flags");
+}
+
+// ---- batched serving ----------------------------------------------------------
+struct fdtd_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int rows = 4;
+    FusedArgs* table_dev = nullptr;
+    int* offsets_dev = nullptr;
+    FusedArgs* table_host = nullptr;   // pinned staging
+    int* offsets_host = nullptr;
+    int cap = 0;
+    int n = 0;
+    int total_waves = 0;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+};
+
+static void batch_drop_graph(fdtd_batch* b) {
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->graph_steps = 0;
+    }
+}
+
+int fdtd_batch_create(int device, void* stream, int rows, fdtd_batch** out) {
+    if (!out) return fail(-1, "null out");
+    if (rows != 2 && rows != 4 && rows != 8) return fail(-2, "batch rows must be 2, 4 or 8");
+    auto* b = new fdtd_batch();
+    b->device = device;
+    b->stream = static_cast<hipStream_t>(stream);
+    b->rows = rows;
+    *out = b;
+    return 0;
+}
+
+int fdtd_batch_destroy(fdtd_batch* b) {
+    if (!b) return 0;
+    DeviceGuard g(b->device);
+    batch_drop_graph(b);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    else (void)hipDeviceSynchronize();
+    if (b->table_dev) (void)hipFree(b->table_dev);
+    if (b->offsets_dev) (void)hipFree(b->offsets_dev);
+    if (b->table_host) (void)hipHostFree(b->table_host);
+    if (b->offsets_host) (void)hipHostFree(b->offsets_host);
+    delete b;
+    return 0;
+}
+
+int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_limits, int n) {
+    if (!b) return fail(-1, "null batch");
+    if (n < 0 || (n > 0 && (!ctxs || !step_limits))) return fail(-2, "bad batch arrays");
+    DeviceGuard g(b->device);
+    // the staging buffers may still feed an in-flight copy: wait for the stream
+    if (int rc = check(b->stream ? hipStreamSynchronize(b->stream) : hipDeviceSynchronize(), "sync")) return rc;
+    batch_drop_graph(b);
+    if (n > b->cap) {
+        const int cap = std::max(n, 2 * b->cap);
+        if (b->table_dev) (void)hipFree(b->table_dev);
+        if (b->offsets_dev) (void)hipFree(b->offsets_dev);
+        if (b->table_host) (void)hipHostFree(b->table_host);
+        if (b->offsets_host) (void)hipHostFree(b->offsets_host);
+        b->table_dev = nullptr; b->offsets_dev = nullptr; b->table_host = nullptr; b->offsets_host = nullptr;
+        b->cap = 0;
+        if (int rc = check(hipMalloc(&b->table_dev, sizeof(FusedArgs) * cap), "hipMalloc")) return rc;
+        if (int rc = check(hipMalloc(&b->offsets_dev, sizeof(int) * (cap + 1)), "hipMalloc")) return rc;
+        if (int rc = check(hipHostMalloc(&b->table_host, sizeof(FusedArgs) * cap, 0), "hipHostMalloc")) return rc;
+        if (int rc = check(hipHostMalloc(&b->offsets_host, sizeof(int) * (cap + 1), 0), "hipHostMalloc")) return rc;
+        b->cap = cap;
+    }
+    int total = 0;
+    b->offsets_host[0] = 0;
+    for (int q = 0; q < n; ++q) {
+        fdtd_ctx* c = ctxs[q];
+        if (int rc = ready(c)) return rc;
+        if (c->nonuniform || c->a.n_pole > 0)
+            return fail(-2, "batch entry %d: nonuniform or ADE solves run on their own context", q);
+        if (c->device != b->device) return fail(-2, "batch entry %d is on another device", q);
+        FusedArgs a = c->a;
+        fused_grid(a, b->rows, c->planes);
+        a.koff = 0;
+        a.step_limit = step_limits[q];
+        b->table_host[q] = a;
+        total += a.nwaves;
+        b->offsets_host[q + 1] = total;
+    }
+    b->n = n;
+    b->total_waves = total;
+    if (n == 0) return 0;
+    if (int rc = check(hipMemcpyAsync(b->table_dev, b->table_host, sizeof(FusedArgs) * n,
+                                      hipMemcpyHostToDevice, b->stream), "table upload")) return rc;
+    return check(hipMemcpyAsync(b->offsets_dev, b->offsets_host, sizeof(int) * (n + 1),
+                                hipMemcpyHostToDevice, b->stream), "offsets upload");
+}
+
+static int batch_enqueue(fdtd_batch* b, hipStream_t st, int n_steps) {
+    for (int s = 0; s < n_steps; ++s)
+        if (int rc = check(launch_batch(b->table_dev, b->offsets_dev, b->n, b->total_waves, b->rows, st),
+                           "batch step launch"))
+            return rc;
+    return 0;
+}
+
+int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph) {
+    if (!b) return fail(-1, "null batch");
+    if (n_steps < 0) return fail(-2, "negative step count");
+    if (b->n == 0 || n_steps == 0) return 0;
+    DeviceGuard g(b->device);
+    if (!use_graph) return batch_enqueue(b, b->stream, n_steps);
+    if (!b->graph || b->graph_steps != n_steps) {
+        batch_drop_graph(b);
+        hipStream_t cap = nullptr;
+        if (int rc = check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate")) return rc;
+        hipGraph_t gr = nullptr;
+        if (int rc = check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture")) {
+            (void)hipStreamDestroy(cap);
+            return rc;
+        }
+        int rc = batch_enqueue(b, cap, n_steps);
+        hipError_t e = hipStreamEndCapture(cap, &gr);
+        (void)hipStreamDestroy(cap);
+        if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+        if (int r2 = check(e, "end capture")) return r2;
+        e = hipGraphInstantiate(&b->graph, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (int r3 = check(e, "graph instantiate")) return r3;
+        b->graph_steps = n_steps;
+    }
+    return check(hipGraphLaunch(b->graph, b->stream), "graph launch");
+}
+
+int fdtd_batch_info(fdtd_batch* b, int* n, int* total_waves) {
+    if (!b) return fail(-1, "null batch");
+    if (n) *n = b->n;
+    if (total_waves) *total_waves = b->total_waves;
+    return 0;
 }
 
 }  // extern "C"

@@ -33,15 +33,11 @@ struct FusedArgs {
     int nstrip_i;                   // ceil(nx / planes)
     int nwaves;                     // nseg * nstrip_j * nstrip_i
     int blocks_per_xcd;             // gridDim.x / 8
-    // strip sub-box [lo, hi) in (segment, j-strip, i-chunk) coordinates holding
-    // every strip that touches the ADE box. sub_mode 0: whole grid, skipping the
-    // sub-box when skip_sub != 0; sub_mode 1: only the sub-box.
-    int sub_mode, skip_sub;
-    int sub_lo[3], sub_hi[3];
 
     // time
     const int64_t* step_ptr;        // device counter: index of step koff==0
     int koff;                       // offset of this launch inside a chunk
+    int64_t step_limit;             // steps at or past this index are not taken
 
     // sources
     int n_src;

@@ -1,6 +1,6 @@
-// Fused FDTD time step for gfx950: velocity update, ADE density correction,
-// rigid-wall mask, modulus ADE update, pressure update, PML damping, source
-// injection and probe gather in ONE sweep over HBM.
+// Fused FDTD time step for gfx950: velocity update, rigid-wall mask, pressure
+// update, PML damping, source injection and probe gather in ONE sweep over HBM,
+// plus the small kernels that add frequency-dependent (ADE) materials.
 //
 // Reference semantics (per cell, in this order; core/solver.py:2003-2193 with
 // the native arithmetic of _kernels/src/fdtd_step.cpp, pml.cpp and ade.cpp):
@@ -18,10 +18,18 @@
 // Per plane it reads p(i+1) once (p(i) is the previous plane's p(i+1), held in
 // registers), v/flags once, and writes p, v once: 33 B per cell-update plus
 // small halos (j-1/j+R rows, k0-1/k0+64 columns, one plane at the strip start).
-// k neighbours inside the segment move across lanes with DPP/ds_bpermute
-// shuffles. No LDS, no barriers: waves are independent, so occupancy alone
-// hides HBM latency. Ping-pong buffers (read set s&1, write the other) make the
-// single sweep race-free.
+// k neighbours inside the segment move across lanes with shuffles. No LDS, no
+// barriers: waves are independent, so occupancy alone hides HBM latency.
+// Ping-pong buffers (read set s&1, write the other) make the single sweep
+// race-free.
+//
+// ADE materials: a face is corrected only when both of its cells hold the same
+// pole material, and only material cells get a modulus correction, so only
+// material cells' outputs differ from the plain update. A step with poles is
+// therefore: the density-pole update over the material box (k_ade_density),
+// the plain fused sweep over the whole grid, then k_ade_fixup, which recomputes
+// the complete ADE update of every material cell from the (untouched) input
+// buffers and overwrites its outputs.
 
 #include <algorithm>
 
@@ -32,53 +40,10 @@ namespace fdtd_amd {
 __device__ __forceinline__ float shfl_down1(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWave); }
 
-// Density-pole correction of one face between cells lo and hi of the ADE box.
-// Native iteration order: Debye poles, then Lorentz poles (ade.cpp:240-401).
-template <bool NU>
-__device__ __forceinline__ float face_density_corr(const FusedArgs& a, float v, int m_lo, int m_hi,
-                                                   int64_t b_lo, int64_t b_hi, int64_t box_n,
-                                                   float inv_face) {
-    if (m_lo == 0 || m_lo != m_hi) return v;
-#pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int q = 0; q < a.n_pole; ++q) {
-            if (a.pole_target[q] != 0 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m_lo) continue;
-            const float* Jq = a.J + q * box_n;
-            const float cf = NU ? a.pole_corr[q] * inv_face : a.pole_corr[q];
-            v += cf * (Jq[b_hi] - Jq[b_lo]);
-        }
-    }
-    return v;
-}
-
-template <int R, bool NU, bool ADE>
-__global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
-    const int lane = threadIdx.x & (kWave - 1);
-    // XCD-aware remap: hardware block b runs on XCD b % 8; give each XCD a
-    // contiguous range of logical blocks so strips that share halo rows share
-    // an L2. gridDim.x is a multiple of 8.
-    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
-    const int w = lb * kBlockWaves + (threadIdx.x >> 6);
-    int seg, sj, si;
-    if (a.sub_mode) {
-        const int n0 = a.sub_hi[0] - a.sub_lo[0], n1 = a.sub_hi[1] - a.sub_lo[1],
-                  n2 = a.sub_hi[2] - a.sub_lo[2];
-        if (w >= n0 * n1 * n2) return;
-        seg = a.sub_lo[0] + w % n0;
-        const int t = w / n0;
-        sj = a.sub_lo[1] + t % n1;
-        si = a.sub_lo[2] + t / n1;
-    } else {
-        if (w >= a.nwaves) return;
-        seg = w % a.nseg;
-        const int t = w / a.nseg;
-        sj = t % a.nstrip_j;
-        si = t / a.nstrip_j;
-        if (a.skip_sub && seg >= a.sub_lo[0] && seg < a.sub_hi[0] && sj >= a.sub_lo[1] &&
-            sj < a.sub_hi[1] && si >= a.sub_lo[2] && si < a.sub_hi[2])
-            return;
-    }
-
+// One wave's work: the strip (segment seg, j-strip sj, i-chunk si) of the
+// solve described by `a` (kernel argument, or a row of a device batch table).
+template <int R, bool NU>
+__device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj, int si, int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = seg * kWave;
     const int k = k0 + lane;
@@ -86,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     const int j0 = sj * R;
     const int i0 = si * a.planes;
     const int i1 = min(i0 + a.planes, nx);
-    const int64_t sy = nz, sx = static_cast<int64_t>(ny) * nz;
+    const int64_t sx = static_cast<int64_t>(ny) * nz;
 
     const int64_t s = *a.step_ptr + a.koff;
     const int par = static_cast<int>(s & 1);
@@ -129,25 +94,6 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
         if (lane == 0) atomicOr(a.health, 2u);
     }
 
-    // ADE box extents (only used when ADE is instantiated)
-    int64_t box_n = 0, bsy = 0, bsx = 0;
-    bool ade_strip = false;
-    if (ADE) {
-        const int by = a.box_hi[1] - a.box_lo[1], bz = a.box_hi[2] - a.box_lo[2];
-        bsy = bz;
-        bsx = static_cast<int64_t>(by) * bz;
-        box_n = bsx * (a.box_hi[0] - a.box_lo[0]);
-        ade_strip = (i1 > a.box_lo[0] && i0 - 1 < a.box_hi[0]) &&
-                    (j0 + R > a.box_lo[1] && j0 - 1 < a.box_hi[1]) &&
-                    (k0 + kWave > a.box_lo[2] && k0 - 1 < a.box_hi[2]);
-    }
-    auto box_idx = [&](int ii, int jj, int kk) -> int64_t {
-        return (ii - a.box_lo[0]) * bsx + (jj - a.box_lo[1]) * bsy + (kk - a.box_lo[2]);
-    };
-    auto mat_at = [&](int ii, int jj, int kk) -> int {
-        return a.material[lin(ii, jj, kk, ny, nz)];
-    };
-
     const float cv = a.cv, cp = a.cp;
     const float ifz_k = (NU && kv && k < nz - 1) ? a.ifc[2][k] : 1.0f;
     const float icz_k = (NU && kv) ? a.icl[2][k] : 1.0f;
@@ -175,15 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
             const int64_t c = lin(im, j0 + r, k, ny, nz);
             const float pm = P[c], vo = VX[c];
             const uint32_t f = FL[c];
-            float v = vo + cvx * (pc[r] - pm);
-            if (ADE && ade_strip) {
-                const int jr = j0 + r;
-                const int m_lo = mat_at(im, jr, k), m_hi = mat_at(i0, jr, k);
-                if (m_lo && m_lo == m_hi)
-                    v = face_density_corr<NU>(a, v, m_lo, m_hi, box_idx(im, jr, k),
-                                              box_idx(i0, jr, k), box_n,
-                                              NU ? a.ifc[0][im] : 1.f);
-            }
+            const float v = vo + cvx * (pc[r] - pm);
             vxm[r] = (f & FDTD_FLAG_OPEN_X) ? v : 0.f;
         }
     }
@@ -236,15 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
         // vy' on the face (j0-1 | j0)
         float vyprev = 0.f;
         if (j0 > 0) {
-            const int jm = j0 - 1;
-            const float cvy = NU ? cv * a.ifc[1][jm] : cv;
-            float v = vy_jm + cvy * (pc[0] - p_jm);
-            if (ADE && ade_strip && kv) {
-                const int m_lo = mat_at(i, jm, k), m_hi = mat_at(i, j0, k);
-                if (m_lo && m_lo == m_hi)
-                    v = face_density_corr<NU>(a, v, m_lo, m_hi, box_idx(i, jm, k),
-                                              box_idx(i, j0, k), box_n, NU ? a.ifc[1][jm] : 1.f);
-            }
+            const float cvy = NU ? cv * a.ifc[1][j0 - 1] : cv;
+            const float v = vy_jm + cvy * (pc[0] - p_jm);
             vyprev = (f_jm & FDTD_FLAG_OPEN_Y) ? v : 0.f;
         }
 
@@ -256,36 +187,18 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
             const float pr = pc[r];
             const float p_up = (r + 1 < R) ? pc[r + 1] : p_jp;
             const uint32_t f = fl[r];
-            const bool row_ade = ADE && ade_strip && i >= a.box_lo[0] - 1 && i < a.box_hi[0] &&
-                                 jr >= a.box_lo[1] - 1 && jr < a.box_hi[1];
-            int m_c = 0;
-            if (ADE && row_ade && kv) m_c = mat_at(i, jr, k);
 
             // x face (i | i+1)
             float vxn = vxo[r];
             if (has_next) {
-                float v = vxo[r] + cvx * (pn[r] - pr);
-                if (ADE && row_ade && kv && m_c) {
-                    const int m_hi = mat_at(i + 1, jr, k);
-                    if (m_hi == m_c)
-                        v = face_density_corr<NU>(a, v, m_c, m_hi, box_idx(i, jr, k),
-                                                  box_idx(i + 1, jr, k), box_n,
-                                                  NU ? a.ifc[0][i] : 1.f);
-                }
+                const float v = vxo[r] + cvx * (pn[r] - pr);
                 vxn = (f & FDTD_FLAG_OPEN_X) ? v : 0.f;
             }
             // y face (j | j+1)
             float vyn = vyo[r];
             if (jr + 1 < ny) {
                 const float cvy = NU ? cv * a.ifc[1][jr] : cv;
-                float v = vyo[r] + cvy * (p_up - pr);
-                if (ADE && row_ade && kv && m_c) {
-                    const int m_hi = mat_at(i, jr + 1, k);
-                    if (m_hi == m_c)
-                        v = face_density_corr<NU>(a, v, m_c, m_hi, box_idx(i, jr, k),
-                                                  box_idx(i, jr + 1, k), box_n,
-                                                  NU ? a.ifc[1][jr] : 1.f);
-                }
+                const float v = vyo[r] + cvy * (p_up - pr);
                 vyn = (f & FDTD_FLAG_OPEN_Y) ? v : 0.f;
             }
             // z face (k | k+1): p(k+1) from the next lane, or the right halo
@@ -293,13 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
             if (lane == kWave - 1) pk1 = hr_p[r];
             float vzn = vzo[r];
             if (k + 1 < nz) {
-                float v = vzo[r] + cvz * (pk1 - pr);
-                if (ADE && row_ade && kv && m_c) {
-                    const int m_hi = mat_at(i, jr, k + 1);
-                    if (m_hi == m_c)
-                        v = face_density_corr<NU>(a, v, m_c, m_hi, box_idx(i, jr, k),
-                                                  box_idx(i, jr, k + 1), box_n, ifz_k);
-                }
+                const float v = vzo[r] + cvz * (pk1 - pr);
                 vzn = (f & FDTD_FLAG_OPEN_Z) ? v : 0.f;
             }
             // z face (k-1 | k): previous lane, or the left halo for lane 0
@@ -307,16 +214,8 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
             if (lane == 0) {
                 vzm = 0.f;
                 if (k0 > 0) {
-                    const int km = k0 - 1;
-                    const float cz = NU ? cv * a.ifc[2][km] : cv;
-                    float v = hl_vz[r] + cz * (pr - hl_p[r]);
-                    if (ADE && row_ade && kv) {
-                        const int m_lo = mat_at(i, jr, km);
-                        if (m_lo && m_lo == m_c)
-                            v = face_density_corr<NU>(a, v, m_lo, m_c, box_idx(i, jr, km),
-                                                      box_idx(i, jr, k0), box_n,
-                                                      NU ? a.ifc[2][km] : 1.f);
-                    }
+                    const float cz = NU ? cv * a.ifc[2][k0 - 1] : cv;
+                    const float v = hl_vz[r] + cz * (pr - hl_p[r]);
                     vzm = (hl_f[r] & FDTD_FLAG_OPEN_Z) ? v : 0.f;
                 }
             }
@@ -332,40 +231,6 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
                 pnew = pr + cp * ((ddx + ddy) + ddz);
             } else {
                 pnew = pr + cp * ((dvx + dvy) + dvz);
-            }
-            if (ADE && row_ade && kv && m_c && i >= a.box_lo[0] && jr >= a.box_lo[1] &&
-                k >= a.box_lo[2] && k < a.box_hi[2]) {
-                // modulus poles: source is the native divergence of v'
-                const float ix = NU ? icx_i : a.ade_inv_dx;
-                const float iy = NU ? a.icl[1][jr] : a.ade_inv_dx;
-                const float iz = NU ? icz_k : a.ade_inv_dx;
-                float dA = dvx * ix;
-                dA += dvy * iy;
-                dA += dvz * iz;
-                const int64_t bc = box_idx(i, jr, k);
-#pragma unroll
-                for (int pass = 0; pass < 2; ++pass)
-                    for (int q = 0; q < a.n_pole; ++q) {
-                        if (a.pole_target[q] != 1 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m_c)
-                            continue;
-                        float* Jq = a.J + q * box_n;
-                        if (!pass) {
-                            Jq[bc] = a.pole_a[q] * Jq[bc] + a.pole_b[q] * dA;
-                        } else {
-                            float* Jp = a.Jprev + q * box_n;
-                            const float j0v = Jq[bc], j1v = Jp[bc];
-                            const float jn = a.pole_a[q] * j0v + a.pole_b[q] * j1v + a.pole_d[q] * dA;
-                            Jp[bc] = j0v;
-                            Jq[bc] = jn;
-                        }
-                    }
-#pragma unroll
-                for (int pass = 0; pass < 2; ++pass)
-                    for (int q = 0; q < a.n_pole; ++q) {
-                        if (a.pole_target[q] != 1 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m_c)
-                            continue;
-                        pnew += a.pole_corr[q] * a.J[q * box_n + bc];
-                    }
             }
             if (!(f & FDTD_FLAG_AIR)) pnew = 0.f;
 
@@ -409,13 +274,71 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     }
 }
 
-// Density-pole auxiliary update over the ADE box, run before the fused step of
-// the same time step (J <- a J + b p, Lorentz with history), ade.cpp:25-166.
+template <int R, bool NU>
+__global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    // XCD-aware remap: hardware block b runs on XCD b % 8; give each XCD a
+    // contiguous range of logical blocks so strips that share halo rows share
+    // an L2. gridDim.x is a multiple of 8.
+    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
+    // wave index within the block, made provably wave-uniform (scalar registers)
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w >= a.nwaves) return;
+    if (*a.step_ptr + a.koff >= a.step_limit) return;
+    const int seg = w % a.nseg;
+    const int t = w / a.nseg;
+    fused_strip<R, NU>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane);
+}
+
+// Batched step: one launch advances every solve in `table` by one step. Waves
+// are dealt out by solve through the prefix sums `offsets` (n + 1 entries);
+// a solve whose counter reached its limit is left untouched.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_batch_step(const FusedArgs* __restrict__ table,
+                                                       const int* __restrict__ offsets, int n,
+                                                       int blocks_per_xcd) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int lb = (blockIdx.x & 7) * blocks_per_xcd + (blockIdx.x >> 3);
+    const int gw = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (gw >= offsets[n]) return;
+    int lo = 0, hi = n;  // offsets[lo] <= gw < offsets[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (offsets[mid] <= gw) lo = mid; else hi = mid;
+    }
+    const FusedArgs& a = table[lo];
+    const int w = gw - offsets[lo];
+    if (*a.step_ptr >= a.step_limit) return;
+    const int seg = w % a.nseg;
+    const int t = w / a.nseg;
+    fused_strip<R, false>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane);
+}
+
+__global__ void k_batch_advance(const FusedArgs* __restrict__ table, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    int64_t* step = const_cast<int64_t*>(table[e].step_ptr);
+    if (step[0] < table[e].step_limit) step[0] = step[0] + 1;
+}
+
+// ---- ADE --------------------------------------------------------------------
+__device__ __forceinline__ int64_t box_index(const FusedArgs& a, int i, int j, int k) {
+    const int by = a.box_hi[1] - a.box_lo[1], bz = a.box_hi[2] - a.box_lo[2];
+    return ((static_cast<int64_t>(i - a.box_lo[0]) * by) + (j - a.box_lo[1])) * bz + (k - a.box_lo[2]);
+}
+
+__device__ __forceinline__ int64_t box_cells(const FusedArgs& a) {
+    return static_cast<int64_t>(a.box_hi[0] - a.box_lo[0]) * (a.box_hi[1] - a.box_lo[1]) *
+           (a.box_hi[2] - a.box_lo[2]);
+}
+
+// Density-pole auxiliary update over the ADE box, run before the sweep of the
+// same time step (J <- a J + b p, Lorentz with history), ade.cpp:25-166.
 __global__ void k_ade_density(const FusedArgs a) {
-    const int bx = a.box_hi[0] - a.box_lo[0], by = a.box_hi[1] - a.box_lo[1],
-              bz = a.box_hi[2] - a.box_lo[2];
-    const int64_t box_n = static_cast<int64_t>(bx) * by * bz;
+    const int by = a.box_hi[1] - a.box_lo[1], bz = a.box_hi[2] - a.box_lo[2];
+    const int64_t box_n = box_cells(a);
     const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit) return;
     const float* __restrict__ P = (s & 1) ? a.f[1][0] : a.f[0][0];
     for (int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; b < box_n;
          b += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -442,21 +365,141 @@ __global__ void k_ade_density(const FusedArgs a) {
     }
 }
 
+// v' of the face between cells lo (flat c) and hi (c + step along `axis`), with
+// the density-pole correction (Debye poles first, then Lorentz, ade.cpp:240-401).
+template <bool NU>
+__device__ __forceinline__ float ade_face(const FusedArgs& a, const float* __restrict__ P,
+                                          const float* __restrict__ V, int axis, int i, int j, int k,
+                                          int64_t c, int64_t step) {
+    const int n_ax = axis == 0 ? a.nx : (axis == 1 ? a.ny : a.nz);
+    const int idx = axis == 0 ? i : (axis == 1 ? j : k);
+    const float vo = V[c];
+    if (idx + 1 >= n_ax) return vo;  // last face: not updated, not masked
+    const float cf = NU ? a.cv * a.ifc[axis][idx] : a.cv;
+    float v = vo + cf * (P[c + step] - P[c]);
+    const int m_lo = a.material[c], m_hi = a.material[c + step];
+    if (m_lo && m_lo == m_hi) {
+        const int64_t box_n = box_cells(a);
+        const int ih = i + (axis == 0), jh = j + (axis == 1), kh = k + (axis == 2);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int q = 0; q < a.n_pole; ++q) {
+                if (a.pole_target[q] != 0 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m_lo) continue;
+                const float* Jq = a.J + q * box_n;
+                const float corr = NU ? a.pole_corr[q] * a.ifc[axis][idx] : a.pole_corr[q];
+                v += corr * (Jq[box_index(a, ih, jh, kh)] - Jq[box_index(a, i, j, k)]);
+            }
+    }
+    const uint32_t open = axis == 0 ? FDTD_FLAG_OPEN_X : (axis == 1 ? FDTD_FLAG_OPEN_Y : FDTD_FLAG_OPEN_Z);
+    return (a.flags[c] & open) ? v : 0.f;
+}
+
+// Full ADE update of every pole-material cell in the box, from the input set,
+// overwriting what the plain sweep wrote for those cells (see file comment).
+template <bool NU>
+__global__ void k_ade_fixup(const FusedArgs a) {
+    const int ny = a.ny, nz = a.nz;
+    const int by = a.box_hi[1] - a.box_lo[1], bz = a.box_hi[2] - a.box_lo[2];
+    const int64_t box_n = box_cells(a);
+    const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit) return;
+    const int par = static_cast<int>(s & 1);
+    const float* __restrict__ P = par ? a.f[1][0] : a.f[0][0];
+    const float* __restrict__ VX = par ? a.f[1][1] : a.f[0][1];
+    const float* __restrict__ VY = par ? a.f[1][2] : a.f[0][2];
+    const float* __restrict__ VZ = par ? a.f[1][3] : a.f[0][3];
+    float* __restrict__ Po = par ? a.f[0][0] : a.f[1][0];
+    float* __restrict__ VXo = par ? a.f[0][1] : a.f[1][1];
+    float* __restrict__ VYo = par ? a.f[0][2] : a.f[1][2];
+    float* __restrict__ VZo = par ? a.f[0][3] : a.f[1][3];
+    const int64_t sy = nz, sx = static_cast<int64_t>(ny) * nz;
+    for (int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; b < box_n;
+         b += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int k = static_cast<int>(b % bz) + a.box_lo[2];
+        const int j = static_cast<int>((b / bz) % by) + a.box_lo[1];
+        const int i = static_cast<int>(b / (static_cast<int64_t>(by) * bz)) + a.box_lo[0];
+        const int64_t c = lin(i, j, k, ny, nz);
+        const int m = a.material[c];
+        bool has_pole = false;
+        for (int q = 0; q < a.n_pole; ++q) has_pole |= a.pole_mat[q] == m;
+        if (!m || !has_pole) continue;
+        const uint32_t f = a.flags[c];
+        const float vxn = ade_face<NU>(a, P, VX, 0, i, j, k, c, sx);
+        const float vyn = ade_face<NU>(a, P, VY, 1, i, j, k, c, sy);
+        const float vzn = ade_face<NU>(a, P, VZ, 2, i, j, k, c, 1);
+        const float vxm = i > 0 ? ade_face<NU>(a, P, VX, 0, i - 1, j, k, c - sx, sx) : 0.f;
+        const float vym = j > 0 ? ade_face<NU>(a, P, VY, 1, i, j - 1, k, c - sy, sy) : 0.f;
+        const float vzm = k > 0 ? ade_face<NU>(a, P, VZ, 2, i, j, k - 1, c - 1, 1) : 0.f;
+        const float dvx = vxn - vxm, dvy = vyn - vym, dvz = vzn - vzm;
+        const float icx = NU ? a.icl[0][i] : 1.f, icy = NU ? a.icl[1][j] : 1.f, icz = NU ? a.icl[2][k] : 1.f;
+        // modulus poles: source is the native divergence of v'
+        const float ix = NU ? icx : a.ade_inv_dx, iy = NU ? icy : a.ade_inv_dx, iz = NU ? icz : a.ade_inv_dx;
+        float dA = dvx * ix;
+        dA += dvy * iy;
+        dA += dvz * iz;
+        for (int pass = 0; pass < 2; ++pass)
+            for (int q = 0; q < a.n_pole; ++q) {
+                if (a.pole_target[q] != 1 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m) continue;
+                float* Jq = a.J + q * box_n;
+                if (!pass) {
+                    Jq[b] = a.pole_a[q] * Jq[b] + a.pole_b[q] * dA;
+                } else {
+                    float* Jp = a.Jprev + q * box_n;
+                    const float j0v = Jq[b], j1v = Jp[b];
+                    const float jn = a.pole_a[q] * j0v + a.pole_b[q] * j1v + a.pole_d[q] * dA;
+                    Jp[b] = j0v;
+                    Jq[b] = jn;
+                }
+            }
+        float pnew;
+        if (NU) pnew = P[c] + a.cp * (((icx * dvx) + (icy * dvy)) + (icz * dvz));
+        else pnew = P[c] + a.cp * ((dvx + dvy) + dvz);
+        for (int pass = 0; pass < 2; ++pass)
+            for (int q = 0; q < a.n_pole; ++q) {
+                if (a.pole_target[q] != 1 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m) continue;
+                pnew += a.pole_corr[q] * a.J[q * box_n + b];
+            }
+        if (!(f & FDTD_FLAG_AIR)) pnew = 0.f;
+        const float dx = a.dec[0][i], dy = a.dec[1][j], dz = a.dec[2][k];
+        float ps = ((pnew * dx) * dy) * dz;
+        if (s < a.wave_len && (f & FDTD_FLAG_AIR)) {
+            for (int q = 0; q < a.n_src; ++q) {
+                bool on;
+                if (a.src_kind[q] == 0) {
+                    on = a.src_i[q] == i && a.src_j[q] == j && a.src_k[q] == k;
+                } else {
+                    const int ax = a.src_i[q], id = a.src_j[q];
+                    on = ax == 0 ? id == i : (ax == 1 ? id == j : id == k);
+                }
+                if (on) ps = static_cast<float>(static_cast<double>(ps) + a.wave[q * a.wave_len + s]);
+            }
+        }
+        Po[c] = ps;
+        VXo[c] = vxn * dx;
+        VYo[c] = vyn * dy;
+        VZo[c] = vzn * dz;
+        if (s < a.trace_len)
+            for (int q = 0; q < a.n_probe; ++q)
+                if (a.probe_ijk[3 * q] == i && a.probe_ijk[3 * q + 1] == j && a.probe_ijk[3 * q + 2] == k) {
+                    a.trace[s * a.n_probe + q] = ps;
+                    if (!isfinite(ps)) atomicOr(a.health, 1u);
+                }
+    }
+}
+
 __global__ void k_advance(int64_t* step, int n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) step[0] = step[0] + n;
 }
 
 // ---- host-side launch helpers ------------------------------------------------
-template <int R, bool NU, bool ADE>
+template <int R, bool NU>
 static hipError_t launch_r(const FusedArgs& a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_fused_step<R, NU, ADE>), dim3(blocks), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL((k_fused_step<R, NU>), dim3(blocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
 
 template <int R>
-static hipError_t launch_variants(const FusedArgs& a, bool nu, bool ade, int blocks, hipStream_t st) {
-    if (nu) return ade ? launch_r<R, true, true>(a, blocks, st) : launch_r<R, true, false>(a, blocks, st);
-    return ade ? launch_r<R, false, true>(a, blocks, st) : launch_r<R, false, false>(a, blocks, st);
+static hipError_t launch_rows(const FusedArgs& a, bool nu, int blocks, hipStream_t st) {
+    return nu ? launch_r<R, true>(a, blocks, st) : launch_r<R, false>(a, blocks, st);
 }
 
 // Fill the tiling fields of `a` for rows R / planes; returns the grid size.
@@ -466,83 +509,51 @@ int fused_grid(FusedArgs& a, int rows, int planes) {
     a.nstrip_j = (a.ny + rows - 1) / rows;
     a.nstrip_i = (a.nx + planes - 1) / planes;
     a.nwaves = a.nseg * a.nstrip_j * a.nstrip_i;
-    a.sub_mode = 0;
-    a.skip_sub = 0;
     int blocks = (a.nwaves + kBlockWaves - 1) / kBlockWaves;
     blocks = (blocks + 7) & ~7;
     a.blocks_per_xcd = blocks / 8;
     return blocks;
 }
 
-// With an ADE box the step is two launches over disjoint strips that read the
-// same inputs: the plain strips on `st` and the ADE strips on `side`, forked
-// after the density-pole update and joined before the step ends, so the small
-// ADE launch overlaps the large plain one (also inside a captured graph).
-template <int R>
-static hipError_t launch_rows(const FusedArgs& a, bool nu, bool ade, int blocks, hipStream_t st,
-                              hipStream_t side, hipEvent_t fork, hipEvent_t join) {
-    if (!ade) return launch_variants<R>(a, nu, false, blocks, st);
-    // Strips that touch the ADE box run the ADE instantiation; every other strip
-    // runs the plain one (fewer registers, higher occupancy).
-    FusedArgs plain = a;
-    FusedArgs mat = a;
-    const int rows = R;
-    int lo[3] = {a.nseg, a.nstrip_j, a.nstrip_i}, hi[3] = {0, 0, 0};
-    for (int s = 0; s < a.nseg; ++s) {
-        const int k0 = s * kWave;
-        if (k0 + kWave > a.box_lo[2] && k0 - 1 < a.box_hi[2]) { lo[0] = std::min(lo[0], s); hi[0] = s + 1; }
-    }
-    for (int s = 0; s < a.nstrip_j; ++s) {
-        const int j0 = s * rows;
-        if (j0 + rows > a.box_lo[1] && j0 - 1 < a.box_hi[1]) { lo[1] = std::min(lo[1], s); hi[1] = s + 1; }
-    }
-    for (int s = 0; s < a.nstrip_i; ++s) {
-        const int i0 = s * a.planes, i1 = std::min(i0 + a.planes, a.nx);
-        if (i1 > a.box_lo[0] && i0 - 1 < a.box_hi[0]) { lo[2] = std::min(lo[2], s); hi[2] = s + 1; }
-    }
-    for (int d = 0; d < 3; ++d) {
-        if (hi[d] <= lo[d]) return launch_variants<R>(a, nu, false, blocks, st);
-        plain.sub_lo[d] = mat.sub_lo[d] = lo[d];
-        plain.sub_hi[d] = mat.sub_hi[d] = hi[d];
-    }
-    plain.sub_mode = 0;
-    plain.skip_sub = 1;
-    mat.sub_mode = 1;
-    mat.skip_sub = 0;
-    const int n_sub = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
-    int mb = (n_sub + kBlockWaves - 1) / kBlockWaves;
-    mb = (mb + 7) & ~7;
-    mat.blocks_per_xcd = mb / 8;
-    hipError_t e;
-    if (side) {
-        if ((e = hipEventRecord(fork, st)) != hipSuccess) return e;
-        if ((e = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return e;
-        if ((e = launch_variants<R>(mat, nu, true, mb, side)) != hipSuccess) return e;
-        if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
-        if ((e = launch_variants<R>(plain, nu, false, blocks, st)) != hipSuccess) return e;
-        return hipStreamWaitEvent(st, join, 0);
-    }
-    if ((e = launch_variants<R>(plain, nu, false, blocks, st)) != hipSuccess) return e;
-    return launch_variants<R>(mat, nu, true, mb, st);
-}
-
-hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st,
-                        hipStream_t side, hipEvent_t fork, hipEvent_t join) {
+hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st) {
+    int box_blocks = 1;
     if (ade) {
         const int64_t bn = static_cast<int64_t>(a.box_hi[0] - a.box_lo[0]) *
                            (a.box_hi[1] - a.box_lo[1]) * (a.box_hi[2] - a.box_lo[2]);
-        const int nb = static_cast<int>(std::min<int64_t>((bn + 255) / 256, 4096));
-        hipLaunchKernelGGL(k_ade_density, dim3(nb > 0 ? nb : 1), dim3(256), 0, st, a);
+        box_blocks = static_cast<int>(std::min<int64_t>(std::max<int64_t>((bn + 255) / 256, 1), 1 << 16));
+        hipLaunchKernelGGL(k_ade_density, dim3(box_blocks), dim3(256), 0, st, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
+    hipError_t e;
     switch (rows) {
-        case 2: return launch_rows<2>(a, nu, ade, blocks, st, side, fork, join);
-        case 4: return launch_rows<4>(a, nu, ade, blocks, st, side, fork, join);
-        case 8: return launch_rows<8>(a, nu, ade, blocks, st, side, fork, join);
-        case 16: return launch_rows<16>(a, nu, ade, blocks, st, side, fork, join);
+        case 2: e = launch_rows<2>(a, nu, blocks, st); break;
+        case 4: e = launch_rows<4>(a, nu, blocks, st); break;
+        case 8: e = launch_rows<8>(a, nu, blocks, st); break;
+        case 16: e = launch_rows<16>(a, nu, blocks, st); break;
         default: return hipErrorInvalidValue;
     }
+    if (e != hipSuccess || !ade) return e;
+    if (nu) hipLaunchKernelGGL(k_ade_fixup<true>, dim3(box_blocks), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_ade_fixup<false>, dim3(box_blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const FusedArgs* table, const int* offsets, int n, int total_waves, int rows,
+                        hipStream_t st) {
+    int blocks = (total_waves + kBlockWaves - 1) / kBlockWaves;
+    blocks = std::max(8, (blocks + 7) & ~7);
+    const int bpx = blocks / 8;
+    switch (rows) {
+        case 2: hipLaunchKernelGGL(k_batch_step<2>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n, bpx); break;
+        case 4: hipLaunchKernelGGL(k_batch_step<4>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n, bpx); break;
+        case 8: hipLaunchKernelGGL(k_batch_step<8>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n, bpx); break;
+        default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_advance, dim3((n + 63) / 64), dim3(64), 0, st, table, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st) {

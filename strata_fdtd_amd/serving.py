@@ -1,0 +1,290 @@
+"""Continuous-batching server for design sweeps of many independent solves.
+
+Requests (grid size, SDF geometry, source, probes, step count) arrive over
+time. The server keeps each admitted solve's fields and tables resident in HBM
+(one ``FDTDSolver`` per request, built through the public API), admits waiting
+requests whenever an active-slot and the HBM budget allow, and advances all
+active solves together with the batched step kernel (one launch per step over
+every solve's strips, captured in a hipGraph per chunk). While a chunk runs on
+the GPU the host builds the next arrivals' scenes, so set-up overlaps device
+work. A finished solve's probe traces are copied out and its memory released.
+
+The batched kernel executes, for each solve, exactly the per-wave arithmetic of
+the single-solve kernel, so every result is bit-identical alone and in the mix
+(tests/test_gpu_serving.py checks that).
+
+Reported: solves/s and latency (arrival -> result) p50/p99, next to running
+the same requests one at a time through plain ``FDTDSolver(backend="mi355x")``.
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .boundaries import PML
+from .geometry import Box, Cylinder, Difference, SDFPrimitive, Union
+from .grid import UniformGrid
+from .solver import FDTDSolver
+from .sources import GaussianPulse
+from .tuning import pick_tiling
+
+BYTES_PER_CELL_RESIDENT = 2 * 16 + 3   # ping-pong fields + flags + mask + material ids
+
+
+@dataclass
+class SolveRequest:
+    rid: int
+    shape: tuple[int, int, int]
+    resolution: float
+    pml_depth: int
+    geometry: SDFPrimitive | np.ndarray | None
+    source: GaussianPulse
+    probes: list
+    steps: int
+    arrival: float = 0.0          # seconds after the stream starts
+
+    @property
+    def cells(self) -> int:
+        return int(np.prod(self.shape))
+
+    @property
+    def footprint(self) -> int:
+        return self.cells * BYTES_PER_CELL_RESIDENT + self.steps * (len(self.probes) * 4 + 8)
+
+
+@dataclass
+class SolveResult:
+    rid: int
+    traces: dict
+    arrival: float
+    admitted: float
+    finished: float
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+def build_solver(req: SolveRequest, device: int = 0, tiling=None) -> FDTDSolver:
+    """The request as a plain single-solve FDTDSolver (also the one-at-a-time path)."""
+    s = FDTDSolver(grid=UniformGrid(req.shape, req.resolution), c=343.0, rho=1.204, backend="mi355x",
+                   device=device, tiling=tiling)
+    if req.geometry is not None:
+        s.set_geometry(req.geometry)
+    if req.pml_depth:
+        s.add_boundary(PML(depth=req.pml_depth))
+    s.add_source(GaussianPulse(position=tuple(req.source.position), frequency=req.source.frequency,
+                               bandwidth=req.source.bandwidth, amplitude=req.source.amplitude))
+    for q, p in enumerate(req.probes):
+        s.add_probe(f"probe{q}", tuple(int(v) for v in p))
+    return s
+
+
+def sweep_requests(n: int = 64, rate: float = 40.0, seed: int = 1234, steps=(2000, 8000),
+                   dims=tuple(range(64, 193, 16))) -> list[SolveRequest]:
+    """Seeded Poisson stream of enclosure solves (the design-sweep workload).
+
+    Each axis is drawn from 64..192 in steps of 16, 1 mm cells, PML 8-12; a box
+    enclosure with 6-12-cell walls and one cylindrical port (radius 4-10, length
+    10-40) through its -x wall; a Gaussian pulse of 200 Hz-4 kHz at a seeded
+    interior point; 1-4 probes inside; step count uniform in [2000, 8000].
+    Inter-arrival times are exponential with mean 1/rate seconds.
+    """
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    dx = 1e-3
+    for rid in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        shape = tuple(int(rng.choice(dims)) for _ in range(3))
+        pml = int(rng.integers(8, 13))
+        wall = int(rng.integers(6, 13))
+        lo = np.array([pml + 4] * 3)
+        hi = np.array(shape) - pml - 4
+        wall = min(wall, int((hi - lo).min()) // 2 - 3)   # keep a cavity on small test grids
+        outer = Box(min_corner=tuple(lo * dx), max_corner=tuple(hi * dx))
+        inner = Box(min_corner=tuple((lo + wall) * dx), max_corner=tuple((hi - wall) * dx))
+        r = int(rng.integers(4, 11))
+        plen = int(rng.integers(10, 41))
+        cy, cz = (lo[1] + hi[1]) / 2 * dx, (lo[2] + hi[2]) / 2 * dx
+        x0 = lo[0] * dx
+        port = Cylinder(p1=(x0 - 0.5 * plen * dx, cy, cz), p2=(x0 + 0.5 * plen * dx + wall * dx, cy, cz),
+                        radius=r * dx)
+        domain = Box(min_corner=(-1.0, -1.0, -1.0), max_corner=(2.0, 2.0, 2.0))
+        geom = Union(Difference(domain, Difference(outer, inner)), port)
+        cav_lo, cav_hi = lo + wall + 2, hi - wall - 2
+        src = tuple(int(v) for v in rng.integers(cav_lo, cav_hi))
+        f = float(rng.uniform(200.0, 4000.0))
+        probes = [tuple(int(v) for v in rng.integers(cav_lo, cav_hi)) for _ in range(int(rng.integers(1, 5)))]
+        steps_n = int(rng.integers(steps[0], steps[1] + 1))
+        out.append(SolveRequest(rid, shape, dx, pml, geom, GaussianPulse(position=src, frequency=f), probes,
+                                steps_n, t))
+    return out
+
+
+@dataclass
+class _Slot:
+    req: SolveRequest
+    solver: FDTDSolver
+    admitted: float
+    done: int = 0
+
+
+@dataclass
+class ServeStats:
+    results: list = field(default_factory=list)
+    wall: float = 0.0
+    chunks: int = 0
+    max_active: int = 0
+
+    @property
+    def solves_per_s(self) -> float:
+        return len(self.results) / self.wall if self.wall > 0 else 0.0
+
+    def latency_pct(self, q: float) -> float:
+        return float(np.percentile([r.latency for r in self.results], q))
+
+    def summary(self) -> dict:
+        return {"solves": len(self.results), "wall_s": round(self.wall, 3),
+                "solves_per_s": round(self.solves_per_s, 3),
+                "p50_s": round(self.latency_pct(50), 3), "p99_s": round(self.latency_pct(99), 3),
+                "chunks": self.chunks, "max_active": self.max_active}
+
+
+class SweepServer:
+    """Continuous batching of uniform-grid solves on one GPU."""
+
+    def __init__(self, device: int = 0, rows: int = 4, chunk: int = 100, max_active: int = 64,
+                 hbm_budget: int | None = None, use_graph: bool = True):
+        import torch
+        self.device = device
+        self.rows = rows
+        self.chunk = chunk
+        self.max_active = max_active
+        self.use_graph = use_graph
+        if hbm_budget is None:
+            free, _total = torch.cuda.mem_get_info(device)
+            hbm_budget = int(free * 0.6)
+        self.hbm_budget = hbm_budget
+        self._stream = torch.cuda.current_stream(torch.device(f"cuda:{device}"))
+        h = _native._vp()
+        _native.call("fdtd_batch_create", device, self._stream.cuda_stream, rows, _native.C.byref(h))
+        self._batch = h.value
+
+    def close(self) -> None:
+        if self._batch:
+            _native.load().fdtd_batch_destroy(self._batch)
+            self._batch = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _set_batch(self, slots: list[_Slot]) -> None:
+        ctxs = (_native._vp * max(len(slots), 1))(*[s.solver._ctx for s in slots])
+        limits = np.array([s.req.steps for s in slots] or [0], dtype=np.int64)
+        _native.call("fdtd_batch_set", self._batch, ctxs, limits.ctypes.data, len(slots))
+
+    def serve(self, requests: list[SolveRequest], realtime: bool = True) -> ServeStats:
+        """Serve a stream; arrival times are honoured against the wall clock
+        when ``realtime`` (otherwise every request is available at t=0)."""
+        import torch
+        pending = sorted(requests, key=lambda r: r.arrival)
+        stats = ServeStats()
+        active: list[_Slot] = []
+        used = 0
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        built: dict[int, FDTDSolver] = {}
+
+        def prebuild(limit_time: float) -> None:
+            # host-side scene set-up for requests that have arrived, while the GPU runs
+            for r in pending:
+                if r.rid in built:
+                    continue
+                if realtime and r.arrival > limit_time:
+                    break
+                if len(built) >= 4:
+                    break
+                s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1]))
+                s._prepare(r.steps)
+                built[r.rid] = s
+
+        while pending or active:
+            changed = False
+            tnow = now()
+            # admission: arrived requests, slots and HBM permitting
+            while pending and (not realtime or pending[0].arrival <= tnow):
+                r = pending[0]
+                if len(active) >= self.max_active or (active and used + r.footprint > self.hbm_budget):
+                    break
+                pending.pop(0)
+                s = built.pop(r.rid, None)
+                if s is None:
+                    s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1]))
+                    s._prepare(r.steps)
+                active.append(_Slot(r, s, now()))
+                used += r.footprint
+                changed = True
+            if not active:
+                # idle until the next arrival (build it meanwhile)
+                if pending and realtime:
+                    prebuild(pending[0].arrival)
+                    wait = pending[0].arrival - now()
+                    if wait > 0:
+                        time.sleep(wait)
+                continue
+            if changed:
+                self._set_batch(active)
+            stats.max_active = max(stats.max_active, len(active))
+            _native.call("fdtd_batch_step_n", self._batch, self.chunk, 1 if self.use_graph else 0)
+            stats.chunks += 1
+            prebuild(now() + 0.05)          # overlaps the chunk on the GPU
+            torch.cuda.synchronize(self.device)
+            finished = []
+            for sl in active:
+                step = min(self.chunk, sl.req.steps - sl.done)
+                sl.done += step
+                if sl.done >= sl.req.steps:
+                    finished.append(sl)
+            if finished:
+                for sl in finished:
+                    sl.solver._advance_host(sl.req.steps)
+                    traces = sl.solver.get_probe_data()
+                    stats.results.append(SolveResult(sl.req.rid, traces, sl.req.arrival, sl.admitted, now()))
+                    used -= sl.req.footprint
+                    active.remove(sl)
+                    del sl.solver
+                self._set_batch(active)
+        stats.wall = now()
+        for s in built.values():
+            del s
+        return stats
+
+
+def serve_one_at_a_time(requests: list[SolveRequest], device: int = 0, realtime: bool = True,
+                        graph_chunk: int = 64) -> ServeStats:
+    """Baseline: each request runs alone through plain FDTDSolver, in arrival order."""
+    import torch
+    stats = ServeStats()
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0  # noqa: E731
+    for r in sorted(requests, key=lambda x: x.arrival):
+        if realtime and r.arrival > now():
+            time.sleep(r.arrival - now())
+        start = now()
+        s = build_solver(r, device)
+        s.step_n(r.steps, graph_chunk=graph_chunk)
+        traces = s.get_probe_data()
+        torch.cuda.synchronize(device)
+        stats.results.append(SolveResult(r.rid, traces, r.arrival, start, now()))
+        stats.max_active = 1
+        del s
+    stats.wall = now()
+    return stats
