@@ -26,6 +26,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -104,6 +106,108 @@ def cpu_baseline(solver, seconds_budget: float = 20.0):
     }
 
 
+def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
+    """Continuous batching vs one-at-a-time on the seeded Poisson design sweep."""
+    import torch
+    from strata_fdtd_amd.serving import SweepServer, serve_one_at_a_time, sweep_requests
+    reqs = sweep_requests(n=n_requests, rate=rate)
+    # warm both paths (code objects, allocator) on a tiny stream first
+    warm = sweep_requests(n=3, rate=1e6, seed=99, steps=(20, 40), dims=(64,))
+    SweepServer(chunk=10).serve(warm, realtime=False)
+    serve_one_at_a_time(warm, realtime=False)
+    torch.cuda.synchronize()
+    log(f"serving: {len(reqs)} requests, Poisson rate {rate}/s, "
+        f"{sum(r.cells * r.steps for r in reqs) / 1e9:.1f} Gcell-updates in total")
+    srv = SweepServer(chunk=chunk)
+    batched = srv.serve(reqs, realtime=True)
+    srv.close()
+    log(f"serving batched: {batched.summary()}")
+    single = serve_one_at_a_time(reqs, realtime=True)
+    log(f"serving one-at-a-time: {single.summary()}")
+    # spot-check equality of results in the mix and alone
+    same = all(
+        all(np.array_equal(a.traces[k], b.traces[k]) for k in a.traces)
+        for a, b in zip(sorted(batched.results, key=lambda r: r.rid), sorted(single.results, key=lambda r: r.rid)))
+    return {
+        "requests": len(reqs), "arrival_rate_per_s": rate, "chunk_steps": chunk,
+        "gcell_updates_total": round(sum(r.cells * r.steps for r in reqs) / 1e9, 2),
+        "batched": batched.summary(), "one_at_a_time": single.summary(),
+        "speedup_solves_per_s": round(batched.solves_per_s / single.solves_per_s, 3),
+        "results_identical_alone_and_in_mix": bool(same),
+    }
+
+
+def adjoint_bench(n: int = 192, steps: int = 2000, fd_voxels: int = 0):
+    """Gradient cost (forward + checkpointed adjoint) vs a forward solve."""
+    import torch
+    from strata_fdtd_amd import FDTDSolver, PML, UniformGrid
+    from strata_fdtd_amd.adjoint import design_problem_192, forward_traces, probe_traces
+    pb, rho, K, target = design_problem_192(n=n, steps=steps)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, out
+
+    forward_traces(pb, rho, K)  # warm-up
+    t_fwd, _ = timed(lambda: forward_traces(pb, rho, K))
+    plain = FDTDSolver(grid=UniformGrid(pb.shape, pb.resolution), c=pb.c, rho=pb.rho, backend="mi355x")
+    plain.add_boundary(PML(depth=pb.pml_depth))
+    plain.add_source(pb.source)
+    for q, p in enumerate(pb.probes):
+        plain.add_probe(f"p{q}", p)
+    plain._prepare(steps)
+    t_plain, _ = timed(lambda: plain.step_n(steps))
+
+    def grad():
+        r = rho.clone().requires_grad_(True)
+        k = K.clone().requires_grad_(True)
+        tr = probe_traces(r, k, pb)
+        loss = ((tr - target) ** 2).sum()
+        loss.backward()
+        return loss.item(), r.grad, k.grad
+
+    t_grad, (loss, g_r, g_k) = timed(grad)
+    out = {
+        "shape": list(pb.shape), "steps": steps, "design_box": list(pb.box_shape), "checkpoint": pb.checkpoint,
+        "forward_per_voxel_s": round(t_fwd, 4), "forward_plain_fused_s": round(t_plain, 4),
+        "gradient_s": round(t_grad, 4),
+        "gradient_over_forward": round(t_grad / t_fwd, 3),
+        "gradient_over_plain_forward": round(t_grad / t_plain, 3),
+        "loss": loss, "grad_norm_rho": float(g_r.norm()), "grad_norm_K": float(g_k.norm()),
+    }
+    if fd_voxels:
+        rng = np.random.default_rng(17)
+        rows = []
+
+        def loss64(r_, k_):
+            tr, _ = forward_traces(pb, r_, k_, dtype=torch.float64)
+            return float(((tr - target.double()) ** 2).sum())
+
+        for v in range(fd_voxels):
+            which = "rho" if v % 2 == 0 else "K"
+            base, grad_t = (rho, g_r) if which == "rho" else (K, g_k)
+            idx = tuple(int(rng.integers(0, s)) for s in pb.box_shape)
+            b64 = base.double()
+            plus, minus = b64.clone(), b64.clone()
+            plus[idx] *= 1 + 1e-4
+            minus[idx] *= 1 - 1e-4
+            dp = float(np.float32(plus[idx].item())) - float(np.float32(minus[idx].item()))
+            lp = loss64(plus if which == "rho" else rho.double(), plus if which == "K" else K.double())
+            lm = loss64(minus if which == "rho" else rho.double(), minus if which == "K" else K.double())
+            fd = (lp - lm) / dp
+            g = float(grad_t[idx])
+            rows.append({"param": which, "voxel": list(idx), "adjoint": g, "fd": fd,
+                         "rel_err": abs(g - fd) / max(abs(fd), 1e-300)})
+        errs = np.array([r["rel_err"] for r in rows])
+        out["finite_differences"] = {"voxels": fd_voxels, "h_rel": 1e-4, "fd_precision": "f64",
+                                     "median_rel_err": float(np.median(errs)), "max_rel_err": float(errs.max()),
+                                     "rows": rows}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +217,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--graph-chunk", type=int, default=50)
     ap.add_argument("--tiling", default=None, help="rows,planes override")
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving sweep")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference check on N voxels")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -198,10 +305,24 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    del solver
+    torch.cuda.empty_cache()
+    if not args.no_serving:
+        try:
+            line["serving"] = serving_bench()
+        except Exception as exc:
+            line["serving"] = {"error": f"{type(exc).__name__}: {exc}"}
+    if not args.no_adjoint:
+        try:
+            line["adjoint"] = adjoint_bench(fd_voxels=args.adjoint_fd)
+            log(f"adjoint: {json.dumps({k: v for k, v in line['adjoint'].items() if k != 'finite_differences'})}")
+        except Exception as exc:
+            line["adjoint"] = {"error": f"{type(exc).__name__}: {exc}"}
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         log("cpu baseline ...")
         try:
-            line["cpu_baseline"] = cpu_baseline(solver)
+            line["cpu_baseline"] = cpu_baseline(wl.build(**kw))
         except Exception as exc:  # the GPU number stands; report why the baseline is missing
             line["cpu_baseline"] = {"error": f"{type(exc).__name__}: {exc}"}
     print(json.dumps(line), flush=True)

@@ -137,6 +137,30 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
 int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph);
 int fdtd_batch_info(fdtd_batch* b, int* n, int* total_waves);
 
+/* ---- per-voxel density / modulus and the adjoint (no reference counterpart) */
+/* The reference has scalar rho and c (core/solver.py:1584-1592). These two
+ * steps add per-voxel r = rho/rho0 and kappa = K/K0 on a design box [lo, hi)
+ * (float32 box arrays; NULL = 1 everywhere) and the time-reversed adjoint.
+ * dtype 4 = float32 fields, 8 = float64 fields. One step per call, on `stream`.
+ * fdtd_param_step: (p, v) -> (po, vo), probe samples into trace_row[n_probe],
+ * source value w added (float64) at src_cell (-1: none).
+ * fdtd_adjoint_step: (P, V) = adjoint at step n+1 -> (Po, Vo) at step n, with
+ * the forward state n in (p, v), dL/dy_n in g_row[n_probe]; accumulates
+ * dL/dr and dL/dkappa on the box with Kahan compensation (sum, comp pairs). */
+int fdtd_param_step(int dtype, const void* p, const void* vx, const void* vy, const void* vz, void* po,
+                    void* vxo, void* vyo, void* vzo, int nx, int ny, int nz, const uint8_t* flags,
+                    const float* dx, const float* dy, const float* dz, const int32_t* box_lo,
+                    const int32_t* box_hi, const float* r, const float* kappa, double cv0, double cp0,
+                    int64_t src_cell, double w, int n_probe, const int64_t* probe_cell, void* trace_row,
+                    void* stream);
+int fdtd_adjoint_step(int dtype, const void* P, const void* Vx, const void* Vy, const void* Vz, void* Po,
+                      void* Vxo, void* Vyo, void* Vzo, const void* p, const void* vx, const void* vy,
+                      const void* vz, int nx, int ny, int nz, const uint8_t* flags, const float* dx,
+                      const float* dy, const float* dz, const int32_t* box_lo, const int32_t* box_hi,
+                      const float* r, const float* kappa, double cv0, double cp0, int n_probe,
+                      const int64_t* probe_cell, const void* g_row, void* gr_sum, void* gr_comp,
+                      void* gk_sum, void* gk_comp, void* stream);
+
 /* ---- per-phase entry points (one kernel per reference function) --------- */
 /* All in place, on `stream`, float32, uniform unless noted. */
 /* Replaces fdtd::update_velocity (fdtd_step.cpp:15-82). */

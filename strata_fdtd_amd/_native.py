@@ -41,6 +41,10 @@ _SIGS = {
     "fdtd_batch_set": (_i, [_vp, _vp, _vp, _i]),
     "fdtd_batch_step_n": (_i, [_vp, _i, _i]),
     "fdtd_batch_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    "fdtd_param_step": (_i, [_i] + [_vp] * 8 + [_i, _i, _i] + [_vp] * 8
+                        + [C.c_double, C.c_double, _i64, C.c_double, _i, _vp, _vp, _vp]),
+    "fdtd_adjoint_step": (_i, [_i] + [_vp] * 12 + [_i, _i, _i] + [_vp] * 8
+                          + [C.c_double, C.c_double, _i] + [_vp] * 7),
     "fdtd_update_velocity": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
     "fdtd_update_velocity_nonuniform": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "fdtd_apply_rigid": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),

@@ -34,17 +34,7 @@ _vp = C.c_void_p
 
 
 def _lib():
-    lib = _native.load()
-    if not getattr(lib, "_adjoint_sigs", False):
-        i, f, d, i64 = C.c_int, C.c_float, C.c_double, C.c_int64
-        lib.fdtd_param_step.restype = i
-        lib.fdtd_param_step.argtypes = [i] + [_vp] * 8 + [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                                         d, d, i64, d, i, _vp, _vp, _vp]
-        lib.fdtd_adjoint_step.restype = i
-        lib.fdtd_adjoint_step.argtypes = [i] + [_vp] * 12 + [i, i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                                            _vp, d, d, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
-        lib._adjoint_sigs = True
-    return lib
+    return _native.load()
 
 
 def _ptr(t):
