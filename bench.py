@@ -141,7 +141,7 @@ def adjoint_bench(n: int = 192, steps: int = 2000, fd_voxels: int = 0):
     """Gradient cost (forward + checkpointed adjoint) vs a forward solve."""
     import torch
     from strata_fdtd_amd import FDTDSolver, PML, UniformGrid
-    from strata_fdtd_amd.adjoint import design_problem_192, forward_traces, probe_traces
+    from strata_fdtd_amd.adjoint import design_problem_192, design_ratio, forward_traces, probe_traces
     pb, rho, K, target = design_problem_192(n=n, steps=steps)
 
     def timed(fn):
@@ -190,14 +190,16 @@ def adjoint_bench(n: int = 192, steps: int = 2000, fd_voxels: int = 0):
             which = "rho" if v % 2 == 0 else "K"
             base, grad_t = (rho, g_r) if which == "rho" else (K, g_k)
             idx = tuple(int(rng.integers(0, s)) for s in pb.box_shape)
+            ref = pb.rho if which == "rho" else pb.K0
             b64 = base.double()
             plus, minus = b64.clone(), b64.clone()
             plus[idx] *= 1 + 1e-4
             minus[idx] *= 1 - 1e-4
-            dp = float(np.float32(plus[idx].item())) - float(np.float32(minus[idx].item()))
+            # the step that reaches the kernels: difference of the float32 ratios
+            dr = float(design_ratio(plus, ref)[idx].double() - design_ratio(minus, ref)[idx].double())
             lp = loss64(plus if which == "rho" else rho.double(), plus if which == "K" else K.double())
             lm = loss64(minus if which == "rho" else rho.double(), minus if which == "K" else K.double())
-            fd = (lp - lm) / dp
+            fd = (lp - lm) / (dr * ref)
             g = float(grad_t[idx])
             rows.append({"param": which, "voxel": list(idx), "adjoint": g, "fd": fd,
                          "rel_err": abs(g - fd) / max(abs(fd), 1e-300)})
@@ -216,7 +218,7 @@ def main() -> int:
     ap.add_argument("--workload", default="enclosure_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--graph-chunk", type=int, default=50)
-    ap.add_argument("--tiling", default=None, help="rows,planes override")
+    ap.add_argument("--tiling", default=None, help="rows,planes[,stack] override")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference check on N voxels")
@@ -243,10 +245,9 @@ def main() -> int:
     t_setup = time.perf_counter()
     kw = {}
     if args.tiling:
-        r, p = (int(v) for v in args.tiling.split(","))
-        kw["tiling"] = (r, p)
+        kw["tiling"] = tuple(int(v) for v in args.tiling.split(","))
     solver = wl.build(**kw)
-    solver._prepare(args.warmup + args.steps)
+    solver._prepare(args.warmup + 2 * args.steps)
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t_setup:.1f}s; warmup {args.warmup} steps")
     solver.step_n(args.warmup, graph_chunk=args.graph_chunk)
@@ -271,11 +272,20 @@ def main() -> int:
     dev_ms = ev0.elapsed_time(ev1)
     health = solver.check_health()
     log(f"timed {args.steps} steps: device {dev_ms:.2f} ms, wall {wall * 1e3:.2f} ms, health {health}")
+    # the dominant kernel alone: HIP events around each fused sweep launch, on
+    # the solver's stream, over a second window of the same length
+    sweep_ms, eager_ms = solver.time_sweeps(args.steps)
+    torch.cuda.synchronize()
+    rows, planes, stack = solver.tiling
+    kname = "k_fused_step" if stack == 1 else "k_fused_stack"
+    log(f"sweep kernel {kname} rows={rows} planes={planes} stack={stack}: "
+        f"{sweep_ms / args.steps * 1e3:.1f} us/launch; eager step {eager_ms / args.steps * 1e3:.1f} us")
 
     cells = wl.cells
     ms_step = dev_ms / args.steps
     value = cells * args.steps / (dev_ms / 1e3) / 1e9
-    achieved = BYTES_PER_CELL * cells / (ms_step / 1e3) / 1e9
+    ms_sweep = sweep_ms / args.steps
+    achieved = BYTES_PER_CELL * cells / (ms_sweep / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -283,8 +293,12 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(wl.name),
+        "kernel": f"{kname}<{rows},{'false' if solver.grid.is_uniform else 'true'}{'' if stack == 1 else f',{stack}'}>",
+        "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells,
         "bytes_per_cell_update": BYTES_PER_CELL,
-        "kernel": "k_fused_step (+ per-step ADE density and counter kernels inside the timed window)",
+        "avg_launch_us": round(ms_sweep * 1e3, 1),
+        "whole_step_us": round(ms_step * 1e3, 1),
+        "whole_step_frac": round(BYTES_PER_CELL * cells / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
     }
     line = {
         "metric": METRIC,

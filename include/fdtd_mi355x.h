@@ -104,9 +104,12 @@ int fdtd_ctx_set_ade(fdtd_ctx* ctx, int n_pole, const int32_t* mat_id, const int
                      const float* corr, const uint8_t* material, const int32_t* box_lo,
                      const int32_t* box_hi, float* J, float* Jprev, float inv_dx);
 
-/* Launch shape for the fused kernel: rows per wave (2, 4, 8 or 16) and planes
- * per wave (>= 1). 0 picks the built-in default. */
-int fdtd_ctx_set_tiling(fdtd_ctx* ctx, int rows, int planes);
+/* Launch shape for the fused kernel: rows per wave (2, 4, 8 or 16), planes
+ * per wave (>= 1) and stack, the number of waves (1, 2, 4 or 8) of one
+ * workgroup that own consecutive j-strips and share their halo rows through
+ * LDS (1: independent waves). 0 picks the built-in default (4, 8, 1). No
+ * reference counterpart (the reference's OpenMP loops have no launch shape). */
+int fdtd_ctx_set_tiling(fdtd_ctx* ctx, int rows, int planes, int stack);
 
 /* Enqueue n fused time steps. Replaces FDTDSolver.step's native dispatch
  * (core/solver.py:2028-2077: _step_native + boundary.apply_* + sources +
@@ -120,6 +123,11 @@ int fdtd_graph_run(fdtd_ctx* ctx, int n_steps, int replays);
 /* Average device time of the fused step kernel over the last timed window,
  * measured with HIP events on the context stream (milliseconds). */
 int fdtd_time_steps(fdtd_ctx* ctx, int n_steps, int use_graph, float* ms_total);
+
+/* Enqueue n_steps (1..4096) eager steps with HIP events recorded around each
+ * fused sweep launch, wait, and return the summed sweep time and the whole
+ * window (milliseconds). Timing instrumentation; no reference counterpart. */
+int fdtd_time_sweeps(fdtd_ctx* ctx, int n_steps, float* ms_sweep, float* ms_total);
 
 /* ---- batched serving (no counterpart in the reference) ------------------- */
 /* A batch advances many independent solves (uniform grids, no ADE) by one step

@@ -32,10 +32,11 @@ _SIGS = {
     "fdtd_ctx_set_probes": (_i, [_vp, _i, _vp, _vp, _i64]),
     "fdtd_ctx_set_counters": (_i, [_vp, _vp, _vp]),
     "fdtd_ctx_set_ade": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f]),
-    "fdtd_ctx_set_tiling": (_i, [_vp, _i, _i]),
+    "fdtd_ctx_set_tiling": (_i, [_vp, _i, _i, _i]),
     "fdtd_step_n": (_i, [_vp, _i]),
     "fdtd_graph_run": (_i, [_vp, _i, _i]),
     "fdtd_time_steps": (_i, [_vp, _i, _i, C.POINTER(_f)]),
+    "fdtd_time_sweeps": (_i, [_vp, _i, C.POINTER(_f), C.POINTER(_f)]),
     "fdtd_batch_create": (_i, [_i, _vp, _i, C.POINTER(_vp)]),
     "fdtd_batch_destroy": (_i, [_vp]),
     "fdtd_batch_set": (_i, [_vp, _vp, _vp, _i]),

@@ -137,17 +137,22 @@ class _Device:
             raise _native.NativeError(f"fdtd_adjoint_step failed ({rc})")
 
 
+def design_ratio(x, ref: float, device="cuda"):
+    """The per-voxel ratio the kernels read: x / ref formed in float32 on the
+    device (rho/rho0 or K/K0). Finite-difference checks use it to find the
+    perturbation that actually reaches the kernels."""
+    import torch
+    return (x.detach().to(device=device, dtype=torch.float32) / ref).contiguous()
+
+
 def forward_traces(pb: DesignProblem, rho=None, K=None, dtype=None, device="cuda",
                    keep_checkpoints: bool = False):
     """Run the per-voxel forward solve; returns (traces [steps, n_probe], checkpoints)."""
     import torch
     dtype = dtype or torch.float32
     dv = _Device(pb, dtype, torch.device(device))
-    r = kappa = None
-    if rho is not None:
-        r = (rho.detach().to(device=device, dtype=torch.float32) / pb.rho).contiguous()
-    if K is not None:
-        kappa = (K.detach().to(device=device, dtype=torch.float32) / pb.K0).contiguous()
+    r = design_ratio(rho, pb.rho, device) if rho is not None else None
+    kappa = design_ratio(K, pb.K0, device) if K is not None else None
     a, b = dv.zeros_state(), dv.zeros_state()
     traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=dtype, device=device)
     ckpts = []

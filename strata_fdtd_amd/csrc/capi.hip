@@ -9,12 +9,14 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "fdtd_common.hpp"
 
 namespace fdtd_amd {
-int fused_grid(FusedArgs& a, int rows, int planes);
-hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st);
+int fused_grid(FusedArgs& a, int rows, int planes, int stack);
+hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool ade, int blocks,
+                        hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
 hipError_t launch_batch(const FusedArgs* table, const int* offsets, int n, int total_waves, int rows,
                         hipStream_t st);
@@ -72,12 +74,13 @@ struct fdtd_ctx {
     bool nonuniform = false;
     bool have_grid = false, have_fields = false, have_flags = false, have_pml = false;
     bool have_counters = false;
-    int rows = 8, planes = 16;
+    int rows = 4, planes = 8, stack = 1;
     int32_t* probe_dev = nullptr;     // owned: [n_probe][3]
     int64_t* step_counter = nullptr;  // same as a.step_ptr, writable
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> sweep_ev;  // pairs around each sweep (fdtd_time_sweeps)
 };
 
 extern "C" {
@@ -114,6 +117,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->probe_dev) (void)hipFree(c->probe_dev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->sweep_ev) (void)hipEventDestroy(e);
     delete c;
     return 0;
 }
@@ -292,15 +296,18 @@ int fdtd_ctx_set_ade(fdtd_ctx* c, int n_pole, const int32_t* mat_id, const int32
     return 0;
 }
 
-int fdtd_ctx_set_tiling(fdtd_ctx* c, int rows, int planes) {
+int fdtd_ctx_set_tiling(fdtd_ctx* c, int rows, int planes, int stack) {
     if (!c) return fail(-1, "null context");
-    if (rows == 0) rows = 8;
-    if (planes == 0) planes = 16;
+    if (rows == 0) rows = 4;
+    if (planes == 0) planes = 8;
+    if (stack == 0) stack = 1;
     if (rows != 2 && rows != 4 && rows != 8 && rows != 16) return fail(-2, "rows must be 2, 4, 8 or 16");
     if (planes < 1) return fail(-2, "planes must be >= 1");
+    if (stack != 1 && stack != 2 && stack != 4 && stack != 8) return fail(-2, "stack must be 1, 2, 4 or 8");
     drop_graph(c);
     c->rows = rows;
     c->planes = planes;
+    c->stack = stack;
     return 0;
 }
 
@@ -314,10 +321,11 @@ static int ready(fdtd_ctx* c) {
 
 static int enqueue_steps(fdtd_ctx* c, int n) {
     FusedArgs a = c->a;
-    const int blocks = fused_grid(a, c->rows, c->planes);
+    const int blocks = fused_grid(a, c->rows, c->planes, c->stack);
     for (int s = 0; s < n; ++s) {
         a.koff = s;
-        if (int rc = check(launch_fused(a, c->rows, c->nonuniform, a.n_pole > 0, blocks, c->stream),
+        if (int rc = check(launch_fused(a, c->rows, c->stack, c->nonuniform, a.n_pole > 0, blocks, c->stream,
+                                        nullptr, nullptr),
                            "fused step launch"))
             return rc;
     }
@@ -378,6 +386,42 @@ int fdtd_time_steps(fdtd_ctx* c, int n_steps, int use_graph, float* ms_total) {
     if (int r2 = check(hipEventRecord(c->ev1, c->stream), "event record")) return r2;
     if (int r3 = check(hipEventSynchronize(c->ev1), "event sync")) return r3;
     return check(hipEventElapsedTime(ms_total, c->ev0, c->ev1), "elapsed time");
+}
+
+int fdtd_time_sweeps(fdtd_ctx* c, int n_steps, float* ms_sweep, float* ms_total) {
+    if (int rc = ready(c)) return rc;
+    if (!ms_sweep || !ms_total) return fail(-2, "null output");
+    if (n_steps < 1 || n_steps > 4096) return fail(-2, "n_steps must be in [1, 4096]");
+    DeviceGuard g(c->device);
+    while (c->sweep_ev.size() < 2 * static_cast<size_t>(n_steps)) {
+        hipEvent_t e = nullptr;
+        if (int rc = check(hipEventCreate(&e), "event create")) return rc;
+        c->sweep_ev.push_back(e);
+    }
+    FusedArgs a = c->a;
+    const int blocks = fused_grid(a, c->rows, c->planes, c->stack);
+    if (int rc = check(hipEventRecord(c->ev0, c->stream), "event record")) return rc;
+    for (int s = 0; s < n_steps; ++s) {
+        a.koff = s;
+        if (int rc = check(launch_fused(a, c->rows, c->stack, c->nonuniform, a.n_pole > 0, blocks, c->stream,
+                                        c->sweep_ev[2 * s], c->sweep_ev[2 * s + 1]),
+                           "fused step launch"))
+            return rc;
+    }
+    if (int rc = check(launch_advance(c->step_counter, n_steps, c->stream), "advance launch")) return rc;
+    if (int rc = check(hipEventRecord(c->ev1, c->stream), "event record")) return rc;
+    if (int rc = check(hipEventSynchronize(c->ev1), "event sync")) return rc;
+    float total = 0.f, sweep = 0.f;
+    if (int rc = check(hipEventElapsedTime(&total, c->ev0, c->ev1), "elapsed time")) return rc;
+    for (int s = 0; s < n_steps; ++s) {
+        float ms = 0.f;
+        if (int rc = check(hipEventElapsedTime(&ms, c->sweep_ev[2 * s], c->sweep_ev[2 * s + 1]), "elapsed time"))
+            return rc;
+        sweep += ms;
+    }
+    *ms_sweep = sweep;
+    *ms_total = total;
+    return 0;
 }
 
 // ---- per-phase entry points ---------------------------------------------------
@@ -533,7 +577,7 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
             return fail(-2, "batch entry %d: nonuniform or ADE solves run on their own context", q);
         if (c->device != b->device) return fail(-2, "batch entry %d is on another device", q);
         FusedArgs a = c->a;
-        fused_grid(a, b->rows, c->planes);
+        fused_grid(a, b->rows, c->planes, 1);
         a.koff = 0;
         a.step_limit = step_limits[q];
         b->table_host[q] = a;

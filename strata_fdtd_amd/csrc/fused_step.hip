@@ -42,8 +42,16 @@ __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWav
 
 // One wave's work: the strip (segment seg, j-strip sj, i-chunk si) of the
 // solve described by `a` (kernel argument, or a row of a device batch table).
-template <int R, bool NU>
-__device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj, int si, int lane) {
+//
+// W > 1 (stacked): the W waves of a workgroup own W consecutive j-strips of the
+// same segment and i-chunk and march in step; per plane each wave posts its
+// bottom/top rows (p, and vy/flags of the top row) to LDS, so the j halos
+// between them come from LDS and only the workgroup's outer rows are read
+// twice from HBM. `lds` holds 2 (plane parity) x 4 x W x 64 floats; `wi` is
+// the wave's index in the stack. W == 1: every wave reads its own halos.
+template <int R, bool NU, int W>
+__device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj, int si, int lane,
+                                            float* lds, int wi) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = seg * kWave;
     const int k = k0 + lane;
@@ -163,13 +171,27 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
         // j halos: p(j0-1), vy(j0-1), flags(j0-1) and p(j0+R)
         float p_jm = 0.f, vy_jm = 0.f, p_jp = 0.f;
         uint32_t f_jm = 0;
-        if (kv && j0 > 0) {
+        if constexpr (W > 1) {
+            float* sb = lds + (i & 1) * (4 * W * kWave);
+            sb[(0 * W + wi) * kWave + lane] = pc[0];
+            sb[(1 * W + wi) * kWave + lane] = pc[R - 1];
+            sb[(2 * W + wi) * kWave + lane] = vyo[R - 1];
+            sb[(3 * W + wi) * kWave + lane] = __uint_as_float(fl[R - 1]);
+            __syncthreads();
+            if (wi > 0) {
+                p_jm = sb[(1 * W + wi - 1) * kWave + lane];
+                vy_jm = sb[(2 * W + wi - 1) * kWave + lane];
+                f_jm = __float_as_uint(sb[(3 * W + wi - 1) * kWave + lane]);
+            }
+            if (wi < W - 1) p_jp = sb[(0 * W + wi + 1) * kWave + lane];
+        }
+        if ((W == 1 || wi == 0) && kv && j0 > 0) {
             const int64_t c = lin(i, j0 - 1, k, ny, nz);
             p_jm = P[c];
             vy_jm = VY[c];
             f_jm = FL[c];
         }
-        if (kv && j0 + R < ny) p_jp = P[lin(i, j0 + R, k, ny, nz)];
+        if ((W == 1 || wi == W - 1) && kv && j0 + R < ny) p_jp = P[lin(i, j0 + R, k, ny, nz)];
 
         // vy' on the face (j0-1 | j0)
         float vyprev = 0.f;
@@ -287,7 +309,26 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     if (*a.step_ptr + a.koff >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, NU>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane);
+    fused_strip<R, NU, 1>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+}
+
+// Stacked variant: workgroup = W waves over W consecutive j-strips of one
+// segment and i-chunk (halo rows shared through LDS). Logical workgroup
+// index -> (segment, j-stack, i-chunk), segment fastest; `a.nwaves` counts
+// workgroups here. Every wave of a workgroup runs the same number of planes,
+// so the per-plane barrier is reached uniformly.
+template <int R, bool NU, int W>
+__global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
+    __shared__ float lds[2 * 4 * W * kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wi = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
+    if (lb >= a.nwaves) return;                                  // whole workgroup
+    if (*a.step_ptr + a.koff >= a.step_limit) return;            // whole workgroup
+    const int nstack_j = (a.nstrip_j + W - 1) / W;
+    const int seg = lb % a.nseg;
+    const int t = lb / a.nseg;
+    fused_strip<R, NU, W>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
 }
 
 // Batched step: one launch advances every solve in `table` by one step. Waves
@@ -311,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const FusedArgs* __restri
     if (*a.step_ptr >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, false>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane);
+    fused_strip<R, false, 1>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 __global__ void k_batch_advance(const FusedArgs* __restrict__ table, int n) {
@@ -497,25 +538,54 @@ static hipError_t launch_r(const FusedArgs& a, int blocks, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int R>
-static hipError_t launch_rows(const FusedArgs& a, bool nu, int blocks, hipStream_t st) {
-    return nu ? launch_r<R, true>(a, blocks, st) : launch_r<R, false>(a, blocks, st);
+template <int R, bool NU, int W>
+static hipError_t launch_stack(const FusedArgs& a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL((k_fused_stack<R, NU, W>), dim3(blocks), dim3(W * kWave), 0, st, a);
+    return hipGetLastError();
 }
 
-// Fill the tiling fields of `a` for rows R / planes; returns the grid size.
-int fused_grid(FusedArgs& a, int rows, int planes) {
+template <int R, bool NU>
+static hipError_t launch_rn(const FusedArgs& a, int stack, int blocks, hipStream_t st) {
+    switch (stack) {
+        case 1: return launch_r<R, NU>(a, blocks, st);
+        case 2: return launch_stack<R, NU, 2>(a, blocks, st);
+        case 4: return launch_stack<R, NU, 4>(a, blocks, st);
+        case 8: return launch_stack<R, NU, 8>(a, blocks, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int R>
+static hipError_t launch_rows(const FusedArgs& a, bool nu, int stack, int blocks, hipStream_t st) {
+    return nu ? launch_rn<R, true>(a, stack, blocks, st) : launch_rn<R, false>(a, stack, blocks, st);
+}
+
+// Fill the tiling fields of `a` for rows R / planes / stack; returns the grid
+// size (a multiple of 8, the XCD count). stack == 1: one independent wave per
+// strip, 4 waves per workgroup, `nwaves` = strips. stack > 1: one workgroup of
+// `stack` waves per j-stack, `nwaves` = workgroups.
+int fused_grid(FusedArgs& a, int rows, int planes, int stack) {
     a.planes = planes;
     a.nseg = (a.nz + kWave - 1) / kWave;
     a.nstrip_j = (a.ny + rows - 1) / rows;
     a.nstrip_i = (a.nx + planes - 1) / planes;
-    a.nwaves = a.nseg * a.nstrip_j * a.nstrip_i;
-    int blocks = (a.nwaves + kBlockWaves - 1) / kBlockWaves;
+    int blocks;
+    if (stack <= 1) {
+        a.nwaves = a.nseg * a.nstrip_j * a.nstrip_i;
+        blocks = (a.nwaves + kBlockWaves - 1) / kBlockWaves;
+    } else {
+        a.nwaves = a.nseg * ((a.nstrip_j + stack - 1) / stack) * a.nstrip_i;
+        blocks = a.nwaves;
+    }
     blocks = (blocks + 7) & ~7;
     a.blocks_per_xcd = blocks / 8;
     return blocks;
 }
 
-hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blocks, hipStream_t st) {
+// One full step: [ADE density] -> fused sweep -> [ADE fix-up]. ev_before /
+// ev_after (optional) are recorded around the sweep alone, for timing it.
+hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool ade, int blocks,
+                        hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after) {
     int box_blocks = 1;
     if (ade) {
         const int64_t bn = static_cast<int64_t>(a.box_hi[0] - a.box_lo[0]) *
@@ -526,14 +596,17 @@ hipError_t launch_fused(const FusedArgs& a, int rows, bool nu, bool ade, int blo
         if (e != hipSuccess) return e;
     }
     hipError_t e;
+    if (ev_before && (e = hipEventRecord(ev_before, st)) != hipSuccess) return e;
     switch (rows) {
-        case 2: e = launch_rows<2>(a, nu, blocks, st); break;
-        case 4: e = launch_rows<4>(a, nu, blocks, st); break;
-        case 8: e = launch_rows<8>(a, nu, blocks, st); break;
-        case 16: e = launch_rows<16>(a, nu, blocks, st); break;
+        case 2: e = launch_rows<2>(a, nu, stack, blocks, st); break;
+        case 4: e = launch_rows<4>(a, nu, stack, blocks, st); break;
+        case 8: e = launch_rows<8>(a, nu, stack, blocks, st); break;
+        case 16: e = launch_rows<16>(a, nu, stack, blocks, st); break;
         default: return hipErrorInvalidValue;
     }
-    if (e != hipSuccess || !ade) return e;
+    if (e != hipSuccess) return e;
+    if (ev_after && (e = hipEventRecord(ev_after, st)) != hipSuccess) return e;
+    if (!ade) return e;
     if (nu) hipLaunchKernelGGL(k_ade_fixup<true>, dim3(box_blocks), dim3(256), 0, st, a);
     else hipLaunchKernelGGL(k_ade_fixup<false>, dim3(box_blocks), dim3(256), 0, st, a);
     return hipGetLastError();

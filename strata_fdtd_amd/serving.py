@@ -212,7 +212,7 @@ class SweepServer:
                     break
                 if len(built) >= 4:
                     break
-                s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1]))
+                s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1], 1))
                 s._prepare(r.steps)
                 built[r.rid] = s
 
@@ -227,7 +227,7 @@ class SweepServer:
                 pending.pop(0)
                 s = built.pop(r.rid, None)
                 if s is None:
-                    s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1]))
+                    s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1], 1))
                     s._prepare(r.steps)
                 active.append(_Slot(r, s, now()))
                 used += r.footprint

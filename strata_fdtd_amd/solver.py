@@ -30,7 +30,7 @@ from .boundaries import PML
 from .grid import NonuniformGrid, UniformGrid
 from .materials import density_correction_coeff, modulus_correction_coeff
 from .sources import GaussianPulse, Probe, accumulated_times, waveform_table
-from .tuning import pick_tiling
+from .tuning import normalize_tiling, pick_tiling
 
 _FIELDS = ("p", "vx", "vy", "vz")
 
@@ -155,7 +155,7 @@ class FDTDSolver:
         _native.call("fdtd_ctx_create", device, stream, _native.C.byref(h))
         self._ctx = h.value
         self._stream = torch.cuda.current_stream(dev)
-        self._tiling = tiling
+        self._tiling = normalize_tiling(tiling) if tiling is not None else None
         self._bind_static()
 
     def __del__(self):
@@ -233,12 +233,18 @@ class FDTDSolver:
         _native.call("fdtd_ctx_set_flags", ctx, _native.ptr(self._flags))
         _native.call("fdtd_ctx_set_pml", ctx, *[_native.ptr(d) for d in self._decay])
         _native.call("fdtd_ctx_set_counters", ctx, _native.ptr(self._step_dev), _native.ptr(self._health))
-        rows, planes = self._tiling or pick_tiling(self.shape, not self._grid.is_uniform, False)
-        _native.call("fdtd_ctx_set_tiling", ctx, rows, planes)
+        tiling = self._tiling or pick_tiling(self.shape, not self._grid.is_uniform, False)
+        _native.call("fdtd_ctx_set_tiling", ctx, *tiling)
 
-    def set_tiling(self, rows: int, planes: int) -> None:
-        self._tiling = (rows, planes)
-        _native.call("fdtd_ctx_set_tiling", self._ctx, rows, planes)
+    def set_tiling(self, rows: int, planes: int, stack: int = 1) -> None:
+        """Launch shape of the fused kernel: rows per wave, planes per wave and
+        waves per j-stack (halo rows shared through LDS)."""
+        self._tiling = normalize_tiling((rows, planes, stack))
+        _native.call("fdtd_ctx_set_tiling", self._ctx, *self._tiling)
+
+    @property
+    def tiling(self) -> tuple[int, int, int]:
+        return self._tiling or pick_tiling(self.shape, not self._grid.is_uniform, bool(self._ade_tensors))
 
     def set_geometry(self, geometry) -> None:
         """Bool mask (True = air), an SDF primitive, or a MaterializedGeometry
@@ -421,8 +427,8 @@ class FDTDSolver:
                      float(np.float32(1.0 / self.dx)))
         self._ade_tensors = keep
         self._dirty_ade = False
-        rows, planes = self._tiling or pick_tiling(self.shape, not self._grid.is_uniform, True)
-        _native.call("fdtd_ctx_set_tiling", self._ctx, rows, planes)
+        tiling = self._tiling or pick_tiling(self.shape, not self._grid.is_uniform, True)
+        _native.call("fdtd_ctx_set_tiling", self._ctx, *tiling)
 
     def _ensure_capacity(self, total_steps: int) -> None:
         """Waveform and trace tables must cover steps [0, total_steps)."""
@@ -503,6 +509,16 @@ class FDTDSolver:
                 self._velocity_snapshots.append((t_before, *self._centred_velocity()))
         if self._track_energy and self._step_count % self._energy_sample_interval == 0:
             self._energy_history.append((self._step_count, self._time, self.compute_energy()))
+
+    def time_sweeps(self, n: int) -> tuple[float, float]:
+        """Advance n (1..4096) steps eagerly with HIP events around every fused
+        sweep launch; returns (summed sweep ms, whole-window ms)."""
+        import ctypes as C
+        self._prepare(n)
+        ms_sweep, ms_total = C.c_float(), C.c_float()
+        _native.call("fdtd_time_sweeps", self._ctx, n, C.byref(ms_sweep), C.byref(ms_total))
+        self._advance_host(n)
+        return float(ms_sweep.value), float(ms_total.value)
 
     def step_n(self, n: int, graph_chunk: int = 64) -> None:
         """Advance n steps on the device without host round trips."""

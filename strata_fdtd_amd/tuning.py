@@ -1,7 +1,9 @@
 """Launch-shape selection for the fused step kernel.
 
-A launch is described by (rows, planes): a wave owns a 64-wide k segment of
-``rows`` consecutive j rows and marches ``planes`` planes along i. The
+A launch is described by (rows, planes, stack): a wave owns a 64-wide k
+segment of ``rows`` consecutive j rows and marches ``planes`` planes along i;
+``stack`` waves of one workgroup own consecutive j-strips and share their halo
+rows through LDS (1: independent waves). The
 heuristic below gives each of the 256 CUs several resident waves while keeping
 the per-strip prologue (one extra plane) and the j-halo rows small. The
 autotuner (``strata_fdtd_amd.autotune``) measures candidates on the device and
@@ -15,6 +17,8 @@ import os
 
 NUM_CU = 256
 ROW_CHOICES = (2, 4, 8, 16)
+STACK_CHOICES = (1, 2, 4, 8)
+STACK_DEFAULT = 1
 _CACHE_ENV = "STRATA_FDTD_AMD_TUNE_CACHE"
 _DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "tune_cache.json")
 
@@ -35,7 +39,17 @@ def shape_key(shape, nonuniform: bool, ade: bool) -> str:
     return f"{shape[0]}x{shape[1]}x{shape[2]}:{'nu' if nonuniform else 'u'}:{'ade' if ade else 'plain'}"
 
 
-def heuristic_tiling(shape) -> tuple[int, int]:
+def normalize_tiling(t) -> tuple[int, int, int]:
+    """(rows, planes) or (rows, planes, stack) -> (rows, planes, stack)."""
+    t = tuple(int(v) for v in t)
+    if len(t) == 2:
+        return t[0], t[1], 1
+    if len(t) == 3:
+        return t
+    raise ValueError(f"tiling must be (rows, planes) or (rows, planes, stack), got {t}")
+
+
+def heuristic_tiling(shape) -> tuple[int, int, int]:
     """rows=4; the most planes per strip (<= 8) that still leaves >= 8 waves
     per CU. Measured at 512^3: planes 4-16 within 2% of each other, 64+ is 5%
     slower, and very long strips starve the small ADE launch."""
@@ -46,19 +60,22 @@ def heuristic_tiling(shape) -> tuple[int, int]:
     planes = 8
     while planes > 2 and strips_ji * ((nx + planes - 1) // planes) < 8 * NUM_CU:
         planes //= 2
-    return rows, planes
+    return rows, planes, STACK_DEFAULT
 
 
-def pick_tiling(shape, nonuniform: bool = False, ade: bool = False) -> tuple[int, int]:
+def pick_tiling(shape, nonuniform: bool = False, ade: bool = False) -> tuple[int, int, int]:
     hit = _load_cache().get(shape_key(shape, nonuniform, ade))
     if hit:
-        return int(hit[0]), int(hit[1])
+        # entries are [rows, planes, stack, ms]
+        if len(hit) == 4:
+            return int(hit[0]), int(hit[1]), int(hit[2])
     return heuristic_tiling(shape)
 
 
-def store_tiling(shape, nonuniform: bool, ade: bool, rows: int, planes: int, ms: float) -> None:
+def store_tiling(shape, nonuniform: bool, ade: bool, tiling, ms: float) -> None:
+    rows, planes, stack = normalize_tiling(tiling)
     cache = _load_cache()
-    cache[shape_key(shape, nonuniform, ade)] = [rows, planes, ms]
+    cache[shape_key(shape, nonuniform, ade)] = [rows, planes, stack, ms]
     path = cache_path()
     os.makedirs(os.path.dirname(path), exist_ok=True)
     tmp = path + ".tmp"

@@ -117,7 +117,9 @@ VARIANTS = {
 
 
 @pytest.mark.parametrize("name", sorted(VARIANTS))
-@pytest.mark.parametrize("tiling", [(2, 3), (4, 7), (8, 16), (16, 5)])
+# (rows, planes[, stack]): stacks of 2-8 waves share j halos through LDS; with
+# ny = 37 the last stack is partly or wholly past the grid edge
+@pytest.mark.parametrize("tiling", [(2, 3), (4, 7), (8, 16), (16, 5), (2, 3, 4), (4, 7, 2), (4, 5, 8), (8, 16, 4)])
 def test_fused_step_bitexact(oracle_lib, name, tiling):
     print(f"[fused] variant={name} tiling={tiling}", flush=True)
     shape = (26, 37, 70) if tiling[0] != 16 else (22, 40, 130)

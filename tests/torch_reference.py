@@ -22,9 +22,14 @@ def forward(pb, rho_box, K_box, air: np.ndarray, decays, wave, dtype=torch.float
     cp0 = -pb.rho * pb.c**2 * dt / pb.resolution
     lo, hi = pb.box_lo, pb.box_hi
     ones = torch.ones(pb.shape, dtype=dtype, device=device)
-    # the kernels hold r = rho/rho0 and kappa = K/K0 as float32 (formed in float32)
-    r_box = (rho_box.float() / pb.rho).to(dtype)
-    k_box = (K_box.float() / pb.K0).to(dtype)
+    # the kernels hold r = rho/rho0 and kappa = K/K0 as float32 (formed in float32);
+    # the forward takes those rounded values while the derivative stays the exact
+    # 1/rho0 (1/K0) in `dtype` (autograd through the float32 division would round
+    # the gradient itself to float32)
+    r64 = rho_box.to(dtype) / pb.rho
+    k64 = K_box.to(dtype) / pb.K0
+    r_box = r64 + ((rho_box.float() / pb.rho).to(dtype) - r64).detach()
+    k_box = k64 + ((K_box.float() / pb.K0).to(dtype) - k64).detach()
     pad = (lo[2], nz - hi[2], lo[1], ny - hi[1], lo[0], nx - hi[0])
     r = F.pad(r_box - 1.0, pad) + ones
     kap = F.pad(k_box - 1.0, pad) + ones
