@@ -205,6 +205,24 @@ int fdtd_compute_divergence(float* div, const float* vx, const float* vy, const 
 int fdtd_build_flags(const uint8_t* air, const uint8_t* material, uint8_t* flags, int nx, int ny,
                      int nz, void* stream);
 
+/* ---- device voxeliser ------------------------------------------------------
+ * Evaluates CSG programs (boxes, spheres, axis-aligned cylinders; union,
+ * intersection, difference) in float64 at the cell centres xc/yc/zc (device)
+ * and writes air[c] = (sdf <= 0) and, for material programs, ids[c].
+ * Replaces SDFPrimitive.voxelize (geometry/sdf.py:99-127) and
+ * MaterializedGeometry.voxelize_with_materials
+ * (geometry/material_assignment.py:133-168) with bit-identical masks.
+ * ops: host int32 [n_ops][2] postfix instructions ({op, arg}); prog_start:
+ * host int32 [n_prog + 1]; prog_ids: host int32 [n_prog] (entry 0 unused);
+ * params: host float64 [n_params]. The programs are validated, copied into
+ * dev_scratch (device, >= fdtd_voxelize_scratch_bytes bytes; the call waits
+ * for that copy) and evaluated on `stream`. ids may be null when n_prog == 1. */
+int64_t fdtd_voxelize_scratch_bytes(int n_ops, int n_prog, int n_params);
+int fdtd_voxelize(const int32_t* ops, int n_ops, const int32_t* prog_start, const int32_t* prog_ids,
+                  int n_prog, const double* params, int n_params, void* dev_scratch, int64_t scratch_bytes,
+                  const double* xc, const double* yc, const double* zc, int nx, int ny, int nz,
+                  uint8_t* air, uint8_t* ids, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,8 @@ _SIGS = {
     "fdtd_apply_pml_pressure": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "fdtd_compute_divergence": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "fdtd_build_flags": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
+    "fdtd_voxelize_scratch_bytes": (_i64, [_i, _i, _i]),
+    "fdtd_voxelize": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _i64, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
 }
 
 _lib: C.CDLL | None = None

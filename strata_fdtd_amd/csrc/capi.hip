@@ -32,6 +32,17 @@ hipError_t phase_pml_pressure(float*, const float*, const float*, const float*, 
 hipError_t phase_divergence(float*, const float*, const float*, const float*, int, int, int, float,
                             const float*, const float*, const float*, hipStream_t);
 hipError_t phase_build_flags(const uint8_t*, const uint8_t*, uint8_t*, int, int, int, hipStream_t);
+struct VoxProgram {
+    const int32_t* ops;
+    const int32_t* start;
+    const int32_t* ids;
+    const double* params;
+    int n_prog;
+};
+const char* vox_validate(const int32_t* ops, int n_ops, const int32_t* start, int n_prog, const int32_t* ids,
+                         const double* params, int n_params);
+hipError_t launch_voxelize(const VoxProgram& pg, const double* xc, const double* yc, const double* zc, int nx,
+                           int ny, int nz, uint8_t* air, uint8_t* ids, hipStream_t st);
 }  // namespace fdtd_amd
 
 using namespace fdtd_amd;
@@ -422,6 +433,55 @@ int fdtd_time_sweeps(fdtd_ctx* c, int n_steps, float* ms_sweep, float* ms_total)
     *ms_sweep = sweep;
     *ms_total = total;
     return 0;
+}
+
+int64_t fdtd_voxelize_scratch_bytes(int n_ops, int n_prog, int n_params) {
+    if (n_ops < 0 || n_prog < 1 || n_params < 0) return -1;
+    const int64_t ints = 2 * static_cast<int64_t>(n_ops) + (n_prog + 1) + n_prog;
+    return ((ints * 4 + 7) / 8) * 8 + 8 * static_cast<int64_t>(n_params);
+}
+
+int fdtd_voxelize(const int32_t* ops, int n_ops, const int32_t* prog_start, const int32_t* prog_ids, int n_prog,
+                  const double* params, int n_params, void* dev_scratch, int64_t scratch_bytes,
+                  const double* xc, const double* yc, const double* zc, int nx, int ny, int nz,
+                  uint8_t* air, uint8_t* ids, void* stream) {
+    if (!prog_start || !prog_ids || (n_ops > 0 && !ops) || (n_params > 0 && !params))
+        return fail(-1, "fdtd_voxelize: null program array");
+    if (!dev_scratch || !xc || !yc || !zc || !air) return fail(-1, "fdtd_voxelize: null device pointer");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(-2, "fdtd_voxelize: bad dimensions");
+    if (n_ops < 0 || n_ops > (1 << 20) || n_params < 0 || n_params > (1 << 22))
+        return fail(-2, "fdtd_voxelize: program too large");
+    if (n_prog > 1 && !ids) return fail(-1, "fdtd_voxelize: material programs need an ids output");
+    if (const char* msg = vox_validate(ops, n_ops, prog_start, n_prog, prog_ids, params, n_params))
+        return fail(-2, "fdtd_voxelize: %s", msg);
+    const int64_t need = fdtd_voxelize_scratch_bytes(n_ops, n_prog, n_params);
+    if (scratch_bytes < need) return fail(-2, "fdtd_voxelize: scratch %lld B < %lld B", (long long)scratch_bytes,
+                                          (long long)need);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    auto* base = static_cast<char*>(dev_scratch);
+    int32_t* d_ops = reinterpret_cast<int32_t*>(base);
+    int32_t* d_start = d_ops + 2 * static_cast<int64_t>(n_ops);
+    int32_t* d_ids = d_start + (n_prog + 1);
+    const int64_t ints = 2 * static_cast<int64_t>(n_ops) + (n_prog + 1) + n_prog;
+    double* d_params = reinterpret_cast<double*>(base + ((ints * 4 + 7) / 8) * 8);
+    if (n_ops)
+        if (int rc = check(hipMemcpyAsync(d_ops, ops, 8 * static_cast<size_t>(n_ops), hipMemcpyHostToDevice, st),
+                           "voxelize program upload"))
+            return rc;
+    if (int rc = check(hipMemcpyAsync(d_start, prog_start, 4 * static_cast<size_t>(n_prog + 1),
+                                      hipMemcpyHostToDevice, st), "voxelize program upload"))
+        return rc;
+    if (int rc = check(hipMemcpyAsync(d_ids, prog_ids, 4 * static_cast<size_t>(n_prog), hipMemcpyHostToDevice, st),
+                       "voxelize program upload"))
+        return rc;
+    if (n_params)
+        if (int rc = check(hipMemcpyAsync(d_params, params, 8 * static_cast<size_t>(n_params),
+                                          hipMemcpyHostToDevice, st), "voxelize program upload"))
+            return rc;
+    // the host arrays belong to the caller: finish reading them before returning
+    if (int rc = check(hipStreamSynchronize(st), "voxelize upload sync")) return rc;
+    VoxProgram pg{d_ops, d_start, d_ids, d_params, n_prog};
+    return check(launch_voxelize(pg, xc, yc, zc, nx, ny, nz, air, ids, st), "voxelize launch");
 }
 
 // ---- per-phase entry points ---------------------------------------------------

@@ -71,7 +71,8 @@ class FDTDSolver:
                  courant: float = 0.95,
                  backend: Literal["auto", "mi355x"] = "auto",
                  warn_energy_drift: bool = False, energy_drift_threshold: float = 0.01,
-                 device: int = 0, tiling: tuple[int, int] | None = None):
+                 device: int = 0, tiling: tuple[int, int] | None = None,
+                 voxelizer: Literal["auto", "device", "host"] = "auto"):
         if grid is not None:
             self._grid = grid
             self.shape = tuple(grid.shape)
@@ -156,6 +157,9 @@ class FDTDSolver:
         self._ctx = h.value
         self._stream = torch.cuda.current_stream(dev)
         self._tiling = normalize_tiling(tiling) if tiling is not None else None
+        if voxelizer not in ("auto", "device", "host"):
+            raise ValueError(f"voxelizer must be 'auto', 'device' or 'host', got {voxelizer!r}")
+        self._voxelizer = voxelizer
         self._bind_static()
 
     def __del__(self):
@@ -251,7 +255,7 @@ class FDTDSolver:
         (core/solver.py:1682-1780)."""
         from .geometry import MaterializedGeometry, SDFPrimitive
         if isinstance(geometry, MaterializedGeometry):
-            mask, ids = geometry.voxelize_with_materials(self.grid)
+            mask, ids = self._voxelize(geometry)
             if mask.shape != self.shape:
                 raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
             self.geometry = mask
@@ -264,7 +268,7 @@ class FDTDSolver:
                 if np.any(region):
                     self.set_material_region(region, material_id=mid)
         elif isinstance(geometry, SDFPrimitive):
-            mask = geometry.voxelize(self.grid)
+            mask, _ = self._voxelize(geometry)
             if mask.shape != self.shape:
                 raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
             self.geometry = mask
@@ -276,6 +280,28 @@ class FDTDSolver:
                 raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
             self.geometry = mask.astype(bool)
         self._dirty_geometry = True
+
+    def _voxelize(self, geometry):
+        """(air mask bool, material ids uint8 or None) of an SDF tree or a
+        MaterializedGeometry: on the GPU when the tree is one the device
+        voxeliser evaluates exactly (voxelizer="auto" or "device"), else with
+        the host voxeliser. Both give the reference's mask bit for bit."""
+        from .geometry import MaterializedGeometry
+        from . import voxelize_device as VD
+        tree = geometry.geometry if isinstance(geometry, MaterializedGeometry) else geometry
+        use_device = self._voxelizer == "device" or (self._voxelizer == "auto" and VD.supports(tree) and all(
+            VD.supports(r) for r, _m, _i in getattr(geometry, "_material_regions", [])))
+        if use_device:
+            air, ids = VD.voxelize_device(geometry, self.grid, device=self._device, stream=self._stream)
+            self._stream.synchronize()
+            mask = air.cpu().numpy().astype(bool)
+            if isinstance(geometry, MaterializedGeometry):
+                ids_h = ids.cpu().numpy() if ids is not None else np.zeros(self.shape, dtype=np.uint8)
+                return mask, ids_h
+            return mask, None
+        if isinstance(geometry, MaterializedGeometry):
+            return geometry.voxelize_with_materials(self.grid)
+        return geometry.voxelize(self.grid), None
 
     def _position_to_index(self, pos, what: str):
         if isinstance(pos, tuple) and len(pos) == 3:

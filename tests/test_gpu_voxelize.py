@@ -1,0 +1,110 @@
+"""Device voxeliser vs the host voxeliser, bit for bit.
+
+The host voxeliser is pinned to the reference's direct ``sdf <= 0`` evaluation
+(tests/test_host.py); here the HIP kernel's air masks and material ids must
+equal it exactly: random CSG trees on uniform and nonuniform grids (surface
+ties included), material regions, and the full-size scenes of the 512^3
+enclosure and the 320x320x480 nonuniform resonator plate.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_tree(rng, depth=0):
+    from strata_fdtd_amd import Box, Cylinder, Difference, Intersection, Sphere, Union
+
+    def r(a, b, n=None, d=3):
+        return np.round(rng.uniform(a, b, n), d)
+    t = rng.integers(0, 6 if depth < 2 else 3)
+    c = r(0.01, 0.08, 3, 4)
+    if t == 0:
+        return Box(center=tuple(c), size=tuple(r(0.005, 0.05, 3)))
+    if t == 1:
+        return Sphere(center=tuple(c), radius=float(r(0.005, 0.03)))
+    if t == 2:
+        d = np.zeros(3)
+        d[rng.integers(0, 3)] = r(0.01, 0.05) * (1 if rng.random() < 0.5 else -1)
+        return Cylinder(p1=tuple(c), p2=tuple(c + d), radius=float(r(0.003, 0.02)))
+    kids = [_random_tree(rng, depth + 1) for _ in range(rng.integers(1, 4))]
+    return (Union, Intersection, lambda *k: Difference(k[0], *k[1:]))[t - 3](*kids)
+
+
+def _device(geometry, grid):
+    from strata_fdtd_amd.voxelize_device import voxelize_device
+    air, ids = voxelize_device(geometry, grid)
+    return air.cpu().numpy().astype(bool), None if ids is None else ids.cpu().numpy()
+
+
+@pytest.mark.parametrize("nonuniform", [False, True])
+def test_random_trees_equal_host(nonuniform):
+    from strata_fdtd_amd import NonuniformGrid, UniformGrid
+    rng = np.random.default_rng(21)
+    grid = (NonuniformGrid.from_stretch((64, 56, 48), 1e-3, stretch_z=1.03) if nonuniform
+            else UniformGrid((64, 56, 48), 1e-3))
+    print(f"[voxelize] random trees nonuniform={nonuniform}", flush=True)
+    ties = 0
+    for _ in range(60):
+        node = _random_tree(rng)
+        host = node.voxelize(grid)
+        dev, _ = _device(node, grid)
+        assert np.array_equal(dev, host), int((dev != host).sum())
+        ties += int((node.voxelize_direct(grid, None, True) != host).sum())
+    assert ties > 0  # cells exactly on a surface were exercised
+
+
+def test_material_regions_equal_host():
+    from strata_fdtd_amd import MaterializedGeometry, SimpleMaterial, UniformGrid
+    rng = np.random.default_rng(22)
+    grid = UniformGrid((48, 40, 52), 1e-3)
+    for _ in range(8):
+        geo = MaterializedGeometry(_random_tree(rng))
+        for q in range(4):
+            geo.set_material(_random_tree(rng), SimpleMaterial(name=f"m{q}", _rho=1.0 + q, _c=300.0))
+        mask, ids = geo.voxelize_with_materials(grid)
+        dmask, dids = _device(geo, grid)
+        assert np.array_equal(dmask, mask)
+        assert np.array_equal(dids, ids)
+
+
+def test_enclosure_512_scene_equals_host():
+    from strata_fdtd_amd.workloads import enclosure_512_scene
+    from strata_fdtd_amd import UniformGrid
+    geo, *_ = enclosure_512_scene(512)
+    grid = UniformGrid((512, 512, 512), 1e-3)
+    print("[voxelize] enclosure_512 scene", flush=True)
+    mask, ids = geo.voxelize_with_materials(grid)
+    dmask, dids = _device(geo, grid)
+    assert np.array_equal(dmask, mask)
+    assert np.array_equal(dids, ids)
+    assert 0 < mask.sum() < mask.size and ids.any()
+
+
+def test_resonator_plate_scene_equals_host():
+    from strata_fdtd_amd.workloads import resonators_nu_scene
+    grid, geo, *_ = resonators_nu_scene()
+    print("[voxelize] resonators_nu scene", flush=True)
+    host = geo.voxelize(grid)
+    dev, _ = _device(geo, grid)
+    assert np.array_equal(dev, host)
+
+
+def test_solver_uses_device_voxeliser_by_default(monkeypatch):
+    from strata_fdtd_amd import FDTDSolver, MaterializedGeometry, UniformGrid
+    from strata_fdtd_amd import voxelize_device as VD
+    from strata_fdtd_amd.workloads import enclosure_512_scene
+    calls = []
+    real = VD.voxelize_device
+    monkeypatch.setattr(VD, "voxelize_device", lambda *a, **k: calls.append(1) or real(*a, **k))
+    geo, *_ = enclosure_512_scene(96)
+    assert isinstance(geo, MaterializedGeometry)
+    a = FDTDSolver(grid=UniformGrid((96, 96, 96), 1e-3), rho=1.204, backend="mi355x")
+    b = FDTDSolver(grid=UniformGrid((96, 96, 96), 1e-3), rho=1.204, backend="mi355x", voxelizer="host")
+    a.set_geometry(geo)
+    assert len(calls) == 1
+    b.set_geometry(geo)
+    assert len(calls) == 1
+    assert np.array_equal(a.geometry, b.geometry)
+    assert np.array_equal(a._material_id, b._material_id)

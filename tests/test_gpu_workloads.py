@@ -1,12 +1,19 @@
 """The three solve workloads at their real sizes, against the CPU oracle.
 
 * 512^3 enclosure (rigid SDF shell + port + ADE sphere, PML 16, 8 probes):
-  bit-exact fields and traces vs the oracle after 8 steps at full size, then
-  size-independent properties over a longer run: exact linearity in the
-  source amplitude, p == 0 in every solid cell, all probe samples finite.
+  bit-exact fields and traces vs the oracle after 64 steps at full size; the
+  same scene scaled to 128^3 bit-exact over the full 4000-step pulse; then
+  size-independent properties over a longer full-size run: a rerun is bit
+  identical (no races), p == 0 in every solid cell, no health flag, and the
+  traces scale with the source amplitude.
 * 320x320x480 nonuniform resonator plate (PML 16, plane source, 16 probes):
-  bit-exact vs the oracle after 8 steps at full size, plus the same
+  bit-exact vs the oracle after 64 steps at full size, plus the same
   properties.
+
+Amplitude scaling is exact in float32 only while no value is subnormal: once
+gradual underflow rounds a doubled value differently, the two runs carry
+independent rounding noise (measured on the CPU oracle at 96^3, 1200 steps:
+1.3e-6 to 1.9e-6 relative L2 per probe). The bound used is 1e-4.
 """
 
 import numpy as np
@@ -22,7 +29,7 @@ def _oracle_check(solver, n):
     from oracle.from_solver import problem_from_solver
     pb = problem_from_solver(solver, n)
     st, tr = cpu.run(pb, n)
-    solver.step_n(n, graph_chunk=4)
+    solver.step_n(n, graph_chunk=16)
     got = solver.get_probe_data()
     for q, name in enumerate(got):
         assert np.array_equal(got[name], tr[:, q]), name
@@ -31,9 +38,7 @@ def _oracle_check(solver, n):
         assert np.array_equal(a, st[k]), (k, int((a != st[k]).sum()))
 
 
-def _amplitude_linearity(build, steps):
-    """Doubling the source amplitude doubles every sample (power-of-two scaling
-    commutes with rounding): checked to rel-L2 1e-12 over `steps` steps."""
+def _properties(build, steps):
     a = build()
     a.step_n(steps)
     ta = a.get_probe_data()
@@ -41,6 +46,12 @@ def _amplitude_linearity(build, steps):
     p_solid_max = float(np.abs(a.p[~air]).max()) if (~air).any() else 0.0
     health = a.check_health()
     del a
+    again = build()
+    again.step_n(steps)
+    tr = again.get_probe_data()
+    for name in ta:
+        assert np.array_equal(ta[name], tr[name]), f"rerun differs at {name}"
+    del again
     b = build()
     for s in b._sources:
         s.amplitude = 2.0 * s.amplitude
@@ -49,9 +60,9 @@ def _amplitude_linearity(build, steps):
     tb = b.get_probe_data()
     for name in ta:
         x, y = ta[name].astype(np.float64), tb[name].astype(np.float64)
-        assert np.all(np.isfinite(x))
-        err = np.linalg.norm(y - 2 * x) / max(np.linalg.norm(2 * x), 1e-30)
-        assert err < 1e-12, (name, err)
+        assert np.all(np.isfinite(x)) and np.abs(x).max() > 0
+        err = np.linalg.norm(y - 2 * x) / np.linalg.norm(2 * x)
+        assert err < 1e-4, (name, err)
     return p_solid_max, health
 
 
@@ -59,21 +70,29 @@ def _amplitude_linearity(build, steps):
 def test_enclosure_512_full_size():
     from strata_fdtd_amd import workloads as W
     wl = W.enclosure_512()
-    print("[workload] enclosure_512 oracle check (8 steps)", flush=True)
-    _oracle_check(wl.build(), 8)
-    print("[workload] enclosure_512 linearity (1200 steps)", flush=True)
-    p_solid, health = _amplitude_linearity(wl.build, 1200)
+    print("[workload] enclosure_512 oracle check (64 steps)", flush=True)
+    _oracle_check(wl.build(), 64)
+    print("[workload] enclosure_512 properties (1200 steps)", flush=True)
+    p_solid, health = _properties(wl.build, 1200)
     assert p_solid == 0.0
     assert health == 0
+
+
+@pytest.mark.slow
+def test_enclosure_scaled_full_pulse_bitexact():
+    from strata_fdtd_amd import workloads as W
+    wl = W.enclosure_512(n=128, steps=4000)
+    print("[workload] enclosure_128 oracle check (4000 steps)", flush=True)
+    _oracle_check(wl.build(), wl.steps)
 
 
 @pytest.mark.slow
 def test_resonators_nonuniform_full_size():
     from strata_fdtd_amd import workloads as W
     wl = W.resonators_nu()
-    print("[workload] resonators_nu oracle check (8 steps)", flush=True)
-    _oracle_check(wl.build(), 8)
-    print("[workload] resonators_nu linearity (600 steps)", flush=True)
-    p_solid, health = _amplitude_linearity(wl.build, 600)
+    print("[workload] resonators_nu oracle check (64 steps)", flush=True)
+    _oracle_check(wl.build(), 64)
+    print("[workload] resonators_nu properties (600 steps)", flush=True)
+    p_solid, health = _properties(wl.build, 600)
     assert p_solid == 0.0
     assert health == 0

@@ -179,3 +179,77 @@ def test_helmholtz_resonator_union():
                              cavity_size=(0.01, 0.02), neck_length=0.005, neck_radius=0.003)
     grid = UniformGrid((60, 60, 60), 1e-3)
     assert np.array_equal(res.voxelize(grid), res.voxelize_direct(grid, None))
+
+
+# ------------------------------------------------------------ device voxeliser
+def _axis_aligned_tree(rng):
+    while True:
+        node = _random_tree(rng)
+        from strata_fdtd_amd.voxelize_device import supports
+        if supports(node):
+            return node
+
+
+@pytest.mark.parametrize("nonuniform", [False, True])
+def test_device_voxeliser_programs_restate_host_voxeliser(nonuniform):
+    """The postfix programs the device voxeliser runs, evaluated by a numpy
+    restatement of the kernel, give the host voxeliser's mask (itself pinned
+    to the reference's direct sdf <= 0 evaluation) on random CSG trees,
+    surface ties included; Difference flips the subtrahend to a strict test."""
+    from strata_fdtd_amd.voxelize_device import compile_programs, evaluate_host
+    rng = np.random.default_rng(5)
+    grid = (NonuniformGrid.from_stretch((64, 56, 48), 1e-3, stretch_z=1.03) if nonuniform
+            else UniformGrid((64, 56, 48), 1e-3))
+    for _ in range(30):
+        node = _axis_aligned_tree(rng)
+        air, _ = evaluate_host(compile_programs(node), grid.x_coords, grid.y_coords, grid.z_coords)
+        assert np.array_equal(air, node.voxelize(grid))
+
+
+def test_device_voxeliser_programs_material_ids():
+    from strata_fdtd_amd.voxelize_device import compile_programs, evaluate_host
+    rng = np.random.default_rng(9)
+    grid = UniformGrid((40, 44, 36), 1e-3)
+    for _ in range(6):
+        geo = MaterializedGeometry(_axis_aligned_tree(rng))
+        for q in range(3):
+            geo.set_material(_axis_aligned_tree(rng), SimpleMaterial(name=f"m{q}", _rho=1.0 + q, _c=300.0))
+        mask, ids = geo.voxelize_with_materials(grid)
+        ps = compile_programs(geo.geometry, [(r, mid) for r, _m, mid in geo._material_regions])
+        air, ids_p = evaluate_host(ps, grid.x_coords, grid.y_coords, grid.z_coords)
+        assert np.array_equal(air, mask) and np.array_equal(ids_p, ids)
+
+
+def test_device_voxeliser_rejects_oblique_cylinder():
+    from strata_fdtd_amd.voxelize_device import Unsupported, compile_programs, supports
+    cyl = Cylinder(p1=(0.01, 0.01, 0.01), p2=(0.02, 0.03, 0.04), radius=0.004)
+    assert not supports(Union(Box(center=(0.02, 0.02, 0.02), size=(0.01, 0.01, 0.01)), cyl))
+    with pytest.raises(Unsupported):
+        compile_programs(cyl)
+    assert supports(Cylinder(p1=(0.01, 0.01, 0.04), p2=(0.01, 0.01, 0.01), radius=0.004))
+
+
+def test_voxelize_entry_point_validates_programs():
+    """fdtd_voxelize rejects malformed programs before touching the device."""
+    from strata_fdtd_amd import _native
+    lib = _native.load()
+    fake = 4096  # never dereferenced: validation fails first
+
+    def run(ops, start, ids, params):
+        ops = np.asarray(ops, np.int32).reshape(-1, 2)
+        start = np.asarray(start, np.int32)
+        ids = np.asarray(ids, np.int32)
+        params = np.asarray(params, np.float64)
+        rc = lib.fdtd_voxelize(ops.ctypes.data, len(ops), start.ctypes.data, ids.ctypes.data, len(start) - 1,
+                               params.ctypes.data, len(params), fake, 1 << 20, fake, fake, fake, 4, 4, 4,
+                               fake, fake, None)
+        return rc, lib.fdtd_last_error().decode()
+
+    box = [0.0] * 6
+    assert run([(9, 0)], [0, 1], [0], box)[1].endswith("unknown opcode")
+    assert "underflow" in run([(2, 0), (5, 0)], [0, 2], [0], box)[1]
+    assert "out of range" in run([(2, 3)], [0, 1], [0], box)[1]
+    assert "exactly one value" in run([(2, 0), (2, 0)], [0, 2], [0], box)[1]
+    cyl = [0, 0, 0, 0.6, 0.8, 0, 1.0, 0.1, 0]
+    assert "signed unit axis" in run([(4, 0)], [0, 1], [0], cyl)[1]
+    assert "material ids" in run([(2, 0), (2, 0)], [0, 1, 2], [0, 0], box)[1]
