@@ -33,6 +33,7 @@ extern "C" {
 
 #define FDTD_MAX_SOURCES 8
 #define FDTD_MAX_PROBES  64
+#define FDTD_MAX_MICS 4096
 #define FDTD_MAX_POLES   8
 
 typedef struct fdtd_ctx fdtd_ctx;
@@ -86,6 +87,17 @@ int fdtd_ctx_set_sources(fdtd_ctx* ctx, int n, const int32_t* kind, const int64_
  * writes row s. Replaces _record_probes (core/solver.py:2435-2439). */
 int fdtd_ctx_set_probes(fdtd_ctx* ctx, int n, const int64_t* flat_idx, float* trace,
                         int64_t trace_len);
+
+/* Microphones: n (<= FDTD_MAX_MICS) trilinear gathers recorded after every
+ * step into out [out_len][n][4] (p, vx, vy, vz at the microphone, float32).
+ * idx: host int32 [n][4][8] flat cell indices; w: host float32 [n][4][8]
+ * weights (corner order and weights of Microphone._compute_trilinear_weights,
+ * core/solver.py:962-1003; velocity rows as _interpolate_velocity :1060-1113).
+ * Replaces _record_microphones (core/solver.py:2441-2518) and the native
+ * precompute_microphone_data / record_microphones_batch
+ * (_kernels/src/microphones.cpp:16-116). The tables are copied to the device. */
+int fdtd_ctx_set_microphones(fdtd_ctx* ctx, int n, const int32_t* idx, const float* w, float* out,
+                             int64_t out_len);
 
 /* Device int64 holding the index of the next step (read by every step kernel,
  * advanced by fdtd_step_n) and a device uint32 health word (bit0: a probe

@@ -15,6 +15,7 @@ from .geometry import (Box, Cylinder, Difference, HelmholtzResonator, Intersecti
                        MaterializedGeometry, SDFPrimitive, Sphere, Union)
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
 from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, Pole, PoleType, SimpleMaterial
+from .microphones import POLAR_PATTERNS, Microphone
 from .solver import FDTDSolver, has_mi355x_backend
 from .sources import GaussianPulse, Probe
 
@@ -24,5 +25,6 @@ __all__ = [
     "FDTDSolver", "has_mi355x_backend", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
     "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder",
     "Union", "Intersection", "Difference", "HelmholtzResonator", "MaterializedGeometry", "Pole",
-    "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C",
+    "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
+    "POLAR_PATTERNS",
 ]

@@ -87,6 +87,9 @@ struct fdtd_ctx {
     bool have_counters = false;
     int rows = 4, planes = 8, stack = 1;
     int32_t* probe_dev = nullptr;     // owned: [n_probe][3]
+    int32_t* mic_idx_dev = nullptr;   // owned: [mic_cap][4][8]
+    float* mic_w_dev = nullptr;       // owned: [mic_cap][4][8]
+    int mic_cap = 0;
     int64_t* step_counter = nullptr;  // same as a.step_ptr, writable
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -126,6 +129,8 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     DeviceGuard g(c->device);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->probe_dev) (void)hipFree(c->probe_dev);
+    if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
+    if (c->mic_w_dev) (void)hipFree(c->mic_w_dev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->sweep_ev) (void)hipEventDestroy(e);
@@ -257,6 +262,45 @@ int fdtd_ctx_set_probes(fdtd_ctx* c, int n, const int64_t* idx, float* trace, in
     c->a.probe_ijk = c->probe_dev;
     c->a.trace = trace;
     c->a.trace_len = n > 0 ? trace_len : 0;
+    return 0;
+}
+
+int fdtd_ctx_set_microphones(fdtd_ctx* c, int n, const int32_t* idx, const float* w, float* out,
+                             int64_t out_len) {
+    if (!c) return fail(-1, "null context");
+    if (!c->have_grid) return fail(-3, "set the grid before the microphones");
+    if (n < 0 || n > FDTD_MAX_MICS) return fail(-2, "%d microphones (max %d)", n, FDTD_MAX_MICS);
+    if (n > 0 && (!idx || !w || !out || out_len < 1)) return fail(-2, "bad microphone arrays");
+    const int64_t cells = static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz;
+    for (int64_t q = 0; q < 32 * static_cast<int64_t>(n); ++q)
+        if (idx[q] < 0 || idx[q] >= cells)
+            return fail(-2, "microphone %lld: flat index %d outside the grid", (long long)(q / 32), idx[q]);
+    DeviceGuard g(c->device);
+    drop_graph(c);
+    if (n > c->mic_cap) {
+        // set-up allocation (never on a step path): the gather tables
+        if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
+        if (c->mic_w_dev) (void)hipFree(c->mic_w_dev);
+        c->mic_idx_dev = nullptr;
+        c->mic_w_dev = nullptr;
+        c->mic_cap = 0;
+        if (int rc = check(hipMalloc(&c->mic_idx_dev, sizeof(int32_t) * 32 * n), "hipMalloc")) return rc;
+        if (int rc = check(hipMalloc(&c->mic_w_dev, sizeof(float) * 32 * n), "hipMalloc")) return rc;
+        c->mic_cap = n;
+    }
+    if (n > 0) {
+        if (int rc = check(hipMemcpy(c->mic_idx_dev, idx, sizeof(int32_t) * 32 * n, hipMemcpyHostToDevice),
+                           "hipMemcpy"))
+            return rc;
+        if (int rc = check(hipMemcpy(c->mic_w_dev, w, sizeof(float) * 32 * n, hipMemcpyHostToDevice),
+                           "hipMemcpy"))
+            return rc;
+    }
+    c->a.n_mic = n;
+    c->a.mic_idx = c->mic_idx_dev;
+    c->a.mic_w = c->mic_w_dev;
+    c->a.mic_out = out;
+    c->a.mic_len = n > 0 ? out_len : 0;
     return 0;
 }
 
@@ -633,8 +677,8 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
     for (int q = 0; q < n; ++q) {
         fdtd_ctx* c = ctxs[q];
         if (int rc = ready(c)) return rc;
-        if (c->nonuniform || c->a.n_pole > 0)
-            return fail(-2, "batch entry %d: nonuniform or ADE solves run on their own context", q);
+        if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0)
+            return fail(-2, "batch entry %d: nonuniform, ADE or microphone solves run on their own context", q);
         if (c->device != b->device) return fail(-2, "batch entry %d is on another device", q);
         FusedArgs a = c->a;
         fused_grid(a, b->rows, c->planes, 1);

@@ -55,6 +55,13 @@ struct FusedArgs {
     int64_t trace_len;
     uint32_t* health;               // bit0 non-finite probe, bit1 table overrun
 
+    // microphones: 4 trilinear gathers (p, vx, vy, vz) of 8 corners each
+    int n_mic;
+    const int32_t* mic_idx;         // device [n_mic][4][8] flat cell indices
+    const float* mic_w;             // device [n_mic][4][8] weights
+    float* mic_out;                 // [mic_len][n_mic][4]
+    int64_t mic_len;
+
     // ADE (materials with poles)
     int n_pole;
     int pole_mat[FDTD_MAX_POLES];

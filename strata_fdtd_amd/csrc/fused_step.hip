@@ -527,6 +527,26 @@ __global__ void k_ade_fixup(const FusedArgs a) {
     }
 }
 
+// Microphones: after step s the state is in set (s+1)&1; each thread gathers
+// one microphone's p, vx, vy, vz from 8 corners each, summing in float32 in
+// the corner order of core/solver.py:962-1003 (acc = 0, then acc += w * f).
+__global__ void k_mic_gather(const FusedArgs a) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= a.n_mic) return;
+    const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit || s >= a.mic_len) return;
+    const int par = static_cast<int>((s + 1) & 1);
+    for (int g = 0; g < 4; ++g) {
+        const float* __restrict__ F = a.f[par][g];
+        const int32_t* idx = a.mic_idx + (static_cast<int64_t>(m) * 4 + g) * 8;
+        const float* w = a.mic_w + (static_cast<int64_t>(m) * 4 + g) * 8;
+        float acc = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc = acc + w[c] * F[idx[c]];
+        a.mic_out[(s * a.n_mic + m) * 4 + g] = acc;
+    }
+}
+
 __global__ void k_advance(int64_t* step, int n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) step[0] = step[0] + n;
 }
@@ -606,10 +626,16 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
     }
     if (e != hipSuccess) return e;
     if (ev_after && (e = hipEventRecord(ev_after, st)) != hipSuccess) return e;
-    if (!ade) return e;
-    if (nu) hipLaunchKernelGGL(k_ade_fixup<true>, dim3(box_blocks), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(k_ade_fixup<false>, dim3(box_blocks), dim3(256), 0, st, a);
-    return hipGetLastError();
+    if (ade) {
+        if (nu) hipLaunchKernelGGL(k_ade_fixup<true>, dim3(box_blocks), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL(k_ade_fixup<false>, dim3(box_blocks), dim3(256), 0, st, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (a.n_mic > 0) {
+        hipLaunchKernelGGL(k_mic_gather, dim3((a.n_mic + 63) / 64), dim3(64), 0, st, a);
+        e = hipGetLastError();
+    }
+    return e;
 }
 
 hipError_t launch_batch(const FusedArgs* table, const int* offsets, int n, int total_waves, int rows,

@@ -123,6 +123,10 @@ class FDTDSolver:
         self._sources: list = []
         self._probes: dict[str, Probe] = {}
         self._probe_start: dict[str, int] = {}
+        self._microphones: dict = {}
+        self._mic_start: dict[str, int] = {}   # first step recorded on the device
+        self._mic_synced: dict[str, int] = {}  # steps already moved into the Microphone
+        self._step_times: list[float] = []     # solver time at which each step was taken
         self._boundaries: list = []
         self._step_count = 0
         self._time = 0.0
@@ -146,6 +150,8 @@ class FDTDSolver:
         self._wave = None
         self._trace = None
         self._probe_order: list[str] = []
+        self._mic_out = None
+        self._mic_order: list[str] = []
         self._dirty_geometry = True
         self._dirty_sources = True
         self._dirty_ade = True
@@ -332,8 +338,46 @@ class FDTDSolver:
         self._probe_start[name] = self._step_count
         self._dirty_sources = True
 
-    def add_microphone(self, *args, **kwargs):
-        raise NotImplementedError("microphones (trilinear gather) are not implemented on the mi355x backend")
+    def add_microphone(self, position, name: str | None = None, pattern="omni", direction=None, up=None):
+        """Virtual microphone at a physical position (metres), recorded on the
+        device after every step (core/solver.py:1887-1971)."""
+        from .microphones import Microphone
+        mic = position if isinstance(position, Microphone) else Microphone(
+            position=position, name=name, pattern=pattern, direction=direction, up=up)
+        mic_name = mic.name or f"mic_{len(self._microphones)}"
+        mic.name = mic_name
+        if mic_name in self._microphones:
+            raise ValueError(f"Microphone '{mic_name}' already exists")
+        mic._initialize(self)
+        self._sync_microphones()
+        self._microphones[mic_name] = mic
+        self._mic_start[mic_name] = self._step_count
+        self._mic_synced[mic_name] = self._step_count
+        mic._solver = self
+        self._dirty_sources = True
+        return mic
+
+    @property
+    def microphones(self) -> dict:
+        return self._microphones
+
+    def _sync_microphones(self) -> None:
+        """Move device-recorded microphone samples into the Microphone objects."""
+        if not self._microphones or self._mic_out is None:
+            return
+        pending = [n for n in self._mic_order if self._mic_synced.get(n, 0) < self._step_count]
+        if not pending:
+            return
+        lo = min(self._mic_synced[n] for n in pending)
+        rec = self._mic_out[lo:self._step_count].cpu().numpy()
+        for name in pending:
+            mic = self._microphones[name]
+            q = self._mic_order.index(name)
+            a = self._mic_synced[name]
+            rows = rec[a - lo:, q, :]
+            mic._data.extend(mic._combine(rows))
+            mic._times.extend(self._step_times[a:self._step_count])
+            self._mic_synced[name] = self._step_count
 
     def add_boundary(self, boundary) -> None:
         boundary.initialize(self)
@@ -481,6 +525,25 @@ class FDTDSolver:
                     new_trace[:n_old, q_new] = self._trace[:n_old, q_old]
         self._trace = new_trace
         self._probe_order = names
+        self._sync_microphones()
+        mics = list(self._microphones)
+        new_mic = None
+        if mics:
+            new_mic = torch.zeros((cap, len(mics), 4), dtype=torch.float32, device=self._device)
+            if self._mic_out is not None and self._step_count:
+                n_old = min(self._step_count, self._mic_out.shape[0])
+                for q_new, name in enumerate(mics):
+                    if name in self._mic_order:
+                        new_mic[:n_old, q_new] = self._mic_out[:n_old, self._mic_order.index(name)]
+            tabs = [self._microphones[n].gather_table(self.shape) for n in mics]
+            idx = np.ascontiguousarray(np.stack([t[0] for t in tabs]), dtype=np.int32)
+            wts = np.ascontiguousarray(np.stack([t[1] for t in tabs]), dtype=np.float32)
+            _native.call("fdtd_ctx_set_microphones", self._ctx, len(mics), idx.ctypes.data, wts.ctypes.data,
+                         _native.ptr(new_mic), cap)
+        else:
+            _native.call("fdtd_ctx_set_microphones", self._ctx, 0, None, None, None, 0)
+        self._mic_out = new_mic
+        self._mic_order = mics
         kind, a, b = [], [], []
         for s in srcs:
             if s.source_type == "point":
@@ -514,6 +577,7 @@ class FDTDSolver:
 
     def _advance_host(self, n: int) -> None:
         for _ in range(n):
+            self._step_times.append(self._time)
             self._time += self.dt
         self._step_count += n
         self._mirror_valid = False
@@ -691,6 +755,14 @@ class FDTDSolver:
         self._mirror_valid = False
         for name in self._probe_start:
             self._probe_start[name] = 0
+        self._step_times.clear()
+        if self._mic_out is not None:
+            self._mic_out.zero_()
+        for name, mic in self._microphones.items():
+            mic._data.clear()
+            mic._times.clear()
+            self._mic_start[name] = 0
+            self._mic_synced[name] = 0
         for b in self._boundaries:
             b.reset()
 
