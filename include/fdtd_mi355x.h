@@ -89,15 +89,18 @@ int fdtd_ctx_set_probes(fdtd_ctx* ctx, int n, const int64_t* flat_idx, float* tr
                         int64_t trace_len);
 
 /* Microphones: n (<= FDTD_MAX_MICS) trilinear gathers recorded after every
- * step into out [out_len][n][4] (p, vx, vy, vz at the microphone, float32).
- * idx: host int32 [n][4][8] flat cell indices; w: host float32 [n][4][8]
+ * step into out [out_len][n][4] (p, vx, vy, vz at the microphone, float64).
+ * idx: host int32 [n][4][8] flat cell indices; w: host float64 [n][4][8]
  * weights (corner order and weights of Microphone._compute_trilinear_weights,
- * core/solver.py:962-1003; velocity rows as _interpolate_velocity :1060-1113).
+ * core/solver.py:962-1003; velocity rows as _interpolate_velocity :1060-1113);
+ * f64: host int32 [n][4], 1 = sum in float64 (the reference's weights are
+ * numpy float64), 0 = weights rounded to float32 and summed in float32 (the
+ * reference's weights are Python floats).
  * Replaces _record_microphones (core/solver.py:2441-2518) and the native
  * precompute_microphone_data / record_microphones_batch
  * (_kernels/src/microphones.cpp:16-116). The tables are copied to the device. */
-int fdtd_ctx_set_microphones(fdtd_ctx* ctx, int n, const int32_t* idx, const float* w, float* out,
-                             int64_t out_len);
+int fdtd_ctx_set_microphones(fdtd_ctx* ctx, int n, const int32_t* idx, const double* w, const int32_t* f64,
+                             double* out, int64_t out_len);
 
 /* Device int64 holding the index of the next step (read by every step kernel,
  * advanced by fdtd_step_n) and a device uint32 health word (bit0: a probe
@@ -180,6 +183,26 @@ int fdtd_adjoint_step(int dtype, const void* P, const void* Vx, const void* Vy, 
                       const float* r, const float* kappa, double cv0, double cp0, int n_probe,
                       const int64_t* probe_cell, const void* g_row, void* gr_sum, void* gr_comp,
                       void* gk_sum, void* gk_comp, void* stream);
+/* Many steps per call (the loops run in C, not per step from Python).
+ * fdtd_param_run: states = host array of 4 * n_states device pointers
+ * {p, vx, vy, vz}; pingpong: step m reads states[m & 1], writes
+ * states[(m + 1) & 1] (n_states = 2), else reads states[m], writes
+ * states[m + 1] (n_states = n_steps + 1); wave: host float64 waveform by
+ * absolute step; trace: device [steps][n_probe], row step0 + m.
+ * fdtd_adjoint_run: lam = host array of 8 device pointers (two {P, Vx, Vy, Vz}
+ * buffers, lambda_{n_hi} in buffer 0; lambda_{n_lo} ends in buffer
+ * (n_hi - n_lo) & 1); states = 4 * (n_hi - n_lo) pointers, forward state of
+ * step n_lo + m at entry m; g = device [steps][n_probe]. */
+int fdtd_param_run(int dtype, void* const* states, int n_states, int64_t step0, int n_steps, int pingpong,
+                   int nx, int ny, int nz, const uint8_t* flags, const float* dx, const float* dy,
+                   const float* dz, const int32_t* box_lo, const int32_t* box_hi, const float* r,
+                   const float* kappa, double cv0, double cp0, int64_t src_cell, const double* wave,
+                   int n_probe, const int64_t* probe_cell, void* trace, void* stream);
+int fdtd_adjoint_run(int dtype, void* const* lam, void* const* states, int64_t n_lo, int64_t n_hi, int nx,
+                     int ny, int nz, const uint8_t* flags, const float* dx, const float* dy, const float* dz,
+                     const int32_t* box_lo, const int32_t* box_hi, const float* r, const float* kappa,
+                     double cv0, double cp0, int n_probe, const int64_t* probe_cell, const void* g,
+                     void* gr_sum, void* gr_comp, void* gk_sum, void* gk_comp, void* stream);
 
 /* ---- per-phase entry points (one kernel per reference function) --------- */
 /* All in place, on `stream`, float32, uniform unless noted. */

@@ -30,7 +30,7 @@ _SIGS = {
     "fdtd_ctx_set_pml": (_i, [_vp, _vp, _vp, _vp]),
     "fdtd_ctx_set_sources": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64]),
     "fdtd_ctx_set_probes": (_i, [_vp, _i, _vp, _vp, _i64]),
-    "fdtd_ctx_set_microphones": (_i, [_vp, _i, _vp, _vp, _vp, _i64]),
+    "fdtd_ctx_set_microphones": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64]),
     "fdtd_ctx_set_counters": (_i, [_vp, _vp, _vp]),
     "fdtd_ctx_set_ade": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f]),
     "fdtd_ctx_set_tiling": (_i, [_vp, _i, _i, _i]),
@@ -47,6 +47,10 @@ _SIGS = {
                         + [C.c_double, C.c_double, _i64, C.c_double, _i, _vp, _vp, _vp]),
     "fdtd_adjoint_step": (_i, [_i] + [_vp] * 12 + [_i, _i, _i] + [_vp] * 8
                           + [C.c_double, C.c_double, _i] + [_vp] * 7),
+    "fdtd_param_run": (_i, [_i, _vp, _i, _i64, _i, _i, _i, _i, _i] + [_vp] * 8
+                       + [C.c_double, C.c_double, _i64, _vp, _i, _vp, _vp, _vp]),
+    "fdtd_adjoint_run": (_i, [_i, _vp, _vp, _i64, _i64, _i, _i, _i] + [_vp] * 8
+                         + [C.c_double, C.c_double, _i] + [_vp] * 7),
     "fdtd_update_velocity": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
     "fdtd_update_velocity_nonuniform": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "fdtd_apply_rigid": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),

@@ -105,7 +105,8 @@ class _Device:
         src = pb.source
         self.src_cell = int(np.ravel_multi_index(tuple(src.position), pb.shape)) if src is not None else -1
         times = accumulated_times(self.dt, pb.steps)
-        self.wave = src.waveform(times, self.dt) if src is not None else np.zeros(pb.steps)
+        self.wave = np.ascontiguousarray(src.waveform(times, self.dt) if src is not None else np.zeros(pb.steps),
+                                         dtype=np.float64)
         self.lo = np.array(pb.box_lo, dtype=np.int32)
         self.hi = np.array(pb.box_hi, dtype=np.int32)
         self.code = 8 if dtype == torch.float64 else 4
@@ -124,6 +125,32 @@ class _Device:
                                     _ptr(self.probe_cells), _ptr(trace_row), self.stream)
         if rc != 0:
             raise _native.NativeError(f"fdtd_param_step failed ({rc})")
+
+    def _common(self, r, kappa):
+        pb = self.pb
+        return (*pb.shape, _ptr(self.flags), *[_ptr(d) for d in self.dec], self.lo.ctypes.data,
+                self.hi.ctypes.data, _ptr(r), _ptr(kappa), self.cv0, self.cp0)
+
+    def run(self, states, step0, n_steps, pingpong, r, kappa, trace):
+        """n_steps forward steps in one native call (see fdtd_param_run)."""
+        if n_steps <= 0:
+            return
+        ptrs = [_ptr(t) for st in states for t in st]
+        arr = (_vp * len(ptrs))(*ptrs)
+        _native.call("fdtd_param_run", self.code, arr, len(states), step0, n_steps, 1 if pingpong else 0,
+                     *self._common(r, kappa), self.src_cell, self.wave.ctypes.data, self.n_probe,
+                     _ptr(self.probe_cells), _ptr(trace), self.stream)
+
+    def adjoint_run(self, lam_pair, states, n_lo, n_hi, r, kappa, g, acc):
+        """Adjoint steps n_hi-1 .. n_lo in one native call; returns the index of
+        the buffer of `lam_pair` holding lambda_{n_lo}."""
+        ptrs = [_ptr(t) for st in lam_pair for t in st]
+        lam_arr = (_vp * 8)(*ptrs)
+        sp = [_ptr(t) for st in states for t in st]
+        st_arr = (_vp * max(len(sp), 1))(*sp) if sp else (_vp * 1)()
+        _native.call("fdtd_adjoint_run", self.code, lam_arr, st_arr, n_lo, n_hi, *self._common(r, kappa),
+                     self.n_probe, _ptr(self.probe_cells), _ptr(g), *[_ptr(a) for a in acc], self.stream)
+        return (n_hi - n_lo) & 1
 
     def adjoint(self, lam, lam_out, state, r, kappa, g_row, acc):
         pb = self.pb
@@ -156,11 +183,14 @@ def forward_traces(pb: DesignProblem, rho=None, K=None, dtype=None, device="cuda
     a, b = dv.zeros_state(), dv.zeros_state()
     traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=dtype, device=device)
     ckpts = []
-    for n in range(pb.steps):
-        if keep_checkpoints and n % pb.checkpoint == 0:
+    K = pb.checkpoint if keep_checkpoints else pb.steps
+    for n0 in range(0, pb.steps, K):
+        if keep_checkpoints:
             ckpts.append([t.clone() for t in a])
-        dv.step(a, b, n, r, kappa, traces[n])
-        a, b = b, a
+        m = min(K, pb.steps - n0)
+        dv.run([a, b], n0, m, True, r, kappa, traces)
+        if m & 1:
+            a, b = b, a
     return traces[:, :dv.n_probe], (dv, r, kappa, ckpts)
 
 
@@ -175,17 +205,15 @@ def adjoint_gradients(saved, grad_traces):
     g = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=dv.dtype, device=dv.dev)
     g[:, :dv.n_probe] = grad_traces.to(dv.dtype)
     slots = [dv.zeros_state() for _ in range(K)]
-    scratch = torch.zeros(max(dv.n_probe, 1), dtype=dv.dtype, device=dv.dev)
+    scratch = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=dv.dtype, device=dv.dev)
     n_seg = (pb.steps + K - 1) // K
     for seg in reversed(range(n_seg)):
         n0 = seg * K
         n1 = min(pb.steps, n0 + K)
         for t, c in zip(slots[0], ckpts[seg]):
             t.copy_(c)
-        for m in range(n1 - n0 - 1):
-            dv.step(slots[m], slots[m + 1], n0 + m, r, kappa, scratch)
-        for n in range(n1 - 1, n0 - 1, -1):
-            dv.adjoint(lam, lam_out, slots[n - n0], r, kappa, g[n], acc)
+        dv.run(slots[:n1 - n0], n0, n1 - n0 - 1, False, r, kappa, scratch)
+        if dv.adjoint_run([lam, lam_out], slots[:n1 - n0], n0, n1, r, kappa, g, acc):
             lam, lam_out = lam_out, lam
     # Kahan: the compensation holds minus the low-order part the sums lost
     return acc[0] - acc[1], acc[2] - acc[3]

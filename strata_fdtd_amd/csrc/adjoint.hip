@@ -369,4 +369,59 @@ int fdtd_adjoint_step(int dtype, const void* P, const void* Vx, const void* Vy, 
     return e == hipSuccess ? 0 : -5;
 }
 
+// n_steps forward steps in one call. states: host array of 4 * n_states
+// device pointers {p, vx, vy, vz}; pingpong != 0: step m reads states[m & 1]
+// and writes states[(m + 1) & 1] (n_states = 2); otherwise step m reads
+// states[m] and writes states[m + 1] (n_states = n_steps + 1). wave: host
+// float64 waveform indexed by the absolute step; trace: device
+// [steps][n_probe] (row step0 + m written by step m).
+int fdtd_param_run(int dtype, void* const* states, int n_states, int64_t step0, int n_steps, int pingpong,
+                   int nx, int ny, int nz, const uint8_t* flags, const float* dx, const float* dy,
+                   const float* dz, const int32_t* box_lo, const int32_t* box_hi, const float* r,
+                   const float* kappa, double cv0, double cp0, int64_t src_cell, const double* wave,
+                   int n_probe, const int64_t* probe_cell, void* trace, void* stream) {
+    if (!states || n_steps < 0 || !wave || (n_probe > 0 && (!trace || !probe_cell))) return -2;
+    if (pingpong ? n_states != 2 : n_states != n_steps + 1) return -2;
+    const size_t es = dtype == 8 ? 8 : 4;
+    for (int m = 0; m < n_steps; ++m) {
+        void* const* a = states + 4 * (pingpong ? (m & 1) : m);
+        void* const* b = states + 4 * (pingpong ? ((m + 1) & 1) : m + 1);
+        void* row = trace ? static_cast<char*>(trace) + (step0 + m) * n_probe * es : nullptr;
+        if (int rc = fdtd_param_step(dtype, a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], nx, ny, nz, flags,
+                                     dx, dy, dz, box_lo, box_hi, r, kappa, cv0, cp0, src_cell,
+                                     wave[step0 + m], n_probe, probe_cell, row, stream))
+            return rc;
+    }
+    return 0;
+}
+
+// Adjoint steps n = n_hi - 1 down to n_lo. lam: host array of 8 device
+// pointers, {P, Vx, Vy, Vz} of buffer 0 then buffer 1; buffer 0 holds
+// lambda_{n_hi} on entry, and the result lambda_{n_lo} ends in buffer
+// (n_hi - n_lo) & 1. states: host array of 4 * (n_hi - n_lo) pointers, the
+// forward state of step n_lo + m at entry m. g: device [steps][n_probe]
+// probe-trace gradients; acc: 4 device box arrays (Kahan sums).
+int fdtd_adjoint_run(int dtype, void* const* lam, void* const* states, int64_t n_lo, int64_t n_hi, int nx,
+                     int ny, int nz, const uint8_t* flags, const float* dx, const float* dy, const float* dz,
+                     const int32_t* box_lo, const int32_t* box_hi, const float* r, const float* kappa,
+                     double cv0, double cp0, int n_probe, const int64_t* probe_cell, const void* g,
+                     void* gr_sum, void* gr_comp, void* gk_sum, void* gk_comp, void* stream) {
+    if (!lam || !states || n_hi < n_lo || (n_probe > 0 && (!g || !probe_cell))) return -2;
+    const size_t es = dtype == 8 ? 8 : 4;
+    int cur = 0;
+    for (int64_t n = n_hi - 1; n >= n_lo; --n) {
+        void* const* a = lam + 4 * cur;
+        void* const* b = lam + 4 * (cur ^ 1);
+        void* const* st = states + 4 * (n - n_lo);
+        const void* row = g ? static_cast<const char*>(g) + n * n_probe * es : nullptr;
+        if (int rc = fdtd_adjoint_step(dtype, a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3], st[0], st[1],
+                                       st[2], st[3], nx, ny, nz, flags, dx, dy, dz, box_lo, box_hi, r, kappa,
+                                       cv0, cp0, n_probe, probe_cell, row, gr_sum, gr_comp, gk_sum, gk_comp,
+                                       stream))
+            return rc;
+        cur ^= 1;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -88,7 +88,8 @@ struct fdtd_ctx {
     int rows = 4, planes = 8, stack = 1;
     int32_t* probe_dev = nullptr;     // owned: [n_probe][3]
     int32_t* mic_idx_dev = nullptr;   // owned: [mic_cap][4][8]
-    float* mic_w_dev = nullptr;       // owned: [mic_cap][4][8]
+    double* mic_w_dev = nullptr;      // owned: [mic_cap][4][8]
+    int32_t* mic_f64_dev = nullptr;   // owned: [mic_cap][4]
     int mic_cap = 0;
     int64_t* step_counter = nullptr;  // same as a.step_ptr, writable
     hipGraphExec_t graph = nullptr;
@@ -131,6 +132,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->probe_dev) (void)hipFree(c->probe_dev);
     if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
     if (c->mic_w_dev) (void)hipFree(c->mic_w_dev);
+    if (c->mic_f64_dev) (void)hipFree(c->mic_f64_dev);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->sweep_ev) (void)hipEventDestroy(e);
@@ -265,12 +267,12 @@ int fdtd_ctx_set_probes(fdtd_ctx* c, int n, const int64_t* idx, float* trace, in
     return 0;
 }
 
-int fdtd_ctx_set_microphones(fdtd_ctx* c, int n, const int32_t* idx, const float* w, float* out,
-                             int64_t out_len) {
+int fdtd_ctx_set_microphones(fdtd_ctx* c, int n, const int32_t* idx, const double* w, const int32_t* f64,
+                             double* out, int64_t out_len) {
     if (!c) return fail(-1, "null context");
     if (!c->have_grid) return fail(-3, "set the grid before the microphones");
     if (n < 0 || n > FDTD_MAX_MICS) return fail(-2, "%d microphones (max %d)", n, FDTD_MAX_MICS);
-    if (n > 0 && (!idx || !w || !out || out_len < 1)) return fail(-2, "bad microphone arrays");
+    if (n > 0 && (!idx || !w || !f64 || !out || out_len < 1)) return fail(-2, "bad microphone arrays");
     const int64_t cells = static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz;
     for (int64_t q = 0; q < 32 * static_cast<int64_t>(n); ++q)
         if (idx[q] < 0 || idx[q] >= cells)
@@ -281,24 +283,31 @@ int fdtd_ctx_set_microphones(fdtd_ctx* c, int n, const int32_t* idx, const float
         // set-up allocation (never on a step path): the gather tables
         if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
         if (c->mic_w_dev) (void)hipFree(c->mic_w_dev);
+        if (c->mic_f64_dev) (void)hipFree(c->mic_f64_dev);
         c->mic_idx_dev = nullptr;
         c->mic_w_dev = nullptr;
+        c->mic_f64_dev = nullptr;
         c->mic_cap = 0;
         if (int rc = check(hipMalloc(&c->mic_idx_dev, sizeof(int32_t) * 32 * n), "hipMalloc")) return rc;
-        if (int rc = check(hipMalloc(&c->mic_w_dev, sizeof(float) * 32 * n), "hipMalloc")) return rc;
+        if (int rc = check(hipMalloc(&c->mic_w_dev, sizeof(double) * 32 * n), "hipMalloc")) return rc;
+        if (int rc = check(hipMalloc(&c->mic_f64_dev, sizeof(int32_t) * 4 * n), "hipMalloc")) return rc;
         c->mic_cap = n;
     }
     if (n > 0) {
         if (int rc = check(hipMemcpy(c->mic_idx_dev, idx, sizeof(int32_t) * 32 * n, hipMemcpyHostToDevice),
                            "hipMemcpy"))
             return rc;
-        if (int rc = check(hipMemcpy(c->mic_w_dev, w, sizeof(float) * 32 * n, hipMemcpyHostToDevice),
+        if (int rc = check(hipMemcpy(c->mic_w_dev, w, sizeof(double) * 32 * n, hipMemcpyHostToDevice),
+                           "hipMemcpy"))
+            return rc;
+        if (int rc = check(hipMemcpy(c->mic_f64_dev, f64, sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice),
                            "hipMemcpy"))
             return rc;
     }
     c->a.n_mic = n;
     c->a.mic_idx = c->mic_idx_dev;
     c->a.mic_w = c->mic_w_dev;
+    c->a.mic_f64 = c->mic_f64_dev;
     c->a.mic_out = out;
     c->a.mic_len = n > 0 ? out_len : 0;
     return 0;

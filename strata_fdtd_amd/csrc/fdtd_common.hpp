@@ -58,8 +58,9 @@ struct FusedArgs {
     // microphones: 4 trilinear gathers (p, vx, vy, vz) of 8 corners each
     int n_mic;
     const int32_t* mic_idx;         // device [n_mic][4][8] flat cell indices
-    const float* mic_w;             // device [n_mic][4][8] weights
-    float* mic_out;                 // [mic_len][n_mic][4]
+    const double* mic_w;            // device [n_mic][4][8] weights
+    const int32_t* mic_f64;         // device [n_mic][4]: 1 = float64 sum, 0 = float32 sum
+    double* mic_out;                // [mic_len][n_mic][4]
     int64_t mic_len;
 
     // ADE (materials with poles)

@@ -528,8 +528,12 @@ __global__ void k_ade_fixup(const FusedArgs a) {
 }
 
 // Microphones: after step s the state is in set (s+1)&1; each thread gathers
-// one microphone's p, vx, vy, vz from 8 corners each, summing in float32 in
-// the corner order of core/solver.py:962-1003 (acc = 0, then acc += w * f).
+// one microphone's p, vx, vy, vz from 8 corners each in the corner order of
+// core/solver.py:962-1003 (acc = 0, then acc += w * f). The reference sums in
+// float32 when its weights are Python floats (numpy rounds the weight to
+// float32 and multiplies in float32) and in float64 when they are numpy
+// float64 scalars; mic_f64 selects which, per gather. Results are stored as
+// float64 (a float32 sum exactly).
 __global__ void k_mic_gather(const FusedArgs a) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= a.n_mic) return;
@@ -538,12 +542,22 @@ __global__ void k_mic_gather(const FusedArgs a) {
     const int par = static_cast<int>((s + 1) & 1);
     for (int g = 0; g < 4; ++g) {
         const float* __restrict__ F = a.f[par][g];
-        const int32_t* idx = a.mic_idx + (static_cast<int64_t>(m) * 4 + g) * 8;
-        const float* w = a.mic_w + (static_cast<int64_t>(m) * 4 + g) * 8;
-        float acc = 0.0f;
+        const int64_t row = static_cast<int64_t>(m) * 4 + g;
+        const int32_t* idx = a.mic_idx + row * 8;
+        const double* w = a.mic_w + row * 8;
+        double out;
+        if (a.mic_f64[row]) {
+            double acc = 0.0;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc = acc + w[c] * F[idx[c]];
-        a.mic_out[(s * a.n_mic + m) * 4 + g] = acc;
+            for (int c = 0; c < 8; ++c) acc = acc + w[c] * static_cast<double>(F[idx[c]]);
+            out = acc;
+        } else {
+            float acc = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) acc = acc + static_cast<float>(w[c]) * F[idx[c]];
+            out = acc;
+        }
+        a.mic_out[(s * a.n_mic + m) * 4 + g] = out;
     }
 }
 

@@ -10,8 +10,10 @@ gather tables, and the combination of the four recorded values into the
 microphone output with the reference's dtype rules:
 
 * a Python float times a numpy float32 scalar is a float32 product of the
-  float32-rounded weight (numpy 2 weak scalars), so the device uses float32
-  weights and float32 sums;
+  float32-rounded weight (numpy 2 weak scalars), so for positions given as
+  Python floats the device sums float32 products of float32 weights; for
+  positions given as numpy float64 the weights are float64 scalars and the
+  reference's sums are float64, which the device reproduces in double;
 * directional output (1 - k) P + k Z (v . d): the first term is float32, the
   dot product and the second term are float64 (float32 times a float64
   direction component), the sum is float64, and ``get_waveform`` rounds to
@@ -129,9 +131,22 @@ class Microphone:
         corners = [tuple(min(c, n - 1) for c, n in zip(cc, shape)) for cc in corners]
         return corners, weights
 
-    def gather_table(self, shape) -> tuple[NDArray[np.int32], NDArray[np.float32]]:
-        """Flat cell indices int32 [4][8] and float32 weights [4][8] of the
-        p, vx, vy, vz gathers."""
+    @staticmethod
+    def _sum_in_float64(weights) -> bool:
+        """The reference sums ``w * field`` (field float32) in the weights' type:
+        numpy float64 weights give float64 sums; Python floats (weak scalars)
+        and float32 weights give float32 sums of the float32-rounded weight."""
+        kinds = {isinstance(w, np.float64) for w in weights}
+        if len(kinds) != 1:
+            raise NotImplementedError("mixed float64/float32 interpolation weights")
+        if any(isinstance(w, np.floating) and np.dtype(type(w)).itemsize > 8 for w in weights):
+            raise NotImplementedError("extended-precision microphone positions")
+        return kinds.pop()
+
+    def gather_table(self, shape) -> tuple[NDArray[np.int32], NDArray[np.float64], NDArray[np.int32]]:
+        """Flat cell indices int32 [4][8], weights float64 [4][8] and the
+        float64-sum flags int32 [4] of the p, vx, vy, vz gathers (a float32
+        gather's weights are stored already rounded to float32)."""
         if self._interpolation_weights is None:
             raise RuntimeError("Microphone not initialized. Add to solver first.")
         rows = [(self._interpolation_weights["indices"], self._interpolation_weights["weights"])]
@@ -139,25 +154,27 @@ class Microphone:
         idx = np.array([[np.ravel_multi_index(c, shape) for c in corners] for corners, _ in rows], dtype=np.int64)
         if idx.max() >= 2**31:
             raise ValueError("microphone gathers need flat indices below 2**31")
-        w = np.array([[np.float32(x) for x in weights] for _, weights in rows], dtype=np.float32)
-        return idx.astype(np.int32), w
+        f64 = np.array([self._sum_in_float64(weights) for _, weights in rows], dtype=np.int32)
+        w = np.array([[float(x) if wide else float(np.float32(x)) for x in weights]
+                      for (_, weights), wide in zip(rows, f64)], dtype=np.float64)
+        return idx.astype(np.int32), w, f64
 
     # -------------------------------------------------------------- recording
-    def _combine(self, rec: NDArray[np.float32]) -> list:
-        """Microphone outputs from device records [n][4] (p, vx, vy, vz), with
-        the reference's scalar types."""
-        p = rec[:, 0]
+    def _combine(self, rec: NDArray[np.float64], f64) -> list:
+        """Microphone outputs from device records [n][4] (p, vx, vy, vz, each
+        a float32 or float64 sum as `f64` says), with the reference's scalar
+        types and arithmetic."""
+        vals = [rec[:, g] if f64[g] else rec[:, g].astype(np.float32) for g in range(4)]
+        p = vals[0]
         if not self.is_directional():
             return list(p)
         if self._custom_pattern is None:
             k = self._pattern_k
             Z = self._solver_rho * self._solver_c
             d = self._direction
-            v_dot_d = rec[:, 1].astype(np.float64) * d[0] + rec[:, 2].astype(np.float64) * d[1] \
-                + rec[:, 3].astype(np.float64) * d[2]
-            out = (np.float32(1 - k) * p).astype(np.float64) + (k * Z) * v_dot_d
-            return list(out)
-        return [self._compute_directional_output(r[0], r[1], r[2], r[3]) for r in rec]
+            v_dot_d = vals[1] * d[0] + vals[2] * d[1] + vals[3] * d[2]
+            return list((1 - k) * p + k * Z * v_dot_d)
+        return [self._compute_directional_output(a, b, c, e) for a, b, c, e in zip(*vals)]
 
     def _compute_directional_output(self, pressure, vel_x, vel_y, vel_z):
         Z = self._solver_rho * self._solver_c

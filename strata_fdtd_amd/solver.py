@@ -152,6 +152,7 @@ class FDTDSolver:
         self._probe_order: list[str] = []
         self._mic_out = None
         self._mic_order: list[str] = []
+        self._mic_f64 = np.zeros((0, 4), dtype=np.int32)
         self._dirty_geometry = True
         self._dirty_sources = True
         self._dirty_ade = True
@@ -375,7 +376,7 @@ class FDTDSolver:
             q = self._mic_order.index(name)
             a = self._mic_synced[name]
             rows = rec[a - lo:, q, :]
-            mic._data.extend(mic._combine(rows))
+            mic._data.extend(mic._combine(rows, self._mic_f64[q]))
             mic._times.extend(self._step_times[a:self._step_count])
             self._mic_synced[name] = self._step_count
 
@@ -529,7 +530,7 @@ class FDTDSolver:
         mics = list(self._microphones)
         new_mic = None
         if mics:
-            new_mic = torch.zeros((cap, len(mics), 4), dtype=torch.float32, device=self._device)
+            new_mic = torch.zeros((cap, len(mics), 4), dtype=torch.float64, device=self._device)
             if self._mic_out is not None and self._step_count:
                 n_old = min(self._step_count, self._mic_out.shape[0])
                 for q_new, name in enumerate(mics):
@@ -537,11 +538,12 @@ class FDTDSolver:
                         new_mic[:n_old, q_new] = self._mic_out[:n_old, self._mic_order.index(name)]
             tabs = [self._microphones[n].gather_table(self.shape) for n in mics]
             idx = np.ascontiguousarray(np.stack([t[0] for t in tabs]), dtype=np.int32)
-            wts = np.ascontiguousarray(np.stack([t[1] for t in tabs]), dtype=np.float32)
+            wts = np.ascontiguousarray(np.stack([t[1] for t in tabs]), dtype=np.float64)
+            self._mic_f64 = np.ascontiguousarray(np.stack([t[2] for t in tabs]), dtype=np.int32)
             _native.call("fdtd_ctx_set_microphones", self._ctx, len(mics), idx.ctypes.data, wts.ctypes.data,
-                         _native.ptr(new_mic), cap)
+                         self._mic_f64.ctypes.data, _native.ptr(new_mic), cap)
         else:
-            _native.call("fdtd_ctx_set_microphones", self._ctx, 0, None, None, None, 0)
+            _native.call("fdtd_ctx_set_microphones", self._ctx, 0, None, None, None, None, 0)
         self._mic_out = new_mic
         self._mic_order = mics
         kind, a, b = [], [], []

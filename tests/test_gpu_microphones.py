@@ -24,7 +24,7 @@ def _solver(shape=(24, 26, 28), dx=2e-3, geometry=True):
 PATTERNS = {
     "omni": dict(pattern="omni"),
     "cardioid": dict(pattern="cardioid", direction=(1.0, 0.5, 0.0)),
-    "figure8": dict(pattern="figure8", direction=(0.0, 0.0, 1.0)),
+    "figure8": dict(pattern="figure8", direction=(0.0, 1.0, 0.0)),
     "hyper": dict(pattern="hypercardioid", direction=(-0.3, 0.9, 0.2)),
     "custom": dict(pattern=lambda th: 0.7 + 0.3 * np.cos(th), direction=(0.2, 0.2, 0.9)),
 }
@@ -35,6 +35,9 @@ def _positions(shape, dx):
     pos = [tuple(rng.uniform(0.0, np.array(shape) - 1.0001) * dx) for _ in range(len(PATTERNS))]
     pos[0] = (0.0, 0.0, 0.0)              # velocity gathers clamp at the low edges
     pos[1] = (5 * dx, 6 * dx, 7 * dx)     # on a grid point
+    # numpy float64 positions make the reference sum in float64; Python
+    # floats make it sum float32 products of float32-rounded weights
+    pos[3] = tuple(float(v) for v in pos[3])
     return pos
 
 

@@ -109,14 +109,17 @@ def test_product_combine_equals_oracle(pattern):
     rng = np.random.default_rng(11)
     rec = (rng.standard_normal((64, 4)) * 1e-2).astype(np.float32)
     rec[5, 1:] = 0.0  # |v| == 0: theta = 0 branch
-    out = m._combine(rec)
-    dd = np.asarray(d) / np.linalg.norm(d)
-    for r, o in zip(rec, out):
-        if pattern == "omni":
-            want = r[0]
-        else:
-            want = OM.directional(r[0], (r[1], r[2], r[3]), dd, pat, g.rho, g.c)
-        assert np.float32(o) == np.float32(want) and float(o) == float(want)
+    for f64 in ([0, 0, 0, 0], [1, 1, 1, 1], [0, 1, 0, 1]):
+        rec_t = [r.astype(np.float64) if f else r for r, f in zip(rec.T, f64)]
+        out = m._combine(rec.astype(np.float64), np.array(f64))
+        dd = np.asarray(d) / np.linalg.norm(d)
+        for q, o in enumerate(out):
+            r = [col[q] for col in rec_t]
+            if pattern == "omni":
+                want = r[0]
+            else:
+                want = OM.directional(r[0], (r[1], r[2], r[3]), dd, pat, g.rho, g.c)
+            assert type(o) is type(want) and float(o) == float(want), (f64, q)
 
 
 def test_product_gather_tables_match_oracle_corners():
@@ -128,11 +131,12 @@ def test_product_gather_tables_match_oracle_corners():
             gp[t] = 0.0
         pos = tuple(gp * g.dx)
         m = _mic(g, pos)
-        idx, w = m.gather_table(g.shape)
+        idx, w, f64 = m.gather_table(g.shape)
+        assert list(f64) == [1, 1, 1, 1]  # numpy float64 positions: float64 weights
         gg = OM.grid_position(pos, g.dx)
         corners, weights = OM.trilinear(*gg)
         assert list(idx[0]) == [np.ravel_multi_index(c, g.shape) for c in corners]
-        assert np.array_equal(w[0], np.array([np.float32(x) for x in weights], np.float32))
+        assert np.array_equal(w[0], np.array(weights, np.float64))
         for ax in range(3):
             h = list(gg)
             h[ax] -= 0.5
@@ -140,7 +144,7 @@ def test_product_gather_tables_match_oracle_corners():
             corners, weights = OM.trilinear(*h)
             corners = [tuple(min(c, n - 1) for c, n in zip(cc, g.shape)) for cc in corners]
             assert list(idx[1 + ax]) == [np.ravel_multi_index(c, g.shape) for c in corners]
-            assert np.array_equal(w[1 + ax], np.array([np.float32(x) for x in weights], np.float32))
+            assert np.array_equal(w[1 + ax], np.array(weights, np.float64))
 
 
 def test_microphone_validation():
@@ -153,3 +157,35 @@ def test_microphone_validation():
         Microphone(position=(0.05, 0.05, 0.05), direction=(0, 0, 1), up=(0, 0, 2))
     with pytest.raises(TypeError):
         Microphone(position=(0.05, 0.05, 0.05), pattern=3)
+
+
+@pytest.mark.parametrize("kind", ["python", "float64", "float32"])
+def test_device_gather_arithmetic_equals_oracle(kind):
+    """Restates the device gather on the CPU (float32 sum of float32-rounded
+    weights, or float64 sum of float64 weights, per the table's flag) and
+    compares it with the oracle for positions given as Python floats, numpy
+    float64 and numpy float32 scalars."""
+    g = _Grid(shape=(24, 26, 28), dx=2e-3)
+    rng = np.random.default_rng(2)
+    for t in range(200):
+        raw = rng.uniform(0.0, np.array(g.shape) - 1.0001) * g.dx
+        pos = {"python": tuple(float(v) for v in raw), "float64": tuple(raw),
+               "float32": tuple(np.float32(v) for v in raw)}[kind]
+        m = _mic(g, pos)
+        idx, w, f64 = m.gather_table(g.shape)
+        F = [(rng.standard_normal(g.shape) * 1e-4).astype(np.float32) for _ in range(4)]
+        gg = OM.grid_position(pos, g.dx)
+        for a in range(4):
+            flat = F[a].ravel()
+            if f64[a]:
+                acc = 0.0
+                for c in range(8):
+                    acc = acc + w[a][c] * float(flat[idx[a][c]])
+                got = np.float64(acc)
+            else:
+                acc = np.float32(0)
+                for c in range(8):
+                    acc = np.float32(acc + np.float32(np.float32(w[a][c]) * flat[idx[a][c]]))
+                got = acc
+            want = OM.interp_pressure(F[0], gg) if a == 0 else OM.interp_velocity(F[a], gg, a - 1)
+            assert got == want and (f64[a] == 1) == isinstance(want, np.float64), (kind, t, a)
