@@ -153,6 +153,7 @@ def adjoint_bench(n: int = 192, steps: int = 2000, fd_voxels: int = 0):
 
     forward_traces(pb, rho, K)  # warm-up
     t_fwd, _ = timed(lambda: forward_traces(pb, rho, K))
+    t_fwd_cell, _ = timed(lambda: forward_traces(pb, rho, K, impl="cell"))
     plain = FDTDSolver(grid=UniformGrid(pb.shape, pb.resolution), c=pb.c, rho=pb.rho, backend="mi355x")
     plain.add_boundary(PML(depth=pb.pml_depth))
     plain.add_source(pb.source)
@@ -161,20 +162,31 @@ def adjoint_bench(n: int = 192, steps: int = 2000, fd_voxels: int = 0):
     plain._prepare(steps)
     t_plain, _ = timed(lambda: plain.step_n(steps))
 
-    def grad():
+    def grad(history):
         r = rho.clone().requires_grad_(True)
         k = K.clone().requires_grad_(True)
-        tr = probe_traces(r, k, pb)
+        tr = probe_traces(r, k, pb, history=history)
         loss = ((tr - target) ** 2).sum()
         loss.backward()
         return loss.item(), r.grad, k.grad
 
-    t_grad, (loss, g_r, g_k) = timed(grad)
+    grad("all")  # warm-up (history allocation, code objects)
+    torch.cuda.empty_cache()
+    t_grad, (loss, g_r, g_k) = timed(lambda: grad("all"))
+    torch.cuda.empty_cache()
+    t_grad_ck, (loss_ck, g_r_ck, g_k_ck) = timed(lambda: grad("checkpoint"))
+    same = bool(torch.equal(g_r, g_r_ck) and torch.equal(g_k, g_k_ck))
+    from strata_fdtd_amd.adjoint import history_bytes
     out = {
         "shape": list(pb.shape), "steps": steps, "design_box": list(pb.box_shape), "checkpoint": pb.checkpoint,
-        "forward_per_voxel_s": round(t_fwd, 4), "forward_plain_fused_s": round(t_plain, 4),
-        "gradient_s": round(t_grad, 4),
+        "forward_s": round(t_fwd, 4), "forward_cell_kernels_s": round(t_fwd_cell, 4),
+        "forward_plain_fused_s": round(t_plain, 4),
+        "gradient_s": round(t_grad, 4), "gradient_history": "all steps in HBM",
+        "history_gb": round(history_bytes(pb) / 1e9, 2),
         "gradient_over_forward": round(t_grad / t_fwd, 3),
+        "gradient_checkpoint_recompute_s": round(t_grad_ck, 4),
+        "gradient_checkpoint_over_forward": round(t_grad_ck / t_fwd, 3),
+        "checkpoint_and_history_gradients_identical": same,
         "gradient_over_plain_forward": round(t_grad / t_plain, 3),
         "loss": loss, "grad_norm_rho": float(g_r.norm()), "grad_norm_K": float(g_k.norm()),
     }

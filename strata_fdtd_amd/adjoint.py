@@ -164,6 +164,114 @@ class _Device:
             raise _native.NativeError(f"fdtd_adjoint_step failed ({rc})")
 
 
+class _Fast:
+    """Float32 march kernels of the design solve (csrc/design.hip): forward
+    sweep with optional box history, adjoint sweep reading that history."""
+
+    PLANES = 8
+
+    def __init__(self, dv: _Device):
+        import torch
+        self.dv = dv
+        pb = dv.pb
+        self.wave_dev = torch.from_numpy(np.ascontiguousarray(dv.wave, dtype=np.float64)).to(dv.dev)
+        src = pb.source
+        self.src = (C.c_int32 * 3)(*[int(v) for v in src.position]) if src is not None else None
+        self.probe_ijk = np.ascontiguousarray(np.array(pb.probes or [(0, 0, 0)], dtype=np.int32).reshape(-1, 3))
+        self.dec = (_vp * 3)(*[_ptr(d) for d in dv.dec])
+        lo, hi = np.array(pb.box_lo), np.array(pb.box_hi)
+        self.ecells = int(np.prod(hi - lo + 2))
+        self.bcells = int(np.prod(hi - lo))
+
+    @staticmethod
+    def _sets(sets):
+        return (_vp * 8)(*[_ptr(t) for st in sets for t in st])
+
+    def _common(self, r, kappa):
+        dv, pb = self.dv, self.dv.pb
+        return (_ptr(dv.flags), self.dec, *pb.shape, dv.cv0, dv.cp0, dv.lo.ctypes.data, dv.hi.ctypes.data,
+                _ptr(r), _ptr(kappa))
+
+    def history(self, steps):
+        import torch
+        return (torch.empty(steps * self.ecells, dtype=torch.float32, device=self.dv.dev),
+                torch.empty(steps * self.bcells, dtype=torch.float32, device=self.dv.dev))
+
+    def forward(self, sets, r, kappa, trace, step0, n_steps, hist=None, hist0=0):
+        hp, hd = hist if hist is not None else (None, None)
+        hist_steps = (hp.numel() // self.ecells) if hp is not None else 0
+        _native.call("fdtd_design_forward", self._sets(sets), *self._common(r, kappa), self.src,
+                     _ptr(self.wave_dev), self.dv.n_probe, self.probe_ijk.ctypes.data, _ptr(trace), step0, n_steps,
+                     _ptr(hp), _ptr(hd), hist0, hist_steps, self.PLANES, self.dv.stream)
+
+    def adjoint(self, lam, r, kappa, g, n_lo, n_hi, hist, hist0, acc):
+        hp, hd = hist
+        _native.call("fdtd_design_adjoint", self._sets(lam), *self._common(r, kappa), self.dv.n_probe,
+                     self.probe_ijk.ctypes.data, _ptr(g), n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
+                     hp.numel() // self.ecells, *[_ptr(a) for a in acc], self.PLANES, self.dv.stream)
+
+
+def history_bytes(pb: DesignProblem, steps: int | None = None) -> int:
+    """HBM bytes of the float32 forward history (p on the box grown by one
+    cell, div on the box) for `steps` steps."""
+    lo, hi = np.array(pb.box_lo), np.array(pb.box_hi)
+    return 4 * int(np.prod(hi - lo + 2) + np.prod(hi - lo)) * (pb.steps if steps is None else steps)
+
+
+def _pick_history(pb: DesignProblem, history: str, device) -> str:
+    if history != "auto":
+        return history
+    import torch
+    free, _total = torch.cuda.mem_get_info(device)
+    return "all" if history_bytes(pb) <= 0.5 * free else "checkpoint"
+
+
+def _forward_fast(pb, r, kappa, dv, history: str):
+    import torch
+    fast = _Fast(dv)
+    sets = [dv.zeros_state(), dv.zeros_state()]
+    traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
+    ckpts, hist = [], None
+    if history == "all":
+        hist = fast.history(pb.steps)
+        fast.forward(sets, r, kappa, traces, 0, pb.steps, hist, 0)
+    elif history == "checkpoint":
+        K = pb.checkpoint
+        for n0 in range(0, pb.steps, K):
+            ckpts.append([t.clone() for t in sets[n0 & 1]])
+            fast.forward(sets, r, kappa, traces, n0, min(K, pb.steps - n0))
+    else:
+        fast.forward(sets, r, kappa, traces, 0, pb.steps)
+    return traces[:, :dv.n_probe], ("fast", dv, r, kappa, fast, history, hist, ckpts)
+
+
+def _adjoint_fast(saved, grad_traces):
+    import torch
+    _, dv, r, kappa, fast, history, hist, ckpts = saved
+    pb = dv.pb
+    lam = [dv.zeros_state(), dv.zeros_state()]
+    acc = [torch.zeros(pb.box_shape, dtype=torch.float32, device=dv.dev) for _ in range(4)]
+    g = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
+    g[:, :dv.n_probe] = grad_traces.to(torch.float32)
+    if history == "all":
+        fast.adjoint(lam, r, kappa, g, 0, pb.steps, hist, 0, acc)
+    elif history == "checkpoint":
+        K = pb.checkpoint
+        rec = [dv.zeros_state(), dv.zeros_state()]
+        seg_hist = fast.history(K)
+        scratch = torch.zeros_like(g)
+        for seg in reversed(range(len(ckpts))):
+            n0 = seg * K
+            n1 = min(pb.steps, n0 + K)
+            for t, c in zip(rec[n0 & 1], ckpts[seg]):
+                t.copy_(c)
+            fast.forward(rec, r, kappa, scratch, n0, n1 - n0, seg_hist, n0)
+            fast.adjoint(lam, r, kappa, g, n0, n1, seg_hist, n0, acc)
+    else:
+        raise RuntimeError("forward ran without history or checkpoints")
+    return acc[0] - acc[1], acc[2] - acc[3]
+
+
 def design_ratio(x, ref: float, device="cuda"):
     """The per-voxel ratio the kernels read: x / ref formed in float32 on the
     device (rho/rho0 or K/K0). Finite-difference checks use it to find the
@@ -173,13 +281,27 @@ def design_ratio(x, ref: float, device="cuda"):
 
 
 def forward_traces(pb: DesignProblem, rho=None, K=None, dtype=None, device="cuda",
-                   keep_checkpoints: bool = False):
-    """Run the per-voxel forward solve; returns (traces [steps, n_probe], checkpoints)."""
+                   keep_checkpoints: bool = False, impl: str = "auto", history: str = "auto"):
+    """Run the per-voxel forward solve; returns (traces [steps, n_probe], saved
+    state for the adjoint). impl "fast" (float32 march kernels) or "cell" (one
+    thread per cell, float32 or float64); "auto" picks "fast" for float32.
+    history (fast, with keep_checkpoints): "all" keeps the box history of every
+    step in HBM, "checkpoint" keeps the state every `pb.checkpoint` steps and
+    recomputes each segment's history in the backward pass; "auto" takes "all"
+    when it fits in half the free HBM."""
     import torch
     dtype = dtype or torch.float32
-    dv = _Device(pb, dtype, torch.device(device))
+    dev = torch.device(device)
+    dv = _Device(pb, dtype, dev)
     r = design_ratio(rho, pb.rho, device) if rho is not None else None
     kappa = design_ratio(K, pb.K0, device) if K is not None else None
+    if impl == "auto":
+        impl = "fast" if dtype == torch.float32 else "cell"
+    if impl == "fast":
+        if dtype != torch.float32:
+            raise ValueError("the fast design kernels are float32")
+        mode = _pick_history(pb, history, dev) if keep_checkpoints else "none"
+        return _forward_fast(pb, r, kappa, dv, mode)
     a, b = dv.zeros_state(), dv.zeros_state()
     traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=dtype, device=device)
     ckpts = []
@@ -191,13 +313,15 @@ def forward_traces(pb: DesignProblem, rho=None, K=None, dtype=None, device="cuda
         dv.run([a, b], n0, m, True, r, kappa, traces)
         if m & 1:
             a, b = b, a
-    return traces[:, :dv.n_probe], (dv, r, kappa, ckpts)
+    return traces[:, :dv.n_probe], ("cell", dv, r, kappa, ckpts)
 
 
 def adjoint_gradients(saved, grad_traces):
     """Backward pass: returns (dL/dr, dL/dkappa) on the design box."""
     import torch
-    dv, r, kappa, ckpts = saved
+    if saved[0] == "fast":
+        return _adjoint_fast(saved, grad_traces)
+    _, dv, r, kappa, ckpts = saved
     pb = dv.pb
     K = pb.checkpoint
     lam, lam_out = dv.zeros_state(), dv.zeros_state()
@@ -225,18 +349,19 @@ class _ProbeTraces:
     fn = None
 
 
-def probe_traces(rho, K, problem: DesignProblem):
+def probe_traces(rho, K, problem: DesignProblem, impl: str = "auto", history: str = "auto"):
     """Differentiable probe traces [steps, n_probe] as a function of the design
-    region's density and bulk modulus (torch tensors of the box shape)."""
+    region's density and bulk modulus (torch tensors of the box shape). impl and
+    history as in forward_traces."""
     import torch
 
     if _ProbeTraces.fn is None:
         class ProbeTraces(torch.autograd.Function):
             @staticmethod
-            def forward(ctx, rho_, K_, pb):
+            def forward(ctx, rho_, K_, pb, impl_, history_):
                 traces, saved = forward_traces(pb, rho_, K_, dtype=rho_.dtype if rho_.dtype == torch.float64
                                                else torch.float32, device=rho_.device,
-                                               keep_checkpoints=True)
+                                               keep_checkpoints=True, impl=impl_, history=history_)
                 ctx.saved = saved
                 ctx.pb = pb
                 return traces
@@ -246,10 +371,10 @@ def probe_traces(rho, K, problem: DesignProblem):
                 gr, gk = adjoint_gradients(ctx.saved, grad_traces.contiguous())
                 pb = ctx.pb
                 ctx.saved = None
-                return (gr / pb.rho).to(grad_traces.dtype), (gk / pb.K0).to(grad_traces.dtype), None
+                return (gr / pb.rho).to(grad_traces.dtype), (gk / pb.K0).to(grad_traces.dtype), None, None, None
 
         _ProbeTraces.fn = ProbeTraces
-    return _ProbeTraces.fn.apply(rho, K, problem)
+    return _ProbeTraces.fn.apply(rho, K, problem, impl, history)
 
 
 def design_problem_192(n: int = 192, steps: int = 2000, seed: int = 5, pml: int = 12,

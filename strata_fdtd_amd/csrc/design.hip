@@ -1,0 +1,563 @@
+// Fast float32 path of the per-voxel design solve and its adjoint (gfx950):
+// the forward step and the time-reversed adjoint step as 2.5D i-marching
+// sweeps (the layout of the fused solver sweep: a wave owns a 64-wide k
+// segment of R rows and marches `planes` planes; k neighbours by lane
+// shuffles, j neighbours within the wave or from one halo row, i neighbours
+// carried from the previous plane).
+//
+// Both perform, cell by cell, the operations of the reference-checked kernels
+// in adjoint.hip (k_param_step / k_adjoint_step) in the same order, so their
+// float32 results are identical to those kernels' (tests compare them bit for
+// bit). No counterpart in the reference (scalar rho and c only,
+// core/solver.py:1584-1592).
+//
+// The adjoint needs the forward state only inside the design box: the forward
+// sweep records, for step n, p^n on the box grown by one cell on every side
+// (the pressure differences across the box cells' faces) and div^n on the box
+// (the divergence of the updated velocity), so the adjoint sweep reads no full
+// forward state. That history is kept for every step when it fits in HBM
+// (no recomputation) or for one checkpoint segment at a time.
+
+#include <algorithm>
+
+#include "fdtd_common.hpp"
+
+namespace fdtd_amd {
+
+struct DesignArgs {
+    // fields: ping-pong sets {p, vx, vy, vz}; step s reads set s & 1
+    float* f[2][4];
+    const uint8_t* flags;
+    const float* dec[3];
+    int nx, ny, nz;
+    float cv0, cp0;
+    // design box [lo, hi) with r = rho/rho0 and kappa = K/K0 (float32)
+    int lo[3], hi[3];
+    const float* r;
+    const float* kappa;
+    // source (one point source; -1 none) and probes
+    int src_i, src_j, src_k;
+    const double* wave;              // device [steps]
+    int n_probe;
+    int probe[FDTD_MAX_PROBES][3];
+    float* trace;                    // device [steps][n_probe] (forward)
+    const float* g;                  // device [steps][n_probe] (adjoint: dL/dy)
+    // history: p on [lo-1, hi+1), div on [lo, hi); slot of step s = s - hist0
+    float* hist_p;
+    float* hist_div;
+    int64_t hist0;
+    int64_t hist_steps;
+    // gradient accumulators on the box (Kahan sum / compensation)
+    float* gr_sum;
+    float* gr_comp;
+    float* gk_sum;
+    float* gk_comp;
+    // tiling
+    int planes, nseg, nstrip_j, nwaves;
+};
+
+__device__ __forceinline__ bool dbox(const DesignArgs& a, int i, int j, int k) {
+    return i >= a.lo[0] && i < a.hi[0] && j >= a.lo[1] && j < a.hi[1] && k >= a.lo[2] && k < a.hi[2];
+}
+
+__device__ __forceinline__ int64_t dbox_lin(const DesignArgs& a, int i, int j, int k) {
+    return ((static_cast<int64_t>(i - a.lo[0]) * (a.hi[1] - a.lo[1])) + (j - a.lo[1])) * (a.hi[2] - a.lo[2]) +
+           (k - a.lo[2]);
+}
+
+__device__ __forceinline__ float rv(const DesignArgs& a, int i, int j, int k) {
+    return (a.r && dbox(a, i, j, k)) ? a.r[dbox_lin(a, i, j, k)] : 1.0f;
+}
+
+__device__ __forceinline__ float kv_(const DesignArgs& a, int i, int j, int k) {
+    return (a.kappa && dbox(a, i, j, k)) ? a.kappa[dbox_lin(a, i, j, k)] : 1.0f;
+}
+
+// the extended box [lo-1, hi+1) of the pressure history
+__device__ __forceinline__ bool ebox(const DesignArgs& a, int i, int j, int k) {
+    return i >= a.lo[0] - 1 && i <= a.hi[0] && j >= a.lo[1] - 1 && j <= a.hi[1] && k >= a.lo[2] - 1 &&
+           k <= a.hi[2];
+}
+
+__device__ __forceinline__ int64_t ebox_lin(const DesignArgs& a, int i, int j, int k) {
+    const int ey = a.hi[1] - a.lo[1] + 2, ez = a.hi[2] - a.lo[2] + 2;
+    return ((static_cast<int64_t>(i - a.lo[0] + 1) * ey) + (j - a.lo[1] + 1)) * ez + (k - a.lo[2] + 1);
+}
+
+__device__ __forceinline__ int64_t ebox_cells(const DesignArgs& a) {
+    return static_cast<int64_t>(a.hi[0] - a.lo[0] + 2) * (a.hi[1] - a.lo[1] + 2) * (a.hi[2] - a.lo[2] + 2);
+}
+
+__device__ __forceinline__ int64_t dbox_cells(const DesignArgs& a) {
+    return static_cast<int64_t>(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]) * (a.hi[2] - a.lo[2]);
+}
+
+__device__ __forceinline__ float cvf(const DesignArgs& a, float r_lo, float r_hi) {
+    return a.cv0 / ((r_lo + r_hi) * 0.5f);
+}
+
+__device__ __forceinline__ float sdown(float v) { return __shfl_down(v, 1, kWave); }
+__device__ __forceinline__ float sup(float v) { return __shfl_up(v, 1, kWave); }
+
+// ---------------------------------------------------------------- forward
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, int64_t s) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = blockIdx.x * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w >= a.nwaves) return;
+    const int seg = w % a.nseg, t = w / a.nseg;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = seg * kWave, k = k0 + lane;
+    const bool kin = k < nz;
+    const int j0 = (t % a.nstrip_j) * R;
+    const int i0 = (t / a.nstrip_j) * a.planes, i1 = min(i0 + a.planes, nx);
+    const int64_t sx = static_cast<int64_t>(ny) * nz;
+    const int par = static_cast<int>(s & 1);
+    const float* __restrict__ P = a.f[par][0];
+    const float* __restrict__ VX = a.f[par][1];
+    const float* __restrict__ VY = a.f[par][2];
+    const float* __restrict__ VZ = a.f[par][3];
+    float* __restrict__ Po = a.f[par ^ 1][0];
+    float* __restrict__ VXo = a.f[par ^ 1][1];
+    float* __restrict__ VYo = a.f[par ^ 1][2];
+    float* __restrict__ VZo = a.f[par ^ 1][3];
+    const uint8_t* __restrict__ FL = a.flags;
+    const bool hist = a.hist_p && s >= a.hist0 && s < a.hist0 + a.hist_steps;
+    float* hp = hist ? a.hist_p + (s - a.hist0) * ebox_cells(a) : nullptr;
+    float* hd = hist ? a.hist_div + (s - a.hist0) * dbox_cells(a) : nullptr;
+    const double wsrc = a.wave ? a.wave[s] : 0.0;
+    const float dz_k = kin ? a.dec[2][k] : 1.0f;
+
+    bool rvld[R];
+    float pc[R], vxm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        rvld[r] = j0 + r < ny;
+        pc[r] = (rvld[r] && kin) ? P[lin(i0, j0 + r, k, ny, nz)] : 0.f;
+        vxm[r] = 0.f;
+        if (i0 > 0 && rvld[r] && kin) {
+            const int64_t c = lin(i0 - 1, j0 + r, k, ny, nz);
+            const float v = VX[c] + cvf(a, rv(a, i0 - 1, j0 + r, k), rv(a, i0, j0 + r, k)) * (pc[r] - P[c]);
+            vxm[r] = (FL[c] & FDTD_FLAG_OPEN_X) ? v : 0.f;
+        }
+    }
+    for (int i = i0; i < i1; ++i) {
+        const bool nxt = i + 1 < nx;
+        const float dx_i = a.dec[0][i];
+        float pn[R], vxo[R], vyo[R], vzo[R], hr[R];
+        uint32_t fl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pn[r] = vxo[r] = vyo[r] = vzo[r] = hr[r] = 0.f;
+            fl[r] = 0;
+            if (!(rvld[r] && kin)) continue;
+            const int64_t c = lin(i, j0 + r, k, ny, nz);
+            if (nxt) pn[r] = P[c + sx];
+            vxo[r] = VX[c];
+            vyo[r] = VY[c];
+            vzo[r] = VZ[c];
+            fl[r] = FL[c];
+        }
+        // right k halo (lane 63) and the face below the first row
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rvld[r] && lane == kWave - 1 && k + 1 < nz) hr[r] = P[lin(i, j0 + r, k + 1, ny, nz)];
+        float p_up_halo = 0.f;
+        if (kin && j0 + R < ny) p_up_halo = P[lin(i, j0 + R, k, ny, nz)];
+        float vyprev = 0.f;
+        if (kin && j0 > 0) {
+            const int64_t c = lin(i, j0 - 1, k, ny, nz);
+            const float v = VY[c] + cvf(a, rv(a, i, j0 - 1, k), rv(a, i, j0, k)) * (pc[0] - P[c]);
+            vyprev = (FL[c] & FDTD_FLAG_OPEN_Y) ? v : 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!rvld[r]) continue;
+            const int j = j0 + r;
+            const int64_t c = lin(i, j, k, ny, nz);
+            const uint32_t f = fl[r];
+            const float pr = pc[r];
+            const float rc = rv(a, i, j, k);
+            float vxn = vxo[r];
+            if (nxt) {
+                const float v = vxo[r] + cvf(a, rc, rv(a, i + 1, j, k)) * (pn[r] - pr);
+                vxn = (f & FDTD_FLAG_OPEN_X) ? v : 0.f;
+            }
+            float vyn = vyo[r];
+            if (j + 1 < ny) {
+                const float pu = (r + 1 < R) ? pc[r + 1] : p_up_halo;
+                const float v = vyo[r] + cvf(a, rc, rv(a, i, j + 1, k)) * (pu - pr);
+                vyn = (f & FDTD_FLAG_OPEN_Y) ? v : 0.f;
+            }
+            float pk1 = sdown(pr);
+            if (lane == kWave - 1) pk1 = hr[r];
+            float vzn = vzo[r];
+            if (k + 1 < nz) {
+                const float v = vzo[r] + cvf(a, rc, rv(a, i, j, k + 1)) * (pk1 - pr);
+                vzn = (f & FDTD_FLAG_OPEN_Z) ? v : 0.f;
+            }
+            float vzm = sup(vzn);
+            if (lane == 0) {
+                vzm = 0.f;
+                if (k0 > 0 && kin) {
+                    const int64_t cl = c - 1;
+                    const float v = VZ[cl] + cvf(a, rv(a, i, j, k - 1), rc) * (pr - P[cl]);
+                    vzm = (FL[cl] & FDTD_FLAG_OPEN_Z) ? v : 0.f;
+                }
+            }
+            const float div = ((vxn - vxm[r]) + (vyn - vyprev)) + (vzn - vzm);
+            float pnew = (f & FDTD_FLAG_AIR) ? pr + (a.cp0 * kv_(a, i, j, k)) * div : 0.f;
+            const float dy_j = a.dec[1][j];
+            pnew = ((pnew * dx_i) * dy_j) * dz_k;
+            if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
+                pnew = static_cast<float>(static_cast<double>(pnew) + wsrc);
+            if (kin) {
+                Po[c] = pnew;
+                VXo[c] = vxn * dx_i;
+                VYo[c] = vyn * dy_j;
+                VZo[c] = vzn * dz_k;
+                for (int q = 0; q < a.n_probe; ++q)
+                    if (a.probe[q][0] == i && a.probe[q][1] == j && a.probe[q][2] == k)
+                        a.trace[s * a.n_probe + q] = pnew;
+                if (hist) {
+                    if (ebox(a, i, j, k)) hp[ebox_lin(a, i, j, k)] = pr;
+                    if (dbox(a, i, j, k)) hd[dbox_lin(a, i, j, k)] = div;
+                }
+            }
+            vxm[r] = vxn;
+            vyprev = vyn;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) pc[r] = pn[r];
+    }
+}
+
+// ---------------------------------------------------------------- adjoint
+// pb = dX dY dZ (P + g at probe cells); q = air cp0 kappa pb.
+__device__ __forceinline__ float design_pb(const DesignArgs& a, const float* __restrict__ P, int i, int j, int k,
+                                           int64_t c, int64_t n) {
+    float v = P[c];
+    for (int q = 0; q < a.n_probe; ++q)
+        if (a.probe[q][0] == i && a.probe[q][1] == j && a.probe[q][2] == k) v += a.g[n * a.n_probe + q];
+    const float lam = (a.dec[0][i] * a.dec[1][j]) * a.dec[2][k];
+    return lam * v;
+}
+
+__device__ __forceinline__ float design_q(const DesignArgs& a, const float* __restrict__ P, int i, int j, int k,
+                                          int64_t c, int64_t n) {
+    if (!(a.flags[c] & FDTD_FLAG_AIR)) return 0.f;
+    return a.cp0 * kv_(a, i, j, k) * design_pb(a, P, i, j, k, c, n);
+}
+
+__device__ __forceinline__ void kahan(float* sum, float* comp, int64_t b, float x) {
+    const float y = x - comp[b];
+    const float t = sum[b] + y;
+    comp[b] = (t - sum[b]) - y;
+    sum[b] = t;
+}
+
+// Adjoint step n: (P, V) = lambda_{n+1} in set (n+1)&1 -> lambda_n in set n&1.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, int64_t n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = blockIdx.x * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w >= a.nwaves) return;
+    const int seg = w % a.nseg, t = w / a.nseg;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = seg * kWave, k = k0 + lane;
+    const bool kin = k < nz;
+    const int j0 = (t % a.nstrip_j) * R;
+    const int i0 = (t / a.nstrip_j) * a.planes, i1 = min(i0 + a.planes, nx);
+    const int64_t sx = static_cast<int64_t>(ny) * nz;
+    const int src = static_cast<int>((n + 1) & 1);
+    const float* __restrict__ P = a.f[src][0];
+    const float* __restrict__ VX = a.f[src][1];
+    const float* __restrict__ VY = a.f[src][2];
+    const float* __restrict__ VZ = a.f[src][3];
+    float* __restrict__ Po = a.f[src ^ 1][0];
+    float* __restrict__ VXo = a.f[src ^ 1][1];
+    float* __restrict__ VYo = a.f[src ^ 1][2];
+    float* __restrict__ VZo = a.f[src ^ 1][3];
+    const uint8_t* __restrict__ FL = a.flags;
+    const bool grad = a.gr_sum && n >= a.hist0 && n < a.hist0 + a.hist_steps;
+    const float* hp = grad ? a.hist_p + (n - a.hist0) * ebox_cells(a) : nullptr;
+    const float* hd = grad ? a.hist_div + (n - a.hist0) * dbox_cells(a) : nullptr;
+    const float dz_k = kin ? a.dec[2][k] : 1.0f;
+
+    bool rvld[R];
+    float pbc[R], qc[R], wxm[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        rvld[r] = j0 + r < ny;
+        pbc[r] = qc[r] = wxm[r] = 0.f;
+        if (!(rvld[r] && kin)) continue;
+        const int j = j0 + r;
+        const int64_t c = lin(i0, j, k, ny, nz);
+        pbc[r] = design_pb(a, P, i0, j, k, c, n);
+        qc[r] = (FL[c] & FDTD_FLAG_AIR) ? a.cp0 * kv_(a, i0, j, k) * pbc[r] : 0.f;
+        if (i0 > 0) {
+            const int64_t cl = c - sx;
+            wxm[r] = (FL[cl] & FDTD_FLAG_OPEN_X)
+                         ? a.dec[0][i0 - 1] * VX[cl] + design_q(a, P, i0 - 1, j, k, cl, n) - qc[r]
+                         : 0.f;
+        }
+    }
+    for (int i = i0; i < i1; ++i) {
+        const bool nxt = i + 1 < nx;
+        const float dx_i = a.dec[0][i];
+        float pbn[R], qn[R], vx[R], vy[R], vz[R], qhr[R];
+        uint32_t fl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pbn[r] = qn[r] = vx[r] = vy[r] = vz[r] = qhr[r] = 0.f;
+            fl[r] = 0;
+            if (!(rvld[r] && kin)) continue;
+            const int j = j0 + r;
+            const int64_t c = lin(i, j, k, ny, nz);
+            if (nxt) {
+                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n);
+                qn[r] = (FL[c + sx] & FDTD_FLAG_AIR) ? a.cp0 * kv_(a, i + 1, j, k) * pbn[r] : 0.f;
+            }
+            vx[r] = VX[c];
+            vy[r] = VY[c];
+            vz[r] = VZ[c];
+            fl[r] = FL[c];
+            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n);
+        }
+        float q_up_halo = 0.f;
+        if (kin && j0 + R < ny) q_up_halo = design_q(a, P, i, j0 + R, k, lin(i, j0 + R, k, ny, nz), n);
+        float wyprev = 0.f;
+        if (kin && j0 > 0) {
+            const int64_t cl = lin(i, j0 - 1, k, ny, nz);
+            wyprev = (FL[cl] & FDTD_FLAG_OPEN_Y)
+                         ? a.dec[1][j0 - 1] * VY[cl] + design_q(a, P, i, j0 - 1, k, cl, n) - qc[0]
+                         : 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!rvld[r]) continue;
+            const int j = j0 + r;
+            const int64_t c = lin(i, j, k, ny, nz);
+            const uint32_t f = fl[r];
+            const float rc = rv(a, i, j, k);
+            const bool air = f & FDTD_FLAG_AIR;
+            float acc = air ? pbc[r] : 0.f;
+            float gr = 0.f;
+            const bool gbox = grad && kin && dbox(a, i, j, k);
+            const float hpc = gbox ? hp[ebox_lin(a, i, j, k)] : 0.f;
+            // own faces (c = lo)
+            float wx, wy, wz;
+            const float dy_j = a.dec[1][j];
+            if (nxt) {
+                const float rh = rv(a, i + 1, j, k);
+                const float cv = cvf(a, rc, rh);
+                wx = (f & FDTD_FLAG_OPEN_X) ? dx_i * vx[r] + qc[r] - qn[r] : 0.f;
+                acc -= cv * wx;
+                if (gbox) gr += wx * (hp[ebox_lin(a, i + 1, j, k)] - hpc) * (-cv / (rc + rh));
+            } else {
+                wx = dx_i * vx[r] + qc[r];
+            }
+            if (j + 1 < ny) {
+                const float rh = rv(a, i, j + 1, k);
+                const float cv = cvf(a, rc, rh);
+                const float qu = (r + 1 < R) ? qc[r + 1] : q_up_halo;
+                wy = (f & FDTD_FLAG_OPEN_Y) ? dy_j * vy[r] + qc[r] - qu : 0.f;
+                acc -= cv * wy;
+                if (gbox) gr += wy * (hp[ebox_lin(a, i, j + 1, k)] - hpc) * (-cv / (rc + rh));
+            } else {
+                wy = dy_j * vy[r] + qc[r];
+            }
+            float qk1 = sdown(qc[r]);
+            if (lane == kWave - 1) qk1 = qhr[r];
+            if (k + 1 < nz) {
+                const float rh = rv(a, i, j, k + 1);
+                const float cv = cvf(a, rc, rh);
+                wz = (f & FDTD_FLAG_OPEN_Z) ? dz_k * vz[r] + qc[r] - qk1 : 0.f;
+                acc -= cv * wz;
+                if (gbox) gr += wz * (hp[ebox_lin(a, i, j, k + 1)] - hpc) * (-cv / (rc + rh));
+            } else {
+                wz = dz_k * vz[r] + qc[r];
+            }
+            // faces below (c = hi)
+            float wzm = sup(wz);
+            if (lane == 0) {
+                wzm = 0.f;
+                if (k0 > 0 && kin) {
+                    const int64_t cl = c - 1;
+                    wzm = (FL[cl] & FDTD_FLAG_OPEN_Z)
+                              ? a.dec[2][k - 1] * VZ[cl] + design_q(a, P, i, j, k - 1, cl, n) - qc[r]
+                              : 0.f;
+                }
+            }
+            if (i > 0) {
+                const float rl = rv(a, i - 1, j, k);
+                const float cv = cvf(a, rl, rc);
+                acc += cv * wxm[r];
+                if (gbox) gr += wxm[r] * (hpc - hp[ebox_lin(a, i - 1, j, k)]) * (-cv / (rl + rc));
+            }
+            if (j > 0) {
+                const float rl = rv(a, i, j - 1, k);
+                const float cv = cvf(a, rl, rc);
+                acc += cv * wyprev;
+                if (gbox) gr += wyprev * (hpc - hp[ebox_lin(a, i, j - 1, k)]) * (-cv / (rl + rc));
+            }
+            if (k > 0) {
+                const float rl = rv(a, i, j, k - 1);
+                const float cv = cvf(a, rl, rc);
+                acc += cv * wzm;
+                if (gbox) gr += wzm * (hpc - hp[ebox_lin(a, i, j, k - 1)]) * (-cv / (rl + rc));
+            }
+            if (kin) {
+                Po[c] = acc;
+                VXo[c] = wx;
+                VYo[c] = wy;
+                VZo[c] = wz;
+                if (gbox) {
+                    const int64_t b = dbox_lin(a, i, j, k);
+                    kahan(a.gr_sum, a.gr_comp, b, gr);
+                    if (air) kahan(a.gk_sum, a.gk_comp, b, pbc[r] * a.cp0 * hd[b]);
+                }
+            }
+            wxm[r] = wx;
+            wyprev = wy;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pbc[r] = pbn[r];
+            qc[r] = qn[r];
+        }
+    }
+}
+
+constexpr int kDesignRows = 4;
+
+static int design_grid(DesignArgs& a, int planes) {
+    a.planes = planes;
+    a.nseg = (a.nz + kWave - 1) / kWave;
+    a.nstrip_j = (a.ny + kDesignRows - 1) / kDesignRows;
+    const int nstrip_i = (a.nx + planes - 1) / planes;
+    a.nwaves = a.nseg * a.nstrip_j * nstrip_i;
+    return (a.nwaves + kBlockWaves - 1) / kBlockWaves;
+}
+
+}  // namespace fdtd_amd
+
+using namespace fdtd_amd;
+
+namespace {
+
+int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny,
+                int nz, float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
+                const float* kappa, const int32_t* src_ijk, const double* wave, int n_probe,
+                const int32_t* probe_ijk) {
+    if (nx < 2 || ny < 2 || nz < 2 || !fields || !flags || !dec) return -2;
+    if (n_probe < 0 || n_probe > FDTD_MAX_PROBES || (n_probe > 0 && !probe_ijk)) return -2;
+    a = DesignArgs{};
+    for (int s = 0; s < 2; ++s)
+        for (int q = 0; q < 4; ++q) {
+            if (!fields[4 * s + q]) return -2;
+            a.f[s][q] = fields[4 * s + q];
+        }
+    a.flags = flags;
+    for (int d = 0; d < 3; ++d) {
+        if (!dec[d]) return -2;
+        a.dec[d] = dec[d];
+    }
+    a.nx = nx;
+    a.ny = ny;
+    a.nz = nz;
+    a.cv0 = cv0;
+    a.cp0 = cp0;
+    const int n[3] = {nx, ny, nz};
+    for (int d = 0; d < 3; ++d) {
+        a.lo[d] = box_lo ? box_lo[d] : 0;
+        a.hi[d] = box_hi ? box_hi[d] : 0;
+        if (a.lo[d] < 1 || a.hi[d] > n[d] - 1 || a.hi[d] <= a.lo[d]) return -3;  // box + 1-cell margin inside
+    }
+    a.r = r;
+    a.kappa = kappa;
+    a.src_i = a.src_j = a.src_k = -1;
+    if (src_ijk) {
+        a.src_i = src_ijk[0];
+        a.src_j = src_ijk[1];
+        a.src_k = src_ijk[2];
+    }
+    a.wave = wave;
+    a.n_probe = n_probe;
+    for (int q = 0; q < n_probe; ++q)
+        for (int d = 0; d < 3; ++d) {
+            a.probe[q][d] = probe_ijk[3 * q + d];
+            if (a.probe[q][d] < 0 || a.probe[q][d] >= n[d]) return -2;
+        }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t fdtd_design_history_bytes(const int32_t* box_lo, const int32_t* box_hi, int64_t steps) {
+    int64_t e = 1, b = 1;
+    for (int d = 0; d < 3; ++d) {
+        e *= box_hi[d] - box_lo[d] + 2;
+        b *= box_hi[d] - box_lo[d];
+    }
+    return 4 * (e + b) * steps;
+}
+
+int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
+                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
+                        const float* kappa, const int32_t* src_ijk, const double* wave, int n_probe,
+                        const int32_t* probe_ijk, float* trace, int64_t step0, int n_steps, float* hist_p,
+                        float* hist_div, int64_t hist0, int64_t hist_steps, int planes, void* stream) {
+    DesignArgs a;
+    if (int rc = fill_design(a, fields, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, r, kappa, src_ijk, wave,
+                             n_probe, probe_ijk))
+        return rc;
+    if (n_steps < 0 || step0 < 0 || planes < 1 || (n_probe > 0 && !trace) || (src_ijk && !wave)) return -2;
+    if (hist_p && (!hist_div || hist0 < 0 || hist_steps < 1)) return -2;
+    a.trace = trace;
+    a.hist_p = hist_p;
+    a.hist_div = hist_div;
+    a.hist0 = hist0;
+    a.hist_steps = hist_p ? hist_steps : 0;
+    const int blocks = design_grid(a, planes);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    for (int m = 0; m < n_steps; ++m) {
+        hipLaunchKernelGGL(k_design_forward<kDesignRows>, dim3(blocks), dim3(kBlock), 0, st, a, step0 + m);
+        if (hipGetLastError() != hipSuccess) return -5;
+    }
+    return 0;
+}
+
+int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
+                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
+                        const float* kappa, int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo,
+                        int64_t n_hi, const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps,
+                        float* gr_sum, float* gr_comp, float* gk_sum, float* gk_comp, int planes, void* stream) {
+    DesignArgs a;
+    if (int rc = fill_design(a, lam, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, r, kappa, nullptr, nullptr,
+                             n_probe, probe_ijk))
+        return rc;
+    if (n_hi < n_lo || n_lo < 0 || planes < 1 || (n_probe > 0 && !g)) return -2;
+    if (!hist_p || !hist_div || !gr_sum || !gr_comp || !gk_sum || !gk_comp) return -2;
+    if (n_lo < hist0 || n_hi > hist0 + hist_steps) return -2;  // every step needs its history
+    a.g = g;
+    a.hist_p = const_cast<float*>(hist_p);
+    a.hist_div = const_cast<float*>(hist_div);
+    a.hist0 = hist0;
+    a.hist_steps = hist_steps;
+    a.gr_sum = gr_sum;
+    a.gr_comp = gr_comp;
+    a.gk_sum = gk_sum;
+    a.gk_comp = gk_comp;
+    const int blocks = design_grid(a, planes);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    for (int64_t n = n_hi - 1; n >= n_lo; --n) {
+        hipLaunchKernelGGL(k_design_adjoint<kDesignRows>, dim3(blocks), dim3(kBlock), 0, st, a, n);
+        if (hipGetLastError() != hipSuccess) return -5;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -134,3 +134,38 @@ def test_checkpoint_interval_does_not_change_gradient():
     _, a_r, a_k = _grad(pb7, rho, K, target)
     _, b_r, b_k = _grad(pb40, rho, K, target)
     assert torch.equal(a_r, b_r) and torch.equal(a_k, b_k)
+
+
+def test_fast_forward_equals_cell_kernels():
+    """The float32 march forward performs the per-cell kernel's operations in
+    the same order: identical traces, with and without recorded history."""
+    import torch
+    from strata_fdtd_amd.adjoint import forward_traces
+    pb = _problem(steps=45, geometry=True)
+    rho, K, _ = _params(pb, torch.float32)
+    a, _ = forward_traces(pb, rho, K, impl="cell")
+    for hist in ("all", "checkpoint"):
+        b, _ = forward_traces(pb, rho, K, impl="fast", keep_checkpoints=True, history=hist)
+        assert torch.equal(a, b), hist
+    c, _ = forward_traces(pb, rho, K, impl="fast")
+    assert torch.equal(a, c)
+
+
+@pytest.mark.parametrize("history", ["all", "checkpoint"])
+def test_fast_gradient_equals_cell_kernels(history):
+    """The float32 march adjoint (forward state from the recorded box history)
+    gives the per-cell adjoint's gradients bit for bit."""
+    import torch
+    from strata_fdtd_amd.adjoint import probe_traces
+    pb = _problem(steps=45, checkpoint=10, geometry=True)
+    rho, K, target = _params(pb, torch.float32)
+    out = {}
+    for impl in ("cell", "fast"):
+        r = rho.clone().requires_grad_(True)
+        k = K.clone().requires_grad_(True)
+        tr = probe_traces(r, k, pb, impl=impl, history=history)
+        ((tr - target) ** 2).sum().backward()
+        out[impl] = (tr.detach(), r.grad, k.grad)
+    for x, y in zip(out["cell"], out["fast"]):
+        assert torch.equal(x, y), history
+    assert out["fast"][1].abs().max() > 0 and out["fast"][2].abs().max() > 0
