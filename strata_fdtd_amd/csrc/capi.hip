@@ -18,8 +18,8 @@ int fused_grid(FusedArgs& a, int rows, int planes, int stack);
 hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool ade, int blocks,
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
-hipError_t launch_batch(const FusedArgs* table, const int* offsets, int n, int total_waves, int rows,
-                        hipStream_t st);
+hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
+                        int cap_waves, int rows, hipStream_t st);
 hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
                           const float*, const float*, hipStream_t);
 hipError_t phase_rigid(float*, float*, float*, const uint8_t*, int, int, int, hipStream_t);
@@ -616,15 +616,19 @@ struct fdtd_batch {
     int device = 0;
     hipStream_t stream = nullptr;
     int rows = 4;
-    FusedArgs* table_dev = nullptr;
-    int* offsets_dev = nullptr;
-    FusedArgs* table_host = nullptr;   // pinned staging
-    int* offsets_host = nullptr;
-    int cap = 0;
+    int cap = 0;                        // solves the buffers hold
+    FusedArgs* table_dev = nullptr;     // [cap]
+    int* meta_dev = nullptr;            // [cap + 2]: n, then offsets[0..n]
+    FusedArgs* table_host[2] = {};      // pinned staging, used alternately
+    int* meta_host[2] = {};
+    hipEvent_t copied[2] = {};          // the staging buffer's last upload has run
+    bool pending[2] = {};
+    int stage = 0;
     int n = 0;
     int total_waves = 0;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    int graph_waves = 0;                // wave capacity of the captured grid
 };
 
 static void batch_drop_graph(fdtd_batch* b) {
@@ -632,16 +636,38 @@ static void batch_drop_graph(fdtd_batch* b) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
         b->graph_steps = 0;
+        b->graph_waves = 0;
     }
+}
+
+static void batch_free(fdtd_batch* b) {
+    if (b->table_dev) (void)hipFree(b->table_dev);
+    if (b->meta_dev) (void)hipFree(b->meta_dev);
+    for (int i = 0; i < 2; ++i) {
+        if (b->table_host[i]) (void)hipHostFree(b->table_host[i]);
+        if (b->meta_host[i]) (void)hipHostFree(b->meta_host[i]);
+        b->table_host[i] = nullptr;
+        b->meta_host[i] = nullptr;
+        b->pending[i] = false;
+    }
+    b->table_dev = nullptr;
+    b->meta_dev = nullptr;
+    b->cap = 0;
 }
 
 int fdtd_batch_create(int device, void* stream, int rows, fdtd_batch** out) {
     if (!out) return fail(-1, "null out");
     if (rows != 2 && rows != 4 && rows != 8) return fail(-2, "batch rows must be 2, 4 or 8");
+    DeviceGuard g(device);
     auto* b = new fdtd_batch();
     b->device = device;
     b->stream = static_cast<hipStream_t>(stream);
     b->rows = rows;
+    for (int i = 0; i < 2; ++i)
+        if (int rc = check(hipEventCreateWithFlags(&b->copied[i], hipEventDisableTiming), "hipEventCreate")) {
+            delete b;
+            return rc;
+        }
     *out = b;
     return 0;
 }
@@ -652,10 +678,9 @@ int fdtd_batch_destroy(fdtd_batch* b) {
     batch_drop_graph(b);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     else (void)hipDeviceSynchronize();
-    if (b->table_dev) (void)hipFree(b->table_dev);
-    if (b->offsets_dev) (void)hipFree(b->offsets_dev);
-    if (b->table_host) (void)hipHostFree(b->table_host);
-    if (b->offsets_host) (void)hipHostFree(b->offsets_host);
+    batch_free(b);
+    for (int i = 0; i < 2; ++i)
+        if (b->copied[i]) (void)hipEventDestroy(b->copied[i]);
     delete b;
     return 0;
 }
@@ -664,51 +689,65 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
     if (!b) return fail(-1, "null batch");
     if (n < 0 || (n > 0 && (!ctxs || !step_limits))) return fail(-2, "bad batch arrays");
     DeviceGuard g(b->device);
-    // the staging buffers may still feed an in-flight copy: wait for the stream
-    if (int rc = check(b->stream ? hipStreamSynchronize(b->stream) : hipDeviceSynchronize(), "sync")) return rc;
-    batch_drop_graph(b);
-    if (n > b->cap) {
-        const int cap = std::max(n, 2 * b->cap);
-        if (b->table_dev) (void)hipFree(b->table_dev);
-        if (b->offsets_dev) (void)hipFree(b->offsets_dev);
-        if (b->table_host) (void)hipHostFree(b->table_host);
-        if (b->offsets_host) (void)hipHostFree(b->offsets_host);
-        b->table_dev = nullptr; b->offsets_dev = nullptr; b->table_host = nullptr; b->offsets_host = nullptr;
-        b->cap = 0;
-        if (int rc = check(hipMalloc(&b->table_dev, sizeof(FusedArgs) * cap), "hipMalloc")) return rc;
-        if (int rc = check(hipMalloc(&b->offsets_dev, sizeof(int) * (cap + 1)), "hipMalloc")) return rc;
-        if (int rc = check(hipHostMalloc(&b->table_host, sizeof(FusedArgs) * cap, 0), "hipHostMalloc")) return rc;
-        if (int rc = check(hipHostMalloc(&b->offsets_host, sizeof(int) * (cap + 1), 0), "hipHostMalloc")) return rc;
-        b->cap = cap;
-    }
-    int total = 0;
-    b->offsets_host[0] = 0;
     for (int q = 0; q < n; ++q) {
         fdtd_ctx* c = ctxs[q];
         if (int rc = ready(c)) return rc;
         if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0)
             return fail(-2, "batch entry %d: nonuniform, ADE or microphone solves run on their own context", q);
         if (c->device != b->device) return fail(-2, "batch entry %d is on another device", q);
-        FusedArgs a = c->a;
-        fused_grid(a, b->rows, c->planes, 1);
+    }
+    if (n > b->cap || !b->meta_dev) {
+        // growth (rare): no queued work may still read the old buffers
+        if (int rc = check(b->stream ? hipStreamSynchronize(b->stream) : hipDeviceSynchronize(), "sync")) return rc;
+        batch_drop_graph(b);
+        const int cap = std::max({n, 2 * b->cap, 16});
+        batch_free(b);
+        if (int rc = check(hipMalloc(&b->table_dev, sizeof(FusedArgs) * cap), "hipMalloc")) return rc;
+        if (int rc = check(hipMalloc(&b->meta_dev, sizeof(int) * (cap + 2)), "hipMalloc")) return rc;
+        for (int i = 0; i < 2; ++i) {
+            if (int rc = check(hipHostMalloc(&b->table_host[i], sizeof(FusedArgs) * cap, 0), "hipHostMalloc"))
+                return rc;
+            if (int rc = check(hipHostMalloc(&b->meta_host[i], sizeof(int) * (cap + 2), 0), "hipHostMalloc"))
+                return rc;
+        }
+        b->cap = cap;
+    }
+    // staging buffer i may still feed an upload queued two sets ago
+    const int i = b->stage;
+    if (b->pending[i])
+        if (int rc = check(hipEventSynchronize(b->copied[i]), "staging wait")) return rc;
+    FusedArgs* tab = b->table_host[i];
+    int* meta = b->meta_host[i];
+    int total = 0;
+    meta[0] = n;
+    meta[1] = 0;
+    for (int q = 0; q < n; ++q) {
+        FusedArgs a = ctxs[q]->a;
+        fused_grid(a, b->rows, ctxs[q]->planes, 1);
         a.koff = 0;
         a.step_limit = step_limits[q];
-        b->table_host[q] = a;
+        tab[q] = a;
         total += a.nwaves;
-        b->offsets_host[q + 1] = total;
+        meta[q + 2] = total;
     }
+    if (n > 0)
+        if (int rc = check(hipMemcpyAsync(b->table_dev, tab, sizeof(FusedArgs) * n, hipMemcpyHostToDevice, b->stream),
+                           "table upload"))
+            return rc;
+    if (int rc = check(hipMemcpyAsync(b->meta_dev, meta, sizeof(int) * (n + 2), hipMemcpyHostToDevice, b->stream),
+                       "offsets upload"))
+        return rc;
+    if (int rc = check(hipEventRecord(b->copied[i], b->stream), "event record")) return rc;
+    b->pending[i] = true;
+    b->stage ^= 1;
     b->n = n;
     b->total_waves = total;
-    if (n == 0) return 0;
-    if (int rc = check(hipMemcpyAsync(b->table_dev, b->table_host, sizeof(FusedArgs) * n,
-                                      hipMemcpyHostToDevice, b->stream), "table upload")) return rc;
-    return check(hipMemcpyAsync(b->offsets_dev, b->offsets_host, sizeof(int) * (n + 1),
-                                hipMemcpyHostToDevice, b->stream), "offsets upload");
+    return 0;
 }
 
-static int batch_enqueue(fdtd_batch* b, hipStream_t st, int n_steps) {
+static int batch_enqueue(fdtd_batch* b, hipStream_t st, int n_steps, int cap_waves) {
     for (int s = 0; s < n_steps; ++s)
-        if (int rc = check(launch_batch(b->table_dev, b->offsets_dev, b->n, b->total_waves, b->rows, st),
+        if (int rc = check(launch_batch(b->table_dev, b->meta_dev + 1, b->meta_dev, b->cap, cap_waves, b->rows, st),
                            "batch step launch"))
             return rc;
     return 0;
@@ -719,8 +758,10 @@ int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph) {
     if (n_steps < 0) return fail(-2, "negative step count");
     if (b->n == 0 || n_steps == 0) return 0;
     DeviceGuard g(b->device);
-    if (!use_graph) return batch_enqueue(b, b->stream, n_steps);
-    if (!b->graph || b->graph_steps != n_steps) {
+    if (!use_graph) return batch_enqueue(b, b->stream, n_steps, b->total_waves);
+    if (!b->graph || b->graph_steps != n_steps || b->graph_waves < b->total_waves) {
+        // capture with head room so that later tables of up to this many waves replay it
+        const int cap_waves = std::max(b->total_waves + b->total_waves / 2, b->graph_waves * 2);
         batch_drop_graph(b);
         hipStream_t cap = nullptr;
         if (int rc = check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate")) return rc;
@@ -729,7 +770,7 @@ int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph) {
             (void)hipStreamDestroy(cap);
             return rc;
         }
-        int rc = batch_enqueue(b, cap, n_steps);
+        int rc = batch_enqueue(b, cap, n_steps, cap_waves);
         hipError_t e = hipStreamEndCapture(cap, &gr);
         (void)hipStreamDestroy(cap);
         if (rc) { if (gr) (void)hipGraphDestroy(gr); return rc; }
@@ -738,6 +779,7 @@ int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph) {
         (void)hipGraphDestroy(gr);
         if (int r3 = check(e, "graph instantiate")) return r3;
         b->graph_steps = n_steps;
+        b->graph_waves = cap_waves;
     }
     return check(hipGraphLaunch(b->graph, b->stream), "graph launch");
 }

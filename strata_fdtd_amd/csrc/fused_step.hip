@@ -37,6 +37,17 @@
 
 namespace fdtd_amd {
 
+#define GMEM __attribute__((address_space(1)))
+
+template <typename T>
+__device__ __forceinline__ GMEM T* gmem(T* p) {
+    return (GMEM T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const GMEM T* gmem(const T* p) {
+    return (const GMEM T*)p;
+}
+
 __device__ __forceinline__ float shfl_down1(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWave); }
 
@@ -63,18 +74,30 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
 
     const int64_t s = *a.step_ptr + a.koff;
     const int par = static_cast<int>(s & 1);
-    const float* __restrict__ P = par ? a.f[1][0] : a.f[0][0];
-    const float* __restrict__ VX = par ? a.f[1][1] : a.f[0][1];
-    const float* __restrict__ VY = par ? a.f[1][2] : a.f[0][2];
-    const float* __restrict__ VZ = par ? a.f[1][3] : a.f[0][3];
-    float* __restrict__ Po = par ? a.f[0][0] : a.f[1][0];
-    float* __restrict__ VXo = par ? a.f[0][1] : a.f[1][1];
-    float* __restrict__ VYo = par ? a.f[0][2] : a.f[1][2];
-    float* __restrict__ VZo = par ? a.f[0][3] : a.f[1][3];
-    const uint8_t* __restrict__ FL = a.flags;
-    const float* __restrict__ dX = a.dec[0];
-    const float* __restrict__ dY = a.dec[1];
-    const float* __restrict__ dZ = a.dec[2];
+    // Every buffer is device (global) memory. Saying so lets the compiler use
+    // global loads with a scalar base even when the arguments come from a
+    // batch table in memory (otherwise it must assume flat addresses).
+    const GMEM float* __restrict__ P = gmem(par ? a.f[1][0] : a.f[0][0]);
+    const GMEM float* __restrict__ VX = gmem(par ? a.f[1][1] : a.f[0][1]);
+    const GMEM float* __restrict__ VY = gmem(par ? a.f[1][2] : a.f[0][2]);
+    const GMEM float* __restrict__ VZ = gmem(par ? a.f[1][3] : a.f[0][3]);
+    GMEM float* __restrict__ Po = gmem(par ? a.f[0][0] : a.f[1][0]);
+    GMEM float* __restrict__ VXo = gmem(par ? a.f[0][1] : a.f[1][1]);
+    GMEM float* __restrict__ VYo = gmem(par ? a.f[0][2] : a.f[1][2]);
+    GMEM float* __restrict__ VZo = gmem(par ? a.f[0][3] : a.f[1][3]);
+    const GMEM uint8_t* __restrict__ FL = gmem(a.flags);
+    const GMEM float* __restrict__ dX = gmem(a.dec[0]);
+    const GMEM float* __restrict__ dY = gmem(a.dec[1]);
+    const GMEM float* __restrict__ dZ = gmem(a.dec[2]);
+    const GMEM float* __restrict__ ifc0 = gmem(a.ifc[0]);
+    const GMEM float* __restrict__ ifc1 = gmem(a.ifc[1]);
+    const GMEM float* __restrict__ ifc2 = gmem(a.ifc[2]);
+    const GMEM float* __restrict__ icl0 = gmem(a.icl[0]);
+    const GMEM float* __restrict__ icl1 = gmem(a.icl[1]);
+    const GMEM float* __restrict__ icl2 = gmem(a.icl[2]);
+    const GMEM double* __restrict__ WAVE = gmem(a.wave);
+    const GMEM int32_t* __restrict__ PIJK = gmem(a.probe_ijk);
+    GMEM float* __restrict__ TRACE = gmem(a.trace);
 
     // Which sources / probes can fall inside this strip (scalar work, once).
     uint32_t src_mask = 0;
@@ -92,7 +115,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     }
     uint64_t probe_mask = 0;
     for (int q = 0; q < a.n_probe; ++q) {
-        const int pi = a.probe_ijk[3 * q], pj = a.probe_ijk[3 * q + 1], pk = a.probe_ijk[3 * q + 2];
+        const int pi = PIJK[3 * q], pj = PIJK[3 * q + 1], pk = PIJK[3 * q + 2];
         if (pi >= i0 && pi < i1 && pj >= j0 && pj < j0 + R && pk >= k0 && pk < k0 + kWave)
             probe_mask |= 1ull << q;
     }
@@ -103,8 +126,8 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     }
 
     const float cv = a.cv, cp = a.cp;
-    const float ifz_k = (NU && kv && k < nz - 1) ? a.ifc[2][k] : 1.0f;
-    const float icz_k = (NU && kv) ? a.icl[2][k] : 1.0f;
+    const float ifz_k = (NU && kv && k < nz - 1) ? ifc2[k] : 1.0f;
+    const float icz_k = (NU && kv) ? icl2[k] : 1.0f;
     const float cvz = NU ? cv * ifz_k : cv;
     const float dz_k = kv ? dZ[k] : 1.0f;
 
@@ -122,7 +145,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     }
     if (i0 > 0) {
         const int im = i0 - 1;
-        const float cvx = NU ? cv * a.ifc[0][im] : cv;
+        const float cvx = NU ? cv * ifc0[im] : cv;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!(rv[r] && kv)) continue;
@@ -138,8 +161,8 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     for (int i = i0; i < i1; ++i) {
         const bool has_next = (i + 1) < nx;
         const float dx_i = dX[i];
-        const float cvx = NU ? cv * (has_next ? a.ifc[0][i] : 1.f) : cv;
-        const float icx_i = NU ? a.icl[0][i] : 1.f;
+        const float cvx = NU ? cv * (has_next ? ifc0[i] : 1.f) : cv;
+        const float icx_i = NU ? icl0[i] : 1.f;
 
         float vxo[R], vyo[R], vzo[R];
         uint32_t fl[R];
@@ -196,7 +219,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
         // vy' on the face (j0-1 | j0)
         float vyprev = 0.f;
         if (j0 > 0) {
-            const float cvy = NU ? cv * a.ifc[1][j0 - 1] : cv;
+            const float cvy = NU ? cv * ifc1[j0 - 1] : cv;
             const float v = vy_jm + cvy * (pc[0] - p_jm);
             vyprev = (f_jm & FDTD_FLAG_OPEN_Y) ? v : 0.f;
         }
@@ -219,7 +242,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             // y face (j | j+1)
             float vyn = vyo[r];
             if (jr + 1 < ny) {
-                const float cvy = NU ? cv * a.ifc[1][jr] : cv;
+                const float cvy = NU ? cv * ifc1[jr] : cv;
                 const float v = vyo[r] + cvy * (p_up - pr);
                 vyn = (f & FDTD_FLAG_OPEN_Y) ? v : 0.f;
             }
@@ -236,7 +259,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             if (lane == 0) {
                 vzm = 0.f;
                 if (k0 > 0) {
-                    const float cz = NU ? cv * a.ifc[2][k0 - 1] : cv;
+                    const float cz = NU ? cv * ifc2[k0 - 1] : cv;
                     const float v = hl_vz[r] + cz * (pr - hl_p[r]);
                     vzm = (hl_f[r] & FDTD_FLAG_OPEN_Z) ? v : 0.f;
                 }
@@ -248,7 +271,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             const float dvz = vzn - vzm;
             float pnew;
             if (NU) {
-                const float icy = a.icl[1][jr];
+                const float icy = icl1[jr];
                 const float ddx = icx_i * dvx, ddy = icy * dvy, ddz = icz_k * dvz;
                 pnew = pr + cp * ((ddx + ddy) + ddz);
             } else {
@@ -269,7 +292,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
                         const int ax = a.src_i[q], id = a.src_j[q];
                         on = ax == 0 ? id == i : (ax == 1 ? id == jr : id == k);
                     }
-                    if (on) ps = static_cast<float>(static_cast<double>(ps) + a.wave[q * a.wave_len + s]);
+                    if (on) ps = static_cast<float>(static_cast<double>(ps) + WAVE[q * a.wave_len + s]);
                 }
             }
             if (kv) {
@@ -281,9 +304,9 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             if (probe_mask && trace_ok) {
                 for (int q = 0; q < a.n_probe; ++q) {
                     if (!((probe_mask >> q) & 1ull)) continue;
-                    if (a.probe_ijk[3 * q] == i && a.probe_ijk[3 * q + 1] == jr &&
-                        a.probe_ijk[3 * q + 2] == k) {
-                        a.trace[s * a.n_probe + q] = ps;
+                    if (PIJK[3 * q] == i && PIJK[3 * q + 1] == jr &&
+                        PIJK[3 * q + 2] == k) {
+                        TRACE[s * a.n_probe + q] = ps;
                         if (!isfinite(ps)) atomicOr(a.health, 1u);
                     }
                 }
@@ -334,13 +357,17 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
 // Batched step: one launch advances every solve in `table` by one step. Waves
 // are dealt out by solve through the prefix sums `offsets` (n + 1 entries);
 // a solve whose counter reached its limit is left untouched.
+// The number of solves is read from device memory (n_ptr), so a captured
+// graph stays valid while the host rewrites the table between chunks; the
+// grid is sized for the table's wave capacity and surplus waves return.
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_batch_step(const FusedArgs* __restrict__ table,
-                                                       const int* __restrict__ offsets, int n,
-                                                       int blocks_per_xcd) {
+                                                       const int* __restrict__ offsets,
+                                                       const int* __restrict__ n_ptr, int blocks_per_xcd) {
     const int lane = threadIdx.x & (kWave - 1);
     const int lb = (blockIdx.x & 7) * blocks_per_xcd + (blockIdx.x >> 3);
     const int gw = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int n = *n_ptr;
     if (gw >= offsets[n]) return;
     int lo = 0, hi = n;  // offsets[lo] <= gw < offsets[hi]
     while (hi - lo > 1) {
@@ -355,9 +382,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const FusedArgs* __restri
     fused_strip<R, false, 1>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
-__global__ void k_batch_advance(const FusedArgs* __restrict__ table, int n) {
+__global__ void k_batch_advance(const FusedArgs* __restrict__ table, const int* __restrict__ n_ptr) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    if (e >= *n_ptr) return;
     int64_t* step = const_cast<int64_t*>(table[e].step_ptr);
     if (step[0] < table[e].step_limit) step[0] = step[0] + 1;
 }
@@ -652,20 +679,22 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
     return e;
 }
 
-hipError_t launch_batch(const FusedArgs* table, const int* offsets, int n, int total_waves, int rows,
-                        hipStream_t st) {
-    int blocks = (total_waves + kBlockWaves - 1) / kBlockWaves;
+// One batched step over a table of up to `cap_solves` solves holding at most
+// `cap_waves` waves (the grid covers the capacity; n is read on the device).
+hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
+                        int cap_waves, int rows, hipStream_t st) {
+    int blocks = (cap_waves + kBlockWaves - 1) / kBlockWaves;
     blocks = std::max(8, (blocks + 7) & ~7);
     const int bpx = blocks / 8;
     switch (rows) {
-        case 2: hipLaunchKernelGGL(k_batch_step<2>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n, bpx); break;
-        case 4: hipLaunchKernelGGL(k_batch_step<4>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n, bpx); break;
-        case 8: hipLaunchKernelGGL(k_batch_step<8>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n, bpx); break;
+        case 2: hipLaunchKernelGGL(k_batch_step<2>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, bpx); break;
+        case 4: hipLaunchKernelGGL(k_batch_step<4>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, bpx); break;
+        case 8: hipLaunchKernelGGL(k_batch_step<8>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, bpx); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_batch_advance, dim3((n + 63) / 64), dim3(64), 0, st, table, n);
+    hipLaunchKernelGGL(k_batch_advance, dim3((cap_solves + 63) / 64), dim3(64), 0, st, table, n_ptr);
     return hipGetLastError();
 }
 

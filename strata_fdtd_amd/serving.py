@@ -129,9 +129,10 @@ def sweep_requests(n: int = 64, rate: float = 40.0, seed: int = 1234, steps=(200
 @dataclass
 class _Slot:
     req: SolveRequest
-    solver: FDTDSolver
+    solver: FDTDSolver | None
     admitted: float
     done: int = 0
+    host_trace: object = None
 
 
 @dataclass
@@ -193,78 +194,113 @@ class SweepServer:
 
     def serve(self, requests: list[SolveRequest], realtime: bool = True) -> ServeStats:
         """Serve a stream; arrival times are honoured against the wall clock
-        when ``realtime`` (otherwise every request is available at t=0)."""
+        when ``realtime`` (otherwise every request is available at t=0).
+
+        The loop never waits for the chunk it has just launched: up to two
+        chunks are queued on the stream; table updates, trace copies of
+        finished solves (into pinned memory) and the next chunk are enqueued
+        behind the running one, and new scenes are built on a second stream
+        meanwhile. A solve that finishes is dropped from the table for the next
+        chunk and freed once its trace copy has completed."""
+        import collections
+
         import torch
         pending = sorted(requests, key=lambda r: r.arrival)
         stats = ServeStats()
         active: list[_Slot] = []
+        retiring: collections.deque = collections.deque()   # (event, [slots])
+        inflight: collections.deque = collections.deque()   # chunk events
         used = 0
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
-        built: dict[int, FDTDSolver] = {}
+        dev = torch.device(f"cuda:{self.device}")
+        build_stream = torch.cuda.Stream(dev)
+        built: dict[int, tuple] = {}
+
+        def build(r: SolveRequest):
+            with torch.cuda.stream(build_stream):
+                s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1], 1))
+                s._prepare(r.steps)
+            ev = torch.cuda.Event()
+            ev.record(build_stream)
+            return s, ev
 
         def prebuild(limit_time: float) -> None:
-            # host-side scene set-up for requests that have arrived, while the GPU runs
             for r in pending:
                 if r.rid in built:
                     continue
-                if realtime and r.arrival > limit_time:
+                if (realtime and r.arrival > limit_time) or len(built) >= 4:
                     break
-                if len(built) >= 4:
-                    break
-                s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1], 1))
-                s._prepare(r.steps)
-                built[r.rid] = s
+                built[r.rid] = build(r)
 
-        while pending or active:
-            changed = False
+        def harvest(block: bool) -> None:
+            while retiring and (block or retiring[0][0].query()):
+                ev, slots = retiring.popleft()
+                ev.synchronize()
+                t = now()
+                for sl in slots:
+                    sv = sl.solver
+                    sv._advance_host(sl.req.steps)
+                    host = sl.host_trace
+                    traces = {name: np.ascontiguousarray(host[:, q].numpy(), dtype=np.float32)
+                              for q, name in enumerate(sv._probe_order)}
+                    stats.results.append(SolveResult(sl.req.rid, traces, sl.req.arrival, sl.admitted, t))
+                    sl.solver = None
+                    sl.host_trace = None
+                block = False
+
+        while pending or active or retiring:
             tnow = now()
-            # admission: arrived requests, slots and HBM permitting
+            changed = False
             while pending and (not realtime or pending[0].arrival <= tnow):
                 r = pending[0]
                 if len(active) >= self.max_active or (active and used + r.footprint > self.hbm_budget):
                     break
                 pending.pop(0)
-                s = built.pop(r.rid, None)
-                if s is None:
-                    s = build_solver(r, self.device, tiling=(self.rows, pick_tiling(r.shape)[1], 1))
-                    s._prepare(r.steps)
+                s, ev = built.pop(r.rid, None) or build(r)
+                self._stream.wait_event(ev)
                 active.append(_Slot(r, s, now()))
                 used += r.footprint
                 changed = True
-            if not active:
-                # idle until the next arrival (build it meanwhile)
-                if pending and realtime:
+            if changed:
+                self._set_batch(active)
+            if active:
+                stats.max_active = max(stats.max_active, len(active))
+                _native.call("fdtd_batch_step_n", self._batch, self.chunk, 1 if self.use_graph else 0)
+                stats.chunks += 1
+                finished = []
+                for sl in active:
+                    sl.done += min(self.chunk, sl.req.steps - sl.done)
+                    if sl.done >= sl.req.steps:
+                        finished.append(sl)
+                if finished:
+                    for sl in finished:
+                        tr = sl.solver._trace[:sl.req.steps]
+                        sl.host_trace = torch.empty(tr.shape, dtype=tr.dtype, pin_memory=True)
+                        sl.host_trace.copy_(tr, non_blocking=True)
+                        active.remove(sl)
+                        used -= sl.req.footprint
+                    ev = torch.cuda.Event()
+                    ev.record(self._stream)
+                    retiring.append((ev, finished))
+                    self._set_batch(active)
+                cev = torch.cuda.Event()
+                cev.record(self._stream)
+                inflight.append(cev)
+                prebuild(now() + 0.05)
+                while len(inflight) > 2:
+                    inflight.popleft().synchronize()
+                harvest(False)
+            else:
+                harvest(bool(retiring) and not pending)
+                if pending and realtime and not retiring:
                     prebuild(pending[0].arrival)
                     wait = pending[0].arrival - now()
                     if wait > 0:
-                        time.sleep(wait)
-                continue
-            if changed:
-                self._set_batch(active)
-            stats.max_active = max(stats.max_active, len(active))
-            _native.call("fdtd_batch_step_n", self._batch, self.chunk, 1 if self.use_graph else 0)
-            stats.chunks += 1
-            prebuild(now() + 0.05)          # overlaps the chunk on the GPU
-            torch.cuda.synchronize(self.device)
-            finished = []
-            for sl in active:
-                step = min(self.chunk, sl.req.steps - sl.done)
-                sl.done += step
-                if sl.done >= sl.req.steps:
-                    finished.append(sl)
-            if finished:
-                for sl in finished:
-                    sl.solver._advance_host(sl.req.steps)
-                    traces = sl.solver.get_probe_data()
-                    stats.results.append(SolveResult(sl.req.rid, traces, sl.req.arrival, sl.admitted, now()))
-                    used -= sl.req.footprint
-                    active.remove(sl)
-                    del sl.solver
-                self._set_batch(active)
+                        time.sleep(min(wait, 0.01))
+        torch.cuda.synchronize(self.device)
         stats.wall = now()
-        for s in built.values():
-            del s
+        built.clear()
         return stats
 
 

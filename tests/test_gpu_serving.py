@@ -18,12 +18,12 @@ def _small_requests(n=6, seed=7):
     return reqs
 
 
-@pytest.mark.parametrize("chunk,use_graph", [(16, True), (7, False)])
-def test_served_results_equal_solo_runs(chunk, use_graph):
+@pytest.mark.parametrize("chunk,use_graph,rows", [(16, True, 4), (7, False, 4), (16, True, 2)])
+def test_served_results_equal_solo_runs(chunk, use_graph, rows):
     from strata_fdtd_amd.serving import SweepServer, build_solver
     reqs = _small_requests()
-    print(f"[serving] {len(reqs)} requests, chunk={chunk}, graph={use_graph}", flush=True)
-    srv = SweepServer(chunk=chunk, use_graph=use_graph)
+    print(f"[serving] {len(reqs)} requests, chunk={chunk}, graph={use_graph}, rows={rows}", flush=True)
+    srv = SweepServer(chunk=chunk, use_graph=use_graph, rows=rows)
     stats = srv.serve(reqs, realtime=False)
     srv.close()
     assert len(stats.results) == len(reqs)
