@@ -24,6 +24,8 @@
 
 namespace fdtd_amd {
 
+constexpr int kDesignMaxProbes = 8;
+
 struct DesignArgs {
     // fields: ping-pong sets {p, vx, vy, vz}; step s reads set s & 1
     float* f[2][4];
@@ -39,7 +41,7 @@ struct DesignArgs {
     int src_i, src_j, src_k;
     const double* wave;              // device [steps]
     int n_probe;
-    int probe[FDTD_MAX_PROBES][3];
+    int probe[8][3];                 // at most kDesignMaxProbes
     float* trace;                    // device [steps][n_probe] (forward)
     const float* g;                  // device [steps][n_probe] (adjoint: dL/dy)
     // history: p on [lo-1, hi+1), div on [lo, hi); slot of step s = s - hist0
@@ -96,6 +98,31 @@ __device__ __forceinline__ float cvf(const DesignArgs& a, float r_lo, float r_hi
     return a.cv0 / ((r_lo + r_hi) * 0.5f);
 }
 
+// Does the region [i0-1, i1] x [j0-1, j1] x [k0-1, k1] (a strip and its
+// halo) meet the design box? Outside the box r = kappa = 1, so the face
+// coefficient cv0 / ((1 + 1) * 0.5) is exactly cv0 and cp0 * 1 is exactly cp0:
+// a strip that misses the box takes those constants without looking anything up.
+__device__ __forceinline__ bool strip_meets_box(const DesignArgs& a, int i0, int i1, int j0, int j1, int k0,
+                                                int k1) {
+    return i0 - 1 < a.hi[0] && i1 >= a.lo[0] && j0 - 1 < a.hi[1] && j1 >= a.lo[1] && k0 - 1 < a.hi[2] &&
+           k1 >= a.lo[2];
+}
+
+__device__ __forceinline__ bool strip_has_probe(const DesignArgs& a, int i0, int i1, int j0, int j1, int k0,
+                                                int k1) {
+    for (int q = 0; q < a.n_probe; ++q)
+        if (a.probe[q][0] >= i0 - 1 && a.probe[q][0] <= i1 && a.probe[q][1] >= j0 - 1 && a.probe[q][1] <= j1 &&
+            a.probe[q][2] >= k0 - 1 && a.probe[q][2] <= k1)
+            return true;
+    return false;
+}
+
+#define GMEM __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GMEM T* gm(T* p) { return (GMEM T*)p; }
+template <typename T>
+__device__ __forceinline__ const GMEM T* gm(const T* p) { return (const GMEM T*)p; }
+
 __device__ __forceinline__ float sdown(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float sup(float v) { return __shfl_up(v, 1, kWave); }
 
@@ -113,20 +140,24 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
     const int i0 = (t / a.nstrip_j) * a.planes, i1 = min(i0 + a.planes, nx);
     const int64_t sx = static_cast<int64_t>(ny) * nz;
     const int par = static_cast<int>(s & 1);
-    const float* __restrict__ P = a.f[par][0];
-    const float* __restrict__ VX = a.f[par][1];
-    const float* __restrict__ VY = a.f[par][2];
-    const float* __restrict__ VZ = a.f[par][3];
-    float* __restrict__ Po = a.f[par ^ 1][0];
-    float* __restrict__ VXo = a.f[par ^ 1][1];
-    float* __restrict__ VYo = a.f[par ^ 1][2];
-    float* __restrict__ VZo = a.f[par ^ 1][3];
-    const uint8_t* __restrict__ FL = a.flags;
+    const GMEM float* __restrict__ P = gm(a.f[par][0]);
+    const GMEM float* __restrict__ VX = gm(a.f[par][1]);
+    const GMEM float* __restrict__ VY = gm(a.f[par][2]);
+    const GMEM float* __restrict__ VZ = gm(a.f[par][3]);
+    GMEM float* __restrict__ Po = gm(a.f[par ^ 1][0]);
+    GMEM float* __restrict__ VXo = gm(a.f[par ^ 1][1]);
+    GMEM float* __restrict__ VYo = gm(a.f[par ^ 1][2]);
+    GMEM float* __restrict__ VZo = gm(a.f[par ^ 1][3]);
+    const GMEM uint8_t* __restrict__ FL = gm(a.flags);
     const bool hist = a.hist_p && s >= a.hist0 && s < a.hist0 + a.hist_steps;
     float* hp = hist ? a.hist_p + (s - a.hist0) * ebox_cells(a) : nullptr;
     float* hd = hist ? a.hist_div + (s - a.hist0) * dbox_cells(a) : nullptr;
     const double wsrc = a.wave ? a.wave[s] : 0.0;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
+    const bool tb = strip_meets_box(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
+    auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
+    auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
+    auto KP = [&](int ii, int jj, int kk) { return tb ? kv_(a, ii, jj, kk) : 1.0f; };
 
     bool rvld[R];
     float pc[R], vxm[R];
@@ -137,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
         vxm[r] = 0.f;
         if (i0 > 0 && rvld[r] && kin) {
             const int64_t c = lin(i0 - 1, j0 + r, k, ny, nz);
-            const float v = VX[c] + cvf(a, rv(a, i0 - 1, j0 + r, k), rv(a, i0, j0 + r, k)) * (pc[r] - P[c]);
+            const float v = VX[c] + CV(RV(i0 - 1, j0 + r, k), RV(i0, j0 + r, k)) * (pc[r] - P[c]);
             vxm[r] = (FL[c] & FDTD_FLAG_OPEN_X) ? v : 0.f;
         }
     }
@@ -167,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
         float vyprev = 0.f;
         if (kin && j0 > 0) {
             const int64_t c = lin(i, j0 - 1, k, ny, nz);
-            const float v = VY[c] + cvf(a, rv(a, i, j0 - 1, k), rv(a, i, j0, k)) * (pc[0] - P[c]);
+            const float v = VY[c] + CV(RV(i, j0 - 1, k), RV(i, j0, k)) * (pc[0] - P[c]);
             vyprev = (FL[c] & FDTD_FLAG_OPEN_Y) ? v : 0.f;
         }
 #pragma unroll
@@ -177,23 +208,23 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
             const int64_t c = lin(i, j, k, ny, nz);
             const uint32_t f = fl[r];
             const float pr = pc[r];
-            const float rc = rv(a, i, j, k);
+            const float rc = RV(i, j, k);
             float vxn = vxo[r];
             if (nxt) {
-                const float v = vxo[r] + cvf(a, rc, rv(a, i + 1, j, k)) * (pn[r] - pr);
+                const float v = vxo[r] + CV(rc, RV(i + 1, j, k)) * (pn[r] - pr);
                 vxn = (f & FDTD_FLAG_OPEN_X) ? v : 0.f;
             }
             float vyn = vyo[r];
             if (j + 1 < ny) {
                 const float pu = (r + 1 < R) ? pc[r + 1] : p_up_halo;
-                const float v = vyo[r] + cvf(a, rc, rv(a, i, j + 1, k)) * (pu - pr);
+                const float v = vyo[r] + CV(rc, RV(i, j + 1, k)) * (pu - pr);
                 vyn = (f & FDTD_FLAG_OPEN_Y) ? v : 0.f;
             }
             float pk1 = sdown(pr);
             if (lane == kWave - 1) pk1 = hr[r];
             float vzn = vzo[r];
             if (k + 1 < nz) {
-                const float v = vzo[r] + cvf(a, rc, rv(a, i, j, k + 1)) * (pk1 - pr);
+                const float v = vzo[r] + CV(rc, RV(i, j, k + 1)) * (pk1 - pr);
                 vzn = (f & FDTD_FLAG_OPEN_Z) ? v : 0.f;
             }
             float vzm = sup(vzn);
@@ -201,12 +232,12 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
                 vzm = 0.f;
                 if (k0 > 0 && kin) {
                     const int64_t cl = c - 1;
-                    const float v = VZ[cl] + cvf(a, rv(a, i, j, k - 1), rc) * (pr - P[cl]);
+                    const float v = VZ[cl] + CV(RV(i, j, k - 1), rc) * (pr - P[cl]);
                     vzm = (FL[cl] & FDTD_FLAG_OPEN_Z) ? v : 0.f;
                 }
             }
             const float div = ((vxn - vxm[r]) + (vyn - vyprev)) + (vzn - vzm);
-            float pnew = (f & FDTD_FLAG_AIR) ? pr + (a.cp0 * kv_(a, i, j, k)) * div : 0.f;
+            float pnew = (f & FDTD_FLAG_AIR) ? pr + (a.cp0 * KP(i, j, k)) * div : 0.f;
             const float dy_j = a.dec[1][j];
             pnew = ((pnew * dx_i) * dy_j) * dz_k;
             if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
@@ -219,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
                 for (int q = 0; q < a.n_probe; ++q)
                     if (a.probe[q][0] == i && a.probe[q][1] == j && a.probe[q][2] == k)
                         a.trace[s * a.n_probe + q] = pnew;
-                if (hist) {
+                if (hist && tb) {
                     if (ebox(a, i, j, k)) hp[ebox_lin(a, i, j, k)] = pr;
                     if (dbox(a, i, j, k)) hd[dbox_lin(a, i, j, k)] = div;
                 }
@@ -234,19 +265,21 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
 
 // ---------------------------------------------------------------- adjoint
 // pb = dX dY dZ (P + g at probe cells); q = air cp0 kappa pb.
-__device__ __forceinline__ float design_pb(const DesignArgs& a, const float* __restrict__ P, int i, int j, int k,
-                                           int64_t c, int64_t n) {
+// pm: the strip (with its halo) holds a probe; tb: it meets the design box.
+__device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
+                                           int64_t c, int64_t n, bool pm) {
     float v = P[c];
-    for (int q = 0; q < a.n_probe; ++q)
-        if (a.probe[q][0] == i && a.probe[q][1] == j && a.probe[q][2] == k) v += a.g[n * a.n_probe + q];
+    if (pm)
+        for (int q = 0; q < a.n_probe; ++q)
+            if (a.probe[q][0] == i && a.probe[q][1] == j && a.probe[q][2] == k) v += a.g[n * a.n_probe + q];
     const float lam = (a.dec[0][i] * a.dec[1][j]) * a.dec[2][k];
     return lam * v;
 }
 
-__device__ __forceinline__ float design_q(const DesignArgs& a, const float* __restrict__ P, int i, int j, int k,
-                                          int64_t c, int64_t n) {
+__device__ __forceinline__ float design_q(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
+                                          int64_t c, int64_t n, bool pm, bool tb) {
     if (!(a.flags[c] & FDTD_FLAG_AIR)) return 0.f;
-    return a.cp0 * kv_(a, i, j, k) * design_pb(a, P, i, j, k, c, n);
+    return a.cp0 * (tb ? kv_(a, i, j, k) : 1.0f) * design_pb(a, P, i, j, k, c, n, pm);
 }
 
 __device__ __forceinline__ void kahan(float* sum, float* comp, int64_t b, float x) {
@@ -270,19 +303,24 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
     const int i0 = (t / a.nstrip_j) * a.planes, i1 = min(i0 + a.planes, nx);
     const int64_t sx = static_cast<int64_t>(ny) * nz;
     const int src = static_cast<int>((n + 1) & 1);
-    const float* __restrict__ P = a.f[src][0];
-    const float* __restrict__ VX = a.f[src][1];
-    const float* __restrict__ VY = a.f[src][2];
-    const float* __restrict__ VZ = a.f[src][3];
-    float* __restrict__ Po = a.f[src ^ 1][0];
-    float* __restrict__ VXo = a.f[src ^ 1][1];
-    float* __restrict__ VYo = a.f[src ^ 1][2];
-    float* __restrict__ VZo = a.f[src ^ 1][3];
-    const uint8_t* __restrict__ FL = a.flags;
+    const GMEM float* __restrict__ P = gm(a.f[src][0]);
+    const GMEM float* __restrict__ VX = gm(a.f[src][1]);
+    const GMEM float* __restrict__ VY = gm(a.f[src][2]);
+    const GMEM float* __restrict__ VZ = gm(a.f[src][3]);
+    GMEM float* __restrict__ Po = gm(a.f[src ^ 1][0]);
+    GMEM float* __restrict__ VXo = gm(a.f[src ^ 1][1]);
+    GMEM float* __restrict__ VYo = gm(a.f[src ^ 1][2]);
+    GMEM float* __restrict__ VZo = gm(a.f[src ^ 1][3]);
+    const GMEM uint8_t* __restrict__ FL = gm(a.flags);
     const bool grad = a.gr_sum && n >= a.hist0 && n < a.hist0 + a.hist_steps;
     const float* hp = grad ? a.hist_p + (n - a.hist0) * ebox_cells(a) : nullptr;
     const float* hd = grad ? a.hist_div + (n - a.hist0) * dbox_cells(a) : nullptr;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
+    const bool tb = strip_meets_box(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
+    const bool pm = strip_has_probe(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
+    auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
+    auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
+    auto KP = [&](int ii, int jj, int kk) { return tb ? kv_(a, ii, jj, kk) : 1.0f; };
 
     bool rvld[R];
     float pbc[R], qc[R], wxm[R];
@@ -293,12 +331,12 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
         if (!(rvld[r] && kin)) continue;
         const int j = j0 + r;
         const int64_t c = lin(i0, j, k, ny, nz);
-        pbc[r] = design_pb(a, P, i0, j, k, c, n);
-        qc[r] = (FL[c] & FDTD_FLAG_AIR) ? a.cp0 * kv_(a, i0, j, k) * pbc[r] : 0.f;
+        pbc[r] = design_pb(a, P, i0, j, k, c, n, pm);
+        qc[r] = (FL[c] & FDTD_FLAG_AIR) ? a.cp0 * KP(i0, j, k) * pbc[r] : 0.f;
         if (i0 > 0) {
             const int64_t cl = c - sx;
             wxm[r] = (FL[cl] & FDTD_FLAG_OPEN_X)
-                         ? a.dec[0][i0 - 1] * VX[cl] + design_q(a, P, i0 - 1, j, k, cl, n) - qc[r]
+                         ? a.dec[0][i0 - 1] * VX[cl] + design_q(a, P, i0 - 1, j, k, cl, n, pm, tb) - qc[r]
                          : 0.f;
         }
     }
@@ -315,22 +353,22 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
             const int j = j0 + r;
             const int64_t c = lin(i, j, k, ny, nz);
             if (nxt) {
-                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n);
-                qn[r] = (FL[c + sx] & FDTD_FLAG_AIR) ? a.cp0 * kv_(a, i + 1, j, k) * pbn[r] : 0.f;
+                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm);
+                qn[r] = (FL[c + sx] & FDTD_FLAG_AIR) ? a.cp0 * KP(i + 1, j, k) * pbn[r] : 0.f;
             }
             vx[r] = VX[c];
             vy[r] = VY[c];
             vz[r] = VZ[c];
             fl[r] = FL[c];
-            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n);
+            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, tb);
         }
         float q_up_halo = 0.f;
-        if (kin && j0 + R < ny) q_up_halo = design_q(a, P, i, j0 + R, k, lin(i, j0 + R, k, ny, nz), n);
+        if (kin && j0 + R < ny) q_up_halo = design_q(a, P, i, j0 + R, k, lin(i, j0 + R, k, ny, nz), n, pm, tb);
         float wyprev = 0.f;
         if (kin && j0 > 0) {
             const int64_t cl = lin(i, j0 - 1, k, ny, nz);
             wyprev = (FL[cl] & FDTD_FLAG_OPEN_Y)
-                         ? a.dec[1][j0 - 1] * VY[cl] + design_q(a, P, i, j0 - 1, k, cl, n) - qc[0]
+                         ? a.dec[1][j0 - 1] * VY[cl] + design_q(a, P, i, j0 - 1, k, cl, n, pm, tb) - qc[0]
                          : 0.f;
         }
 #pragma unroll
@@ -339,18 +377,18 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
             const int j = j0 + r;
             const int64_t c = lin(i, j, k, ny, nz);
             const uint32_t f = fl[r];
-            const float rc = rv(a, i, j, k);
+            const float rc = RV(i, j, k);
             const bool air = f & FDTD_FLAG_AIR;
             float acc = air ? pbc[r] : 0.f;
             float gr = 0.f;
-            const bool gbox = grad && kin && dbox(a, i, j, k);
+            const bool gbox = grad && tb && kin && dbox(a, i, j, k);
             const float hpc = gbox ? hp[ebox_lin(a, i, j, k)] : 0.f;
             // own faces (c = lo)
             float wx, wy, wz;
             const float dy_j = a.dec[1][j];
             if (nxt) {
-                const float rh = rv(a, i + 1, j, k);
-                const float cv = cvf(a, rc, rh);
+                const float rh = RV(i + 1, j, k);
+                const float cv = CV(rc, rh);
                 wx = (f & FDTD_FLAG_OPEN_X) ? dx_i * vx[r] + qc[r] - qn[r] : 0.f;
                 acc -= cv * wx;
                 if (gbox) gr += wx * (hp[ebox_lin(a, i + 1, j, k)] - hpc) * (-cv / (rc + rh));
@@ -358,8 +396,8 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
                 wx = dx_i * vx[r] + qc[r];
             }
             if (j + 1 < ny) {
-                const float rh = rv(a, i, j + 1, k);
-                const float cv = cvf(a, rc, rh);
+                const float rh = RV(i, j + 1, k);
+                const float cv = CV(rc, rh);
                 const float qu = (r + 1 < R) ? qc[r + 1] : q_up_halo;
                 wy = (f & FDTD_FLAG_OPEN_Y) ? dy_j * vy[r] + qc[r] - qu : 0.f;
                 acc -= cv * wy;
@@ -370,8 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
             float qk1 = sdown(qc[r]);
             if (lane == kWave - 1) qk1 = qhr[r];
             if (k + 1 < nz) {
-                const float rh = rv(a, i, j, k + 1);
-                const float cv = cvf(a, rc, rh);
+                const float rh = RV(i, j, k + 1);
+                const float cv = CV(rc, rh);
                 wz = (f & FDTD_FLAG_OPEN_Z) ? dz_k * vz[r] + qc[r] - qk1 : 0.f;
                 acc -= cv * wz;
                 if (gbox) gr += wz * (hp[ebox_lin(a, i, j, k + 1)] - hpc) * (-cv / (rc + rh));
@@ -385,25 +423,25 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
                 if (k0 > 0 && kin) {
                     const int64_t cl = c - 1;
                     wzm = (FL[cl] & FDTD_FLAG_OPEN_Z)
-                              ? a.dec[2][k - 1] * VZ[cl] + design_q(a, P, i, j, k - 1, cl, n) - qc[r]
+                              ? a.dec[2][k - 1] * VZ[cl] + design_q(a, P, i, j, k - 1, cl, n, pm, tb) - qc[r]
                               : 0.f;
                 }
             }
             if (i > 0) {
-                const float rl = rv(a, i - 1, j, k);
-                const float cv = cvf(a, rl, rc);
+                const float rl = RV(i - 1, j, k);
+                const float cv = CV(rl, rc);
                 acc += cv * wxm[r];
                 if (gbox) gr += wxm[r] * (hpc - hp[ebox_lin(a, i - 1, j, k)]) * (-cv / (rl + rc));
             }
             if (j > 0) {
-                const float rl = rv(a, i, j - 1, k);
-                const float cv = cvf(a, rl, rc);
+                const float rl = RV(i, j - 1, k);
+                const float cv = CV(rl, rc);
                 acc += cv * wyprev;
                 if (gbox) gr += wyprev * (hpc - hp[ebox_lin(a, i, j - 1, k)]) * (-cv / (rl + rc));
             }
             if (k > 0) {
-                const float rl = rv(a, i, j, k - 1);
-                const float cv = cvf(a, rl, rc);
+                const float rl = RV(i, j, k - 1);
+                const float cv = CV(rl, rc);
                 acc += cv * wzm;
                 if (gbox) gr += wzm * (hpc - hp[ebox_lin(a, i, j, k - 1)]) * (-cv / (rl + rc));
             }
@@ -451,7 +489,7 @@ int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const
                 const float* kappa, const int32_t* src_ijk, const double* wave, int n_probe,
                 const int32_t* probe_ijk) {
     if (nx < 2 || ny < 2 || nz < 2 || !fields || !flags || !dec) return -2;
-    if (n_probe < 0 || n_probe > FDTD_MAX_PROBES || (n_probe > 0 && !probe_ijk)) return -2;
+    if (n_probe < 0 || n_probe > kDesignMaxProbes || (n_probe > 0 && !probe_ijk)) return -2;
     a = DesignArgs{};
     for (int s = 0; s < 2; ++s)
         for (int q = 0; q < 4; ++q) {
