@@ -94,6 +94,12 @@ def cpu_baseline(solver, seconds_budget: float = 20.0):
     if per_step <= 0:
         per_step = (t3 - t2) / n_long
     threads = cpu.num_threads(impl if impl == "oracle" else "ref")
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), model)
+    except OSError:
+        pass
     return {
         "value": round(cells / per_step / 1e9, 4),
         "unit": "Gcell-updates/s",
@@ -101,7 +107,7 @@ def cpu_baseline(solver, seconds_budget: float = 20.0):
         "kind": kind,
         "sample": f"{n_long} steps of the same {solver.shape[0]}x{solver.shape[1]}x{solver.shape[2]} "
                   f"workload (setup cost removed by differencing a {n_short}-step run), "
-                  f"OMP_NUM_THREADS={threads}",
+                  f"OMP_NUM_THREADS={threads}, {model}, g++ -O3 -march=x86-64-v2 -ffp-contract=off -fopenmp",
         "seconds_per_step": round(per_step, 4),
     }
 
