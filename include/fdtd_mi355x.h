@@ -209,7 +209,7 @@ int fdtd_adjoint_run(int dtype, void* const* lam, void* const* states, int64_t n
  * device pointers (two {p, vx, vy, vz} sets; step s reads set s & 1, the
  * adjoint step n reads set (n + 1) & 1). dec: host array of 3 device decay
  * tables. The design box must keep one cell to every grid face. src_ijk: host
- * int32[3] or NULL; wave: device float64 [steps]; probe_ijk: host int32
+ * int32[3] or NULL; wave: device float64 [steps]; probe_ijk: device int32
  * [n_probe][3]; trace and g: device float32 [steps][n_probe].
  * History (device float32): p on the box grown by one cell
  * ((hx+2)(hy+2)(hz+2) per step) and div on the box (hx hy hz per step) for

@@ -177,7 +177,8 @@ class _Fast:
         self.wave_dev = torch.from_numpy(np.ascontiguousarray(dv.wave, dtype=np.float64)).to(dv.dev)
         src = pb.source
         self.src = (C.c_int32 * 3)(*[int(v) for v in src.position]) if src is not None else None
-        self.probe_ijk = np.ascontiguousarray(np.array(pb.probes or [(0, 0, 0)], dtype=np.int32).reshape(-1, 3))
+        self.probe_ijk = torch.tensor(np.array(pb.probes or [(0, 0, 0)], dtype=np.int32).reshape(-1, 3),
+                                      device=dv.dev)
         self.dec = (_vp * 3)(*[_ptr(d) for d in dv.dec])
         lo, hi = np.array(pb.box_lo), np.array(pb.box_hi)
         self.ecells = int(np.prod(hi - lo + 2))
@@ -201,13 +202,13 @@ class _Fast:
         hp, hd = hist if hist is not None else (None, None)
         hist_steps = (hp.numel() // self.ecells) if hp is not None else 0
         _native.call("fdtd_design_forward", self._sets(sets), *self._common(r, kappa), self.src,
-                     _ptr(self.wave_dev), self.dv.n_probe, self.probe_ijk.ctypes.data, _ptr(trace), step0, n_steps,
+                     _ptr(self.wave_dev), self.dv.n_probe, _ptr(self.probe_ijk), _ptr(trace), step0, n_steps,
                      _ptr(hp), _ptr(hd), hist0, hist_steps, self.PLANES, self.dv.stream)
 
     def adjoint(self, lam, r, kappa, g, n_lo, n_hi, hist, hist0, acc):
         hp, hd = hist
         _native.call("fdtd_design_adjoint", self._sets(lam), *self._common(r, kappa), self.dv.n_probe,
-                     self.probe_ijk.ctypes.data, _ptr(g), n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
+                     _ptr(self.probe_ijk), _ptr(g), n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
                      hp.numel() // self.ecells, *[_ptr(a) for a in acc], self.PLANES, self.dv.stream)
 
 

@@ -41,7 +41,7 @@ struct DesignArgs {
     int src_i, src_j, src_k;
     const double* wave;              // device [steps]
     int n_probe;
-    int probe[8][3];                 // at most kDesignMaxProbes
+    const int32_t* probe;            // device [n_probe][3] (i, j, k)
     float* trace;                    // device [steps][n_probe] (forward)
     const float* g;                  // device [steps][n_probe] (adjoint: dL/dy)
     // history: p on [lo-1, hi+1), div on [lo, hi); slot of step s = s - hist0
@@ -55,7 +55,7 @@ struct DesignArgs {
     float* gk_sum;
     float* gk_comp;
     // tiling
-    int planes, nseg, nstrip_j, nwaves;
+    int planes, nseg, nstrip_j, nwaves, blocks_per_xcd;
 };
 
 __device__ __forceinline__ bool dbox(const DesignArgs& a, int i, int j, int k) {
@@ -108,13 +108,18 @@ __device__ __forceinline__ bool strip_meets_box(const DesignArgs& a, int i0, int
            k1 >= a.lo[2];
 }
 
-__device__ __forceinline__ bool strip_has_probe(const DesignArgs& a, int i0, int i1, int j0, int j1, int k0,
-                                                int k1) {
-    for (int q = 0; q < a.n_probe; ++q)
-        if (a.probe[q][0] >= i0 - 1 && a.probe[q][0] <= i1 && a.probe[q][1] >= j0 - 1 && a.probe[q][1] <= j1 &&
-            a.probe[q][2] >= k0 - 1 && a.probe[q][2] <= k1)
-            return true;
-    return false;
+// Probes in [i0-1, i1] x [j0-1, j1] x [k0-1, k1] (halo = 1), or in the strip
+// itself [i0, i1) x [j0, j1) x [k0, k1) (halo = 0), as a bit mask.
+__device__ __forceinline__ uint32_t strip_probes(const DesignArgs& a, int i0, int i1, int j0, int j1, int k0, int k1,
+                                                 int halo) {
+    uint32_t m = 0;
+    for (int q = 0; q < a.n_probe; ++q) {
+        const int pi = a.probe[3 * q], pj = a.probe[3 * q + 1], pk = a.probe[3 * q + 2];
+        if (pi >= i0 - halo && pi < i1 + halo && pj >= j0 - halo && pj < j1 + halo && pk >= k0 - halo &&
+            pk < k1 + halo)
+            m |= 1u << q;
+    }
+    return m;
 }
 
 #define GMEM __attribute__((address_space(1)))
@@ -130,7 +135,10 @@ __device__ __forceinline__ float sup(float v) { return __shfl_up(v, 1, kWave); }
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, int64_t s) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = blockIdx.x * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // XCD-aware: hardware block b runs on XCD b % 8; each XCD gets a contiguous
+    // range of strips so neighbouring strips share their halo rows in one L2
+    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (w >= a.nwaves) return;
     const int seg = w % a.nseg, t = w / a.nseg;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -155,6 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
     const double wsrc = a.wave ? a.wave[s] : 0.0;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
     const bool tb = strip_meets_box(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
+    const uint32_t pmask = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 0);
     auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
     auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
     auto KP = [&](int ii, int jj, int kk) { return tb ? kv_(a, ii, jj, kk) : 1.0f; };
@@ -247,9 +256,11 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
                 VXo[c] = vxn * dx_i;
                 VYo[c] = vyn * dy_j;
                 VZo[c] = vzn * dz_k;
-                for (int q = 0; q < a.n_probe; ++q)
-                    if (a.probe[q][0] == i && a.probe[q][1] == j && a.probe[q][2] == k)
-                        a.trace[s * a.n_probe + q] = pnew;
+                if (pmask)
+                    for (int q = 0; q < a.n_probe; ++q)
+                        if (((pmask >> q) & 1u) && a.probe[3 * q] == i && a.probe[3 * q + 1] == j &&
+                            a.probe[3 * q + 2] == k)
+                            a.trace[s * a.n_probe + q] = pnew;
                 if (hist && tb) {
                     if (ebox(a, i, j, k)) hp[ebox_lin(a, i, j, k)] = pr;
                     if (dbox(a, i, j, k)) hd[dbox_lin(a, i, j, k)] = div;
@@ -271,7 +282,8 @@ __device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float
     float v = P[c];
     if (pm)
         for (int q = 0; q < a.n_probe; ++q)
-            if (a.probe[q][0] == i && a.probe[q][1] == j && a.probe[q][2] == k) v += a.g[n * a.n_probe + q];
+            if (a.probe[3 * q] == i && a.probe[3 * q + 1] == j && a.probe[3 * q + 2] == k)
+                v += a.g[n * a.n_probe + q];
     const float lam = (a.dec[0][i] * a.dec[1][j]) * a.dec[2][k];
     return lam * v;
 }
@@ -293,7 +305,10 @@ __device__ __forceinline__ void kahan(float* sum, float* comp, int64_t b, float 
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, int64_t n) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = blockIdx.x * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // XCD-aware: hardware block b runs on XCD b % 8; each XCD gets a contiguous
+    // range of strips so neighbouring strips share their halo rows in one L2
+    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (w >= a.nwaves) return;
     const int seg = w % a.nseg, t = w / a.nseg;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -317,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
     const float* hd = grad ? a.hist_div + (n - a.hist0) * dbox_cells(a) : nullptr;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
     const bool tb = strip_meets_box(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
-    const bool pm = strip_has_probe(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
+    const bool pm = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 1) != 0;
     auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
     auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
     auto KP = [&](int ii, int jj, int kk) { return tb ? kv_(a, ii, jj, kk) : 1.0f; };
@@ -475,7 +490,9 @@ static int design_grid(DesignArgs& a, int planes) {
     a.nstrip_j = (a.ny + kDesignRows - 1) / kDesignRows;
     const int nstrip_i = (a.nx + planes - 1) / planes;
     a.nwaves = a.nseg * a.nstrip_j * nstrip_i;
-    return (a.nwaves + kBlockWaves - 1) / kBlockWaves;
+    const int blocks = (((a.nwaves + kBlockWaves - 1) / kBlockWaves) + 7) & ~7;
+    a.blocks_per_xcd = blocks / 8;
+    return blocks;
 }
 
 }  // namespace fdtd_amd
@@ -522,11 +539,7 @@ int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const
     }
     a.wave = wave;
     a.n_probe = n_probe;
-    for (int q = 0; q < n_probe; ++q)
-        for (int d = 0; d < 3; ++d) {
-            a.probe[q][d] = probe_ijk[3 * q + d];
-            if (a.probe[q][d] < 0 || a.probe[q][d] >= n[d]) return -2;
-        }
+    a.probe = probe_ijk;  // device; only compared with cell coordinates
     return 0;
 }
 

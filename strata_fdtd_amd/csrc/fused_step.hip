@@ -60,7 +60,13 @@ __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWav
 // between them come from LDS and only the workgroup's outer rows are read
 // twice from HBM. `lds` holds 2 (plane parity) x 4 x W x 64 floats; `wi` is
 // the wave's index in the stack. W == 1: every wave reads its own halos.
-template <int R, bool NU, int W>
+//
+// NTS: store the step outputs with non-temporal hints. They are not read again
+// within the step, and on a grid much larger than the caches this keeps L2 for
+// the halo rows and columns the neighbouring strips re-read (about 5% faster at
+// 512^3). The batched kernel (small grids, which can stay cache-resident from
+// step to step) uses ordinary stores.
+template <int R, bool NU, int W, bool NTS>
 __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj, int si, int lane,
                                             float* lds, int wi) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -296,10 +302,17 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
                 }
             }
             if (kv) {
-                Po[c] = ps;
-                VXo[c] = vxn * dx_i;
-                VYo[c] = vyn * dy_j;
-                VZo[c] = vzn * dz_k;
+                if constexpr (NTS) {
+                    __builtin_nontemporal_store(ps, &Po[c]);
+                    __builtin_nontemporal_store(vxn * dx_i, &VXo[c]);
+                    __builtin_nontemporal_store(vyn * dy_j, &VYo[c]);
+                    __builtin_nontemporal_store(vzn * dz_k, &VZo[c]);
+                } else {
+                    Po[c] = ps;
+                    VXo[c] = vxn * dx_i;
+                    VYo[c] = vyn * dy_j;
+                    VZo[c] = vzn * dz_k;
+                }
             }
             if (probe_mask && trace_ok) {
                 for (int q = 0; q < a.n_probe; ++q) {
@@ -332,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     if (*a.step_ptr + a.koff >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, NU, 1>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, NU, 1, true>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 // Stacked variant: workgroup = W waves over W consecutive j-strips of one
@@ -351,7 +364,7 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
     const int nstack_j = (a.nstrip_j + W - 1) / W;
     const int seg = lb % a.nseg;
     const int t = lb / a.nseg;
-    fused_strip<R, NU, W>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
+    fused_strip<R, NU, W, true>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
 }
 
 // Batched step: one launch advances every solve in `table` by one step. Waves
@@ -363,12 +376,18 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
 template <int R>
 __global__ __launch_bounds__(kBlock) void k_batch_step(const FusedArgs* __restrict__ table,
                                                        const int* __restrict__ offsets,
-                                                       const int* __restrict__ n_ptr, int blocks_per_xcd) {
+                                                       const int* __restrict__ n_ptr) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int lb = (blockIdx.x & 7) * blocks_per_xcd + (blockIdx.x >> 3);
-    const int gw = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // XCD-aware remap over the blocks the live table needs (the grid may be
+    // larger): XCD x takes logical blocks [x * bpx, (x + 1) * bpx)
     const int n = *n_ptr;
-    if (gw >= offsets[n]) return;
+    const int total = offsets[n];
+    const int bpx = ((total + kBlockWaves - 1) / kBlockWaves + 7) / 8;
+    const int xb = blockIdx.x >> 3;
+    if (xb >= bpx) return;
+    const int lb = (blockIdx.x & 7) * bpx + xb;
+    const int gw = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (gw >= total) return;
     int lo = 0, hi = n;  // offsets[lo] <= gw < offsets[hi]
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -379,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const FusedArgs* __restri
     if (*a.step_ptr >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, false, 1>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, false, 1, false>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 __global__ void k_batch_advance(const FusedArgs* __restrict__ table, const int* __restrict__ n_ptr) {
@@ -685,11 +704,10 @@ hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n
                         int cap_waves, int rows, hipStream_t st) {
     int blocks = (cap_waves + kBlockWaves - 1) / kBlockWaves;
     blocks = std::max(8, (blocks + 7) & ~7);
-    const int bpx = blocks / 8;
     switch (rows) {
-        case 2: hipLaunchKernelGGL(k_batch_step<2>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, bpx); break;
-        case 4: hipLaunchKernelGGL(k_batch_step<4>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, bpx); break;
-        case 8: hipLaunchKernelGGL(k_batch_step<8>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, bpx); break;
+        case 2: hipLaunchKernelGGL(k_batch_step<2>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
+        case 4: hipLaunchKernelGGL(k_batch_step<4>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
+        case 8: hipLaunchKernelGGL(k_batch_step<8>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

@@ -50,14 +50,14 @@ def normalize_tiling(t) -> tuple[int, int, int]:
 
 
 def heuristic_tiling(shape) -> tuple[int, int, int]:
-    """rows=4; the most planes per strip (<= 8) that still leaves >= 8 waves
-    per CU. Measured at 512^3: planes 4-16 within 2% of each other, 64+ is 5%
-    slower, and very long strips starve the small ADE launch."""
+    """rows=4; the most planes per strip (<= 4) that still leaves >= 8 waves
+    per CU. Measured at 512^3 (plain and enclosure): planes 4 is 1% faster
+    than 8 and 3% faster than 16; rows 2 and LDS stacks are slower."""
     nx, ny, nz = shape
     rows = 4
     segs = (nz + 63) // 64
     strips_ji = segs * ((ny + rows - 1) // rows)
-    planes = 8
+    planes = 4
     while planes > 2 and strips_ji * ((nx + planes - 1) // planes) < 8 * NUM_CU:
         planes //= 2
     return rows, planes, STACK_DEFAULT
