@@ -54,8 +54,10 @@ struct DesignArgs {
     float* gr_comp;
     float* gk_sum;
     float* gk_comp;
-    // tiling
+    // tiling; the strips that meet the design box form the sub-grid
+    // [b_seg0, b_seg1) x [b_sj0, b_sj1) x [b_si0, b_si1)
     int planes, nseg, nstrip_j, nwaves, blocks_per_xcd;
+    int b_seg0, b_seg1, b_sj0, b_sj1, b_si0, b_si1, b_blocks_per_xcd;
 };
 
 __device__ __forceinline__ bool dbox(const DesignArgs& a, int i, int j, int k) {
@@ -132,20 +134,16 @@ __device__ __forceinline__ float sdown(float v) { return __shfl_down(v, 1, kWave
 __device__ __forceinline__ float sup(float v) { return __shfl_up(v, 1, kWave); }
 
 // ---------------------------------------------------------------- forward
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, int64_t s) {
-    const int lane = threadIdx.x & (kWave - 1);
-    // XCD-aware: hardware block b runs on XCD b % 8; each XCD gets a contiguous
-    // range of strips so neighbouring strips share their halo rows in one L2
-    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
-    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (w >= a.nwaves) return;
-    const int seg = w % a.nseg, t = w / a.nseg;
+// TB: the strip (with its halo) meets the design box. Strips that do not
+// (most of the grid) run the code with the constant coefficients only.
+template <int R, bool TB>
+__device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_t s, int seg, int sj, int si,
+                                                     int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = seg * kWave, k = k0 + lane;
     const bool kin = k < nz;
-    const int j0 = (t % a.nstrip_j) * R;
-    const int i0 = (t / a.nstrip_j) * a.planes, i1 = min(i0 + a.planes, nx);
+    const int j0 = sj * R;
+    const int i0 = si * a.planes, i1 = min(i0 + a.planes, nx);
     const int64_t sx = static_cast<int64_t>(ny) * nz;
     const int par = static_cast<int>(s & 1);
     const GMEM float* __restrict__ P = gm(a.f[par][0]);
@@ -162,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
     float* hd = hist ? a.hist_div + (s - a.hist0) * dbox_cells(a) : nullptr;
     const double wsrc = a.wave ? a.wave[s] : 0.0;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
-    const bool tb = strip_meets_box(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
+    constexpr bool tb = TB;
     const uint32_t pmask = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 0);
     auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
     auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
@@ -274,6 +272,35 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
     }
 }
 
+// Two launches per step: `box` = 0 runs every strip that misses the design
+// box (constant coefficients, the register budget of the plain sweep) and
+// skips the others; `box` = 1 runs only the strips that meet it, over the
+// sub-grid [b_seg0, b_seg1) x [b_sj0, b_sj1) x [b_si0, b_si1) of strips.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, int64_t s) {
+    const int lane = threadIdx.x & (kWave - 1);
+    // XCD-aware: hardware block b runs on XCD b % 8; each XCD gets a contiguous
+    // range of strips so neighbouring strips share their halo rows in one L2
+    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w >= a.nwaves) return;
+    const int seg = w % a.nseg, t = w / a.nseg, sj = t % a.nstrip_j, si = t / a.nstrip_j;
+    if (seg >= a.b_seg0 && seg < a.b_seg1 && sj >= a.b_sj0 && sj < a.b_sj1 && si >= a.b_si0 && si < a.b_si1)
+        return;
+    design_forward_strip<R, false>(a, s, seg, sj, si, lane);
+}
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_design_forward_box(const DesignArgs a, int64_t s) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int lb = (blockIdx.x & 7) * a.b_blocks_per_xcd + (blockIdx.x >> 3);
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nseg = a.b_seg1 - a.b_seg0, nsj = a.b_sj1 - a.b_sj0;
+    if (w >= nseg * nsj * (a.b_si1 - a.b_si0)) return;
+    const int t = w / nseg;
+    design_forward_strip<R, true>(a, s, a.b_seg0 + w % nseg, a.b_sj0 + t % nsj, a.b_si0 + t / nsj, lane);
+}
+
 // ---------------------------------------------------------------- adjoint
 // pb = dX dY dZ (P + g at probe cells); q = air cp0 kappa pb.
 // pm: the strip (with its halo) holds a probe; tb: it meets the design box.
@@ -302,20 +329,14 @@ __device__ __forceinline__ void kahan(float* sum, float* comp, int64_t b, float 
 }
 
 // Adjoint step n: (P, V) = lambda_{n+1} in set (n+1)&1 -> lambda_n in set n&1.
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, int64_t n) {
-    const int lane = threadIdx.x & (kWave - 1);
-    // XCD-aware: hardware block b runs on XCD b % 8; each XCD gets a contiguous
-    // range of strips so neighbouring strips share their halo rows in one L2
-    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
-    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (w >= a.nwaves) return;
-    const int seg = w % a.nseg, t = w / a.nseg;
+template <int R, bool TB>
+__device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_t n, int seg, int sj, int si,
+                                                     int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = seg * kWave, k = k0 + lane;
     const bool kin = k < nz;
-    const int j0 = (t % a.nstrip_j) * R;
-    const int i0 = (t / a.nstrip_j) * a.planes, i1 = min(i0 + a.planes, nx);
+    const int j0 = sj * R;
+    const int i0 = si * a.planes, i1 = min(i0 + a.planes, nx);
     const int64_t sx = static_cast<int64_t>(ny) * nz;
     const int src = static_cast<int>((n + 1) & 1);
     const GMEM float* __restrict__ P = gm(a.f[src][0]);
@@ -331,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
     const float* hp = grad ? a.hist_p + (n - a.hist0) * ebox_cells(a) : nullptr;
     const float* hd = grad ? a.hist_div + (n - a.hist0) * dbox_cells(a) : nullptr;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
-    const bool tb = strip_meets_box(a, i0, i1, j0, j0 + R, k0, k0 + kWave);
+    constexpr bool tb = TB;
     const bool pm = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 1) != 0;
     auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
     auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
@@ -482,17 +503,61 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
     }
 }
 
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, int64_t n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w >= a.nwaves) return;
+    const int seg = w % a.nseg, t = w / a.nseg, sj = t % a.nstrip_j, si = t / a.nstrip_j;
+    if (seg >= a.b_seg0 && seg < a.b_seg1 && sj >= a.b_sj0 && sj < a.b_sj1 && si >= a.b_si0 && si < a.b_si1)
+        return;
+    design_adjoint_strip<R, false>(a, n, seg, sj, si, lane);
+}
+
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_design_adjoint_box(const DesignArgs a, int64_t n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int lb = (blockIdx.x & 7) * a.b_blocks_per_xcd + (blockIdx.x >> 3);
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nseg = a.b_seg1 - a.b_seg0, nsj = a.b_sj1 - a.b_sj0;
+    if (w >= nseg * nsj * (a.b_si1 - a.b_si0)) return;
+    const int t = w / nseg;
+    design_adjoint_strip<R, true>(a, n, a.b_seg0 + w % nseg, a.b_sj0 + t % nsj, a.b_si0 + t / nsj, lane);
+}
+
 constexpr int kDesignRows = 4;
 
-static int design_grid(DesignArgs& a, int planes) {
+// Strips [c0, c1) of extent `ext` along an axis of n cells whose span with a
+// one-cell halo, [c * ext - 1, min((c + 1) * ext, n)], meets [lo, hi).
+static void meeting_strips(int n, int ext, int lo, int hi, int count, int& c0, int& c1) {
+    c0 = count;
+    c1 = 0;
+    for (int c = 0; c < count; ++c) {
+        const int a0 = c * ext - 1, a1 = std::min((c + 1) * ext, n);
+        if (a0 < hi && a1 >= lo) {
+            c0 = std::min(c0, c);
+            c1 = c + 1;
+        }
+    }
+    if (c1 <= c0) c0 = c1 = 0;
+}
+
+// Returns {blocks of the full-grid launch, blocks of the box launch}.
+static void design_grid(DesignArgs& a, int planes, int& blocks, int& box_blocks) {
     a.planes = planes;
     a.nseg = (a.nz + kWave - 1) / kWave;
     a.nstrip_j = (a.ny + kDesignRows - 1) / kDesignRows;
     const int nstrip_i = (a.nx + planes - 1) / planes;
     a.nwaves = a.nseg * a.nstrip_j * nstrip_i;
-    const int blocks = (((a.nwaves + kBlockWaves - 1) / kBlockWaves) + 7) & ~7;
+    blocks = (((a.nwaves + kBlockWaves - 1) / kBlockWaves) + 7) & ~7;
     a.blocks_per_xcd = blocks / 8;
-    return blocks;
+    meeting_strips(a.nz, kWave, a.lo[2], a.hi[2], a.nseg, a.b_seg0, a.b_seg1);
+    meeting_strips(a.ny, kDesignRows, a.lo[1], a.hi[1], a.nstrip_j, a.b_sj0, a.b_sj1);
+    meeting_strips(a.nx, planes, a.lo[0], a.hi[0], nstrip_i, a.b_si0, a.b_si1);
+    const int bw = (a.b_seg1 - a.b_seg0) * (a.b_sj1 - a.b_sj0) * (a.b_si1 - a.b_si0);
+    box_blocks = (((bw + kBlockWaves - 1) / kBlockWaves) + 7) & ~7;
+    a.b_blocks_per_xcd = box_blocks / 8;
 }
 
 }  // namespace fdtd_amd
@@ -572,10 +637,14 @@ int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float*
     a.hist_div = hist_div;
     a.hist0 = hist0;
     a.hist_steps = hist_p ? hist_steps : 0;
-    const int blocks = design_grid(a, planes);
+    int blocks = 0, box_blocks = 0;
+    design_grid(a, planes, blocks, box_blocks);
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (int m = 0; m < n_steps; ++m) {
         hipLaunchKernelGGL(k_design_forward<kDesignRows>, dim3(blocks), dim3(kBlock), 0, st, a, step0 + m);
+        if (box_blocks)
+            hipLaunchKernelGGL(k_design_forward_box<kDesignRows>, dim3(box_blocks), dim3(kBlock), 0, st, a,
+                               step0 + m);
         if (hipGetLastError() != hipSuccess) return -5;
     }
     return 0;
@@ -602,10 +671,13 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
     a.gr_comp = gr_comp;
     a.gk_sum = gk_sum;
     a.gk_comp = gk_comp;
-    const int blocks = design_grid(a, planes);
+    int blocks = 0, box_blocks = 0;
+    design_grid(a, planes, blocks, box_blocks);
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (int64_t n = n_hi - 1; n >= n_lo; --n) {
         hipLaunchKernelGGL(k_design_adjoint<kDesignRows>, dim3(blocks), dim3(kBlock), 0, st, a, n);
+        if (box_blocks)
+            hipLaunchKernelGGL(k_design_adjoint_box<kDesignRows>, dim3(box_blocks), dim3(kBlock), 0, st, a, n);
         if (hipGetLastError() != hipSuccess) return -5;
     }
     return 0;
