@@ -64,9 +64,10 @@ __device__ __forceinline__ bool dbox(const DesignArgs& a, int i, int j, int k) {
     return i >= a.lo[0] && i < a.hi[0] && j >= a.lo[1] && j < a.hi[1] && k >= a.lo[2] && k < a.hi[2];
 }
 
-__device__ __forceinline__ int64_t dbox_lin(const DesignArgs& a, int i, int j, int k) {
-    return ((static_cast<int64_t>(i - a.lo[0]) * (a.hi[1] - a.lo[1])) + (j - a.lo[1])) * (a.hi[2] - a.lo[2]) +
-           (k - a.lo[2]);
+// Box indices fit in 32 bits (checked on the host), which keeps the address
+// arithmetic of the many box lookups short.
+__device__ __forceinline__ int dbox_lin(const DesignArgs& a, int i, int j, int k) {
+    return ((i - a.lo[0]) * (a.hi[1] - a.lo[1]) + (j - a.lo[1])) * (a.hi[2] - a.lo[2]) + (k - a.lo[2]);
 }
 
 __device__ __forceinline__ float rv(const DesignArgs& a, int i, int j, int k) {
@@ -83,9 +84,9 @@ __device__ __forceinline__ bool ebox(const DesignArgs& a, int i, int j, int k) {
            k <= a.hi[2];
 }
 
-__device__ __forceinline__ int64_t ebox_lin(const DesignArgs& a, int i, int j, int k) {
+__device__ __forceinline__ int ebox_lin(const DesignArgs& a, int i, int j, int k) {
     const int ey = a.hi[1] - a.lo[1] + 2, ez = a.hi[2] - a.lo[2] + 2;
-    return ((static_cast<int64_t>(i - a.lo[0] + 1) * ey) + (j - a.lo[1] + 1)) * ez + (k - a.lo[2] + 1);
+    return ((i - a.lo[0] + 1) * ey + (j - a.lo[1] + 1)) * ez + (k - a.lo[2] + 1);
 }
 
 __device__ __forceinline__ int64_t ebox_cells(const DesignArgs& a) {
@@ -487,7 +488,7 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
                 VYo[c] = wy;
                 VZo[c] = wz;
                 if (gbox) {
-                    const int64_t b = dbox_lin(a, i, j, k);
+                    const int b = dbox_lin(a, i, j, k);
                     kahan(a.gr_sum, a.gr_comp, b, gr);
                     if (air) kahan(a.gk_sum, a.gk_comp, b, pbc[r] * a.cp0 * hd[b]);
                 }
@@ -594,6 +595,8 @@ int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const
         a.hi[d] = box_hi ? box_hi[d] : 0;
         if (a.lo[d] < 1 || a.hi[d] > n[d] - 1 || a.hi[d] <= a.lo[d]) return -3;  // box + 1-cell margin inside
     }
+    if (static_cast<int64_t>(a.hi[0] - a.lo[0] + 2) * (a.hi[1] - a.lo[1] + 2) * (a.hi[2] - a.lo[2] + 2) >= (1ll << 31))
+        return -3;  // box indices are 32-bit
     a.r = r;
     a.kappa = kappa;
     a.src_i = a.src_j = a.src_k = -1;
