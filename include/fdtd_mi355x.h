@@ -208,26 +208,29 @@ int fdtd_adjoint_run(int dtype, void* const* lam, void* const* states, int64_t n
  * as fdtd_param_step / fdtd_adjoint_step). fields / lam: host arrays of 8
  * device pointers (two {p, vx, vy, vz} sets; step s reads set s & 1, the
  * adjoint step n reads set (n + 1) & 1). dec: host array of 3 device decay
- * tables. The design box must keep one cell to every grid face. src_ijk: host
- * int32[3] or NULL; wave: device float64 [steps]; probe_ijk: device int32
- * [n_probe][3]; trace and g: device float32 [steps][n_probe].
+ * tables. Grids below 2^30 cells; the design box must keep one cell to every
+ * grid face. rk: device float32 [hx][hy][hz][2], (rho/rho0, K/K0) per box
+ * voxel, or NULL (air). src_ijk: host int32[3] or NULL; wave: device float64
+ * [steps]; probe_ijk: device int32 [n_probe][3]; trace and g: device float32
+ * [steps][n_probe].
  * History (device float32): p on the box grown by one cell
  * ((hx+2)(hy+2)(hz+2) per step) and div on the box (hx hy hz per step) for
  * steps [hist0, hist0 + hist_steps); fdtd_design_history_bytes gives the size.
  * The forward records it when hist_p is not NULL; the adjoint (steps
- * n_hi - 1 down to n_lo) reads it and Kahan-accumulates dL/dr and dL/dkappa
- * into the four box arrays. */
+ * n_hi - 1 down to n_lo) reads it and Kahan-accumulates into acc, device
+ * float32 [hx][hy][hz][4]: (sum, compensation) of dL/d(rho/rho0), then of
+ * dL/d(K/K0); the gradient is sum - compensation. */
 int64_t fdtd_design_history_bytes(const int32_t* box_lo, const int32_t* box_hi, int64_t steps);
 int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
-                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
-                        const float* kappa, const int32_t* src_ijk, const double* wave, int n_probe,
-                        const int32_t* probe_ijk, float* trace, int64_t step0, int n_steps, float* hist_p,
-                        float* hist_div, int64_t hist0, int64_t hist_steps, int planes, void* stream);
+                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
+                        const int32_t* src_ijk, const double* wave, int n_probe, const int32_t* probe_ijk,
+                        float* trace, int64_t step0, int n_steps, float* hist_p, float* hist_div, int64_t hist0,
+                        int64_t hist_steps, int planes, void* stream);
 int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
-                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
-                        const float* kappa, int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo,
-                        int64_t n_hi, const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps,
-                        float* gr_sum, float* gr_comp, float* gk_sum, float* gk_comp, int planes, void* stream);
+                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
+                        int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo, int64_t n_hi,
+                        const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps, float* acc,
+                        int planes, void* stream);
 
 /* ---- per-phase entry points (one kernel per reference function) --------- */
 /* All in place, on `stream`, float32, uniform unless noted. */

@@ -52,10 +52,10 @@ _SIGS = {
     "fdtd_adjoint_run": (_i, [_i, _vp, _vp, _i64, _i64, _i, _i, _i] + [_vp] * 8
                          + [C.c_double, C.c_double, _i] + [_vp] * 7),
     "fdtd_design_history_bytes": (_i64, [_vp, _vp, _i64]),
-    "fdtd_design_forward": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp,
+    "fdtd_design_forward": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp,
                                  _i64, _i, _vp, _vp, _i64, _i64, _i, _vp]),
-    "fdtd_design_adjoint": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i64, _i64,
-                                 _vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp, _i, _vp]),
+    "fdtd_design_adjoint": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _i, _vp, _vp, _i64, _i64,
+                                 _vp, _vp, _i64, _i64, _vp, _i, _vp]),
     "fdtd_update_velocity": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
     "fdtd_update_velocity_nonuniform": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "fdtd_apply_rigid": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),

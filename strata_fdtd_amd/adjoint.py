@@ -168,7 +168,7 @@ class _Fast:
     """Float32 march kernels of the design solve (csrc/design.hip): forward
     sweep with optional box history, adjoint sweep reading that history."""
 
-    PLANES = 8
+    PLANES = 4
 
     def __init__(self, dv: _Device):
         import torch
@@ -188,28 +188,39 @@ class _Fast:
     def _sets(sets):
         return (_vp * 8)(*[_ptr(t) for st in sets for t in st])
 
-    def _common(self, r, kappa):
+    @staticmethod
+    def pack(r, kappa):
+        """(r, kappa) interleaved per box voxel, as the kernels read them (one
+        8-byte load per voxel); None when both are absent (air)."""
+        import torch
+        if r is None and kappa is None:
+            return None
+        one = torch.ones_like(r if r is not None else kappa)
+        return torch.stack((r if r is not None else one, kappa if kappa is not None else one), dim=-1).contiguous()
+
+    def _common(self, rk):
         dv, pb = self.dv, self.dv.pb
         return (_ptr(dv.flags), self.dec, *pb.shape, dv.cv0, dv.cp0, dv.lo.ctypes.data, dv.hi.ctypes.data,
-                _ptr(r), _ptr(kappa))
+                _ptr(rk))
 
     def history(self, steps):
         import torch
         return (torch.empty(steps * self.ecells, dtype=torch.float32, device=self.dv.dev),
                 torch.empty(steps * self.bcells, dtype=torch.float32, device=self.dv.dev))
 
-    def forward(self, sets, r, kappa, trace, step0, n_steps, hist=None, hist0=0):
+    def forward(self, sets, rk, trace, step0, n_steps, hist=None, hist0=0):
         hp, hd = hist if hist is not None else (None, None)
         hist_steps = (hp.numel() // self.ecells) if hp is not None else 0
-        _native.call("fdtd_design_forward", self._sets(sets), *self._common(r, kappa), self.src,
+        _native.call("fdtd_design_forward", self._sets(sets), *self._common(rk), self.src,
                      _ptr(self.wave_dev), self.dv.n_probe, _ptr(self.probe_ijk), _ptr(trace), step0, n_steps,
                      _ptr(hp), _ptr(hd), hist0, hist_steps, self.PLANES, self.dv.stream)
 
-    def adjoint(self, lam, r, kappa, g, n_lo, n_hi, hist, hist0, acc):
+    def adjoint(self, lam, rk, g, n_lo, n_hi, hist, hist0, acc):
+        """acc: float32 [box][4], Kahan (sum, compensation) of dL/dr, then dL/dkappa."""
         hp, hd = hist
-        _native.call("fdtd_design_adjoint", self._sets(lam), *self._common(r, kappa), self.dv.n_probe,
+        _native.call("fdtd_design_adjoint", self._sets(lam), *self._common(rk), self.dv.n_probe,
                      _ptr(self.probe_ijk), _ptr(g), n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
-                     hp.numel() // self.ecells, *[_ptr(a) for a in acc], self.PLANES, self.dv.stream)
+                     hp.numel() // self.ecells, _ptr(acc), self.PLANES, self.dv.stream)
 
 
 def history_bytes(pb: DesignProblem, steps: int | None = None) -> int:
@@ -230,32 +241,33 @@ def _pick_history(pb: DesignProblem, history: str, device) -> str:
 def _forward_fast(pb, r, kappa, dv, history: str):
     import torch
     fast = _Fast(dv)
+    rk = fast.pack(r, kappa)
     sets = [dv.zeros_state(), dv.zeros_state()]
     traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
     ckpts, hist = [], None
     if history == "all":
         hist = fast.history(pb.steps)
-        fast.forward(sets, r, kappa, traces, 0, pb.steps, hist, 0)
+        fast.forward(sets, rk, traces, 0, pb.steps, hist, 0)
     elif history == "checkpoint":
         K = pb.checkpoint
         for n0 in range(0, pb.steps, K):
             ckpts.append([t.clone() for t in sets[n0 & 1]])
-            fast.forward(sets, r, kappa, traces, n0, min(K, pb.steps - n0))
+            fast.forward(sets, rk, traces, n0, min(K, pb.steps - n0))
     else:
-        fast.forward(sets, r, kappa, traces, 0, pb.steps)
-    return traces[:, :dv.n_probe], ("fast", dv, r, kappa, fast, history, hist, ckpts)
+        fast.forward(sets, rk, traces, 0, pb.steps)
+    return traces[:, :dv.n_probe], ("fast", dv, rk, fast, history, hist, ckpts)
 
 
 def _adjoint_fast(saved, grad_traces):
     import torch
-    _, dv, r, kappa, fast, history, hist, ckpts = saved
+    _, dv, rk, fast, history, hist, ckpts = saved
     pb = dv.pb
     lam = [dv.zeros_state(), dv.zeros_state()]
-    acc = [torch.zeros(pb.box_shape, dtype=torch.float32, device=dv.dev) for _ in range(4)]
+    acc = torch.zeros(tuple(pb.box_shape) + (4,), dtype=torch.float32, device=dv.dev)
     g = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
     g[:, :dv.n_probe] = grad_traces.to(torch.float32)
     if history == "all":
-        fast.adjoint(lam, r, kappa, g, 0, pb.steps, hist, 0, acc)
+        fast.adjoint(lam, rk, g, 0, pb.steps, hist, 0, acc)
     elif history == "checkpoint":
         K = pb.checkpoint
         rec = [dv.zeros_state(), dv.zeros_state()]
@@ -266,11 +278,12 @@ def _adjoint_fast(saved, grad_traces):
             n1 = min(pb.steps, n0 + K)
             for t, c in zip(rec[n0 & 1], ckpts[seg]):
                 t.copy_(c)
-            fast.forward(rec, r, kappa, scratch, n0, n1 - n0, seg_hist, n0)
-            fast.adjoint(lam, r, kappa, g, n0, n1, seg_hist, n0, acc)
+            fast.forward(rec, rk, scratch, n0, n1 - n0, seg_hist, n0)
+            fast.adjoint(lam, rk, g, n0, n1, seg_hist, n0, acc)
     else:
         raise RuntimeError("forward ran without history or checkpoints")
-    return acc[0] - acc[1], acc[2] - acc[3]
+    # Kahan: the compensation holds minus the low-order part the sums lost
+    return acc[..., 0] - acc[..., 1], acc[..., 2] - acc[..., 3]
 
 
 def design_ratio(x, ref: float, device="cuda"):

@@ -26,6 +26,17 @@ namespace fdtd_amd {
 
 constexpr int kDesignMaxProbes = 8;
 
+// (r, kappa) of a voxel and the four Kahan words of its gradients, loaded
+// and stored whole
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+// strips: rows per strip of the plain launches and of the forward box launch,
+// rows per strip of the adjoint box launch (its register budget), planes per
+// strip of the box launches at most
+constexpr int kDesignRows = 4;
+constexpr int kDesignAdjBoxRows = 2;
+constexpr int kDesignBoxPlanes = 4;
+
 struct DesignArgs {
     // fields: ping-pong sets {p, vx, vy, vz}; step s reads set s & 1
     float* f[2][4];
@@ -35,8 +46,7 @@ struct DesignArgs {
     float cv0, cp0;
     // design box [lo, hi) with r = rho/rho0 and kappa = K/K0 (float32)
     int lo[3], hi[3];
-    const float* r;
-    const float* kappa;
+    const f2v* rk;                   // device [box][2]: (r, kappa) per voxel, or NULL (air)
     // source (one point source; -1 none) and probes
     int src_i, src_j, src_k;
     const double* wave;              // device [steps]
@@ -49,45 +59,16 @@ struct DesignArgs {
     float* hist_div;
     int64_t hist0;
     int64_t hist_steps;
-    // gradient accumulators on the box (Kahan sum / compensation)
-    float* gr_sum;
-    float* gr_comp;
-    float* gk_sum;
-    float* gk_comp;
+    // gradient accumulators on the box: Kahan (sum, compensation) of dL/dr,
+    // then of dL/dkappa, per voxel
+    f4v* acc;
     // tiling; the strips that meet the design box form the sub-grid
     // [b_seg0, b_seg1) x [b_sj0, b_sj1) x [b_si0, b_si1)
     int planes, nseg, nstrip_j, nwaves, blocks_per_xcd;
     int b_seg0, b_seg1, b_sj0, b_sj1, b_si0, b_si1, b_blocks_per_xcd;
+    // the box launch: planes [b_i0, b_i1) in nb_i strips of box_planes planes
+    int box_planes, b_i0, b_i1, nb_i;
 };
-
-__device__ __forceinline__ bool dbox(const DesignArgs& a, int i, int j, int k) {
-    return i >= a.lo[0] && i < a.hi[0] && j >= a.lo[1] && j < a.hi[1] && k >= a.lo[2] && k < a.hi[2];
-}
-
-// Box indices fit in 32 bits (checked on the host), which keeps the address
-// arithmetic of the many box lookups short.
-__device__ __forceinline__ int dbox_lin(const DesignArgs& a, int i, int j, int k) {
-    return ((i - a.lo[0]) * (a.hi[1] - a.lo[1]) + (j - a.lo[1])) * (a.hi[2] - a.lo[2]) + (k - a.lo[2]);
-}
-
-__device__ __forceinline__ float rv(const DesignArgs& a, int i, int j, int k) {
-    return (a.r && dbox(a, i, j, k)) ? a.r[dbox_lin(a, i, j, k)] : 1.0f;
-}
-
-__device__ __forceinline__ float kv_(const DesignArgs& a, int i, int j, int k) {
-    return (a.kappa && dbox(a, i, j, k)) ? a.kappa[dbox_lin(a, i, j, k)] : 1.0f;
-}
-
-// the extended box [lo-1, hi+1) of the pressure history
-__device__ __forceinline__ bool ebox(const DesignArgs& a, int i, int j, int k) {
-    return i >= a.lo[0] - 1 && i <= a.hi[0] && j >= a.lo[1] - 1 && j <= a.hi[1] && k >= a.lo[2] - 1 &&
-           k <= a.hi[2];
-}
-
-__device__ __forceinline__ int ebox_lin(const DesignArgs& a, int i, int j, int k) {
-    const int ey = a.hi[1] - a.lo[1] + 2, ez = a.hi[2] - a.lo[2] + 2;
-    return ((i - a.lo[0] + 1) * ey + (j - a.lo[1] + 1)) * ez + (k - a.lo[2] + 1);
-}
 
 __device__ __forceinline__ int64_t ebox_cells(const DesignArgs& a) {
     return static_cast<int64_t>(a.hi[0] - a.lo[0] + 2) * (a.hi[1] - a.lo[1] + 2) * (a.hi[2] - a.lo[2] + 2);
@@ -99,16 +80,6 @@ __device__ __forceinline__ int64_t dbox_cells(const DesignArgs& a) {
 
 __device__ __forceinline__ float cvf(const DesignArgs& a, float r_lo, float r_hi) {
     return a.cv0 / ((r_lo + r_hi) * 0.5f);
-}
-
-// Does the region [i0-1, i1] x [j0-1, j1] x [k0-1, k1] (a strip and its
-// halo) meet the design box? Outside the box r = kappa = 1, so the face
-// coefficient cv0 / ((1 + 1) * 0.5) is exactly cv0 and cp0 * 1 is exactly cp0:
-// a strip that misses the box takes those constants without looking anything up.
-__device__ __forceinline__ bool strip_meets_box(const DesignArgs& a, int i0, int i1, int j0, int j1, int k0,
-                                                int k1) {
-    return i0 - 1 < a.hi[0] && i1 >= a.lo[0] && j0 - 1 < a.hi[1] && j1 >= a.lo[1] && k0 - 1 < a.hi[2] &&
-           k1 >= a.lo[2];
 }
 
 // Probes in [i0-1, i1] x [j0-1, j1] x [k0-1, k1] (halo = 1), or in the strip
@@ -134,18 +105,80 @@ __device__ __forceinline__ const GMEM T* gm(const T* p) { return (const GMEM T*)
 __device__ __forceinline__ float sdown(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float sup(float v) { return __shfl_up(v, 1, kWave); }
 
+// Cell offsets are 32-bit (grids below 2^30 cells, checked on the host) and
+// accesses form their byte offset in 32 bits, so they use the scalar-base +
+// 32-bit vector-offset address form: one VGPR per offset, shared by every
+// array indexed alike, and no 64-bit address arithmetic per access.
+__device__ __forceinline__ uint32_t cix(int i, int j, int k, int ny, int nz) {
+    return (static_cast<uint32_t>(i) * static_cast<uint32_t>(ny) + static_cast<uint32_t>(j)) *
+               static_cast<uint32_t>(nz) +
+           static_cast<uint32_t>(k);
+}
+template <typename T>
+__device__ __forceinline__ T ldo(const GMEM T* base, uint32_t idx) {
+    return *reinterpret_cast<const GMEM T*>(reinterpret_cast<const GMEM char*>(base) +
+                                            idx * static_cast<uint32_t>(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void sto(GMEM T* base, uint32_t idx, T v) {
+    *reinterpret_cast<GMEM T*>(reinterpret_cast<GMEM char*>(base) + idx * static_cast<uint32_t>(sizeof(T))) = v;
+}
+template <typename T>
+__device__ __forceinline__ void stnt(GMEM T* base, uint32_t idx, T v) {
+    __builtin_nontemporal_store(
+        v, reinterpret_cast<GMEM T*>(reinterpret_cast<GMEM char*>(base) + idx * static_cast<uint32_t>(sizeof(T))));
+}
+
+// ---------------------------------------------------------------- box lookups
+// Inside the box strips every box quantity a cell needs is loaded at the top
+// of its plane, together with the field loads, from an offset clamped into
+// the box and then selected (1 or 0 outside), so the loads are unconditional
+// and overlap instead of waiting one after another behind branches.
+
+// Position of a coordinate along one axis of the design box [lo, lo + n)
+// (off clamped into [0, n)) or of the extended box [lo - 1, lo + n + 1).
+struct Ax {
+    int off;
+    bool in;
+};
+__device__ __forceinline__ Ax dax(int x, int lo, int n) {
+    const int b = x - lo;
+    return {min(max(b, 0), n - 1), static_cast<unsigned>(b) < static_cast<unsigned>(n)};
+}
+__device__ __forceinline__ Ax eax(int x, int lo, int n) {
+    const int b = x - lo + 1;
+    return {min(max(b, 0), n + 1), static_cast<unsigned>(b) < static_cast<unsigned>(n + 2)};
+}
+
+// An absent (r, kappa) array means (1, 1) everywhere: its loads read this.
+__device__ __align__(8) float kDesignOne[2] = {1.0f, 1.0f};
+
+struct BoxTab {
+    const GMEM f2v* p;
+    uint32_t mask;  // offsets are masked to 0 when the array is absent
+    bool has;
+};
+__device__ __forceinline__ BoxTab box_tab(const f2v* p) {
+    if (p) return {gm(p), 0xffffffffu, true};
+    return {gm(reinterpret_cast<const f2v*>(kDesignOne)), 0u, false};
+}
+// (r, kappa) of a voxel, (1, 1) outside the box
+__device__ __forceinline__ f2v box_val(const BoxTab& t, int off, bool in) {
+    const f2v v = ldo(t.p, static_cast<uint32_t>(off) & t.mask);
+    const f2v one = {1.0f, 1.0f};
+    return (in && t.has) ? v : one;
+}
+
 // ---------------------------------------------------------------- forward
-// TB: the strip (with its halo) meets the design box. Strips that do not
-// (most of the grid) run the code with the constant coefficients only.
-template <int R, bool TB>
-__device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_t s, int seg, int sj, int si,
+// Strips that miss the design box: r = kappa = 1, so every face coefficient
+// is cv0 / ((1 + 1) * 0.5) = cv0 exactly and every cell coefficient cp0.
+template <int R>
+__device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_t s, int seg, int j0, int i0, int i1,
                                                      int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = seg * kWave, k = k0 + lane;
     const bool kin = k < nz;
-    const int j0 = sj * R;
-    const int i0 = si * a.planes, i1 = min(i0 + a.planes, nx);
-    const int64_t sx = static_cast<int64_t>(ny) * nz;
+    const uint32_t sx = static_cast<uint32_t>(ny) * static_cast<uint32_t>(nz);
     const int par = static_cast<int>(s & 1);
     const GMEM float* __restrict__ P = gm(a.f[par][0]);
     const GMEM float* __restrict__ VX = gm(a.f[par][1]);
@@ -156,28 +189,22 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
     GMEM float* __restrict__ VYo = gm(a.f[par ^ 1][2]);
     GMEM float* __restrict__ VZo = gm(a.f[par ^ 1][3]);
     const GMEM uint8_t* __restrict__ FL = gm(a.flags);
-    const bool hist = a.hist_p && s >= a.hist0 && s < a.hist0 + a.hist_steps;
-    float* hp = hist ? a.hist_p + (s - a.hist0) * ebox_cells(a) : nullptr;
-    float* hd = hist ? a.hist_div + (s - a.hist0) * dbox_cells(a) : nullptr;
     const double wsrc = a.wave ? a.wave[s] : 0.0;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
-    constexpr bool tb = TB;
+    const float cv = a.cv0;
     const uint32_t pmask = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 0);
-    auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
-    auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
-    auto KP = [&](int ii, int jj, int kk) { return tb ? kv_(a, ii, jj, kk) : 1.0f; };
 
     bool rvld[R];
     float pc[R], vxm[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         rvld[r] = j0 + r < ny;
-        pc[r] = (rvld[r] && kin) ? P[lin(i0, j0 + r, k, ny, nz)] : 0.f;
+        pc[r] = (rvld[r] && kin) ? ldo(P, cix(i0, j0 + r, k, ny, nz)) : 0.f;
         vxm[r] = 0.f;
         if (i0 > 0 && rvld[r] && kin) {
-            const int64_t c = lin(i0 - 1, j0 + r, k, ny, nz);
-            const float v = VX[c] + CV(RV(i0 - 1, j0 + r, k), RV(i0, j0 + r, k)) * (pc[r] - P[c]);
-            vxm[r] = (FL[c] & FDTD_FLAG_OPEN_X) ? v : 0.f;
+            const uint32_t c = cix(i0 - 1, j0 + r, k, ny, nz);
+            const float v = ldo(VX, c) + cv * (pc[r] - ldo(P, c));
+            vxm[r] = (ldo(FL, c) & FDTD_FLAG_OPEN_X) ? v : 0.f;
         }
     }
     for (int i = i0; i < i1; ++i) {
@@ -190,80 +217,74 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
             pn[r] = vxo[r] = vyo[r] = vzo[r] = hr[r] = 0.f;
             fl[r] = 0;
             if (!(rvld[r] && kin)) continue;
-            const int64_t c = lin(i, j0 + r, k, ny, nz);
-            if (nxt) pn[r] = P[c + sx];
-            vxo[r] = VX[c];
-            vyo[r] = VY[c];
-            vzo[r] = VZ[c];
-            fl[r] = FL[c];
+            const uint32_t c = cix(i, j0 + r, k, ny, nz);
+            if (nxt) pn[r] = ldo(P, c + sx);
+            vxo[r] = ldo(VX, c);
+            vyo[r] = ldo(VY, c);
+            vzo[r] = ldo(VZ, c);
+            fl[r] = ldo(FL, c);
         }
-        // right k halo (lane 63) and the face below the first row
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (rvld[r] && lane == kWave - 1 && k + 1 < nz) hr[r] = P[lin(i, j0 + r, k + 1, ny, nz)];
+            if (rvld[r] && lane == kWave - 1 && k + 1 < nz) hr[r] = ldo(P, cix(i, j0 + r, k + 1, ny, nz));
         float p_up_halo = 0.f;
-        if (kin && j0 + R < ny) p_up_halo = P[lin(i, j0 + R, k, ny, nz)];
+        if (kin && j0 + R < ny) p_up_halo = ldo(P, cix(i, j0 + R, k, ny, nz));
         float vyprev = 0.f;
         if (kin && j0 > 0) {
-            const int64_t c = lin(i, j0 - 1, k, ny, nz);
-            const float v = VY[c] + CV(RV(i, j0 - 1, k), RV(i, j0, k)) * (pc[0] - P[c]);
-            vyprev = (FL[c] & FDTD_FLAG_OPEN_Y) ? v : 0.f;
+            const uint32_t c = cix(i, j0 - 1, k, ny, nz);
+            const float v = ldo(VY, c) + cv * (pc[0] - ldo(P, c));
+            vyprev = (ldo(FL, c) & FDTD_FLAG_OPEN_Y) ? v : 0.f;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!rvld[r]) continue;
             const int j = j0 + r;
-            const int64_t c = lin(i, j, k, ny, nz);
+            const uint32_t c = cix(i, j, k, ny, nz);
             const uint32_t f = fl[r];
             const float pr = pc[r];
-            const float rc = RV(i, j, k);
             float vxn = vxo[r];
             if (nxt) {
-                const float v = vxo[r] + CV(rc, RV(i + 1, j, k)) * (pn[r] - pr);
+                const float v = vxo[r] + cv * (pn[r] - pr);
                 vxn = (f & FDTD_FLAG_OPEN_X) ? v : 0.f;
             }
             float vyn = vyo[r];
             if (j + 1 < ny) {
                 const float pu = (r + 1 < R) ? pc[r + 1] : p_up_halo;
-                const float v = vyo[r] + CV(rc, RV(i, j + 1, k)) * (pu - pr);
+                const float v = vyo[r] + cv * (pu - pr);
                 vyn = (f & FDTD_FLAG_OPEN_Y) ? v : 0.f;
             }
             float pk1 = sdown(pr);
             if (lane == kWave - 1) pk1 = hr[r];
             float vzn = vzo[r];
             if (k + 1 < nz) {
-                const float v = vzo[r] + CV(rc, RV(i, j, k + 1)) * (pk1 - pr);
+                const float v = vzo[r] + cv * (pk1 - pr);
                 vzn = (f & FDTD_FLAG_OPEN_Z) ? v : 0.f;
             }
             float vzm = sup(vzn);
             if (lane == 0) {
                 vzm = 0.f;
                 if (k0 > 0 && kin) {
-                    const int64_t cl = c - 1;
-                    const float v = VZ[cl] + CV(RV(i, j, k - 1), rc) * (pr - P[cl]);
-                    vzm = (FL[cl] & FDTD_FLAG_OPEN_Z) ? v : 0.f;
+                    const uint32_t cl = c - 1;
+                    const float v = ldo(VZ, cl) + cv * (pr - ldo(P, cl));
+                    vzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z) ? v : 0.f;
                 }
             }
             const float div = ((vxn - vxm[r]) + (vyn - vyprev)) + (vzn - vzm);
-            float pnew = (f & FDTD_FLAG_AIR) ? pr + (a.cp0 * KP(i, j, k)) * div : 0.f;
+            float pnew = (f & FDTD_FLAG_AIR) ? pr + a.cp0 * div : 0.f;
             const float dy_j = a.dec[1][j];
             pnew = ((pnew * dx_i) * dy_j) * dz_k;
             if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
                 pnew = static_cast<float>(static_cast<double>(pnew) + wsrc);
             if (kin) {
-                Po[c] = pnew;
-                VXo[c] = vxn * dx_i;
-                VYo[c] = vyn * dy_j;
-                VZo[c] = vzn * dz_k;
+                stnt(Po, c, pnew);
+                stnt(VXo, c, vxn * dx_i);
+                stnt(VYo, c, vyn * dy_j);
+                stnt(VZo, c, vzn * dz_k);
                 if (pmask)
                     for (int q = 0; q < a.n_probe; ++q)
                         if (((pmask >> q) & 1u) && a.probe[3 * q] == i && a.probe[3 * q + 1] == j &&
                             a.probe[3 * q + 2] == k)
                             a.trace[s * a.n_probe + q] = pnew;
-                if (hist && tb) {
-                    if (ebox(a, i, j, k)) hp[ebox_lin(a, i, j, k)] = pr;
-                    if (dbox(a, i, j, k)) hd[dbox_lin(a, i, j, k)] = div;
-                }
             }
             vxm[r] = vxn;
             vyprev = vyn;
@@ -273,41 +294,254 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
     }
 }
 
-// Two launches per step: `box` = 0 runs every strip that misses the design
-// box (constant coefficients, the register budget of the plain sweep) and
-// skips the others; `box` = 1 runs only the strips that meet it, over the
-// sub-grid [b_seg0, b_seg1) x [b_sj0, b_sj1) x [b_si0, b_si1) of strips.
+// Strips that meet the design box (with their one-cell halo): per-voxel r and
+// kappa, and the forward history of the box. The face coefficient of each
+// face is formed from the same two r values in the same order as in the
+// per-cell kernel, so the results are the same bits.
 template <int R>
-__global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, int64_t s) {
-    const int lane = threadIdx.x & (kWave - 1);
-    // XCD-aware: hardware block b runs on XCD b % 8; each XCD gets a contiguous
-    // range of strips so neighbouring strips share their halo rows in one L2
-    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
-    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (w >= a.nwaves) return;
-    const int seg = w % a.nseg, t = w / a.nseg, sj = t % a.nstrip_j, si = t / a.nstrip_j;
-    if (seg >= a.b_seg0 && seg < a.b_seg1 && sj >= a.b_sj0 && sj < a.b_sj1 && si >= a.b_si0 && si < a.b_si1)
-        return;
-    design_forward_strip<R, false>(a, s, seg, sj, si, lane);
+__device__ __forceinline__ void design_forward_box_strip(const DesignArgs& a, int64_t s, int seg, int j0, int i0,
+                                                         int i1, int lane) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = seg * kWave, k = k0 + lane;
+    const bool kin = k < nz;
+    const uint32_t sx = static_cast<uint32_t>(ny) * static_cast<uint32_t>(nz);
+    const int par = static_cast<int>(s & 1);
+    const GMEM float* __restrict__ P = gm(a.f[par][0]);
+    const GMEM float* __restrict__ VX = gm(a.f[par][1]);
+    const GMEM float* __restrict__ VY = gm(a.f[par][2]);
+    const GMEM float* __restrict__ VZ = gm(a.f[par][3]);
+    GMEM float* __restrict__ Po = gm(a.f[par ^ 1][0]);
+    GMEM float* __restrict__ VXo = gm(a.f[par ^ 1][1]);
+    GMEM float* __restrict__ VYo = gm(a.f[par ^ 1][2]);
+    GMEM float* __restrict__ VZo = gm(a.f[par ^ 1][3]);
+    const GMEM uint8_t* __restrict__ FL = gm(a.flags);
+    const bool hist = a.hist_p && s >= a.hist0 && s < a.hist0 + a.hist_steps;
+    GMEM float* hp = hist ? gm(a.hist_p + (s - a.hist0) * ebox_cells(a)) : nullptr;
+    GMEM float* hd = hist ? gm(a.hist_div + (s - a.hist0) * dbox_cells(a)) : nullptr;
+    const double wsrc = a.wave ? a.wave[s] : 0.0;
+    const float dz_k = kin ? a.dec[2][k] : 1.0f;
+    const uint32_t pmask = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 0);
+
+    const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
+    const BoxTab RK = box_tab(a.rk);
+    // lane halo: lane 0 looks at k - 1, the others at k + 1 (lane 63's is used)
+    const int kh = lane == 0 ? k - 1 : k + 1;
+    const Ax dk = dax(k, a.lo[2], bz), dkh = dax(kh, a.lo[2], bz), ek = eax(k, a.lo[2], bz);
+    Ax dj[R], ej[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        dj[r] = dax(j0 + r, a.lo[1], by);
+        ej[r] = eax(j0 + r, a.lo[1], by);
+    }
+    const Ax djd = dax(j0 - 1, a.lo[1], by), dju = dax(j0 + R, a.lo[1], by);
+    auto boff = [&](const Ax& x, const Ax& y, const Ax& z) { return (x.off * by + y.off) * bz + z.off; };
+    auto RKV = [&](const Ax& x, const Ax& y, const Ax& z) { return box_val(RK, boff(x, y, z), x.in && y.in && z.in); };
+
+    bool rvld[R];
+    float pc[R], vxm[R], rcur[R], kcur[R];
+    {
+        const Ax di0 = dax(i0, a.lo[0], bx), dim = dax(i0 - 1, a.lo[0], bx);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            rvld[r] = j0 + r < ny;
+            pc[r] = (rvld[r] && kin) ? ldo(P, cix(i0, j0 + r, k, ny, nz)) : 0.f;
+            const f2v v0 = RKV(di0, dj[r], dk);
+            rcur[r] = v0.x;
+            kcur[r] = v0.y;
+            const float rm = RKV(dim, dj[r], dk).x;
+            vxm[r] = 0.f;
+            if (i0 > 0 && rvld[r] && kin) {
+                const uint32_t c = cix(i0 - 1, j0 + r, k, ny, nz);
+                const float v = ldo(VX, c) + cvf(a, rm, rcur[r]) * (pc[r] - ldo(P, c));
+                vxm[r] = (ldo(FL, c) & FDTD_FLAG_OPEN_X) ? v : 0.f;
+            }
+        }
+    }
+    for (int i = i0; i < i1; ++i) {
+        const bool nxt = i + 1 < nx;
+        const float dx_i = a.dec[0][i];
+        const Ax di = dax(i, a.lo[0], bx), din = dax(i + 1, a.lo[0], bx), ei = eax(i, a.lo[0], bx);
+        float pn[R], vxo[R], vyo[R], vzo[R], hr[R], rn[R], kn[R], rkh[R];
+        uint32_t fl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pn[r] = vxo[r] = vyo[r] = vzo[r] = hr[r] = 0.f;
+            fl[r] = 0;
+            const f2v vn = RKV(din, dj[r], dk);
+            rn[r] = vn.x;
+            kn[r] = vn.y;
+            rkh[r] = RKV(di, dj[r], dkh).x;
+            if (!(rvld[r] && kin)) continue;
+            const uint32_t c = cix(i, j0 + r, k, ny, nz);
+            if (nxt) pn[r] = ldo(P, c + sx);
+            vxo[r] = ldo(VX, c);
+            vyo[r] = ldo(VY, c);
+            vzo[r] = ldo(VZ, c);
+            fl[r] = ldo(FL, c);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rvld[r] && lane == kWave - 1 && k + 1 < nz) hr[r] = ldo(P, cix(i, j0 + r, k + 1, ny, nz));
+        const float r_up = RKV(di, dju, dk).x, r_dn = RKV(di, djd, dk).x;
+        float p_up_halo = 0.f;
+        if (kin && j0 + R < ny) p_up_halo = ldo(P, cix(i, j0 + R, k, ny, nz));
+        float vyprev = 0.f;
+        if (kin && j0 > 0) {
+            const uint32_t c = cix(i, j0 - 1, k, ny, nz);
+            const float v = ldo(VY, c) + cvf(a, r_dn, rcur[0]) * (pc[0] - ldo(P, c));
+            vyprev = (ldo(FL, c) & FDTD_FLAG_OPEN_Y) ? v : 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!rvld[r]) continue;
+            const int j = j0 + r;
+            const uint32_t c = cix(i, j, k, ny, nz);
+            const uint32_t f = fl[r];
+            const float pr = pc[r];
+            const float rc = rcur[r];
+            const float rc_dn = sdown(rc);  // every lane takes part in the shuffle
+            const float r_k1 = (lane == kWave - 1) ? rkh[r] : rc_dn;
+            float vxn = vxo[r];
+            if (nxt) {
+                const float v = vxo[r] + cvf(a, rc, rn[r]) * (pn[r] - pr);
+                vxn = (f & FDTD_FLAG_OPEN_X) ? v : 0.f;
+            }
+            float vyn = vyo[r];
+            if (j + 1 < ny) {
+                const float pu = (r + 1 < R) ? pc[r + 1] : p_up_halo;
+                const float v = vyo[r] + cvf(a, rc, (r + 1 < R) ? rcur[r + 1] : r_up) * (pu - pr);
+                vyn = (f & FDTD_FLAG_OPEN_Y) ? v : 0.f;
+            }
+            float pk1 = sdown(pr);
+            if (lane == kWave - 1) pk1 = hr[r];
+            float vzn = vzo[r];
+            if (k + 1 < nz) {
+                const float v = vzo[r] + cvf(a, rc, r_k1) * (pk1 - pr);
+                vzn = (f & FDTD_FLAG_OPEN_Z) ? v : 0.f;
+            }
+            float vzm = sup(vzn);
+            if (lane == 0) {
+                vzm = 0.f;
+                if (k0 > 0 && kin) {
+                    const uint32_t cl = c - 1;
+                    const float v = ldo(VZ, cl) + cvf(a, rkh[r], rc) * (pr - ldo(P, cl));
+                    vzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z) ? v : 0.f;
+                }
+            }
+            const float div = ((vxn - vxm[r]) + (vyn - vyprev)) + (vzn - vzm);
+            float pnew = (f & FDTD_FLAG_AIR) ? pr + (a.cp0 * kcur[r]) * div : 0.f;
+            const float dy_j = a.dec[1][j];
+            pnew = ((pnew * dx_i) * dy_j) * dz_k;
+            if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
+                pnew = static_cast<float>(static_cast<double>(pnew) + wsrc);
+            if (kin) {
+                stnt(Po, c, pnew);
+                stnt(VXo, c, vxn * dx_i);
+                stnt(VYo, c, vyn * dy_j);
+                stnt(VZo, c, vzn * dz_k);
+                if (pmask)
+                    for (int q = 0; q < a.n_probe; ++q)
+                        if (((pmask >> q) & 1u) && a.probe[3 * q] == i && a.probe[3 * q + 1] == j &&
+                            a.probe[3 * q + 2] == k)
+                            a.trace[s * a.n_probe + q] = pnew;
+                if (hist) {
+                    if (ei.in && ej[r].in && ek.in)
+                        sto(hp, static_cast<uint32_t>((ei.off * (by + 2) + ej[r].off) * (bz + 2) + ek.off), pr);
+                    if (di.in && dj[r].in && dk.in) sto(hd, static_cast<uint32_t>(boff(di, dj[r], dk)), div);
+                }
+            }
+            vxm[r] = vxn;
+            vyprev = vyn;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pc[r] = pn[r];
+            rcur[r] = rn[r];
+            kcur[r] = kn[r];
+        }
+    }
 }
 
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_design_forward_box(const DesignArgs a, int64_t s) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int lb = (blockIdx.x & 7) * a.b_blocks_per_xcd + (blockIdx.x >> 3);
-    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nseg = a.b_seg1 - a.b_seg0, nsj = a.b_sj1 - a.b_sj0;
-    if (w >= nseg * nsj * (a.b_si1 - a.b_si0)) return;
+// Strip number w among the strips outside the box sub-grid, numbered in
+// (si, sj, seg) order with seg fastest, as in the full grid, so that every XCD
+// gets an equal, contiguous share of the work.
+__device__ __forceinline__ void plain_strip(const DesignArgs& a, int w, int& seg, int& sj, int& si) {
+    const int nseg = a.nseg;
+    const int full = nseg * a.nstrip_j;  // strips of a plane range the box misses
+    const int bseg = a.b_seg1 - a.b_seg0, bsj = a.b_sj1 - a.b_sj0;
+    const int part = full - bseg * bsj;  // strips of a plane range the box meets
+    const int nbi = a.b_si1 - a.b_si0;
+    if (w < a.b_si0 * full || w >= a.b_si0 * full + nbi * part) {
+        if (w >= a.b_si0 * full) w -= nbi * part - nbi * full;
+        si = w / full;
+        seg = (w % full) % nseg;
+        sj = (w % full) / nseg;
+        return;
+    }
+    w -= a.b_si0 * full;
+    si = a.b_si0 + w / part;
+    int rem = w % part;
+    if (rem < a.b_sj0 * nseg) {
+        sj = rem / nseg;
+        seg = rem % nseg;
+        return;
+    }
+    rem -= a.b_sj0 * nseg;
+    const int oseg = nseg - bseg;
+    if (rem < bsj * oseg) {
+        sj = a.b_sj0 + rem / oseg;
+        const int q = rem % oseg;
+        seg = q < a.b_seg0 ? q : q + bseg;
+        return;
+    }
+    rem -= bsj * oseg;
+    sj = a.b_sj1 + rem / nseg;
+    seg = rem % nseg;
+}
+
+// The box launch covers the rows [b_sj0, b_sj1) x kDesignRows and planes
+// [b_i0, b_i1) of the box sub-grid in strips of `rows` rows (nb_j of them) and
+// box_planes planes (nb_i of them).
+__device__ __forceinline__ bool box_strip(const DesignArgs& a, int rows, int w, int& seg, int& j0, int& i0,
+                                          int& i1) {
+    const int nseg = a.b_seg1 - a.b_seg0, nbj = (a.b_sj1 - a.b_sj0) * (kDesignRows / rows);
+    if (w >= nseg * nbj * a.nb_i) return false;
     const int t = w / nseg;
-    design_forward_strip<R, true>(a, s, a.b_seg0 + w % nseg, a.b_sj0 + t % nsj, a.b_si0 + t / nsj, lane);
+    seg = a.b_seg0 + w % nseg;
+    j0 = a.b_sj0 * kDesignRows + (t % nbj) * rows;
+    if (j0 >= a.ny) return false;  // a split of the last, partial row strip
+    i0 = a.b_i0 + (t / nbj) * a.box_planes;
+    i1 = min(i0 + a.box_planes, a.b_i1);
+    return true;
+}
+
+// One launch per step over every strip: the box strips (heavier) first, then
+// the others; each XCD gets an equal contiguous share of both.
+template <int R>
+__global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, int64_t s) {
+    const int xcd = blockIdx.x & 7, bx = blockIdx.x >> 3;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (bx < a.b_blocks_per_xcd) {
+        int seg, j0, i0, i1;
+        if (box_strip(a, R, (xcd * a.b_blocks_per_xcd + bx) * kBlockWaves + wid, seg, j0, i0, i1))
+            design_forward_box_strip<R>(a, s, seg, j0, i0, i1, lane);
+        return;
+    }
+    const int w = (xcd * a.blocks_per_xcd + bx - a.b_blocks_per_xcd) * kBlockWaves + wid;
+    if (w >= a.nwaves) return;
+    int seg, sj, si;
+    plain_strip(a, w, seg, sj, si);
+    const int i0 = si * a.planes;
+    design_forward_strip<R>(a, s, seg, sj * R, i0, min(i0 + a.planes, a.nx), lane);
 }
 
 // ---------------------------------------------------------------- adjoint
 // pb = dX dY dZ (P + g at probe cells); q = air cp0 kappa pb.
-// pm: the strip (with its halo) holds a probe; tb: it meets the design box.
+// pm: the strip (with its halo) holds a probe.
 __device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
-                                           int64_t c, int64_t n, bool pm) {
-    float v = P[c];
+                                           uint32_t c, int64_t n, bool pm) {
+    float v = ldo(P, c);
     if (pm)
         for (int q = 0; q < a.n_probe; ++q)
             if (a.probe[3 * q] == i && a.probe[3 * q + 1] == j && a.probe[3 * q + 2] == k)
@@ -317,28 +551,20 @@ __device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float
 }
 
 __device__ __forceinline__ float design_q(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
-                                          int64_t c, int64_t n, bool pm, bool tb) {
-    if (!(a.flags[c] & FDTD_FLAG_AIR)) return 0.f;
-    return a.cp0 * (tb ? kv_(a, i, j, k) : 1.0f) * design_pb(a, P, i, j, k, c, n, pm);
-}
-
-__device__ __forceinline__ void kahan(float* sum, float* comp, int64_t b, float x) {
-    const float y = x - comp[b];
-    const float t = sum[b] + y;
-    comp[b] = (t - sum[b]) - y;
-    sum[b] = t;
+                                          uint32_t c, int64_t n, bool pm, float kap) {
+    if (!(ldo(gm(a.flags), c) & FDTD_FLAG_AIR)) return 0.f;
+    return a.cp0 * kap * design_pb(a, P, i, j, k, c, n, pm);
 }
 
 // Adjoint step n: (P, V) = lambda_{n+1} in set (n+1)&1 -> lambda_n in set n&1.
-template <int R, bool TB>
-__device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_t n, int seg, int sj, int si,
+// Strips that miss the design box (r = kappa = 1; no gradient).
+template <int R>
+__device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_t n, int seg, int j0, int i0, int i1,
                                                      int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = seg * kWave, k = k0 + lane;
     const bool kin = k < nz;
-    const int j0 = sj * R;
-    const int i0 = si * a.planes, i1 = min(i0 + a.planes, nx);
-    const int64_t sx = static_cast<int64_t>(ny) * nz;
+    const uint32_t sx = static_cast<uint32_t>(ny) * static_cast<uint32_t>(nz);
     const int src = static_cast<int>((n + 1) & 1);
     const GMEM float* __restrict__ P = gm(a.f[src][0]);
     const GMEM float* __restrict__ VX = gm(a.f[src][1]);
@@ -349,15 +575,9 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
     GMEM float* __restrict__ VYo = gm(a.f[src ^ 1][2]);
     GMEM float* __restrict__ VZo = gm(a.f[src ^ 1][3]);
     const GMEM uint8_t* __restrict__ FL = gm(a.flags);
-    const bool grad = a.gr_sum && n >= a.hist0 && n < a.hist0 + a.hist_steps;
-    const float* hp = grad ? a.hist_p + (n - a.hist0) * ebox_cells(a) : nullptr;
-    const float* hd = grad ? a.hist_div + (n - a.hist0) * dbox_cells(a) : nullptr;
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
-    constexpr bool tb = TB;
+    const float cv = a.cv0;
     const bool pm = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 1) != 0;
-    auto RV = [&](int ii, int jj, int kk) { return tb ? rv(a, ii, jj, kk) : 1.0f; };
-    auto CV = [&](float rl, float rh) { return tb ? cvf(a, rl, rh) : a.cv0; };
-    auto KP = [&](int ii, int jj, int kk) { return tb ? kv_(a, ii, jj, kk) : 1.0f; };
 
     bool rvld[R];
     float pbc[R], qc[R], wxm[R];
@@ -367,13 +587,13 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
         pbc[r] = qc[r] = wxm[r] = 0.f;
         if (!(rvld[r] && kin)) continue;
         const int j = j0 + r;
-        const int64_t c = lin(i0, j, k, ny, nz);
+        const uint32_t c = cix(i0, j, k, ny, nz);
         pbc[r] = design_pb(a, P, i0, j, k, c, n, pm);
-        qc[r] = (FL[c] & FDTD_FLAG_AIR) ? a.cp0 * KP(i0, j, k) * pbc[r] : 0.f;
+        qc[r] = (ldo(FL, c) & FDTD_FLAG_AIR) ? a.cp0 * pbc[r] : 0.f;
         if (i0 > 0) {
-            const int64_t cl = c - sx;
-            wxm[r] = (FL[cl] & FDTD_FLAG_OPEN_X)
-                         ? a.dec[0][i0 - 1] * VX[cl] + design_q(a, P, i0 - 1, j, k, cl, n, pm, tb) - qc[r]
+            const uint32_t cl = c - sx;
+            wxm[r] = (ldo(FL, cl) & FDTD_FLAG_OPEN_X)
+                         ? a.dec[0][i0 - 1] * ldo(VX, cl) + design_q(a, P, i0 - 1, j, k, cl, n, pm, 1.0f) - qc[r]
                          : 0.f;
         }
     }
@@ -388,110 +608,76 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
             fl[r] = 0;
             if (!(rvld[r] && kin)) continue;
             const int j = j0 + r;
-            const int64_t c = lin(i, j, k, ny, nz);
+            const uint32_t c = cix(i, j, k, ny, nz);
             if (nxt) {
                 pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm);
-                qn[r] = (FL[c + sx] & FDTD_FLAG_AIR) ? a.cp0 * KP(i + 1, j, k) * pbn[r] : 0.f;
+                qn[r] = (ldo(FL, c + sx) & FDTD_FLAG_AIR) ? a.cp0 * pbn[r] : 0.f;
             }
-            vx[r] = VX[c];
-            vy[r] = VY[c];
-            vz[r] = VZ[c];
-            fl[r] = FL[c];
-            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, tb);
+            vx[r] = ldo(VX, c);
+            vy[r] = ldo(VY, c);
+            vz[r] = ldo(VZ, c);
+            fl[r] = ldo(FL, c);
+            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, 1.0f);
         }
         float q_up_halo = 0.f;
-        if (kin && j0 + R < ny) q_up_halo = design_q(a, P, i, j0 + R, k, lin(i, j0 + R, k, ny, nz), n, pm, tb);
+        if (kin && j0 + R < ny)
+            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, 1.0f);
         float wyprev = 0.f;
         if (kin && j0 > 0) {
-            const int64_t cl = lin(i, j0 - 1, k, ny, nz);
-            wyprev = (FL[cl] & FDTD_FLAG_OPEN_Y)
-                         ? a.dec[1][j0 - 1] * VY[cl] + design_q(a, P, i, j0 - 1, k, cl, n, pm, tb) - qc[0]
+            const uint32_t cl = cix(i, j0 - 1, k, ny, nz);
+            wyprev = (ldo(FL, cl) & FDTD_FLAG_OPEN_Y)
+                         ? a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, 1.0f) - qc[0]
                          : 0.f;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!rvld[r]) continue;
             const int j = j0 + r;
-            const int64_t c = lin(i, j, k, ny, nz);
+            const uint32_t c = cix(i, j, k, ny, nz);
             const uint32_t f = fl[r];
-            const float rc = RV(i, j, k);
             const bool air = f & FDTD_FLAG_AIR;
             float acc = air ? pbc[r] : 0.f;
-            float gr = 0.f;
-            const bool gbox = grad && tb && kin && dbox(a, i, j, k);
-            const float hpc = gbox ? hp[ebox_lin(a, i, j, k)] : 0.f;
-            // own faces (c = lo)
             float wx, wy, wz;
             const float dy_j = a.dec[1][j];
             if (nxt) {
-                const float rh = RV(i + 1, j, k);
-                const float cv = CV(rc, rh);
                 wx = (f & FDTD_FLAG_OPEN_X) ? dx_i * vx[r] + qc[r] - qn[r] : 0.f;
                 acc -= cv * wx;
-                if (gbox) gr += wx * (hp[ebox_lin(a, i + 1, j, k)] - hpc) * (-cv / (rc + rh));
             } else {
                 wx = dx_i * vx[r] + qc[r];
             }
             if (j + 1 < ny) {
-                const float rh = RV(i, j + 1, k);
-                const float cv = CV(rc, rh);
                 const float qu = (r + 1 < R) ? qc[r + 1] : q_up_halo;
                 wy = (f & FDTD_FLAG_OPEN_Y) ? dy_j * vy[r] + qc[r] - qu : 0.f;
                 acc -= cv * wy;
-                if (gbox) gr += wy * (hp[ebox_lin(a, i, j + 1, k)] - hpc) * (-cv / (rc + rh));
             } else {
                 wy = dy_j * vy[r] + qc[r];
             }
             float qk1 = sdown(qc[r]);
             if (lane == kWave - 1) qk1 = qhr[r];
             if (k + 1 < nz) {
-                const float rh = RV(i, j, k + 1);
-                const float cv = CV(rc, rh);
                 wz = (f & FDTD_FLAG_OPEN_Z) ? dz_k * vz[r] + qc[r] - qk1 : 0.f;
                 acc -= cv * wz;
-                if (gbox) gr += wz * (hp[ebox_lin(a, i, j, k + 1)] - hpc) * (-cv / (rc + rh));
             } else {
                 wz = dz_k * vz[r] + qc[r];
             }
-            // faces below (c = hi)
             float wzm = sup(wz);
             if (lane == 0) {
                 wzm = 0.f;
                 if (k0 > 0 && kin) {
-                    const int64_t cl = c - 1;
-                    wzm = (FL[cl] & FDTD_FLAG_OPEN_Z)
-                              ? a.dec[2][k - 1] * VZ[cl] + design_q(a, P, i, j, k - 1, cl, n, pm, tb) - qc[r]
+                    const uint32_t cl = c - 1;
+                    wzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z)
+                              ? a.dec[2][k - 1] * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, 1.0f) - qc[r]
                               : 0.f;
                 }
             }
-            if (i > 0) {
-                const float rl = RV(i - 1, j, k);
-                const float cv = CV(rl, rc);
-                acc += cv * wxm[r];
-                if (gbox) gr += wxm[r] * (hpc - hp[ebox_lin(a, i - 1, j, k)]) * (-cv / (rl + rc));
-            }
-            if (j > 0) {
-                const float rl = RV(i, j - 1, k);
-                const float cv = CV(rl, rc);
-                acc += cv * wyprev;
-                if (gbox) gr += wyprev * (hpc - hp[ebox_lin(a, i, j - 1, k)]) * (-cv / (rl + rc));
-            }
-            if (k > 0) {
-                const float rl = RV(i, j, k - 1);
-                const float cv = CV(rl, rc);
-                acc += cv * wzm;
-                if (gbox) gr += wzm * (hpc - hp[ebox_lin(a, i, j, k - 1)]) * (-cv / (rl + rc));
-            }
+            if (i > 0) acc += cv * wxm[r];
+            if (j > 0) acc += cv * wyprev;
+            if (k > 0) acc += cv * wzm;
             if (kin) {
-                Po[c] = acc;
-                VXo[c] = wx;
-                VYo[c] = wy;
-                VZo[c] = wz;
-                if (gbox) {
-                    const int b = dbox_lin(a, i, j, k);
-                    kahan(a.gr_sum, a.gr_comp, b, gr);
-                    if (air) kahan(a.gk_sum, a.gk_comp, b, pbc[r] * a.cp0 * hd[b]);
-                }
+                stnt(Po, c, acc);
+                stnt(VXo, c, wx);
+                stnt(VYo, c, wy);
+                stnt(VZo, c, wz);
             }
             wxm[r] = wx;
             wyprev = wy;
@@ -504,30 +690,280 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
     }
 }
 
+// Strips that meet the design box: per-voxel coefficients and the gradient.
+// Each face's coefficient cv, its gradient factor e = -cv / (r_lo + r_hi) and
+// the forward pressure difference dh across it are formed once, where the
+// face is the cell's own (upper) face, and passed on to the cell above it:
+// along i in registers from plane to plane, along j from row to row, along k
+// from lane to lane. Each is the same expression of the same two values as in
+// the per-cell kernel, so the results are the same bits.
 template <int R>
+__device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, int64_t n, int seg, int j0, int i0,
+                                                         int i1, int lane) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int k0 = seg * kWave, k = k0 + lane;
+    const bool kin = k < nz;
+    const uint32_t sx = static_cast<uint32_t>(ny) * static_cast<uint32_t>(nz);
+    const int src = static_cast<int>((n + 1) & 1);
+    const GMEM float* __restrict__ P = gm(a.f[src][0]);
+    const GMEM float* __restrict__ VX = gm(a.f[src][1]);
+    const GMEM float* __restrict__ VY = gm(a.f[src][2]);
+    const GMEM float* __restrict__ VZ = gm(a.f[src][3]);
+    GMEM float* __restrict__ Po = gm(a.f[src ^ 1][0]);
+    GMEM float* __restrict__ VXo = gm(a.f[src ^ 1][1]);
+    GMEM float* __restrict__ VYo = gm(a.f[src ^ 1][2]);
+    GMEM float* __restrict__ VZo = gm(a.f[src ^ 1][3]);
+    const GMEM uint8_t* __restrict__ FL = gm(a.flags);
+    // the entry point admits only steps whose history is held
+    const GMEM float* __restrict__ hp = gm(a.hist_p + (n - a.hist0) * ebox_cells(a));
+    const GMEM float* __restrict__ hd = gm(a.hist_div + (n - a.hist0) * dbox_cells(a));
+    GMEM f4v* __restrict__ ACC = gm(a.acc);
+    const float dz_k = kin ? a.dec[2][k] : 1.0f;
+    const bool pm = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 1) != 0;
+
+    const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
+    const BoxTab RK = box_tab(a.rk);
+    const int kh = lane == 0 ? k - 1 : k + 1;
+    const Ax dk = dax(k, a.lo[2], bz), dkh = dax(kh, a.lo[2], bz);
+    const Ax ek = eax(k, a.lo[2], bz), ekh = eax(kh, a.lo[2], bz);
+    Ax dj[R], ej[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        dj[r] = dax(j0 + r, a.lo[1], by);
+        ej[r] = eax(j0 + r, a.lo[1], by);
+    }
+    const Ax djd = dax(j0 - 1, a.lo[1], by), dju = dax(j0 + R, a.lo[1], by);
+    const Ax ejd = eax(j0 - 1, a.lo[1], by), eju = eax(j0 + R, a.lo[1], by);
+    auto boff = [&](const Ax& x, const Ax& y, const Ax& z) { return (x.off * by + y.off) * bz + z.off; };
+    auto RKV = [&](const Ax& x, const Ax& y, const Ax& z) { return box_val(RK, boff(x, y, z), x.in && y.in && z.in); };
+    // forward pressure on the extended box (0 outside it; never used there)
+    auto HV = [&](const Ax& x, const Ax& y, const Ax& z) {
+        const float v = ldo(hp, static_cast<uint32_t>((x.off * (by + 2) + y.off) * (bz + 2) + z.off));
+        return (x.in && y.in && z.in) ? v : 0.f;
+    };
+
+    bool rvld[R];
+    float pbc[R], qc[R], wxm[R], rcur[R], hcur[R], cvxm[R], exm[R], dhxm[R];
+    {
+        const Ax di0 = dax(i0, a.lo[0], bx), dim = dax(i0 - 1, a.lo[0], bx);
+        const Ax ei0 = eax(i0, a.lo[0], bx), eim = eax(i0 - 1, a.lo[0], bx);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            rvld[r] = j0 + r < ny;
+            pbc[r] = qc[r] = wxm[r] = 0.f;
+            const f2v v0 = RKV(di0, dj[r], dk), vm = RKV(dim, dj[r], dk);
+            rcur[r] = v0.x;
+            hcur[r] = HV(ei0, ej[r], ek);
+            const float rm = vm.x, hm = HV(eim, ej[r], ek);
+            const float km = vm.y, kc = v0.y;
+            cvxm[r] = cvf(a, rm, rcur[r]);
+            exm[r] = -cvxm[r] / (rm + rcur[r]);
+            dhxm[r] = hcur[r] - hm;
+            if (!(rvld[r] && kin)) continue;
+            const int j = j0 + r;
+            const uint32_t c = cix(i0, j, k, ny, nz);
+            pbc[r] = design_pb(a, P, i0, j, k, c, n, pm);
+            qc[r] = (ldo(FL, c) & FDTD_FLAG_AIR) ? a.cp0 * kc * pbc[r] : 0.f;
+            if (i0 > 0) {
+                const uint32_t cl = c - sx;
+                wxm[r] = (ldo(FL, cl) & FDTD_FLAG_OPEN_X)
+                             ? a.dec[0][i0 - 1] * ldo(VX, cl) + design_q(a, P, i0 - 1, j, k, cl, n, pm, km) - qc[r]
+                             : 0.f;
+            }
+        }
+    }
+    for (int i = i0; i < i1; ++i) {
+        const bool nxt = i + 1 < nx;
+        const float dx_i = a.dec[0][i];
+        const Ax di = dax(i, a.lo[0], bx), din = dax(i + 1, a.lo[0], bx);
+        const Ax ei = eax(i, a.lo[0], bx), ein = eax(i + 1, a.lo[0], bx);
+        float pbn[R], qn[R], vx[R], vy[R], vz[R], qhr[R];
+        float rn[R], hn[R], rkh[R], hkh[R], kkh[R];
+        float hdv[R];
+        f4v g4[R];
+        uint32_t fl[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pbn[r] = qn[r] = vx[r] = vy[r] = vz[r] = qhr[r] = 0.f;
+            fl[r] = 0;
+            const f2v vn = RKV(din, dj[r], dk), vh = RKV(di, dj[r], dkh);
+            rn[r] = vn.x;
+            hn[r] = HV(ein, ej[r], ek);
+            rkh[r] = vh.x;
+            hkh[r] = HV(ei, ej[r], ekh);
+            kkh[r] = vh.y;
+            const float kn = vn.y;
+            const uint32_t b = static_cast<uint32_t>(boff(di, dj[r], dk));
+            hdv[r] = ldo(hd, b);
+            g4[r] = ldo(ACC, b);
+            if (!(rvld[r] && kin)) continue;
+            const int j = j0 + r;
+            const uint32_t c = cix(i, j, k, ny, nz);
+            if (nxt) {
+                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm);
+                qn[r] = (ldo(FL, c + sx) & FDTD_FLAG_AIR) ? a.cp0 * kn * pbn[r] : 0.f;
+            }
+            vx[r] = ldo(VX, c);
+            vy[r] = ldo(VY, c);
+            vz[r] = ldo(VZ, c);
+            fl[r] = ldo(FL, c);
+            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, kkh[r]);
+        }
+        const f2v v_up = RKV(di, dju, dk), v_dn = RKV(di, djd, dk);
+        const float r_up = v_up.x, r_dn = v_dn.x, k_up = v_up.y, k_dn = v_dn.y;
+        const float h_up = HV(ei, eju, ek), h_dn = HV(ei, ejd, ek);
+        float q_up_halo = 0.f;
+        if (kin && j0 + R < ny)
+            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, k_up);
+        float wyprev = 0.f;
+        if (kin && j0 > 0) {
+            const uint32_t cl = cix(i, j0 - 1, k, ny, nz);
+            wyprev = (ldo(FL, cl) & FDTD_FLAG_OPEN_Y)
+                         ? a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, k_dn) - qc[0]
+                         : 0.f;
+        }
+        // the face below the first row
+        float cvyp = cvf(a, r_dn, rcur[0]);
+        float eyp = -cvyp / (r_dn + rcur[0]);
+        float dhyp = hcur[0] - h_dn;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!rvld[r]) continue;
+            const int j = j0 + r;
+            const uint32_t c = cix(i, j, k, ny, nz);
+            const uint32_t f = fl[r];
+            const float rc = rcur[r], hpc = hcur[r];
+            const bool air = f & FDTD_FLAG_AIR;
+            float acc = air ? pbc[r] : 0.f;
+            float gr = 0.f;
+            const bool gbox = kin && di.in && dj[r].in && dk.in;
+            const float rc_dn = sdown(rc), hpc_dn = sdown(hpc);  // every lane shuffles
+            const float r_k1 = (lane == kWave - 1) ? rkh[r] : rc_dn;
+            const float h_k1 = (lane == kWave - 1) ? hkh[r] : hpc_dn;
+            // own faces (c = lo)
+            float wx, wy, wz;
+            float cvx = 0.f, ex = 0.f, dhx = 0.f, cvy = 0.f, ey = 0.f, dhy = 0.f, cvz = 0.f, ez = 0.f, dhz = 0.f;
+            const float dy_j = a.dec[1][j];
+            if (nxt) {
+                cvx = cvf(a, rc, rn[r]);
+                wx = (f & FDTD_FLAG_OPEN_X) ? dx_i * vx[r] + qc[r] - qn[r] : 0.f;
+                acc -= cvx * wx;
+                ex = -cvx / (rc + rn[r]);
+                dhx = hn[r] - hpc;
+                if (gbox) gr += wx * dhx * ex;
+            } else {
+                wx = dx_i * vx[r] + qc[r];
+            }
+            if (j + 1 < ny) {
+                const float rh = (r + 1 < R) ? rcur[r + 1] : r_up;
+                cvy = cvf(a, rc, rh);
+                const float qu = (r + 1 < R) ? qc[r + 1] : q_up_halo;
+                wy = (f & FDTD_FLAG_OPEN_Y) ? dy_j * vy[r] + qc[r] - qu : 0.f;
+                acc -= cvy * wy;
+                ey = -cvy / (rc + rh);
+                dhy = ((r + 1 < R) ? hcur[r + 1] : h_up) - hpc;
+                if (gbox) gr += wy * dhy * ey;
+            } else {
+                wy = dy_j * vy[r] + qc[r];
+            }
+            float qk1 = sdown(qc[r]);
+            if (lane == kWave - 1) qk1 = qhr[r];
+            if (k + 1 < nz) {
+                cvz = cvf(a, rc, r_k1);
+                wz = (f & FDTD_FLAG_OPEN_Z) ? dz_k * vz[r] + qc[r] - qk1 : 0.f;
+                acc -= cvz * wz;
+                ez = -cvz / (rc + r_k1);
+                dhz = h_k1 - hpc;
+                if (gbox) gr += wz * dhz * ez;
+            } else {
+                wz = dz_k * vz[r] + qc[r];
+            }
+            // faces below (c = hi): the z face from the lane below
+            float wzm = sup(wz);
+            float cvzm = sup(cvz), ezm = sup(ez), dhzm = sup(dhz);
+            if (lane == 0) {
+                wzm = 0.f;
+                cvzm = cvf(a, rkh[r], rc);
+                ezm = -cvzm / (rkh[r] + rc);
+                dhzm = hpc - hkh[r];
+                if (k0 > 0 && kin) {
+                    const uint32_t cl = c - 1;
+                    wzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z)
+                              ? a.dec[2][k - 1] * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, kkh[r]) - qc[r]
+                              : 0.f;
+                }
+            }
+            if (i > 0) {
+                acc += cvxm[r] * wxm[r];
+                if (gbox) gr += wxm[r] * dhxm[r] * exm[r];
+            }
+            if (j > 0) {
+                acc += cvyp * wyprev;
+                if (gbox) gr += wyprev * dhyp * eyp;
+            }
+            if (k > 0) {
+                acc += cvzm * wzm;
+                if (gbox) gr += wzm * dhzm * ezm;
+            }
+            if (kin) {
+                stnt(Po, c, acc);
+                stnt(VXo, c, wx);
+                stnt(VYo, c, wy);
+                stnt(VZo, c, wz);
+                if (gbox) {
+                    // Kahan: sum += x with the compensation of the lost low part
+                    const uint32_t b = static_cast<uint32_t>(boff(di, dj[r], dk));
+                    f4v g = g4[r];
+                    float y = gr - g.y;
+                    float t = g.x + y;
+                    g.y = (t - g.x) - y;
+                    g.x = t;
+                    if (air) {
+                        y = pbc[r] * a.cp0 * hdv[r] - g.w;
+                        t = g.z + y;
+                        g.w = (t - g.z) - y;
+                        g.z = t;
+                    }
+                    sto(ACC, b, g);
+                }
+            }
+            wxm[r] = wx;
+            cvxm[r] = cvx;
+            exm[r] = ex;
+            dhxm[r] = dhx;
+            wyprev = wy;
+            cvyp = cvy;
+            eyp = ey;
+            dhyp = dhy;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pbc[r] = pbn[r];
+            qc[r] = qn[r];
+            rcur[r] = rn[r];
+            hcur[r] = hn[r];
+        }
+    }
+}
+
+template <int R, int RB>
 __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, int64_t n) {
+    const int xcd = blockIdx.x & 7, bx = blockIdx.x >> 3;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & (kWave - 1);
-    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
-    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (w >= a.nwaves) return;
-    const int seg = w % a.nseg, t = w / a.nseg, sj = t % a.nstrip_j, si = t / a.nstrip_j;
-    if (seg >= a.b_seg0 && seg < a.b_seg1 && sj >= a.b_sj0 && sj < a.b_sj1 && si >= a.b_si0 && si < a.b_si1)
+    if (bx < a.b_blocks_per_xcd) {
+        int seg, j0, i0, i1;
+        if (box_strip(a, RB, (xcd * a.b_blocks_per_xcd + bx) * kBlockWaves + wid, seg, j0, i0, i1))
+            design_adjoint_box_strip<RB>(a, n, seg, j0, i0, i1, lane);
         return;
-    design_adjoint_strip<R, false>(a, n, seg, sj, si, lane);
+    }
+    const int w = (xcd * a.blocks_per_xcd + bx - a.b_blocks_per_xcd) * kBlockWaves + wid;
+    if (w >= a.nwaves) return;
+    int seg, sj, si;
+    plain_strip(a, w, seg, sj, si);
+    const int i0 = si * a.planes;
+    design_adjoint_strip<R>(a, n, seg, sj * R, i0, min(i0 + a.planes, a.nx), lane);
 }
-
-template <int R>
-__global__ __launch_bounds__(kBlock) void k_design_adjoint_box(const DesignArgs a, int64_t n) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int lb = (blockIdx.x & 7) * a.b_blocks_per_xcd + (blockIdx.x >> 3);
-    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int nseg = a.b_seg1 - a.b_seg0, nsj = a.b_sj1 - a.b_sj0;
-    if (w >= nseg * nsj * (a.b_si1 - a.b_si0)) return;
-    const int t = w / nseg;
-    design_adjoint_strip<R, true>(a, n, a.b_seg0 + w % nseg, a.b_sj0 + t % nsj, a.b_si0 + t / nsj, lane);
-}
-
-constexpr int kDesignRows = 4;
 
 // Strips [c0, c1) of extent `ext` along an axis of n cells whose span with a
 // one-cell halo, [c * ext - 1, min((c + 1) * ext, n)], meets [lo, hi).
@@ -544,21 +980,27 @@ static void meeting_strips(int n, int ext, int lo, int hi, int count, int& c0, i
     if (c1 <= c0) c0 = c1 = 0;
 }
 
-// Returns {blocks of the full-grid launch, blocks of the box launch}.
-static void design_grid(DesignArgs& a, int planes, int& blocks, int& box_blocks) {
+// Hardware block b runs on XCD b % 8. Per XCD the launch has
+// b_blocks_per_xcd blocks of box strips and blocks_per_xcd of the others;
+// returns the number of blocks.
+static int design_grid(DesignArgs& a, int planes, int box_rows) {
     a.planes = planes;
     a.nseg = (a.nz + kWave - 1) / kWave;
     a.nstrip_j = (a.ny + kDesignRows - 1) / kDesignRows;
     const int nstrip_i = (a.nx + planes - 1) / planes;
-    a.nwaves = a.nseg * a.nstrip_j * nstrip_i;
-    blocks = (((a.nwaves + kBlockWaves - 1) / kBlockWaves) + 7) & ~7;
-    a.blocks_per_xcd = blocks / 8;
     meeting_strips(a.nz, kWave, a.lo[2], a.hi[2], a.nseg, a.b_seg0, a.b_seg1);
     meeting_strips(a.ny, kDesignRows, a.lo[1], a.hi[1], a.nstrip_j, a.b_sj0, a.b_sj1);
     meeting_strips(a.nx, planes, a.lo[0], a.hi[0], nstrip_i, a.b_si0, a.b_si1);
-    const int bw = (a.b_seg1 - a.b_seg0) * (a.b_sj1 - a.b_sj0) * (a.b_si1 - a.b_si0);
-    box_blocks = (((bw + kBlockWaves - 1) / kBlockWaves) + 7) & ~7;
-    a.b_blocks_per_xcd = box_blocks / 8;
+    const int bseg = a.b_seg1 - a.b_seg0, bsj = a.b_sj1 - a.b_sj0, bsi = a.b_si1 - a.b_si0;
+    a.nwaves = a.nseg * a.nstrip_j * nstrip_i - bseg * bsj * bsi;
+    a.blocks_per_xcd = ((a.nwaves + kBlockWaves - 1) / kBlockWaves + 7) / 8;
+    a.box_planes = std::min(planes, kDesignBoxPlanes);
+    a.b_i0 = a.b_si0 * planes;
+    a.b_i1 = std::min(a.b_si1 * planes, a.nx);
+    a.nb_i = bsi ? (a.b_i1 - a.b_i0 + a.box_planes - 1) / a.box_planes : 0;
+    const int bw = bseg * bsj * (kDesignRows / box_rows) * a.nb_i;
+    a.b_blocks_per_xcd = ((bw + kBlockWaves - 1) / kBlockWaves + 7) / 8;
+    return 8 * (a.blocks_per_xcd + a.b_blocks_per_xcd);
 }
 
 }  // namespace fdtd_amd
@@ -568,10 +1010,11 @@ using namespace fdtd_amd;
 namespace {
 
 int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny,
-                int nz, float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
-                const float* kappa, const int32_t* src_ijk, const double* wave, int n_probe,
-                const int32_t* probe_ijk) {
+                int nz, float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
+                const int32_t* src_ijk, const double* wave, int n_probe, const int32_t* probe_ijk) {
     if (nx < 2 || ny < 2 || nz < 2 || !fields || !flags || !dec) return -2;
+    // 32-bit cell offsets: fewer than 2^30 cells (every byte offset below 4 GiB)
+    if (static_cast<int64_t>(nx) * ny * nz >= (1ll << 30)) return -3;
     if (n_probe < 0 || n_probe > kDesignMaxProbes || (n_probe > 0 && !probe_ijk)) return -2;
     a = DesignArgs{};
     for (int s = 0; s < 2; ++s)
@@ -595,10 +1038,10 @@ int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const
         a.hi[d] = box_hi ? box_hi[d] : 0;
         if (a.lo[d] < 1 || a.hi[d] > n[d] - 1 || a.hi[d] <= a.lo[d]) return -3;  // box + 1-cell margin inside
     }
-    if (static_cast<int64_t>(a.hi[0] - a.lo[0] + 2) * (a.hi[1] - a.lo[1] + 2) * (a.hi[2] - a.lo[2] + 2) >= (1ll << 31))
-        return -3;  // box indices are 32-bit
-    a.r = r;
-    a.kappa = kappa;
+    // 32-bit box offsets: the 16-byte accumulators of the box below 4 GiB
+    if (static_cast<int64_t>(a.hi[0] - a.lo[0] + 2) * (a.hi[1] - a.lo[1] + 2) * (a.hi[2] - a.lo[2] + 2) >= (1ll << 28))
+        return -3;
+    a.rk = reinterpret_cast<const f2v*>(rk);
     a.src_i = a.src_j = a.src_k = -1;
     if (src_ijk) {
         a.src_i = src_ijk[0];
@@ -625,12 +1068,12 @@ int64_t fdtd_design_history_bytes(const int32_t* box_lo, const int32_t* box_hi, 
 }
 
 int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
-                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
-                        const float* kappa, const int32_t* src_ijk, const double* wave, int n_probe,
-                        const int32_t* probe_ijk, float* trace, int64_t step0, int n_steps, float* hist_p,
-                        float* hist_div, int64_t hist0, int64_t hist_steps, int planes, void* stream) {
+                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
+                        const int32_t* src_ijk, const double* wave, int n_probe, const int32_t* probe_ijk,
+                        float* trace, int64_t step0, int n_steps, float* hist_p, float* hist_div, int64_t hist0,
+                        int64_t hist_steps, int planes, void* stream) {
     DesignArgs a;
-    if (int rc = fill_design(a, fields, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, r, kappa, src_ijk, wave,
+    if (int rc = fill_design(a, fields, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, rk, src_ijk, wave,
                              n_probe, probe_ijk))
         return rc;
     if (n_steps < 0 || step0 < 0 || planes < 1 || (n_probe > 0 && !trace) || (src_ijk && !wave)) return -2;
@@ -640,47 +1083,38 @@ int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float*
     a.hist_div = hist_div;
     a.hist0 = hist0;
     a.hist_steps = hist_p ? hist_steps : 0;
-    int blocks = 0, box_blocks = 0;
-    design_grid(a, planes, blocks, box_blocks);
+    const int blocks = design_grid(a, planes, kDesignRows);
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (int m = 0; m < n_steps; ++m) {
         hipLaunchKernelGGL(k_design_forward<kDesignRows>, dim3(blocks), dim3(kBlock), 0, st, a, step0 + m);
-        if (box_blocks)
-            hipLaunchKernelGGL(k_design_forward_box<kDesignRows>, dim3(box_blocks), dim3(kBlock), 0, st, a,
-                               step0 + m);
         if (hipGetLastError() != hipSuccess) return -5;
     }
     return 0;
 }
 
 int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
-                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* r,
-                        const float* kappa, int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo,
-                        int64_t n_hi, const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps,
-                        float* gr_sum, float* gr_comp, float* gk_sum, float* gk_comp, int planes, void* stream) {
+                        float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
+                        int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo, int64_t n_hi,
+                        const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps, float* acc,
+                        int planes, void* stream) {
     DesignArgs a;
-    if (int rc = fill_design(a, lam, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, r, kappa, nullptr, nullptr,
-                             n_probe, probe_ijk))
+    if (int rc = fill_design(a, lam, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, rk, nullptr, nullptr, n_probe,
+                             probe_ijk))
         return rc;
     if (n_hi < n_lo || n_lo < 0 || planes < 1 || (n_probe > 0 && !g)) return -2;
-    if (!hist_p || !hist_div || !gr_sum || !gr_comp || !gk_sum || !gk_comp) return -2;
+    if (!hist_p || !hist_div || !acc) return -2;
     if (n_lo < hist0 || n_hi > hist0 + hist_steps) return -2;  // every step needs its history
     a.g = g;
     a.hist_p = const_cast<float*>(hist_p);
     a.hist_div = const_cast<float*>(hist_div);
     a.hist0 = hist0;
     a.hist_steps = hist_steps;
-    a.gr_sum = gr_sum;
-    a.gr_comp = gr_comp;
-    a.gk_sum = gk_sum;
-    a.gk_comp = gk_comp;
-    int blocks = 0, box_blocks = 0;
-    design_grid(a, planes, blocks, box_blocks);
+    a.acc = reinterpret_cast<f4v*>(acc);
+    const int blocks = design_grid(a, planes, kDesignAdjBoxRows);
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (int64_t n = n_hi - 1; n >= n_lo; --n) {
-        hipLaunchKernelGGL(k_design_adjoint<kDesignRows>, dim3(blocks), dim3(kBlock), 0, st, a, n);
-        if (box_blocks)
-            hipLaunchKernelGGL(k_design_adjoint_box<kDesignRows>, dim3(box_blocks), dim3(kBlock), 0, st, a, n);
+        hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows>), dim3(blocks), dim3(kBlock), 0, st, a,
+                           n);
         if (hipGetLastError() != hipSuccess) return -5;
     }
     return 0;

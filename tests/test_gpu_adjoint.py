@@ -169,3 +169,36 @@ def test_fast_gradient_equals_cell_kernels(history):
     for x, y in zip(out["cell"], out["fast"]):
         assert torch.equal(x, y), history
     assert out["fast"][1].abs().max() > 0 and out["fast"][2].abs().max() > 0
+
+
+def test_fast_equals_cell_kernels_on_many_strips():
+    """A grid of many strips: three 64-wide k segments of which the box meets
+    one, a partial last row strip inside the box's row range, several plane
+    strips; probes inside and outside the box. The strips that miss the box
+    (constant coefficients) and the box strips (split finer, per-voxel
+    coefficients, gradient) together give the per-cell kernels' traces and
+    gradients bit for bit, also with rho and K absent (= air)."""
+    import torch
+    from strata_fdtd_amd import GaussianPulse
+    from strata_fdtd_amd.adjoint import DesignProblem, forward_traces, probe_traces
+    shape = (40, 37, 150)
+    air = np.ones(shape, bool)
+    air[12:15, 20:30, 80:84] = False
+    pb = DesignProblem(shape=shape, pml_depth=3, source=GaussianPulse(position=(20, 18, 60), frequency=9000.0),
+                       probes=[(20, 30, 100), (33, 5, 20), (10, 35, 140), (29, 8, 71)], steps=45,
+                       box_lo=(9, 7, 70), box_hi=(30, 36, 110), checkpoint=10, geometry=air)
+    print(f"[adjoint] many strips {shape}", flush=True)
+    a, _ = forward_traces(pb, None, None, impl="cell")
+    b, _ = forward_traces(pb, None, None, impl="fast", keep_checkpoints=True, history="all")
+    assert torch.equal(a, b)
+    rho, K, target = _params(pb, torch.float32, seed=9)
+    out = {}
+    for impl in ("cell", "fast"):
+        r = rho.clone().requires_grad_(True)
+        k = K.clone().requires_grad_(True)
+        tr = probe_traces(r, k, pb, impl=impl, history="all")
+        ((tr - target) ** 2).sum().backward()
+        out[impl] = (tr.detach(), r.grad, k.grad)
+    for x, y in zip(out["cell"], out["fast"]):
+        assert torch.equal(x, y)
+    assert out["fast"][0].abs().max() > 0 and out["fast"][1].abs().max() > 0
