@@ -380,6 +380,8 @@ static int ready(fdtd_ctx* c) {
     if (!c->have_grid || !c->have_fields || !c->have_flags || !c->have_pml || !c->have_counters)
         return fail(-3, "context incomplete: grid=%d fields=%d flags=%d pml=%d counters=%d", c->have_grid,
                     c->have_fields, c->have_flags, c->have_pml, c->have_counters);
+    if (c->stack > 1 && static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz >= kMaxCells32)
+        return fail(-2, "stacked sweeps need fewer than 2^30 cells; use stack 1");
     return 0;
 }
 
@@ -695,6 +697,8 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
         if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0)
             return fail(-2, "batch entry %d: nonuniform, ADE or microphone solves run on their own context", q);
         if (c->device != b->device) return fail(-2, "batch entry %d is on another device", q);
+        if (static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz >= kMaxCells32)
+            return fail(-2, "batch entry %d: batched solves need fewer than 2^30 cells", q);
     }
     if (n > b->cap || !b->meta_dev) {
         // growth (rare): no queued work may still read the old buffers

@@ -96,38 +96,21 @@ __device__ __forceinline__ uint32_t strip_probes(const DesignArgs& a, int i0, in
     return m;
 }
 
-#define GMEM __attribute__((address_space(1)))
 template <typename T>
-__device__ __forceinline__ GMEM T* gm(T* p) { return (GMEM T*)p; }
+__device__ __forceinline__ GMEM T* gm(T* p) { return gmem(p); }
 template <typename T>
-__device__ __forceinline__ const GMEM T* gm(const T* p) { return (const GMEM T*)p; }
+__device__ __forceinline__ const GMEM T* gm(const T* p) { return gmem(p); }
 
 __device__ __forceinline__ float sdown(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float sup(float v) { return __shfl_up(v, 1, kWave); }
 
-// Cell offsets are 32-bit (grids below 2^30 cells, checked on the host) and
-// accesses form their byte offset in 32 bits, so they use the scalar-base +
-// 32-bit vector-offset address form: one VGPR per offset, shared by every
-// array indexed alike, and no 64-bit address arithmetic per access.
-__device__ __forceinline__ uint32_t cix(int i, int j, int k, int ny, int nz) {
-    return (static_cast<uint32_t>(i) * static_cast<uint32_t>(ny) + static_cast<uint32_t>(j)) *
-               static_cast<uint32_t>(nz) +
-           static_cast<uint32_t>(k);
-}
+__device__ __forceinline__ uint32_t cix(int i, int j, int k, int ny, int nz) { return lidx<uint32_t>(i, j, k, ny, nz); }
 template <typename T>
-__device__ __forceinline__ T ldo(const GMEM T* base, uint32_t idx) {
-    return *reinterpret_cast<const GMEM T*>(reinterpret_cast<const GMEM char*>(base) +
-                                            idx * static_cast<uint32_t>(sizeof(T)));
-}
+__device__ __forceinline__ T ldo(const GMEM T* base, uint32_t idx) { return ldx(base, idx); }
 template <typename T>
-__device__ __forceinline__ void sto(GMEM T* base, uint32_t idx, T v) {
-    *reinterpret_cast<GMEM T*>(reinterpret_cast<GMEM char*>(base) + idx * static_cast<uint32_t>(sizeof(T))) = v;
-}
+__device__ __forceinline__ void sto(GMEM T* base, uint32_t idx, T v) { stx(base, idx, v); }
 template <typename T>
-__device__ __forceinline__ void stnt(GMEM T* base, uint32_t idx, T v) {
-    __builtin_nontemporal_store(
-        v, reinterpret_cast<GMEM T*>(reinterpret_cast<GMEM char*>(base) + idx * static_cast<uint32_t>(sizeof(T))));
-}
+__device__ __forceinline__ void stnt(GMEM T* base, uint32_t idx, T v) { stx_nt(base, idx, v); }
 
 // ---------------------------------------------------------------- box lookups
 // Inside the box strips every box quantity a cell needs is loaded at the top
@@ -1014,7 +997,7 @@ int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const
                 const int32_t* src_ijk, const double* wave, int n_probe, const int32_t* probe_ijk) {
     if (nx < 2 || ny < 2 || nz < 2 || !fields || !flags || !dec) return -2;
     // 32-bit cell offsets: fewer than 2^30 cells (every byte offset below 4 GiB)
-    if (static_cast<int64_t>(nx) * ny * nz >= (1ll << 30)) return -3;
+    if (static_cast<int64_t>(nx) * ny * nz >= kMaxCells32) return -3;
     if (n_probe < 0 || n_probe > kDesignMaxProbes || (n_probe > 0 && !probe_ijk)) return -2;
     a = DesignArgs{};
     for (int s = 0; s < 2; ++s)

@@ -16,6 +16,65 @@ __host__ __device__ inline int64_t lin(int i, int j, int k, int ny, int nz) {
     return (static_cast<int64_t>(i) * ny + j) * nz + k;
 }
 
+// Device (global) memory pointers: saying so lets the compiler use global
+// loads with a scalar base even when a pointer comes from memory.
+#define GMEM __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ GMEM T* gmem(T* p) {
+    return (GMEM T*)p;
+}
+template <typename T>
+__device__ __forceinline__ const GMEM T* gmem(const T* p) {
+    return (const GMEM T*)p;
+}
+
+// Cell offsets of grids below 2^30 cells are 32-bit (uint32_t) and accesses
+// with them form the byte offset in 32 bits, so they use the scalar-base +
+// 32-bit vector-offset address form: one VGPR per offset, shared by every
+// array indexed alike, and no 64-bit address arithmetic per access. int64_t
+// offsets index larger grids the ordinary way. ldx / stx / stx_nt pick the
+// form from the offset type.
+template <typename IX>
+__device__ __forceinline__ IX lidx(int i, int j, int k, int ny, int nz) {
+    return (static_cast<IX>(i) * static_cast<IX>(ny) + static_cast<IX>(j)) * static_cast<IX>(nz) +
+           static_cast<IX>(k);
+}
+template <typename T>
+__device__ __forceinline__ const GMEM T* at32(const GMEM T* base, uint32_t idx) {
+    return reinterpret_cast<const GMEM T*>(reinterpret_cast<const GMEM char*>(base) +
+                                           idx * static_cast<uint32_t>(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ GMEM T* at32(GMEM T* base, uint32_t idx) {
+    return reinterpret_cast<GMEM T*>(reinterpret_cast<GMEM char*>(base) + idx * static_cast<uint32_t>(sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ T ldx(const GMEM T* base, uint32_t idx) {
+    return *at32(base, idx);
+}
+template <typename T>
+__device__ __forceinline__ T ldx(const GMEM T* base, int64_t idx) {
+    return base[idx];
+}
+template <typename T>
+__device__ __forceinline__ void stx(GMEM T* base, uint32_t idx, T v) {
+    *at32(base, idx) = v;
+}
+template <typename T>
+__device__ __forceinline__ void stx(GMEM T* base, int64_t idx, T v) {
+    base[idx] = v;
+}
+template <typename T>
+__device__ __forceinline__ void stx_nt(GMEM T* base, uint32_t idx, T v) {
+    __builtin_nontemporal_store(v, at32(base, idx));
+}
+template <typename T>
+__device__ __forceinline__ void stx_nt(GMEM T* base, int64_t idx, T v) {
+    __builtin_nontemporal_store(v, base + idx);
+}
+// grids whose cell offsets fit the 32-bit form
+constexpr int64_t kMaxCells32 = int64_t(1) << 30;
+
 // Everything one fused step launch needs. Passed by value (kernarg segment).
 struct FusedArgs {
     float* f[2][4];                 // ping-pong sets {p, vx, vy, vz}

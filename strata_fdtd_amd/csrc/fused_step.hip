@@ -37,17 +37,6 @@
 
 namespace fdtd_amd {
 
-#define GMEM __attribute__((address_space(1)))
-
-template <typename T>
-__device__ __forceinline__ GMEM T* gmem(T* p) {
-    return (GMEM T*)p;
-}
-template <typename T>
-__device__ __forceinline__ const GMEM T* gmem(const T* p) {
-    return (const GMEM T*)p;
-}
-
 __device__ __forceinline__ float shfl_down1(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWave); }
 
@@ -66,7 +55,10 @@ __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWav
 // the halo rows and columns the neighbouring strips re-read (about 5% faster at
 // 512^3). The batched kernel (small grids, which can stay cache-resident from
 // step to step) uses ordinary stores.
-template <int R, bool NU, int W, bool NTS>
+//
+// IX: cell offset type, uint32_t for grids below 2^30 cells (scalar base +
+// 32-bit offset addressing), int64_t beyond.
+template <int R, bool NU, int W, bool NTS, typename IX>
 __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj, int si, int lane,
                                             float* lds, int wi) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -76,7 +68,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     const int j0 = sj * R;
     const int i0 = si * a.planes;
     const int i1 = min(i0 + a.planes, nx);
-    const int64_t sx = static_cast<int64_t>(ny) * nz;
+    const IX sx = static_cast<IX>(ny) * static_cast<IX>(nz);
 
     const int64_t s = *a.step_ptr + a.koff;
     const int par = static_cast<int>(s & 1);
@@ -147,7 +139,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     for (int r = 0; r < R; ++r) {
         pc[r] = 0.f;
         vxm[r] = 0.f;
-        if (rv[r] && kv) pc[r] = P[lin(i0, j0 + r, k, ny, nz)];
+        if (rv[r] && kv) pc[r] = ldx(P, lidx<IX>(i0, j0 + r, k, ny, nz));
     }
     if (i0 > 0) {
         const int im = i0 - 1;
@@ -155,9 +147,9 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!(rv[r] && kv)) continue;
-            const int64_t c = lin(im, j0 + r, k, ny, nz);
-            const float pm = P[c], vo = VX[c];
-            const uint32_t f = FL[c];
+            const IX c = lidx<IX>(im, j0 + r, k, ny, nz);
+            const float pm = ldx(P, c), vo = ldx(VX, c);
+            const uint32_t f = ldx(FL, c);
             const float v = vo + cvx * (pc[r] - pm);
             vxm[r] = (f & FDTD_FLAG_OPEN_X) ? v : 0.f;
         }
@@ -182,20 +174,20 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             hl_p[r] = hl_vz[r] = hr_p[r] = 0.f;
             hl_f[r] = 0;
             if (!rv[r]) continue;
-            const int64_t c = lin(i, j0 + r, k, ny, nz);
+            const IX c = lidx<IX>(i, j0 + r, k, ny, nz);
             if (kv) {
-                if (has_next) pn[r] = P[c + sx];
-                vxo[r] = VX[c];
-                vyo[r] = VY[c];
-                vzo[r] = VZ[c];
-                fl[r] = FL[c];
+                if (has_next) pn[r] = ldx(P, c + sx);
+                vxo[r] = ldx(VX, c);
+                vyo[r] = ldx(VY, c);
+                vzo[r] = ldx(VZ, c);
+                fl[r] = ldx(FL, c);
             }
             if (lane == 0 && k0 > 0) {
-                hl_p[r] = P[c - 1];
-                hl_vz[r] = VZ[c - 1];
-                hl_f[r] = FL[c - 1];
+                hl_p[r] = ldx(P, c - 1);
+                hl_vz[r] = ldx(VZ, c - 1);
+                hl_f[r] = ldx(FL, c - 1);
             }
-            if (lane == kWave - 1 && k + 1 < nz) hr_p[r] = P[c + 1];
+            if (lane == kWave - 1 && k + 1 < nz) hr_p[r] = ldx(P, c + 1);
         }
         // j halos: p(j0-1), vy(j0-1), flags(j0-1) and p(j0+R)
         float p_jm = 0.f, vy_jm = 0.f, p_jp = 0.f;
@@ -215,12 +207,12 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             if (wi < W - 1) p_jp = sb[(0 * W + wi + 1) * kWave + lane];
         }
         if ((W == 1 || wi == 0) && kv && j0 > 0) {
-            const int64_t c = lin(i, j0 - 1, k, ny, nz);
-            p_jm = P[c];
-            vy_jm = VY[c];
-            f_jm = FL[c];
+            const IX c = lidx<IX>(i, j0 - 1, k, ny, nz);
+            p_jm = ldx(P, c);
+            vy_jm = ldx(VY, c);
+            f_jm = ldx(FL, c);
         }
-        if ((W == 1 || wi == W - 1) && kv && j0 + R < ny) p_jp = P[lin(i, j0 + R, k, ny, nz)];
+        if ((W == 1 || wi == W - 1) && kv && j0 + R < ny) p_jp = ldx(P, lidx<IX>(i, j0 + R, k, ny, nz));
 
         // vy' on the face (j0-1 | j0)
         float vyprev = 0.f;
@@ -234,7 +226,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
         for (int r = 0; r < R; ++r) {
             if (!rv[r]) continue;  // wave-uniform
             const int jr = j0 + r;
-            const int64_t c = lin(i, jr, k, ny, nz);
+            const IX c = lidx<IX>(i, jr, k, ny, nz);
             const float pr = pc[r];
             const float p_up = (r + 1 < R) ? pc[r + 1] : p_jp;
             const uint32_t f = fl[r];
@@ -303,15 +295,15 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             }
             if (kv) {
                 if constexpr (NTS) {
-                    __builtin_nontemporal_store(ps, &Po[c]);
-                    __builtin_nontemporal_store(vxn * dx_i, &VXo[c]);
-                    __builtin_nontemporal_store(vyn * dy_j, &VYo[c]);
-                    __builtin_nontemporal_store(vzn * dz_k, &VZo[c]);
+                    stx_nt(Po, c, ps);
+                    stx_nt(VXo, c, vxn * dx_i);
+                    stx_nt(VYo, c, vyn * dy_j);
+                    stx_nt(VZo, c, vzn * dz_k);
                 } else {
-                    Po[c] = ps;
-                    VXo[c] = vxn * dx_i;
-                    VYo[c] = vyn * dy_j;
-                    VZo[c] = vzn * dz_k;
+                    stx(Po, c, ps);
+                    stx(VXo, c, vxn * dx_i);
+                    stx(VYo, c, vyn * dy_j);
+                    stx(VZo, c, vzn * dz_k);
                 }
             }
             if (probe_mask && trace_ok) {
@@ -332,7 +324,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     }
 }
 
-template <int R, bool NU>
+template <int R, bool NU, typename IX>
 __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     // XCD-aware remap: hardware block b runs on XCD b % 8; give each XCD a
@@ -345,14 +337,14 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     if (*a.step_ptr + a.koff >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, NU, 1, true>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, NU, 1, true, IX>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 // Stacked variant: workgroup = W waves over W consecutive j-strips of one
 // segment and i-chunk (halo rows shared through LDS). Logical workgroup
 // index -> (segment, j-stack, i-chunk), segment fastest; `a.nwaves` counts
 // workgroups here. Every wave of a workgroup runs the same number of planes,
-// so the per-plane barrier is reached uniformly.
+// so the per-plane barrier is reached uniformly. Grids below 2^30 cells.
 template <int R, bool NU, int W>
 __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
     __shared__ float lds[2 * 4 * W * kWave];
@@ -364,12 +356,13 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
     const int nstack_j = (a.nstrip_j + W - 1) / W;
     const int seg = lb % a.nseg;
     const int t = lb / a.nseg;
-    fused_strip<R, NU, W, true>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
+    fused_strip<R, NU, W, true, uint32_t>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
 }
 
 // Batched step: one launch advances every solve in `table` by one step. Waves
 // are dealt out by solve through the prefix sums `offsets` (n + 1 entries);
-// a solve whose counter reached its limit is left untouched.
+// a solve whose counter reached its limit is left untouched. Solves below 2^30
+// cells.
 // The number of solves is read from device memory (n_ptr), so a captured
 // graph stays valid while the host rewrites the table between chunks; the
 // grid is sized for the table's wave capacity and surplus waves return.
@@ -398,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const FusedArgs* __restri
     if (*a.step_ptr >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, false, 1, false>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, false, 1, false, uint32_t>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 __global__ void k_batch_advance(const FusedArgs* __restrict__ table, const int* __restrict__ n_ptr) {
@@ -614,12 +607,16 @@ __global__ void k_advance(int64_t* step, int n) {
 // ---- host-side launch helpers ------------------------------------------------
 template <int R, bool NU>
 static hipError_t launch_r(const FusedArgs& a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL((k_fused_step<R, NU>), dim3(blocks), dim3(kBlock), 0, st, a);
+    if (static_cast<int64_t>(a.nx) * a.ny * a.nz < kMaxCells32)
+        hipLaunchKernelGGL((k_fused_step<R, NU, uint32_t>), dim3(blocks), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_fused_step<R, NU, int64_t>), dim3(blocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
 
 template <int R, bool NU, int W>
 static hipError_t launch_stack(const FusedArgs& a, int blocks, hipStream_t st) {
+    if (static_cast<int64_t>(a.nx) * a.ny * a.nz >= kMaxCells32) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_fused_stack<R, NU, W>), dim3(blocks), dim3(W * kWave), 0, st, a);
     return hipGetLastError();
 }
