@@ -522,21 +522,25 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
 // ---------------------------------------------------------------- adjoint
 // pb = dX dY dZ (P + g at probe cells); q = air cp0 kappa pb.
 // pm: the strip (with its halo) holds a probe.
+// dz: the k decay of the cell (dec[2][k]), kept in a register by the caller.
 __device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
-                                           uint32_t c, int64_t n, bool pm) {
+                                           uint32_t c, int64_t n, bool pm, float dz) {
     float v = ldo(P, c);
     if (pm)
         for (int q = 0; q < a.n_probe; ++q)
             if (a.probe[3 * q] == i && a.probe[3 * q + 1] == j && a.probe[3 * q + 2] == k)
                 v += a.g[n * a.n_probe + q];
-    const float lam = (a.dec[0][i] * a.dec[1][j]) * a.dec[2][k];
+    const float lam = (a.dec[0][i] * a.dec[1][j]) * dz;
     return lam * v;
 }
 
+// Branch-free: the flag and pressure loads are issued together, not one
+// after the other behind a test of the flag.
 __device__ __forceinline__ float design_q(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
-                                          uint32_t c, int64_t n, bool pm, float kap) {
-    if (!(ldo(gm(a.flags), c) & FDTD_FLAG_AIR)) return 0.f;
-    return a.cp0 * kap * design_pb(a, P, i, j, k, c, n, pm);
+                                          uint32_t c, int64_t n, bool pm, float kap, float dz) {
+    const uint32_t f = ldo(gm(a.flags), c);
+    const float q = a.cp0 * kap * design_pb(a, P, i, j, k, c, n, pm, dz);
+    return (f & FDTD_FLAG_AIR) ? q : 0.f;
 }
 
 // Adjoint step n: (P, V) = lambda_{n+1} in set (n+1)&1 -> lambda_n in set n&1.
@@ -559,6 +563,9 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
     GMEM float* __restrict__ VZo = gm(a.f[src ^ 1][3]);
     const GMEM uint8_t* __restrict__ FL = gm(a.flags);
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
+    // the lane halo's decay: lane 0 at k - 1, the others at k + 1
+    const int kh = lane == 0 ? k - 1 : k + 1;
+    const float dz_kh = (kh >= 0 && kh < nz) ? a.dec[2][kh] : 1.0f;
     const float cv = a.cv0;
     const bool pm = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 1) != 0;
 
@@ -571,13 +578,13 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
         if (!(rvld[r] && kin)) continue;
         const int j = j0 + r;
         const uint32_t c = cix(i0, j, k, ny, nz);
-        pbc[r] = design_pb(a, P, i0, j, k, c, n, pm);
+        pbc[r] = design_pb(a, P, i0, j, k, c, n, pm, dz_k);
         qc[r] = (ldo(FL, c) & FDTD_FLAG_AIR) ? a.cp0 * pbc[r] : 0.f;
         if (i0 > 0) {
             const uint32_t cl = c - sx;
-            wxm[r] = (ldo(FL, cl) & FDTD_FLAG_OPEN_X)
-                         ? a.dec[0][i0 - 1] * ldo(VX, cl) + design_q(a, P, i0 - 1, j, k, cl, n, pm, 1.0f) - qc[r]
-                         : 0.f;
+            const uint32_t fl_m = ldo(FL, cl);
+            const float w = a.dec[0][i0 - 1] * ldo(VX, cl) + design_q(a, P, i0 - 1, j, k, cl, n, pm, 1.0f, dz_k) - qc[r];
+            wxm[r] = (fl_m & FDTD_FLAG_OPEN_X) ? w : 0.f;
         }
     }
     for (int i = i0; i < i1; ++i) {
@@ -593,24 +600,24 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
             const int j = j0 + r;
             const uint32_t c = cix(i, j, k, ny, nz);
             if (nxt) {
-                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm);
+                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm, dz_k);
                 qn[r] = (ldo(FL, c + sx) & FDTD_FLAG_AIR) ? a.cp0 * pbn[r] : 0.f;
             }
             vx[r] = ldo(VX, c);
             vy[r] = ldo(VY, c);
             vz[r] = ldo(VZ, c);
             fl[r] = ldo(FL, c);
-            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, 1.0f);
+            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, 1.0f, dz_kh);
         }
         float q_up_halo = 0.f;
         if (kin && j0 + R < ny)
-            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, 1.0f);
+            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, 1.0f, dz_k);
         float wyprev = 0.f;
         if (kin && j0 > 0) {
             const uint32_t cl = cix(i, j0 - 1, k, ny, nz);
-            wyprev = (ldo(FL, cl) & FDTD_FLAG_OPEN_Y)
-                         ? a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, 1.0f) - qc[0]
-                         : 0.f;
+            const uint32_t fl_m = ldo(FL, cl);
+            const float w = a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, 1.0f, dz_k) - qc[0];
+            wyprev = (fl_m & FDTD_FLAG_OPEN_Y) ? w : 0.f;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -648,9 +655,9 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
                 wzm = 0.f;
                 if (k0 > 0 && kin) {
                     const uint32_t cl = c - 1;
-                    wzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z)
-                              ? a.dec[2][k - 1] * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, 1.0f) - qc[r]
-                              : 0.f;
+                    const uint32_t fl_m = ldo(FL, cl);
+                    const float w = dz_kh * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, 1.0f, dz_kh) - qc[r];
+                    wzm = (fl_m & FDTD_FLAG_OPEN_Z) ? w : 0.f;
                 }
             }
             if (i > 0) acc += cv * wxm[r];
@@ -673,13 +680,11 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
     }
 }
 
-// Strips that meet the design box: per-voxel coefficients and the gradient.
-// Each face's coefficient cv, its gradient factor e = -cv / (r_lo + r_hi) and
-// the forward pressure difference dh across it are formed once, where the
-// face is the cell's own (upper) face, and passed on to the cell above it:
-// along i in registers from plane to plane, along j from row to row, along k
-// from lane to lane. Each is the same expression of the same two values as in
-// the per-cell kernel, so the results are the same bits.
+// Strips that meet the design box: per-voxel coefficients. Each face's
+// coefficient cv is formed once, where the face is the cell's own (upper)
+// face, and passed on to the cell above it: along i in registers from plane
+// to plane, along j from row to row, along k from lane to lane. The gradient
+// terms are accumulated afterwards by k_design_grad from this step's output.
 template <int R>
 __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, int64_t n, int seg, int j0, int i0,
                                                          int i1, int lane) {
@@ -697,61 +702,43 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
     GMEM float* __restrict__ VYo = gm(a.f[src ^ 1][2]);
     GMEM float* __restrict__ VZo = gm(a.f[src ^ 1][3]);
     const GMEM uint8_t* __restrict__ FL = gm(a.flags);
-    // the entry point admits only steps whose history is held
-    const GMEM float* __restrict__ hp = gm(a.hist_p + (n - a.hist0) * ebox_cells(a));
-    const GMEM float* __restrict__ hd = gm(a.hist_div + (n - a.hist0) * dbox_cells(a));
-    GMEM f4v* __restrict__ ACC = gm(a.acc);
     const float dz_k = kin ? a.dec[2][k] : 1.0f;
+    // the lane halo: lane 0 looks at k - 1, the others at k + 1
+    const int kh = lane == 0 ? k - 1 : k + 1;
+    const float dz_kh = (kh >= 0 && kh < nz) ? a.dec[2][kh] : 1.0f;
     const bool pm = strip_probes(a, i0, i1, j0, j0 + R, k0, k0 + kWave, 1) != 0;
 
     const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
     const BoxTab RK = box_tab(a.rk);
-    const int kh = lane == 0 ? k - 1 : k + 1;
     const Ax dk = dax(k, a.lo[2], bz), dkh = dax(kh, a.lo[2], bz);
-    const Ax ek = eax(k, a.lo[2], bz), ekh = eax(kh, a.lo[2], bz);
-    Ax dj[R], ej[R];
+    Ax dj[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        dj[r] = dax(j0 + r, a.lo[1], by);
-        ej[r] = eax(j0 + r, a.lo[1], by);
-    }
+    for (int r = 0; r < R; ++r) dj[r] = dax(j0 + r, a.lo[1], by);
     const Ax djd = dax(j0 - 1, a.lo[1], by), dju = dax(j0 + R, a.lo[1], by);
-    const Ax ejd = eax(j0 - 1, a.lo[1], by), eju = eax(j0 + R, a.lo[1], by);
     auto boff = [&](const Ax& x, const Ax& y, const Ax& z) { return (x.off * by + y.off) * bz + z.off; };
     auto RKV = [&](const Ax& x, const Ax& y, const Ax& z) { return box_val(RK, boff(x, y, z), x.in && y.in && z.in); };
-    // forward pressure on the extended box (0 outside it; never used there)
-    auto HV = [&](const Ax& x, const Ax& y, const Ax& z) {
-        const float v = ldo(hp, static_cast<uint32_t>((x.off * (by + 2) + y.off) * (bz + 2) + z.off));
-        return (x.in && y.in && z.in) ? v : 0.f;
-    };
 
     bool rvld[R];
-    float pbc[R], qc[R], wxm[R], rcur[R], hcur[R], cvxm[R], exm[R], dhxm[R];
+    float pbc[R], qc[R], wxm[R], rcur[R], cvxm[R];
     {
         const Ax di0 = dax(i0, a.lo[0], bx), dim = dax(i0 - 1, a.lo[0], bx);
-        const Ax ei0 = eax(i0, a.lo[0], bx), eim = eax(i0 - 1, a.lo[0], bx);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             rvld[r] = j0 + r < ny;
             pbc[r] = qc[r] = wxm[r] = 0.f;
             const f2v v0 = RKV(di0, dj[r], dk), vm = RKV(dim, dj[r], dk);
             rcur[r] = v0.x;
-            hcur[r] = HV(ei0, ej[r], ek);
-            const float rm = vm.x, hm = HV(eim, ej[r], ek);
-            const float km = vm.y, kc = v0.y;
-            cvxm[r] = cvf(a, rm, rcur[r]);
-            exm[r] = -cvxm[r] / (rm + rcur[r]);
-            dhxm[r] = hcur[r] - hm;
+            cvxm[r] = cvf(a, vm.x, v0.x);
             if (!(rvld[r] && kin)) continue;
             const int j = j0 + r;
             const uint32_t c = cix(i0, j, k, ny, nz);
-            pbc[r] = design_pb(a, P, i0, j, k, c, n, pm);
-            qc[r] = (ldo(FL, c) & FDTD_FLAG_AIR) ? a.cp0 * kc * pbc[r] : 0.f;
+            pbc[r] = design_pb(a, P, i0, j, k, c, n, pm, dz_k);
+            qc[r] = (ldo(FL, c) & FDTD_FLAG_AIR) ? a.cp0 * v0.y * pbc[r] : 0.f;
             if (i0 > 0) {
                 const uint32_t cl = c - sx;
-                wxm[r] = (ldo(FL, cl) & FDTD_FLAG_OPEN_X)
-                             ? a.dec[0][i0 - 1] * ldo(VX, cl) + design_q(a, P, i0 - 1, j, k, cl, n, pm, km) - qc[r]
-                             : 0.f;
+                const uint32_t fl_m = ldo(FL, cl);
+                const float w = a.dec[0][i0 - 1] * ldo(VX, cl) + design_q(a, P, i0 - 1, j, k, cl, n, pm, vm.y, dz_k) - qc[r];
+                wxm[r] = (fl_m & FDTD_FLAG_OPEN_X) ? w : 0.f;
             }
         }
     }
@@ -759,11 +746,7 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
         const Ax di = dax(i, a.lo[0], bx), din = dax(i + 1, a.lo[0], bx);
-        const Ax ei = eax(i, a.lo[0], bx), ein = eax(i + 1, a.lo[0], bx);
-        float pbn[R], qn[R], vx[R], vy[R], vz[R], qhr[R];
-        float rn[R], hn[R], rkh[R], hkh[R], kkh[R];
-        float hdv[R];
-        f4v g4[R];
+        float pbn[R], qn[R], vx[R], vy[R], vz[R], qhr[R], rn[R], rkh[R], kkh[R];
         uint32_t fl[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -771,81 +754,61 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
             fl[r] = 0;
             const f2v vn = RKV(din, dj[r], dk), vh = RKV(di, dj[r], dkh);
             rn[r] = vn.x;
-            hn[r] = HV(ein, ej[r], ek);
             rkh[r] = vh.x;
-            hkh[r] = HV(ei, ej[r], ekh);
             kkh[r] = vh.y;
-            const float kn = vn.y;
-            const uint32_t b = static_cast<uint32_t>(boff(di, dj[r], dk));
-            hdv[r] = ldo(hd, b);
-            g4[r] = ldo(ACC, b);
             if (!(rvld[r] && kin)) continue;
             const int j = j0 + r;
             const uint32_t c = cix(i, j, k, ny, nz);
             if (nxt) {
-                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm);
-                qn[r] = (ldo(FL, c + sx) & FDTD_FLAG_AIR) ? a.cp0 * kn * pbn[r] : 0.f;
+                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm, dz_k);
+                qn[r] = (ldo(FL, c + sx) & FDTD_FLAG_AIR) ? a.cp0 * vn.y * pbn[r] : 0.f;
             }
             vx[r] = ldo(VX, c);
             vy[r] = ldo(VY, c);
             vz[r] = ldo(VZ, c);
             fl[r] = ldo(FL, c);
-            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, kkh[r]);
+            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, kkh[r], dz_kh);
         }
         const f2v v_up = RKV(di, dju, dk), v_dn = RKV(di, djd, dk);
-        const float r_up = v_up.x, r_dn = v_dn.x, k_up = v_up.y, k_dn = v_dn.y;
-        const float h_up = HV(ei, eju, ek), h_dn = HV(ei, ejd, ek);
         float q_up_halo = 0.f;
         if (kin && j0 + R < ny)
-            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, k_up);
+            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, v_up.y, dz_k);
         float wyprev = 0.f;
         if (kin && j0 > 0) {
             const uint32_t cl = cix(i, j0 - 1, k, ny, nz);
-            wyprev = (ldo(FL, cl) & FDTD_FLAG_OPEN_Y)
-                         ? a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, k_dn) - qc[0]
-                         : 0.f;
+            const uint32_t fl_m = ldo(FL, cl);
+            const float w = a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, v_dn.y, dz_k) - qc[0];
+            wyprev = (fl_m & FDTD_FLAG_OPEN_Y) ? w : 0.f;
         }
         // the face below the first row
-        float cvyp = cvf(a, r_dn, rcur[0]);
-        float eyp = -cvyp / (r_dn + rcur[0]);
-        float dhyp = hcur[0] - h_dn;
+        float cvyp = cvf(a, v_dn.x, rcur[0]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (!rvld[r]) continue;
             const int j = j0 + r;
             const uint32_t c = cix(i, j, k, ny, nz);
             const uint32_t f = fl[r];
-            const float rc = rcur[r], hpc = hcur[r];
+            const float rc = rcur[r];
             const bool air = f & FDTD_FLAG_AIR;
             float acc = air ? pbc[r] : 0.f;
-            float gr = 0.f;
-            const bool gbox = kin && di.in && dj[r].in && dk.in;
-            const float rc_dn = sdown(rc), hpc_dn = sdown(hpc);  // every lane shuffles
+            const float rc_dn = sdown(rc);  // every lane takes part in the shuffle
             const float r_k1 = (lane == kWave - 1) ? rkh[r] : rc_dn;
-            const float h_k1 = (lane == kWave - 1) ? hkh[r] : hpc_dn;
             // own faces (c = lo)
             float wx, wy, wz;
-            float cvx = 0.f, ex = 0.f, dhx = 0.f, cvy = 0.f, ey = 0.f, dhy = 0.f, cvz = 0.f, ez = 0.f, dhz = 0.f;
+            float cvx = 0.f, cvy = 0.f, cvz = 0.f;
             const float dy_j = a.dec[1][j];
             if (nxt) {
                 cvx = cvf(a, rc, rn[r]);
                 wx = (f & FDTD_FLAG_OPEN_X) ? dx_i * vx[r] + qc[r] - qn[r] : 0.f;
                 acc -= cvx * wx;
-                ex = -cvx / (rc + rn[r]);
-                dhx = hn[r] - hpc;
-                if (gbox) gr += wx * dhx * ex;
             } else {
                 wx = dx_i * vx[r] + qc[r];
             }
             if (j + 1 < ny) {
-                const float rh = (r + 1 < R) ? rcur[r + 1] : r_up;
-                cvy = cvf(a, rc, rh);
+                cvy = cvf(a, rc, (r + 1 < R) ? rcur[r + 1] : v_up.x);
                 const float qu = (r + 1 < R) ? qc[r + 1] : q_up_halo;
                 wy = (f & FDTD_FLAG_OPEN_Y) ? dy_j * vy[r] + qc[r] - qu : 0.f;
                 acc -= cvy * wy;
-                ey = -cvy / (rc + rh);
-                dhy = ((r + 1 < R) ? hcur[r + 1] : h_up) - hpc;
-                if (gbox) gr += wy * dhy * ey;
             } else {
                 wy = dy_j * vy[r] + qc[r];
             }
@@ -855,78 +818,119 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
                 cvz = cvf(a, rc, r_k1);
                 wz = (f & FDTD_FLAG_OPEN_Z) ? dz_k * vz[r] + qc[r] - qk1 : 0.f;
                 acc -= cvz * wz;
-                ez = -cvz / (rc + r_k1);
-                dhz = h_k1 - hpc;
-                if (gbox) gr += wz * dhz * ez;
             } else {
                 wz = dz_k * vz[r] + qc[r];
             }
             // faces below (c = hi): the z face from the lane below
             float wzm = sup(wz);
-            float cvzm = sup(cvz), ezm = sup(ez), dhzm = sup(dhz);
+            float cvzm = sup(cvz);
             if (lane == 0) {
                 wzm = 0.f;
                 cvzm = cvf(a, rkh[r], rc);
-                ezm = -cvzm / (rkh[r] + rc);
-                dhzm = hpc - hkh[r];
                 if (k0 > 0 && kin) {
                     const uint32_t cl = c - 1;
-                    wzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z)
-                              ? a.dec[2][k - 1] * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, kkh[r]) - qc[r]
-                              : 0.f;
+                    const uint32_t fl_m = ldo(FL, cl);
+                    const float w = dz_kh * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, kkh[r], dz_kh) - qc[r];
+                    wzm = (fl_m & FDTD_FLAG_OPEN_Z) ? w : 0.f;
                 }
             }
-            if (i > 0) {
-                acc += cvxm[r] * wxm[r];
-                if (gbox) gr += wxm[r] * dhxm[r] * exm[r];
-            }
-            if (j > 0) {
-                acc += cvyp * wyprev;
-                if (gbox) gr += wyprev * dhyp * eyp;
-            }
-            if (k > 0) {
-                acc += cvzm * wzm;
-                if (gbox) gr += wzm * dhzm * ezm;
-            }
+            if (i > 0) acc += cvxm[r] * wxm[r];
+            if (j > 0) acc += cvyp * wyprev;
+            if (k > 0) acc += cvzm * wzm;
             if (kin) {
                 stnt(Po, c, acc);
                 stnt(VXo, c, wx);
                 stnt(VYo, c, wy);
                 stnt(VZo, c, wz);
-                if (gbox) {
-                    // Kahan: sum += x with the compensation of the lost low part
-                    const uint32_t b = static_cast<uint32_t>(boff(di, dj[r], dk));
-                    f4v g = g4[r];
-                    float y = gr - g.y;
-                    float t = g.x + y;
-                    g.y = (t - g.x) - y;
-                    g.x = t;
-                    if (air) {
-                        y = pbc[r] * a.cp0 * hdv[r] - g.w;
-                        t = g.z + y;
-                        g.w = (t - g.z) - y;
-                        g.z = t;
-                    }
-                    sto(ACC, b, g);
-                }
             }
             wxm[r] = wx;
             cvxm[r] = cvx;
-            exm[r] = ex;
-            dhxm[r] = dhx;
             wyprev = wy;
             cvyp = cvy;
-            eyp = ey;
-            dhyp = dhy;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             pbc[r] = pbn[r];
             qc[r] = qn[r];
             rcur[r] = rn[r];
-            hcur[r] = hn[r];
         }
     }
+}
+
+// Gradient terms of adjoint step n, one thread per design-box voxel, from the
+// step's output lambda_n (the w of the cell's six faces), the input
+// lambda_{n+1} (its pressure) and the forward history of step n:
+//   dL/dr     += sum over the six faces of w * (p_hi - p_lo) * (-cv / (r_lo + r_hi))
+//   dL/dkappa += air * cp0 * pb * div
+// in the per-cell kernel's order (own x, y, z faces, then the faces below),
+// with Kahan compensation. Every box voxel has all six faces (the box keeps
+// one cell to every grid face).
+__global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t n) {
+    const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
+    const uint32_t b = blockIdx.x * 256u + threadIdx.x;
+    if (b >= static_cast<uint32_t>(bx) * by * bz) return;
+    const int bk = static_cast<int>(b % bz), t = static_cast<int>(b / bz);
+    const int bj = t % by, bi = t / by;
+    const int i = a.lo[0] + bi, j = a.lo[1] + bj, k = a.lo[2] + bk;
+    const int ny = a.ny, nz = a.nz;
+    const uint32_t sx = static_cast<uint32_t>(ny) * static_cast<uint32_t>(nz);
+    const uint32_t c = cix(i, j, k, ny, nz);
+    const int src = static_cast<int>((n + 1) & 1);
+    const GMEM float* __restrict__ P = gm(a.f[src][0]);
+    const GMEM float* __restrict__ WX = gm(a.f[src ^ 1][1]);
+    const GMEM float* __restrict__ WY = gm(a.f[src ^ 1][2]);
+    const GMEM float* __restrict__ WZ = gm(a.f[src ^ 1][3]);
+    const GMEM float* __restrict__ hp = gm(a.hist_p + (n - a.hist0) * ebox_cells(a));
+    const GMEM float* __restrict__ hd = gm(a.hist_div + (n - a.hist0) * dbox_cells(a));
+    GMEM f4v* __restrict__ ACC = gm(a.acc);
+    const BoxTab RK = box_tab(a.rk);
+    const int ey = by + 2, ez = bz + 2;
+    const uint32_t e = static_cast<uint32_t>(((bi + 1) * ey + (bj + 1)) * ez + (bk + 1));
+    // r of the voxel and its six neighbours (1 outside the box)
+    auto RV = [&](int di, int dj, int dk) {
+        const int ii = bi + di, jj = bj + dj, kk = bk + dk;
+        const bool in = ii >= 0 && ii < bx && jj >= 0 && jj < by && kk >= 0 && kk < bz;
+        return box_val(RK, in ? (ii * by + jj) * bz + kk : 0, in).x;
+    };
+    const float rc = box_val(RK, static_cast<int>(b), true).x;
+    const float rxp = RV(1, 0, 0), ryp = RV(0, 1, 0), rzp = RV(0, 0, 1);
+    const float rxm = RV(-1, 0, 0), rym = RV(0, -1, 0), rzm = RV(0, 0, -1);
+    const float hpc = ldo(hp, e);
+    const float hxp = ldo(hp, e + static_cast<uint32_t>(ey * ez)), hyp = ldo(hp, e + static_cast<uint32_t>(ez)),
+                hzp = ldo(hp, e + 1u);
+    const float hxm = ldo(hp, e - static_cast<uint32_t>(ey * ez)), hym = ldo(hp, e - static_cast<uint32_t>(ez)),
+                hzm = ldo(hp, e - 1u);
+    const float wx = ldo(WX, c), wy = ldo(WY, c), wz = ldo(WZ, c);
+    const float wxm = ldo(WX, c - sx), wym = ldo(WY, c - static_cast<uint32_t>(nz)), wzm = ldo(WZ, c - 1u);
+    float gr = 0.f;
+    float cv = cvf(a, rc, rxp);
+    gr += wx * (hxp - hpc) * (-cv / (rc + rxp));
+    cv = cvf(a, rc, ryp);
+    gr += wy * (hyp - hpc) * (-cv / (rc + ryp));
+    cv = cvf(a, rc, rzp);
+    gr += wz * (hzp - hpc) * (-cv / (rc + rzp));
+    cv = cvf(a, rxm, rc);
+    gr += wxm * (hpc - hxm) * (-cv / (rxm + rc));
+    cv = cvf(a, rym, rc);
+    gr += wym * (hpc - hym) * (-cv / (rym + rc));
+    cv = cvf(a, rzm, rc);
+    gr += wzm * (hpc - hzm) * (-cv / (rzm + rc));
+    f4v g = ldo(ACC, b);
+    float y = gr - g.y;
+    float s = g.x + y;
+    g.y = (s - g.x) - y;
+    g.x = s;
+    if (ldo(gm(a.flags), c) & FDTD_FLAG_AIR) {
+        bool pm = false;
+        for (int q = 0; q < a.n_probe; ++q)
+            pm |= a.probe[3 * q] == i && a.probe[3 * q + 1] == j && a.probe[3 * q + 2] == k;
+        const float pb = design_pb(a, P, i, j, k, c, n, pm, a.dec[2][k]);
+        y = pb * a.cp0 * ldo(hd, b) - g.w;
+        s = g.z + y;
+        g.w = (s - g.z) - y;
+        g.z = s;
+    }
+    sto(ACC, b, g);
 }
 
 template <int R, int RB>
@@ -1094,10 +1098,13 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
     a.hist_steps = hist_steps;
     a.acc = reinterpret_cast<f4v*>(acc);
     const int blocks = design_grid(a, planes, kDesignAdjBoxRows);
+    const int64_t box_cells = static_cast<int64_t>(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]) * (a.hi[2] - a.lo[2]);
+    const int grad_blocks = static_cast<int>((box_cells + 255) / 256);
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (int64_t n = n_hi - 1; n >= n_lo; --n) {
         hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows>), dim3(blocks), dim3(kBlock), 0, st, a,
                            n);
+        hipLaunchKernelGGL(k_design_grad, dim3(grad_blocks), dim3(256), 0, st, a, n);
         if (hipGetLastError() != hipSuccess) return -5;
     }
     return 0;
