@@ -128,6 +128,13 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
     batched = srv.serve(reqs, realtime=True)
     srv.close()
     log(f"serving batched: {batched.summary()}")
+    torch.cuda.empty_cache()
+    # latency-first admission: about 16M active cells, least work first
+    srv = SweepServer(chunk=chunk, max_cells=16 << 20, policy="sjf")
+    sjf = srv.serve(reqs, realtime=True)
+    srv.close()
+    log(f"serving batched, sjf admission: {sjf.summary()}")
+    torch.cuda.empty_cache()
     single = serve_one_at_a_time(reqs, realtime=True)
     log(f"serving one-at-a-time: {single.summary()}")
     # spot-check equality of results in the mix and alone
@@ -137,7 +144,9 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
     return {
         "requests": len(reqs), "arrival_rate_per_s": rate, "chunk_steps": chunk,
         "gcell_updates_total": round(sum(r.cells * r.steps for r in reqs) / 1e9, 2),
-        "batched": batched.summary(), "one_at_a_time": single.summary(),
+        "batched": batched.summary(),
+        "batched_sjf_16M_cells": sjf.summary(),
+        "one_at_a_time": single.summary(),
         "speedup_solves_per_s": round(batched.solves_per_s / single.solves_per_s, 3),
         "results_identical_alone_and_in_mix": bool(same),
     }

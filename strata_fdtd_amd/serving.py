@@ -157,15 +157,27 @@ class ServeStats:
 
 
 class SweepServer:
-    """Continuous batching of uniform-grid solves on one GPU."""
+    """Continuous batching of uniform-grid solves on one GPU.
+
+    Admission: a waiting request is admitted while an active slot, the HBM
+    budget and the active-cell cap allow. The cap (``max_cells``) keeps just
+    enough work resident to fill the GPU; beyond it more concurrent solves add
+    no throughput and only stretch every solve's latency (the device is shared).
+    ``policy`` orders the waiting requests: "sjf" (least work, cells x steps,
+    first: lowest mean latency) or "fifo" (arrival order)."""
 
     def __init__(self, device: int = 0, rows: int = 4, chunk: int = 100, max_active: int = 64,
-                 hbm_budget: int | None = None, use_graph: bool = True):
+                 hbm_budget: int | None = None, use_graph: bool = True, max_cells: int | None = None,
+                 policy: str = "fifo"):
         import torch
+        if policy not in ("fifo", "sjf"):
+            raise ValueError(f"policy must be 'fifo' or 'sjf', not {policy!r}")
         self.device = device
         self.rows = rows
         self.chunk = chunk
         self.max_active = max_active
+        self.max_cells = max_cells
+        self.policy = policy
         self.use_graph = use_graph
         if hbm_budget is None:
             free, _total = torch.cuda.mem_get_info(device)
@@ -225,13 +237,19 @@ class SweepServer:
             ev.record(build_stream)
             return s, ev
 
+        def waiting(limit_time: float) -> list[SolveRequest]:
+            """Requests arrived by limit_time, in admission order."""
+            arrived = [r for r in pending if not realtime or r.arrival <= limit_time]
+            if self.policy == "sjf":
+                arrived.sort(key=lambda r: (r.cells * r.steps, r.arrival))
+            return arrived
+
         def prebuild(limit_time: float) -> None:
-            for r in pending:
-                if r.rid in built:
-                    continue
-                if (realtime and r.arrival > limit_time) or len(built) >= 4:
+            for r in waiting(limit_time):
+                if len(built) >= 4:
                     break
-                built[r.rid] = build(r)
+                if r.rid not in built:
+                    built[r.rid] = build(r)
 
         def harvest(block: bool) -> None:
             while retiring and (block or retiring[0][0].query()):
@@ -249,18 +267,21 @@ class SweepServer:
                     sl.host_trace = None
                 block = False
 
+        active_cells = 0
         while pending or active or retiring:
             tnow = now()
             changed = False
-            while pending and (not realtime or pending[0].arrival <= tnow):
-                r = pending[0]
+            for r in waiting(tnow):
                 if len(active) >= self.max_active or (active and used + r.footprint > self.hbm_budget):
                     break
-                pending.pop(0)
+                if active and self.max_cells is not None and active_cells + r.cells > self.max_cells:
+                    break
+                pending.remove(r)
                 s, ev = built.pop(r.rid, None) or build(r)
                 self._stream.wait_event(ev)
                 active.append(_Slot(r, s, now()))
                 used += r.footprint
+                active_cells += r.cells
                 changed = True
             if changed:
                 self._set_batch(active)
@@ -280,6 +301,7 @@ class SweepServer:
                         sl.host_trace.copy_(tr, non_blocking=True)
                         active.remove(sl)
                         used -= sl.req.footprint
+                        active_cells -= sl.req.cells
                     ev = torch.cuda.Event()
                     ev.record(self._stream)
                     retiring.append((ev, finished))
