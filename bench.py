@@ -320,7 +320,8 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(wl.name),
-        "kernel": f"{kname}<{rows},{'false' if solver.grid.is_uniform else 'true'}{'' if stack == 1 else f',{stack}'}>",
+        "kernel": (f"{kname}<{rows},{'false' if solver.grid.is_uniform else 'true'},"
+                   f"{'uint32_t' if stack == 1 else stack}>"),
         "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells,
         "bytes_per_cell_update": BYTES_PER_CELL,
         "avg_launch_us": round(ms_sweep * 1e3, 1),
