@@ -102,6 +102,20 @@ int fdtd_ctx_set_probes(fdtd_ctx* ctx, int n, const int64_t* flat_idx, float* tr
 int fdtd_ctx_set_microphones(fdtd_ctx* ctx, int n, const int32_t* idx, const double* w, const int32_t* f64,
                              double* out, int64_t out_len);
 
+/* Membrane (weighted-mask) sources, applied after each step's sweep, ADE
+ * fix-up and point/plane sources: for every listed cell (flat index, cells
+ * increasing), entries entry_start[e] .. entry_start[e+1]-1 in source order
+ * do field[c] = float(double(field[c]) + wave[src][s] * weight), field 0 p,
+ * 1 vx, 2 vy, 3 vz; then the probes probe_idx[probe_start[e] ..
+ * probe_start[e+1]-1] record p at the cell again. Host arrays except wave
+ * (device float64 [n_src][wave_len], value at step s in column s); the
+ * tables are copied to the device. n_cells = 0 removes them. Replaces the
+ * membrane branch of _inject_sources (core/solver.py:2389-2412). */
+int fdtd_ctx_set_membranes(fdtd_ctx* ctx, int n_cells, const int64_t* cell, const int32_t* entry_start,
+                           int n_entries, const int32_t* entry_src, const int32_t* entry_field,
+                           const double* entry_weight, const int32_t* probe_start, int n_probe_refs,
+                           const int32_t* probe_idx, int n_src, const double* wave, int64_t wave_len);
+
 /* Device int64 holding the index of the next step (read by every step kernel,
  * advanced by fdtd_step_n) and a device uint32 health word (bit0: a probe
  * sample was not finite). */

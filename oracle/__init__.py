@@ -18,6 +18,10 @@ Contents
                  for tiny grids, in float32 or float64.
 ``voxel``        brute-force per-point SDF evaluation used to pin the host
                  voxelizer on small grids.
+``microphones``  the reference's microphone recording (trilinear gathers,
+                 polar patterns).
+``membranes``    membrane-source weights over the whole grid (circular Bessel
+                 and rectangular sine modes).
 
 Parity status of each function is recorded in its docstring ("pinned by ...").
 """

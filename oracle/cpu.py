@@ -47,7 +47,8 @@ class _Problem(C.Structure):
                 ("decay", _fp * 3), ("n_src", C.c_int32), ("src_kind", _i32p),
                 ("src_a", _i64p), ("src_b", _i64p), ("src_wave", _dp), ("wave_len", C.c_int32),
                 ("n_probe", C.c_int32), ("probe_idx", _i64p), ("n_pole", C.c_int32),
-                ("poles", C.POINTER(_Pole)), ("material", _u8p), ("ade_inv_dx", C.c_double)]
+                ("poles", C.POINTER(_Pole)), ("material", _u8p), ("ade_inv_dx", C.c_double),
+                ("src_weight", C.POINTER(_dp))]
 
 
 class _Material(C.Structure):
@@ -75,7 +76,7 @@ class Problem:
     nonuniform: bool = False
     inv_face: tuple = (None, None, None)
     inv_cell: tuple = (None, None, None)
-    sources: list = field(default_factory=list)  # (kind, a, b, wave float64[])
+    sources: list = field(default_factory=list)  # (kind, a, b, wave float64[][, weights float64 shape])
     probes: list = field(default_factory=list)   # flat indices
     poles: list = field(default_factory=list)    # Pole
     material: np.ndarray | None = None
@@ -149,6 +150,15 @@ class _Packed:
         s.n_src, s.wave_len = ns, wl
         s.src_kind, s.src_a, s.src_b = _ptr(kinds, _i32p), _ptr(sa, _i64p), _ptr(sb, _i64p)
         s.src_wave = _ptr(wave, _dp)
+        wptrs = (_dp * max(ns, 1))()
+        for q, src in enumerate(pb.sources):
+            if src[0] == 2:
+                w = np.ascontiguousarray(src[4], dtype=np.float64)
+                assert w.shape == tuple(pb.shape)
+                self.keep.append(w)
+                wptrs[q] = w.ctypes.data_as(_dp)
+        self.keep.append(wptrs)
+        s.src_weight = C.cast(wptrs, C.POINTER(_dp))
         pr = np.array(pb.probes or [0], dtype=np.int64)
         self.keep.append(pr)
         s.n_probe = len(pb.probes)

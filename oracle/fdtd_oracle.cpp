@@ -223,6 +223,9 @@ struct OrcProblem {
     const OrcPole* poles;
     const uint8_t* material;       // per-cell material id (null if no poles)
     double ade_inv_dx;             // uniform grids: 1/dx for the divergence source
+    // membrane sources (src_kind 2, src_a = field 0 p / 1 vx / 2 vy / 3 vz):
+    // per source a float64 weight per cell, null for point and plane sources
+    const double* const* src_weight;
 };
 
 // Per-phase entry points (float) so tests can compare single phases.
@@ -412,8 +415,20 @@ struct Runner {
         }
         pml_velocity<T>(vx, vy, vz, tab(ddx), tab(ddy), tab(ddz), nx, ny, nz);
         pml_pressure<T>(p, tab(ddx), tab(ddy), tab(ddz), nx, ny, nz);
+        // sources in list order (core/solver.py:2386-2433)
         for (int q = 0; q < pb.n_src; ++q) {
             const double w = pb.src_wave[static_cast<int64_t>(q) * pb.wave_len + s];
+            if (pb.src_kind[q] == 2) {
+                // membrane: field[mask] += w * weight[mask], mask = (weight > 0) & air;
+                // numpy adds the float64 product to the element in float64
+                T* fld[4] = {p, vx, vy, vz};
+                T* F = fld[pb.src_a[q]];
+                const double* wt = pb.src_weight[q];
+#pragma omp parallel for schedule(static)
+                for (int64_t c = 0; c < n; ++c)
+                    if (wt[c] > 0.0 && pb.air[c]) F[c] = static_cast<T>(static_cast<double>(F[c]) + w * wt[c]);
+                continue;
+            }
             if (pb.src_kind[q] == 0) {
                 const int64_t c = pb.src_a[q];
                 if (pb.air[c]) p[c] = static_cast<T>(static_cast<double>(p[c]) + w);

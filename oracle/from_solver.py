@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from . import cpu, tables as T
+from . import cpu, membranes as M, tables as T
 
 
 def problem_from_solver(solver, n_steps: int) -> cpu.Problem:
@@ -31,6 +31,11 @@ def problem_from_solver(solver, n_steps: int) -> cpu.Problem:
     times = T.accumulated_times(dt, n_steps)
     srcs = []
     for s in solver._sources:
+        if s.source_type == "membrane":
+            wf = s.waveform
+            w = T.waveform_table(times, wf.frequency, wf.bandwidth, wf.amplitude)
+            srcs.append((2, M.injection_field(s), 0, w, M.weights_of(s, grid)))
+            continue
         w = T.waveform_table(times, s.frequency, s.bandwidth, s.amplitude)
         if s.source_type == "point":
             srcs.append((0, int(np.ravel_multi_index(tuple(int(v) for v in s.position), solver.shape)), 0, w))

@@ -3,6 +3,8 @@ pressure-velocity FDTD time step.
 
 Drop-in surface (the reference's names and semantics, ``backend="mi355x"``):
 ``FDTDSolver``, ``UniformGrid``, ``NonuniformGrid``, ``PML``, ``GaussianPulse``,
+membrane sources (``CircularMembraneSource``, ``RectangularMembraneSource``),
+``Microphone``,
 SDF geometry (``Box``, ``Sphere``, ``Cylinder``, ``Union``, ``Intersection``,
 ``Difference``, ``HelmholtzResonator``, ``MaterializedGeometry``) and ADE
 materials (``Pole``, ``PoleType``, ``SimpleMaterial``, ``AIR_20C``).
@@ -15,6 +17,7 @@ from .geometry import (Box, Cylinder, Difference, HelmholtzResonator, Intersecti
                        MaterializedGeometry, SDFPrimitive, Sphere, Union)
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
 from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, Pole, PoleType, SimpleMaterial
+from .membranes import CircularMembraneSource, MembraneSource, RectangularMembraneSource
 from .microphones import POLAR_PATTERNS, Microphone
 from .solver import FDTDSolver, has_mi355x_backend
 from .sources import GaussianPulse, Probe
@@ -26,5 +29,5 @@ __all__ = [
     "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder",
     "Union", "Intersection", "Difference", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
-    "POLAR_PATTERNS",
+    "POLAR_PATTERNS", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
 ]

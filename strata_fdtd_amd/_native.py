@@ -30,6 +30,7 @@ _SIGS = {
     "fdtd_ctx_set_pml": (_i, [_vp, _vp, _vp, _vp]),
     "fdtd_ctx_set_sources": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64]),
     "fdtd_ctx_set_probes": (_i, [_vp, _i, _vp, _vp, _i64]),
+    "fdtd_ctx_set_membranes": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _i64]),
     "fdtd_ctx_set_microphones": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64]),
     "fdtd_ctx_set_counters": (_i, [_vp, _vp, _vp]),
     "fdtd_ctx_set_ade": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _f]),

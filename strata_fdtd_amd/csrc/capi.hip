@@ -91,6 +91,11 @@ struct fdtd_ctx {
     double* mic_w_dev = nullptr;      // owned: [mic_cap][4][8]
     int32_t* mic_f64_dev = nullptr;   // owned: [mic_cap][4]
     int mic_cap = 0;
+    // owned membrane tables: one allocation each, grown as needed
+    struct DevBuf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    } m_cell, m_estart, m_esrc, m_efield, m_eweight, m_pstart, m_pidx;
     int64_t* step_counter = nullptr;  // same as a.step_ptr, writable
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -133,6 +138,9 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
     if (c->mic_w_dev) (void)hipFree(c->mic_w_dev);
     if (c->mic_f64_dev) (void)hipFree(c->mic_f64_dev);
+    for (fdtd_ctx::DevBuf* b : {&c->m_cell, &c->m_estart, &c->m_esrc, &c->m_efield, &c->m_eweight, &c->m_pstart,
+                                &c->m_pidx})
+        if (b->p) (void)hipFree(b->p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->sweep_ev) (void)hipEventDestroy(e);
@@ -310,6 +318,75 @@ int fdtd_ctx_set_microphones(fdtd_ctx* c, int n, const int32_t* idx, const doubl
     c->a.mic_f64 = c->mic_f64_dev;
     c->a.mic_out = out;
     c->a.mic_len = n > 0 ? out_len : 0;
+    return 0;
+}
+
+// Copy a host table into an owned device buffer (set-up only: grows with
+// hipMalloc when needed; the copy is synchronous).
+static int upload(fdtd_ctx::DevBuf& b, const void* host, size_t bytes) {
+    if (bytes > b.bytes) {
+        if (b.p) (void)hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        if (int rc = check(hipMalloc(&b.p, bytes), "hipMalloc")) return rc;
+        b.bytes = bytes;
+    }
+    if (bytes) return check(hipMemcpy(b.p, host, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    return 0;
+}
+
+int fdtd_ctx_set_membranes(fdtd_ctx* c, int n_cells, const int64_t* cell, const int32_t* entry_start,
+                           int n_entries, const int32_t* entry_src, const int32_t* entry_field,
+                           const double* entry_weight, const int32_t* probe_start, int n_probe_refs,
+                           const int32_t* probe_idx, int n_src, const double* wave, int64_t wave_len) {
+    if (!c) return fail(-1, "null context");
+    if (!c->have_grid) return fail(-3, "set the grid before the membrane sources");
+    if (n_cells < 0 || n_entries < 0 || n_probe_refs < 0 || n_src < 0) return fail(-2, "negative table size");
+    if (n_cells > 0) {
+        if (!cell || !entry_start || !entry_src || !entry_field || !entry_weight || !probe_start ||
+            (n_probe_refs > 0 && !probe_idx) || !wave || wave_len < 1 || n_src < 1)
+            return fail(-2, "bad membrane arrays");
+        const int64_t cells = static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz;
+        if (entry_start[0] != 0 || entry_start[n_cells] != n_entries || probe_start[0] != 0 ||
+            probe_start[n_cells] != n_probe_refs)
+            return fail(-2, "membrane CSR tables do not span their entries");
+        for (int e = 0; e < n_cells; ++e) {
+            if (cell[e] < 0 || cell[e] >= cells || cell[e] > INT32_MAX)
+                return fail(-2, "membrane cell %d: flat index %lld outside the grid", e, (long long)cell[e]);
+            if (e > 0 && cell[e] <= cell[e - 1]) return fail(-2, "membrane cells must be increasing");
+            if (entry_start[e + 1] <= entry_start[e] || probe_start[e + 1] < probe_start[e])
+                return fail(-2, "membrane cell %d: bad entry or probe range", e);
+        }
+        for (int t = 0; t < n_entries; ++t)
+            if (entry_src[t] < 0 || entry_src[t] >= n_src || entry_field[t] < 0 || entry_field[t] > 3)
+                return fail(-2, "membrane entry %d: source %d / field %d out of range", t, entry_src[t],
+                            entry_field[t]);
+        for (int t = 0; t < n_probe_refs; ++t)
+            if (probe_idx[t] < 0 || probe_idx[t] >= c->a.n_probe)
+                return fail(-2, "membrane probe reference %d: probe %d not set", t, probe_idx[t]);
+    }
+    DeviceGuard g(c->device);
+    drop_graph(c);
+    c->a.n_mcell = 0;
+    if (n_cells == 0) return 0;
+    std::vector<int32_t> cell32(cell, cell + n_cells);
+    if (int rc = upload(c->m_cell, cell32.data(), sizeof(int32_t) * n_cells)) return rc;
+    if (int rc = upload(c->m_estart, entry_start, sizeof(int32_t) * (n_cells + 1))) return rc;
+    if (int rc = upload(c->m_esrc, entry_src, sizeof(int32_t) * n_entries)) return rc;
+    if (int rc = upload(c->m_efield, entry_field, sizeof(int32_t) * n_entries)) return rc;
+    if (int rc = upload(c->m_eweight, entry_weight, sizeof(double) * n_entries)) return rc;
+    if (int rc = upload(c->m_pstart, probe_start, sizeof(int32_t) * (n_cells + 1))) return rc;
+    if (int rc = upload(c->m_pidx, probe_idx, sizeof(int32_t) * n_probe_refs)) return rc;
+    c->a.mcell = static_cast<const int32_t*>(c->m_cell.p);
+    c->a.m_estart = static_cast<const int32_t*>(c->m_estart.p);
+    c->a.m_esrc = static_cast<const int32_t*>(c->m_esrc.p);
+    c->a.m_efield = static_cast<const int32_t*>(c->m_efield.p);
+    c->a.m_eweight = static_cast<const double*>(c->m_eweight.p);
+    c->a.m_pstart = static_cast<const int32_t*>(c->m_pstart.p);
+    c->a.m_pidx = static_cast<const int32_t*>(c->m_pidx.p);
+    c->a.m_wave = wave;
+    c->a.m_wave_len = wave_len;
+    c->a.n_mcell = n_cells;
     return 0;
 }
 
@@ -694,8 +771,9 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
     for (int q = 0; q < n; ++q) {
         fdtd_ctx* c = ctxs[q];
         if (int rc = ready(c)) return rc;
-        if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0)
-            return fail(-2, "batch entry %d: nonuniform, ADE or microphone solves run on their own context", q);
+        if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0 || c->a.n_mcell > 0)
+            return fail(-2, "batch entry %d: nonuniform, ADE, microphone or membrane-source solves run on their "
+                            "own context", q);
         if (c->device != b->device) return fail(-2, "batch entry %d is on another device", q);
         if (static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz >= kMaxCells32)
             return fail(-2, "batch entry %d: batched solves need fewer than 2^30 cells", q);

@@ -122,6 +122,20 @@ struct FusedArgs {
     double* mic_out;                // [mic_len][n_mic][4]
     int64_t mic_len;
 
+    // weighted-mask (membrane) sources, applied after the sweep and the ADE
+    // fix-up: per listed cell, its entries in source order, then the probes
+    // on that cell are recorded again
+    int n_mcell;
+    const int32_t* mcell;           // device [n_mcell] flat cell indices, each cell once
+    const int32_t* m_estart;        // device [n_mcell + 1] first entry of each cell
+    const int32_t* m_esrc;          // device [n_entry] membrane (row of m_wave)
+    const int32_t* m_efield;        // device [n_entry] 0 p, 1 vx, 2 vy, 3 vz
+    const double* m_eweight;        // device [n_entry] mode-shape weight
+    const int32_t* m_pstart;        // device [n_mcell + 1] first probe of each cell
+    const int32_t* m_pidx;          // device probe numbers
+    const double* m_wave;           // device [n_msrc][m_wave_len] waveform per step
+    int64_t m_wave_len;
+
     // ADE (materials with poles)
     int n_pole;
     int pole_mat[FDTD_MAX_POLES];

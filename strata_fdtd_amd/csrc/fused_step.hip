@@ -601,6 +601,38 @@ __global__ void k_mic_gather(const FusedArgs a) {
     }
 }
 
+// Membrane sources of step s (core/solver.py:2389-2412): after the sweep and
+// the ADE fix-up the step's output is in set (s+1)&1. One thread per listed
+// cell applies the cell's entries in source order,
+//   field[c] = float(double(field[c]) + wave[src][s] * weight)
+// (numpy adds the float64 product to the float32 element in float64 and
+// stores float32), then records the probes on the cell, which the reference
+// samples after every source.
+__global__ void k_membrane(const FusedArgs a) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.n_mcell) return;
+    const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit) return;
+    if (s >= a.m_wave_len) {
+        atomicOr(a.health, 2u);
+        return;
+    }
+    const int out = static_cast<int>((s + 1) & 1);
+    const int64_t c = a.mcell[e];
+    for (int t = a.m_estart[e]; t < a.m_estart[e + 1]; ++t) {
+        float* F = a.f[out][a.m_efield[t]];
+        const double add = a.m_wave[static_cast<int64_t>(a.m_esrc[t]) * a.m_wave_len + s] * a.m_eweight[t];
+        F[c] = static_cast<float>(static_cast<double>(F[c]) + add);
+    }
+    if (s < a.trace_len) {
+        const float pv = a.f[out][0][c];
+        for (int t = a.m_pstart[e]; t < a.m_pstart[e + 1]; ++t) {
+            a.trace[s * a.n_probe + a.m_pidx[t]] = pv;
+            if (!isfinite(pv)) atomicOr(a.health, 1u);
+        }
+    }
+}
+
 __global__ void k_advance(int64_t* step, int n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) step[0] = step[0] + n;
 }
@@ -660,7 +692,8 @@ int fused_grid(FusedArgs& a, int rows, int planes, int stack) {
     return blocks;
 }
 
-// One full step: [ADE density] -> fused sweep -> [ADE fix-up]. ev_before /
+// One full step: [ADE density] -> fused sweep -> [ADE fix-up] -> [membrane
+// sources] -> [microphones]. ev_before /
 // ev_after (optional) are recorded around the sweep alone, for timing it.
 hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool ade, int blocks,
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after) {
@@ -687,6 +720,10 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
     if (ade) {
         if (nu) hipLaunchKernelGGL(k_ade_fixup<true>, dim3(box_blocks), dim3(256), 0, st, a);
         else hipLaunchKernelGGL(k_ade_fixup<false>, dim3(box_blocks), dim3(256), 0, st, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (a.n_mcell > 0) {
+        hipLaunchKernelGGL(k_membrane, dim3((a.n_mcell + 255) / 256), dim3(256), 0, st, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (a.n_mic > 0) {

@@ -148,6 +148,7 @@ class FDTDSolver:
         # device tables with growable capacity
         self._cap = 0
         self._wave = None
+        self._mwave = None
         self._trace = None
         self._probe_order: list[str] = []
         self._mic_out = None
@@ -287,6 +288,7 @@ class FDTDSolver:
                 raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
             self.geometry = mask.astype(bool)
         self._dirty_geometry = True
+        self._dirty_sources = True   # membrane cells are the air cells of their patch
 
     def _voxelize(self, geometry):
         """(air mask bool, material ids uint8 or None) of an SDF tree or a
@@ -322,9 +324,16 @@ class FDTDSolver:
         return pos
 
     def add_source(self, source) -> None:
-        """Point or plane source (core/solver.py:1782-1841)."""
+        """Point, plane or membrane source (core/solver.py:1782-1841)."""
         if getattr(source, "source_type", "point") == "membrane":
-            raise NotImplementedError("membrane sources are not implemented on the mi355x backend")
+            extent = self._grid.physical_extent()
+            for ax, name in enumerate("xyz"):
+                v = source.center[ax]
+                if not 0 <= v <= extent[ax]:
+                    raise ValueError(f"Membrane center {name}={v:.4f}m is outside grid (0-{extent[ax]:.4f}m)")
+            self._sources.append(source)
+            self._dirty_sources = True
+            return
         if source.source_type == "point":
             source.position = self._position_to_index(source.position, "Source")
         self._sources.append(source)
@@ -509,13 +518,20 @@ class FDTDSolver:
         cap = self._cap
         if total_steps > cap:
             cap = max(total_steps, 2 * cap, 1024)
-        srcs = self._sources
+        srcs = [s for s in self._sources if s.source_type != "membrane"]
+        membranes = [s for s in self._sources if s.source_type == "membrane"]
+        times = accumulated_times(self.dt, cap) if self._sources else None
         if srcs:
-            times = accumulated_times(self.dt, cap)
             wave = np.stack([_fast_waveform(s, times, self.dt) for s in srcs]).astype(np.float64)
             self._wave = torch.from_numpy(np.ascontiguousarray(wave)).to(self._device)
         else:
             self._wave = None
+        if membranes:
+            # each membrane samples its own waveform object
+            wave = np.stack([_fast_waveform(m.waveform, times, self.dt) for m in membranes]).astype(np.float64)
+            self._mwave = torch.from_numpy(np.ascontiguousarray(wave)).to(self._device)
+        else:
+            self._mwave = None
         names = list(self._probes)
         new_trace = torch.zeros((cap, max(len(names), 1)), dtype=torch.float32, device=self._device)
         if self._trace is not None and self._step_count:
@@ -566,8 +582,80 @@ class FDTDSolver:
                         dtype=np.int64)
         _native.call("fdtd_ctx_set_probes", self._ctx, len(names), flat.ctypes.data,
                      _native.ptr(self._trace), cap)
+        self._set_membranes(membranes, names, cap)
         self._cap = cap
         self._dirty_sources = False
+
+    def _set_membranes(self, membranes: list, probe_names: list, cap: int) -> None:
+        """Per-cell tables of the membrane sources: the cells (air, weight > 0)
+        of every membrane, each cell once, with its entries (membrane, field,
+        weight) in source order and the probes recorded on it after them."""
+        if not membranes:
+            _native.call("fdtd_ctx_set_membranes", self._ctx, 0, None, None, 0, None, None, None, None, 0, None,
+                         0, None, 0)
+            return
+        air = np.asarray(self.geometry, dtype=bool).ravel()
+        cells, order, rows, fields, weights = [], [], [], [], []
+        for q, m in enumerate(membranes):
+            if m._cached_weights is None:
+                m._cached_weights = m.get_injection_weights(self._grid)
+                m._cached_mask = m._cached_weights > 0
+            w = m._cached_weights.ravel()
+            idx = np.flatnonzero(m._cached_mask.ravel() & air)
+            cells.append(idx)
+            order.append(np.full(idx.size, q, dtype=np.int64))
+            rows.append(np.full(idx.size, q, dtype=np.int32))
+            fields.append(np.full(idx.size, m.injection_field(), dtype=np.int32))
+            weights.append(w[idx])
+        cells = np.concatenate(cells)
+        order = np.concatenate(order)
+        perm = np.lexsort((order, cells))          # by cell, then source order
+        cells = cells[perm]
+        rows = np.concatenate(rows)[perm]
+        fields = np.concatenate(fields)[perm]
+        weights = np.ascontiguousarray(np.concatenate(weights)[perm], dtype=np.float64)
+        ucells, first = np.unique(cells, return_index=True)
+        estart = np.append(first, cells.size).astype(np.int32)
+        self._check_membrane_order(membranes)
+        # probes re-recorded after the injection on cells whose pressure changes
+        pflat = [(np.ravel_multi_index(self._probes[n].position, self.shape), q) for q, n in enumerate(probe_names)]
+        pstart, pidx = [0], []
+        for e, c in enumerate(ucells):
+            if pflat and np.any(fields[estart[e]:estart[e + 1]] == 0):
+                pidx += [q for f, q in pflat if f == c]
+            pstart.append(len(pidx))
+        ucells = np.ascontiguousarray(ucells, dtype=np.int64)
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        fields = np.ascontiguousarray(fields, dtype=np.int32)
+        pstart = np.array(pstart, dtype=np.int32)
+        pidx = np.array(pidx or [0], dtype=np.int32)
+        _native.call("fdtd_ctx_set_membranes", self._ctx, int(ucells.size), ucells.ctypes.data, estart.ctypes.data,
+                     int(cells.size), rows.ctypes.data, fields.ctypes.data, weights.ctypes.data,
+                     pstart.ctypes.data, int(pstart[-1]), pidx.ctypes.data, len(membranes),
+                     _native.ptr(self._mwave), cap)
+
+    def _check_membrane_order(self, membranes) -> None:
+        """The step adds point and plane sources before membranes. That is the
+        reference's order unless a point or plane source listed after a
+        pressure membrane hits one of its cells; refuse that case."""
+        air = np.asarray(self.geometry, dtype=bool)
+        for k, m in enumerate(self._sources):
+            if m.source_type != "membrane" or m.injection_type != "pressure":
+                continue
+            cells = m._cached_mask & air
+            for s in self._sources[k + 1:]:
+                if s.source_type == "point":
+                    hit = bool(cells[tuple(int(v) for v in s.position)])
+                elif s.source_type == "plane":
+                    sl = [slice(None)] * 3
+                    sl[int(s.position["axis"])] = int(s.position["index"])
+                    hit = bool(np.any(cells[tuple(sl)]))
+                else:
+                    continue
+                if hit:
+                    raise NotImplementedError(
+                        "a point or plane source listed after a membrane source on the same cell is not "
+                        "supported (the step adds point and plane sources first); add it before the membrane")
 
     def _prepare(self, n_steps: int) -> None:
         if self._dirty_geometry:
