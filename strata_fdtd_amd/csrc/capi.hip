@@ -19,7 +19,7 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
 hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
-                        int cap_waves, int rows, hipStream_t st);
+                        int cap_waves, int rows, bool nts, hipStream_t st);
 hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
                           const float*, const float*, hipStream_t);
 hipError_t phase_rigid(float*, float*, float*, const uint8_t*, int, int, int, hipStream_t);
@@ -705,9 +705,11 @@ struct fdtd_batch {
     int stage = 0;
     int n = 0;
     int total_waves = 0;
+    bool nts = true;                    // non-temporal stores (the solves exceed the cache)
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int graph_waves = 0;                // wave capacity of the captured grid
+    bool graph_nts = true;
 };
 
 static void batch_drop_graph(fdtd_batch* b) {
@@ -801,10 +803,12 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
     FusedArgs* tab = b->table_host[i];
     int* meta = b->meta_host[i];
     int total = 0;
+    int64_t cells = 0;
     meta[0] = n;
     meta[1] = 0;
     for (int q = 0; q < n; ++q) {
         FusedArgs a = ctxs[q]->a;
+        cells += static_cast<int64_t>(a.nx) * a.ny * a.nz;
         fused_grid(a, b->rows, ctxs[q]->planes, 1);
         a.koff = 0;
         a.step_limit = step_limits[q];
@@ -824,12 +828,14 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
     b->stage ^= 1;
     b->n = n;
     b->total_waves = total;
+    b->nts = cells > kCacheResidentCells;
     return 0;
 }
 
 static int batch_enqueue(fdtd_batch* b, hipStream_t st, int n_steps, int cap_waves) {
     for (int s = 0; s < n_steps; ++s)
-        if (int rc = check(launch_batch(b->table_dev, b->meta_dev + 1, b->meta_dev, b->cap, cap_waves, b->rows, st),
+        if (int rc = check(launch_batch(b->table_dev, b->meta_dev + 1, b->meta_dev, b->cap, cap_waves, b->rows,
+                                        b->nts, st),
                            "batch step launch"))
             return rc;
     return 0;
@@ -841,9 +847,11 @@ int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph) {
     if (b->n == 0 || n_steps == 0) return 0;
     DeviceGuard g(b->device);
     if (!use_graph) return batch_enqueue(b, b->stream, n_steps, b->total_waves);
-    if (!b->graph || b->graph_steps != n_steps || b->graph_waves < b->total_waves) {
-        // capture with head room so that later tables of up to this many waves replay it
-        const int cap_waves = std::max(b->total_waves + b->total_waves / 2, b->graph_waves * 2);
+    if (!b->graph || b->graph_steps != n_steps || b->graph_waves < b->total_waves || b->graph_nts != b->nts) {
+        // capture with head room so that later tables of up to this many waves
+        // replay it; a recapture for another reason keeps the capacity
+        int cap_waves = b->graph_waves;
+        if (cap_waves < b->total_waves) cap_waves = std::max(b->total_waves + b->total_waves / 2, b->graph_waves * 2);
         batch_drop_graph(b);
         hipStream_t cap = nullptr;
         if (int rc = check(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate")) return rc;
@@ -862,6 +870,7 @@ int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph) {
         if (int r3 = check(e, "graph instantiate")) return r3;
         b->graph_steps = n_steps;
         b->graph_waves = cap_waves;
+        b->graph_nts = b->nts;
     }
     return check(hipGraphLaunch(b->graph, b->stream), "graph launch");
 }

@@ -109,8 +109,14 @@ template <typename T>
 __device__ __forceinline__ T ldo(const GMEM T* base, uint32_t idx) { return ldx(base, idx); }
 template <typename T>
 __device__ __forceinline__ void sto(GMEM T* base, uint32_t idx, T v) { stx(base, idx, v); }
-template <typename T>
-__device__ __forceinline__ void stnt(GMEM T* base, uint32_t idx, T v) { stx_nt(base, idx, v); }
+// A step's field outputs: non-temporal stores for grids beyond the Infinity
+// Cache (kCacheResidentCells), ordinary stores below it so that the next step
+// finds them on chip.
+template <bool NTS, typename T>
+__device__ __forceinline__ void stout(GMEM T* base, uint32_t idx, T v) {
+    if constexpr (NTS) stx_nt(base, idx, v);
+    else stx(base, idx, v);
+}
 
 // ---------------------------------------------------------------- box lookups
 // Inside the box strips every box quantity a cell needs is loaded at the top
@@ -155,7 +161,7 @@ __device__ __forceinline__ f2v box_val(const BoxTab& t, int off, bool in) {
 // ---------------------------------------------------------------- forward
 // Strips that miss the design box: r = kappa = 1, so every face coefficient
 // is cv0 / ((1 + 1) * 0.5) = cv0 exactly and every cell coefficient cp0.
-template <int R>
+template <int R, bool NTS>
 __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_t s, int seg, int j0, int i0, int i1,
                                                      int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -193,12 +199,12 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
     for (int i = i0; i < i1; ++i) {
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
-        float pn[R], vxo[R], vyo[R], vzo[R], hr[R];
-        uint32_t fl[R];
+        float pn[R], vxo[R], vyo[R], vzo[R], hr[R], hl_p[R], hl_vz[R];
+        uint32_t fl[R], hl_f[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            pn[r] = vxo[r] = vyo[r] = vzo[r] = hr[r] = 0.f;
-            fl[r] = 0;
+            pn[r] = vxo[r] = vyo[r] = vzo[r] = hr[r] = hl_p[r] = hl_vz[r] = 0.f;
+            fl[r] = hl_f[r] = 0;
             if (!(rvld[r] && kin)) continue;
             const uint32_t c = cix(i, j0 + r, k, ny, nz);
             if (nxt) pn[r] = ldo(P, c + sx);
@@ -206,6 +212,11 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
             vyo[r] = ldo(VY, c);
             vzo[r] = ldo(VZ, c);
             fl[r] = ldo(FL, c);
+            if (lane == 0 && k0 > 0) {  // the face below the segment: loads only, used in the row loop
+                hl_p[r] = ldo(P, c - 1);
+                hl_vz[r] = ldo(VZ, c - 1);
+                hl_f[r] = ldo(FL, c - 1);
+            }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -247,9 +258,8 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
             if (lane == 0) {
                 vzm = 0.f;
                 if (k0 > 0 && kin) {
-                    const uint32_t cl = c - 1;
-                    const float v = ldo(VZ, cl) + cv * (pr - ldo(P, cl));
-                    vzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z) ? v : 0.f;
+                    const float v = hl_vz[r] + cv * (pr - hl_p[r]);
+                    vzm = (hl_f[r] & FDTD_FLAG_OPEN_Z) ? v : 0.f;
                 }
             }
             const float div = ((vxn - vxm[r]) + (vyn - vyprev)) + (vzn - vzm);
@@ -259,10 +269,10 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
             if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
                 pnew = static_cast<float>(static_cast<double>(pnew) + wsrc);
             if (kin) {
-                stnt(Po, c, pnew);
-                stnt(VXo, c, vxn * dx_i);
-                stnt(VYo, c, vyn * dy_j);
-                stnt(VZo, c, vzn * dz_k);
+                stout<NTS>(Po, c, pnew);
+                stout<NTS>(VXo, c, vxn * dx_i);
+                stout<NTS>(VYo, c, vyn * dy_j);
+                stout<NTS>(VZo, c, vzn * dz_k);
                 if (pmask)
                     for (int q = 0; q < a.n_probe; ++q)
                         if (((pmask >> q) & 1u) && a.probe[3 * q] == i && a.probe[3 * q + 1] == j &&
@@ -281,7 +291,7 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
 // kappa, and the forward history of the box. The face coefficient of each
 // face is formed from the same two r values in the same order as in the
 // per-cell kernel, so the results are the same bits.
-template <int R>
+template <int R, bool NTS>
 __device__ __forceinline__ void design_forward_box_strip(const DesignArgs& a, int64_t s, int seg, int j0, int i0,
                                                          int i1, int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -406,6 +416,8 @@ __device__ __forceinline__ void design_forward_box_strip(const DesignArgs& a, in
             if (lane == 0) {
                 vzm = 0.f;
                 if (k0 > 0 && kin) {
+                    // the face below the segment (loaded here: the box strips'
+                    // register budget has no room to carry it from the load phase)
                     const uint32_t cl = c - 1;
                     const float v = ldo(VZ, cl) + cvf(a, rkh[r], rc) * (pr - ldo(P, cl));
                     vzm = (ldo(FL, cl) & FDTD_FLAG_OPEN_Z) ? v : 0.f;
@@ -418,10 +430,10 @@ __device__ __forceinline__ void design_forward_box_strip(const DesignArgs& a, in
             if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
                 pnew = static_cast<float>(static_cast<double>(pnew) + wsrc);
             if (kin) {
-                stnt(Po, c, pnew);
-                stnt(VXo, c, vxn * dx_i);
-                stnt(VYo, c, vyn * dy_j);
-                stnt(VZo, c, vzn * dz_k);
+                stout<NTS>(Po, c, pnew);
+                stout<NTS>(VXo, c, vxn * dx_i);
+                stout<NTS>(VYo, c, vyn * dy_j);
+                stout<NTS>(VZo, c, vzn * dz_k);
                 if (pmask)
                     for (int q = 0; q < a.n_probe; ++q)
                         if (((pmask >> q) & 1u) && a.probe[3 * q] == i && a.probe[3 * q + 1] == j &&
@@ -429,8 +441,10 @@ __device__ __forceinline__ void design_forward_box_strip(const DesignArgs& a, in
                             a.trace[s * a.n_probe + q] = pnew;
                 if (hist) {
                     if (ei.in && ej[r].in && ek.in)
-                        sto(hp, static_cast<uint32_t>((ei.off * (by + 2) + ej[r].off) * (bz + 2) + ek.off), pr);
-                    if (di.in && dj[r].in && dk.in) sto(hd, static_cast<uint32_t>(boff(di, dj[r], dk)), div);
+                        stx_nt(hp, static_cast<uint32_t>((ei.off * (by + 2) + ej[r].off) * (bz + 2) + ek.off), pr);
+                    // the history is read back only by the adjoint, much later:
+                    // non-temporal, so it does not evict the fields from the caches
+                    if (di.in && dj[r].in && dk.in) stx_nt(hd, static_cast<uint32_t>(boff(di, dj[r], dk)), div);
                 }
             }
             vxm[r] = vxn;
@@ -500,7 +514,7 @@ __device__ __forceinline__ bool box_strip(const DesignArgs& a, int rows, int w, 
 
 // One launch per step over every strip: the box strips (heavier) first, then
 // the others; each XCD gets an equal contiguous share of both.
-template <int R>
+template <int R, bool NTS>
 __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, int64_t s) {
     const int xcd = blockIdx.x & 7, bx = blockIdx.x >> 3;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -508,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
     if (bx < a.b_blocks_per_xcd) {
         int seg, j0, i0, i1;
         if (box_strip(a, R, (xcd * a.b_blocks_per_xcd + bx) * kBlockWaves + wid, seg, j0, i0, i1))
-            design_forward_box_strip<R>(a, s, seg, j0, i0, i1, lane);
+            design_forward_box_strip<R, NTS>(a, s, seg, j0, i0, i1, lane);
         return;
     }
     const int w = (xcd * a.blocks_per_xcd + bx - a.b_blocks_per_xcd) * kBlockWaves + wid;
@@ -516,16 +530,18 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
     int seg, sj, si;
     plain_strip(a, w, seg, sj, si);
     const int i0 = si * a.planes;
-    design_forward_strip<R>(a, s, seg, sj * R, i0, min(i0 + a.planes, a.nx), lane);
+    design_forward_strip<R, NTS>(a, s, seg, sj * R, i0, min(i0 + a.planes, a.nx), lane);
 }
 
 // ---------------------------------------------------------------- adjoint
 // pb = dX dY dZ (P + g at probe cells); q = air cp0 kappa pb.
 // pm: the strip (with its halo) holds a probe.
 // dz: the k decay of the cell (dec[2][k]), kept in a register by the caller.
-__device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
-                                           uint32_t c, int64_t n, bool pm, float dz) {
-    float v = ldo(P, c);
+// The _v forms take the cell's loaded pressure (and flag), so that callers can
+// issue those loads together with the plane's other loads.
+__device__ __forceinline__ float design_pb_v(const DesignArgs& a, float pv, int i, int j, int k, int64_t n, bool pm,
+                                             float dz) {
+    float v = pv;
     if (pm)
         for (int q = 0; q < a.n_probe; ++q)
             if (a.probe[3 * q] == i && a.probe[3 * q + 1] == j && a.probe[3 * q + 2] == k)
@@ -534,18 +550,27 @@ __device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float
     return lam * v;
 }
 
+__device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
+                                           uint32_t c, int64_t n, bool pm, float dz) {
+    return design_pb_v(a, ldo(P, c), i, j, k, n, pm, dz);
+}
+
+__device__ __forceinline__ float design_q_v(const DesignArgs& a, uint32_t f, float pv, int i, int j, int k, int64_t n,
+                                            bool pm, float kap, float dz) {
+    const float q = a.cp0 * kap * design_pb_v(a, pv, i, j, k, n, pm, dz);
+    return (f & FDTD_FLAG_AIR) ? q : 0.f;
+}
+
 // Branch-free: the flag and pressure loads are issued together, not one
 // after the other behind a test of the flag.
 __device__ __forceinline__ float design_q(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
                                           uint32_t c, int64_t n, bool pm, float kap, float dz) {
-    const uint32_t f = ldo(gm(a.flags), c);
-    const float q = a.cp0 * kap * design_pb(a, P, i, j, k, c, n, pm, dz);
-    return (f & FDTD_FLAG_AIR) ? q : 0.f;
+    return design_q_v(a, ldo(gm(a.flags), c), ldo(P, c), i, j, k, n, pm, kap, dz);
 }
 
 // Adjoint step n: (P, V) = lambda_{n+1} in set (n+1)&1 -> lambda_n in set n&1.
 // Strips that miss the design box (r = kappa = 1; no gradient).
-template <int R>
+template <int R, bool NTS>
 __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_t n, int seg, int j0, int i0, int i1,
                                                      int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -590,34 +615,64 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
     for (int i = i0; i < i1; ++i) {
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
-        float pbn[R], qn[R], vx[R], vy[R], vz[R], qhr[R];
-        uint32_t fl[R];
+        // loads only (the halos' arithmetic follows once they are all issued)
+        float pbn[R], qn[R], vx[R], vy[R], vz[R], hl_p[R], hl_vz[R], hr_p[R];
+        uint32_t fl[R], fln[R], hl_f[R], hr_f[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            pbn[r] = qn[r] = vx[r] = vy[r] = vz[r] = qhr[r] = 0.f;
-            fl[r] = 0;
+            pbn[r] = vx[r] = vy[r] = vz[r] = hl_p[r] = hl_vz[r] = hr_p[r] = 0.f;
+            fl[r] = fln[r] = hl_f[r] = hr_f[r] = 0;
             if (!(rvld[r] && kin)) continue;
-            const int j = j0 + r;
-            const uint32_t c = cix(i, j, k, ny, nz);
+            const uint32_t c = cix(i, j0 + r, k, ny, nz);
             if (nxt) {
-                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm, dz_k);
-                qn[r] = (ldo(FL, c + sx) & FDTD_FLAG_AIR) ? a.cp0 * pbn[r] : 0.f;
+                pbn[r] = ldo(P, c + sx);
+                fln[r] = ldo(FL, c + sx);
             }
             vx[r] = ldo(VX, c);
             vy[r] = ldo(VY, c);
             vz[r] = ldo(VZ, c);
             fl[r] = ldo(FL, c);
-            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, 1.0f, dz_kh);
+            if (lane == 0 && k0 > 0) {
+                hl_p[r] = ldo(P, c - 1);
+                hl_vz[r] = ldo(VZ, c - 1);
+                hl_f[r] = ldo(FL, c - 1);
+            }
+            if (lane == kWave - 1 && k + 1 < nz) {
+                hr_p[r] = ldo(P, c + 1);
+                hr_f[r] = ldo(FL, c + 1);
+            }
         }
-        float q_up_halo = 0.f;
-        if (kin && j0 + R < ny)
-            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, 1.0f, dz_k);
-        float wyprev = 0.f;
+        float up_p = 0.f, dn_p = 0.f, dn_vy = 0.f;
+        uint32_t up_f = 0, dn_f = 0;
+        if (kin && j0 + R < ny) {
+            const uint32_t cu = cix(i, j0 + R, k, ny, nz);
+            up_p = ldo(P, cu);
+            up_f = ldo(FL, cu);
+        }
         if (kin && j0 > 0) {
             const uint32_t cl = cix(i, j0 - 1, k, ny, nz);
-            const uint32_t fl_m = ldo(FL, cl);
-            const float w = a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, 1.0f, dz_k) - qc[0];
-            wyprev = (fl_m & FDTD_FLAG_OPEN_Y) ? w : 0.f;
+            dn_p = ldo(P, cl);
+            dn_vy = ldo(VY, cl);
+            dn_f = ldo(FL, cl);
+        }
+        float qhr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            qn[r] = qhr[r] = 0.f;
+            if (!(rvld[r] && kin)) continue;
+            const int j = j0 + r;
+            if (nxt) {
+                pbn[r] = design_pb_v(a, pbn[r], i + 1, j, k, n, pm, dz_k);
+                qn[r] = (fln[r] & FDTD_FLAG_AIR) ? a.cp0 * pbn[r] : 0.f;
+            }
+            qhr[r] = design_q_v(a, hr_f[r], hr_p[r], i, j, k + 1, n, pm, 1.0f, dz_kh);
+        }
+        float q_up_halo = 0.f;
+        if (kin && j0 + R < ny) q_up_halo = design_q_v(a, up_f, up_p, i, j0 + R, k, n, pm, 1.0f, dz_k);
+        float wyprev = 0.f;
+        if (kin && j0 > 0) {
+            const float w = a.dec[1][j0 - 1] * dn_vy + design_q_v(a, dn_f, dn_p, i, j0 - 1, k, n, pm, 1.0f, dz_k) - qc[0];
+            wyprev = (dn_f & FDTD_FLAG_OPEN_Y) ? w : 0.f;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -654,20 +709,19 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
             if (lane == 0) {
                 wzm = 0.f;
                 if (k0 > 0 && kin) {
-                    const uint32_t cl = c - 1;
-                    const uint32_t fl_m = ldo(FL, cl);
-                    const float w = dz_kh * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, 1.0f, dz_kh) - qc[r];
-                    wzm = (fl_m & FDTD_FLAG_OPEN_Z) ? w : 0.f;
+                    const float w = dz_kh * hl_vz[r] + design_q_v(a, hl_f[r], hl_p[r], i, j, k - 1, n, pm, 1.0f, dz_kh) -
+                                    qc[r];
+                    wzm = (hl_f[r] & FDTD_FLAG_OPEN_Z) ? w : 0.f;
                 }
             }
             if (i > 0) acc += cv * wxm[r];
             if (j > 0) acc += cv * wyprev;
             if (k > 0) acc += cv * wzm;
             if (kin) {
-                stnt(Po, c, acc);
-                stnt(VXo, c, wx);
-                stnt(VYo, c, wy);
-                stnt(VZo, c, wz);
+                stout<NTS>(Po, c, acc);
+                stout<NTS>(VXo, c, wx);
+                stout<NTS>(VYo, c, wy);
+                stout<NTS>(VZo, c, wz);
             }
             wxm[r] = wx;
             wyprev = wy;
@@ -685,7 +739,7 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
 // face, and passed on to the cell above it: along i in registers from plane
 // to plane, along j from row to row, along k from lane to lane. The gradient
 // terms are accumulated afterwards by k_design_grad from this step's output.
-template <int R>
+template <int R, bool NTS>
 __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, int64_t n, int seg, int j0, int i0,
                                                          int i1, int lane) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -746,39 +800,71 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
         const Ax di = dax(i, a.lo[0], bx), din = dax(i + 1, a.lo[0], bx);
-        float pbn[R], qn[R], vx[R], vy[R], vz[R], qhr[R], rn[R], rkh[R], kkh[R];
-        uint32_t fl[R];
+        // loads only (the halos' arithmetic follows once they are all issued)
+        float pbn[R], qn[R], vx[R], vy[R], vz[R], rn[R], kn[R], rkh[R], kkh[R], hl_p[R], hl_vz[R], hr_p[R];
+        uint32_t fl[R], fln[R], hl_f[R], hr_f[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            pbn[r] = qn[r] = vx[r] = vy[r] = vz[r] = qhr[r] = 0.f;
-            fl[r] = 0;
+            pbn[r] = vx[r] = vy[r] = vz[r] = hl_p[r] = hl_vz[r] = hr_p[r] = 0.f;
+            fl[r] = fln[r] = hl_f[r] = hr_f[r] = 0;
             const f2v vn = RKV(din, dj[r], dk), vh = RKV(di, dj[r], dkh);
             rn[r] = vn.x;
+            kn[r] = vn.y;
             rkh[r] = vh.x;
             kkh[r] = vh.y;
             if (!(rvld[r] && kin)) continue;
-            const int j = j0 + r;
-            const uint32_t c = cix(i, j, k, ny, nz);
+            const uint32_t c = cix(i, j0 + r, k, ny, nz);
             if (nxt) {
-                pbn[r] = design_pb(a, P, i + 1, j, k, c + sx, n, pm, dz_k);
-                qn[r] = (ldo(FL, c + sx) & FDTD_FLAG_AIR) ? a.cp0 * vn.y * pbn[r] : 0.f;
+                pbn[r] = ldo(P, c + sx);
+                fln[r] = ldo(FL, c + sx);
             }
             vx[r] = ldo(VX, c);
             vy[r] = ldo(VY, c);
             vz[r] = ldo(VZ, c);
             fl[r] = ldo(FL, c);
-            if (lane == kWave - 1 && k + 1 < nz) qhr[r] = design_q(a, P, i, j, k + 1, c + 1, n, pm, kkh[r], dz_kh);
+            if (lane == 0 && k0 > 0) {
+                hl_p[r] = ldo(P, c - 1);
+                hl_vz[r] = ldo(VZ, c - 1);
+                hl_f[r] = ldo(FL, c - 1);
+            }
+            if (lane == kWave - 1 && k + 1 < nz) {
+                hr_p[r] = ldo(P, c + 1);
+                hr_f[r] = ldo(FL, c + 1);
+            }
         }
         const f2v v_up = RKV(di, dju, dk), v_dn = RKV(di, djd, dk);
-        float q_up_halo = 0.f;
-        if (kin && j0 + R < ny)
-            q_up_halo = design_q(a, P, i, j0 + R, k, cix(i, j0 + R, k, ny, nz), n, pm, v_up.y, dz_k);
-        float wyprev = 0.f;
+        float up_p = 0.f, dn_p = 0.f, dn_vy = 0.f;
+        uint32_t up_f = 0, dn_f = 0;
+        if (kin && j0 + R < ny) {
+            const uint32_t cu = cix(i, j0 + R, k, ny, nz);
+            up_p = ldo(P, cu);
+            up_f = ldo(FL, cu);
+        }
         if (kin && j0 > 0) {
             const uint32_t cl = cix(i, j0 - 1, k, ny, nz);
-            const uint32_t fl_m = ldo(FL, cl);
-            const float w = a.dec[1][j0 - 1] * ldo(VY, cl) + design_q(a, P, i, j0 - 1, k, cl, n, pm, v_dn.y, dz_k) - qc[0];
-            wyprev = (fl_m & FDTD_FLAG_OPEN_Y) ? w : 0.f;
+            dn_p = ldo(P, cl);
+            dn_vy = ldo(VY, cl);
+            dn_f = ldo(FL, cl);
+        }
+        float qhr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            qn[r] = qhr[r] = 0.f;
+            if (!(rvld[r] && kin)) continue;
+            const int j = j0 + r;
+            if (nxt) {
+                pbn[r] = design_pb_v(a, pbn[r], i + 1, j, k, n, pm, dz_k);
+                qn[r] = (fln[r] & FDTD_FLAG_AIR) ? a.cp0 * kn[r] * pbn[r] : 0.f;
+            }
+            qhr[r] = design_q_v(a, hr_f[r], hr_p[r], i, j, k + 1, n, pm, kkh[r], dz_kh);
+        }
+        float q_up_halo = 0.f;
+        if (kin && j0 + R < ny) q_up_halo = design_q_v(a, up_f, up_p, i, j0 + R, k, n, pm, v_up.y, dz_k);
+        float wyprev = 0.f;
+        if (kin && j0 > 0) {
+            const float w = a.dec[1][j0 - 1] * dn_vy + design_q_v(a, dn_f, dn_p, i, j0 - 1, k, n, pm, v_dn.y, dz_k) -
+                            qc[0];
+            wyprev = (dn_f & FDTD_FLAG_OPEN_Y) ? w : 0.f;
         }
         // the face below the first row
         float cvyp = cvf(a, v_dn.x, rcur[0]);
@@ -828,20 +914,19 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
                 wzm = 0.f;
                 cvzm = cvf(a, rkh[r], rc);
                 if (k0 > 0 && kin) {
-                    const uint32_t cl = c - 1;
-                    const uint32_t fl_m = ldo(FL, cl);
-                    const float w = dz_kh * ldo(VZ, cl) + design_q(a, P, i, j, k - 1, cl, n, pm, kkh[r], dz_kh) - qc[r];
-                    wzm = (fl_m & FDTD_FLAG_OPEN_Z) ? w : 0.f;
+                    const float w =
+                        dz_kh * hl_vz[r] + design_q_v(a, hl_f[r], hl_p[r], i, j, k - 1, n, pm, kkh[r], dz_kh) - qc[r];
+                    wzm = (hl_f[r] & FDTD_FLAG_OPEN_Z) ? w : 0.f;
                 }
             }
             if (i > 0) acc += cvxm[r] * wxm[r];
             if (j > 0) acc += cvyp * wyprev;
             if (k > 0) acc += cvzm * wzm;
             if (kin) {
-                stnt(Po, c, acc);
-                stnt(VXo, c, wx);
-                stnt(VYo, c, wy);
-                stnt(VZo, c, wz);
+                stout<NTS>(Po, c, acc);
+                stout<NTS>(VXo, c, wx);
+                stout<NTS>(VYo, c, wy);
+                stout<NTS>(VZo, c, wz);
             }
             wxm[r] = wx;
             cvxm[r] = cvx;
@@ -933,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t
     sto(ACC, b, g);
 }
 
-template <int R, int RB>
+template <int R, int RB, bool NTS>
 __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, int64_t n) {
     const int xcd = blockIdx.x & 7, bx = blockIdx.x >> 3;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -941,7 +1026,7 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
     if (bx < a.b_blocks_per_xcd) {
         int seg, j0, i0, i1;
         if (box_strip(a, RB, (xcd * a.b_blocks_per_xcd + bx) * kBlockWaves + wid, seg, j0, i0, i1))
-            design_adjoint_box_strip<RB>(a, n, seg, j0, i0, i1, lane);
+            design_adjoint_box_strip<RB, NTS>(a, n, seg, j0, i0, i1, lane);
         return;
     }
     const int w = (xcd * a.blocks_per_xcd + bx - a.b_blocks_per_xcd) * kBlockWaves + wid;
@@ -949,7 +1034,7 @@ __global__ __launch_bounds__(kBlock) void k_design_adjoint(const DesignArgs a, i
     int seg, sj, si;
     plain_strip(a, w, seg, sj, si);
     const int i0 = si * a.planes;
-    design_adjoint_strip<R>(a, n, seg, sj * R, i0, min(i0 + a.planes, a.nx), lane);
+    design_adjoint_strip<R, NTS>(a, n, seg, sj * R, i0, min(i0 + a.planes, a.nx), lane);
 }
 
 // Strips [c0, c1) of extent `ext` along an axis of n cells whose span with a
@@ -1072,8 +1157,13 @@ int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float*
     a.hist_steps = hist_p ? hist_steps : 0;
     const int blocks = design_grid(a, planes, kDesignRows);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    const bool nts = static_cast<int64_t>(nx) * ny * nz > kCacheResidentCells;
     for (int m = 0; m < n_steps; ++m) {
-        hipLaunchKernelGGL(k_design_forward<kDesignRows>, dim3(blocks), dim3(kBlock), 0, st, a, step0 + m);
+        if (nts)
+            hipLaunchKernelGGL((k_design_forward<kDesignRows, true>), dim3(blocks), dim3(kBlock), 0, st, a, step0 + m);
+        else
+            hipLaunchKernelGGL((k_design_forward<kDesignRows, false>), dim3(blocks), dim3(kBlock), 0, st, a,
+                               step0 + m);
         if (hipGetLastError() != hipSuccess) return -5;
     }
     return 0;
@@ -1100,10 +1190,15 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
     const int blocks = design_grid(a, planes, kDesignAdjBoxRows);
     const int64_t box_cells = static_cast<int64_t>(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]) * (a.hi[2] - a.lo[2]);
     const int grad_blocks = static_cast<int>((box_cells + 255) / 256);
+    const bool nts = static_cast<int64_t>(nx) * ny * nz > kCacheResidentCells;
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (int64_t n = n_hi - 1; n >= n_lo; --n) {
-        hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows>), dim3(blocks), dim3(kBlock), 0, st, a,
-                           n);
+        if (nts)
+            hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows, true>), dim3(blocks), dim3(kBlock), 0,
+                               st, a, n);
+        else
+            hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows, false>), dim3(blocks), dim3(kBlock),
+                               0, st, a, n);
         hipLaunchKernelGGL(k_design_grad, dim3(grad_blocks), dim3(256), 0, st, a, n);
         if (hipGetLastError() != hipSuccess) return -5;
     }

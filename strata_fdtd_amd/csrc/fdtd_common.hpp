@@ -74,6 +74,12 @@ __device__ __forceinline__ void stx_nt(GMEM T* base, int64_t idx, T v) {
 }
 // grids whose cell offsets fit the 32-bit form
 constexpr int64_t kMaxCells32 = int64_t(1) << 30;
+// Working sets whose fields and flags (33 B per cell) fit MI355X's 256 MiB
+// Infinity Cache stay on chip from one step to the next when the sweep writes
+// with ordinary stores (192^3: 154 against 116 Gcell/s with non-temporal
+// stores); larger ones are streamed from HBM and written with non-temporal
+// stores, which keep L2 for halo re-reads (224^3: 136 against 129).
+constexpr int64_t kCacheResidentCells = (int64_t(256) << 20) / 33;
 
 // Everything one fused step launch needs. Passed by value (kernarg segment).
 struct FusedArgs {

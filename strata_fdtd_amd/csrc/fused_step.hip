@@ -53,8 +53,9 @@ __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWav
 // NTS: store the step outputs with non-temporal hints. They are not read again
 // within the step, and on a grid much larger than the caches this keeps L2 for
 // the halo rows and columns the neighbouring strips re-read (about 5% faster at
-// 512^3; about 3% on the batched sweep, whose active solves together far
-// exceed the caches).
+// 512^3; about 3% on a batch of solves that together far exceed the caches).
+// A working set that fits the Infinity Cache (kCacheResidentCells) is written
+// with ordinary stores instead, so the next step finds its inputs on chip.
 //
 // IX: cell offset type, uint32_t for grids below 2^30 cells (scalar base +
 // 32-bit offset addressing), int64_t beyond.
@@ -324,7 +325,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     }
 }
 
-template <int R, bool NU, typename IX>
+template <int R, bool NU, typename IX, bool NTS>
 __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     // XCD-aware remap: hardware block b runs on XCD b % 8; give each XCD a
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     if (*a.step_ptr + a.koff >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, NU, 1, true, IX>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, NU, 1, NTS, IX>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 // Stacked variant: workgroup = W waves over W consecutive j-strips of one
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
 // grid is sized for the table's wave capacity and surplus waves return.
 // Register budget: at most 80 VGPRs (6 waves per SIMD) rather than the 83 the
 // compiler picks unguided; measured about 8% faster on the 64-request sweep.
-template <int R>
+template <int R, bool NTS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_batch_step(
     const FusedArgs* __restrict__ table, const int* __restrict__ offsets, const int* __restrict__ n_ptr) {
     const int lane = threadIdx.x & (kWave - 1);
@@ -392,7 +393,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
     if (*a.step_ptr >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, false, 1, true, uint32_t>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, false, 1, NTS, uint32_t>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 __global__ void k_batch_advance(const FusedArgs* __restrict__ table, const int* __restrict__ n_ptr) {
@@ -640,10 +641,13 @@ __global__ void k_advance(int64_t* step, int n) {
 // ---- host-side launch helpers ------------------------------------------------
 template <int R, bool NU>
 static hipError_t launch_r(const FusedArgs& a, int blocks, hipStream_t st) {
-    if (static_cast<int64_t>(a.nx) * a.ny * a.nz < kMaxCells32)
-        hipLaunchKernelGGL((k_fused_step<R, NU, uint32_t>), dim3(blocks), dim3(kBlock), 0, st, a);
+    const int64_t cells = static_cast<int64_t>(a.nx) * a.ny * a.nz;
+    if (cells <= kCacheResidentCells)
+        hipLaunchKernelGGL((k_fused_step<R, NU, uint32_t, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+    else if (cells < kMaxCells32)
+        hipLaunchKernelGGL((k_fused_step<R, NU, uint32_t, true>), dim3(blocks), dim3(kBlock), 0, st, a);
     else
-        hipLaunchKernelGGL((k_fused_step<R, NU, int64_t>), dim3(blocks), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL((k_fused_step<R, NU, int64_t, true>), dim3(blocks), dim3(kBlock), 0, st, a);
     return hipGetLastError();
 }
 
@@ -735,16 +739,27 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
 
 // One batched step over a table of up to `cap_solves` solves holding at most
 // `cap_waves` waves (the grid covers the capacity; n is read on the device).
-hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
-                        int cap_waves, int rows, hipStream_t st) {
-    int blocks = (cap_waves + kBlockWaves - 1) / kBlockWaves;
-    blocks = std::max(8, (blocks + 7) & ~7);
+template <bool NTS>
+static hipError_t launch_batch_t(const FusedArgs* table, const int* offsets, const int* n_ptr, int blocks, int rows,
+                                 hipStream_t st) {
     switch (rows) {
-        case 2: hipLaunchKernelGGL(k_batch_step<2>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
-        case 4: hipLaunchKernelGGL(k_batch_step<4>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
-        case 8: hipLaunchKernelGGL(k_batch_step<8>, dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
+        case 2: hipLaunchKernelGGL((k_batch_step<2, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
+        case 4: hipLaunchKernelGGL((k_batch_step<4, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
+        case 8: hipLaunchKernelGGL((k_batch_step<8, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
         default: return hipErrorInvalidValue;
     }
+    return hipSuccess;
+}
+
+// nts: non-temporal stores (a batch whose solves together exceed the
+// Infinity Cache), else ordinary stores.
+hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
+                        int cap_waves, int rows, bool nts, hipStream_t st) {
+    int blocks = (cap_waves + kBlockWaves - 1) / kBlockWaves;
+    blocks = std::max(8, (blocks + 7) & ~7);
+    const hipError_t le = nts ? launch_batch_t<true>(table, offsets, n_ptr, blocks, rows, st)
+                              : launch_batch_t<false>(table, offsets, n_ptr, blocks, rows, st);
+    if (le != hipSuccess) return le;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch_advance, dim3((cap_solves + 63) / 64), dim3(64), 0, st, table, n_ptr);

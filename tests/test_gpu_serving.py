@@ -54,3 +54,21 @@ def test_admission_under_tight_memory_budget():
         s.step_n(r.steps)
         for name, tr in s.get_probe_data().items():
             assert np.array_equal(res.traces[name], tr)
+
+
+def test_batch_beyond_the_cache_equals_solo_runs():
+    """Three 160^3 solves (12M cells together, beyond the Infinity-Cache
+    working set: the batch is written with non-temporal stores) admitted at
+    once, FIFO: every result equals the solo run's."""
+    from strata_fdtd_amd.serving import SweepServer, build_solver, sweep_requests
+    reqs = sweep_requests(n=3, rate=1e6, seed=3, steps=(150, 220), dims=(160,))
+    srv = SweepServer(chunk=16, max_cells=None, policy="fifo")
+    stats = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert stats.max_active == 3
+    for res in stats.results:
+        r = next(x for x in reqs if x.rid == res.rid)
+        s = build_solver(r)
+        s.step_n(r.steps)
+        for name, tr in s.get_probe_data().items():
+            assert np.array_equal(res.traces[name], tr)
