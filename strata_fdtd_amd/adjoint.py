@@ -217,10 +217,14 @@ class _Fast:
 
     def adjoint(self, lam, rk, g, n_lo, n_hi, hist, hist0, acc):
         """acc: float32 [box][4], Kahan (sum, compensation) of dL/dr, then dL/dkappa."""
+        import torch
         hp, hd = hist
+        if getattr(self, "_faces", None) is None:
+            n = _native.load().fdtd_design_face_count(self.dv.lo.ctypes.data, self.dv.hi.ctypes.data)
+            self._faces = torch.empty(n, dtype=torch.float32, device=self.dv.dev)
         _native.call("fdtd_design_adjoint", self._sets(lam), *self._common(rk), self.dv.n_probe,
                      _ptr(self.probe_ijk), _ptr(g), n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
-                     hp.numel() // self.ecells, _ptr(acc), self.PLANES, self.dv.stream)
+                     hp.numel() // self.ecells, _ptr(acc), _ptr(self._faces), self.PLANES, self.dv.stream)
 
 
 def history_bytes(pb: DesignProblem, steps: int | None = None) -> int:

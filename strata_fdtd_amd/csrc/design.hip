@@ -62,6 +62,10 @@ struct DesignArgs {
     // gradient accumulators on the box: Kahan (sum, compensation) of dL/dr,
     // then of dL/dkappa, per voxel
     f4v* acc;
+    // gradient factor -cv / (r_lo + r_hi) of every face of the box, formed once
+    // per adjoint call: x faces of cells [lo0, hi0] x [lo1, hi1) x [lo2, hi2)
+    // (the face below each cell), likewise y and z; see face_offsets()
+    float* face_e;
     // tiling; the strips that meet the design box form the sub-grid
     // [b_seg0, b_seg1) x [b_sj0, b_sj1) x [b_si0, b_si1)
     int planes, nseg, nstrip_j, nwaves, blocks_per_xcd;
@@ -539,25 +543,26 @@ __global__ __launch_bounds__(kBlock) void k_design_forward(const DesignArgs a, i
 // dz: the k decay of the cell (dec[2][k]), kept in a register by the caller.
 // The _v forms take the cell's loaded pressure (and flag), so that callers can
 // issue those loads together with the plane's other loads.
+// dx, dy: the cell's i and j decays (dec[0][i], dec[1][j]).
 __device__ __forceinline__ float design_pb_v(const DesignArgs& a, float pv, int i, int j, int k, int64_t n, bool pm,
-                                             float dz) {
+                                             float dx, float dy, float dz) {
     float v = pv;
     if (pm)
         for (int q = 0; q < a.n_probe; ++q)
             if (a.probe[3 * q] == i && a.probe[3 * q + 1] == j && a.probe[3 * q + 2] == k)
                 v += a.g[n * a.n_probe + q];
-    const float lam = (a.dec[0][i] * a.dec[1][j]) * dz;
+    const float lam = (dx * dy) * dz;
     return lam * v;
 }
 
 __device__ __forceinline__ float design_pb(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
                                            uint32_t c, int64_t n, bool pm, float dz) {
-    return design_pb_v(a, ldo(P, c), i, j, k, n, pm, dz);
+    return design_pb_v(a, ldo(P, c), i, j, k, n, pm, a.dec[0][i], a.dec[1][j], dz);
 }
 
 __device__ __forceinline__ float design_q_v(const DesignArgs& a, uint32_t f, float pv, int i, int j, int k, int64_t n,
-                                            bool pm, float kap, float dz) {
-    const float q = a.cp0 * kap * design_pb_v(a, pv, i, j, k, n, pm, dz);
+                                            bool pm, float kap, float dx, float dy, float dz) {
+    const float q = a.cp0 * kap * design_pb_v(a, pv, i, j, k, n, pm, dx, dy, dz);
     return (f & FDTD_FLAG_AIR) ? q : 0.f;
 }
 
@@ -565,7 +570,7 @@ __device__ __forceinline__ float design_q_v(const DesignArgs& a, uint32_t f, flo
 // after the other behind a test of the flag.
 __device__ __forceinline__ float design_q(const DesignArgs& a, const GMEM float* __restrict__ P, int i, int j, int k,
                                           uint32_t c, int64_t n, bool pm, float kap, float dz) {
-    return design_q_v(a, ldo(gm(a.flags), c), ldo(P, c), i, j, k, n, pm, kap, dz);
+    return design_q_v(a, ldo(gm(a.flags), c), ldo(P, c), i, j, k, n, pm, kap, a.dec[0][i], a.dec[1][j], dz);
 }
 
 // Adjoint step n: (P, V) = lambda_{n+1} in set (n+1)&1 -> lambda_n in set n&1.
@@ -612,9 +617,16 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
             wxm[r] = (fl_m & FDTD_FLAG_OPEN_X) ? w : 0.f;
         }
     }
+    // decays of the strip's rows and its two halo rows, and per plane of i and i + 1
+    float dyr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) dyr[r] = rvld[r] ? a.dec[1][j0 + r] : 1.0f;
+    const float dy_up = (j0 + R < ny) ? a.dec[1][j0 + R] : 1.0f;
+    const float dy_dn = (j0 > 0) ? a.dec[1][j0 - 1] : 1.0f;
     for (int i = i0; i < i1; ++i) {
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
+        const float dx_n = nxt ? a.dec[0][i + 1] : 1.0f;
         // loads only (the halos' arithmetic follows once they are all issued)
         float pbn[R], qn[R], vx[R], vy[R], vz[R], hl_p[R], hl_vz[R], hr_p[R];
         uint32_t fl[R], fln[R], hl_f[R], hr_f[R];
@@ -662,16 +674,16 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
             if (!(rvld[r] && kin)) continue;
             const int j = j0 + r;
             if (nxt) {
-                pbn[r] = design_pb_v(a, pbn[r], i + 1, j, k, n, pm, dz_k);
+                pbn[r] = design_pb_v(a, pbn[r], i + 1, j, k, n, pm, dx_n, dyr[r], dz_k);
                 qn[r] = (fln[r] & FDTD_FLAG_AIR) ? a.cp0 * pbn[r] : 0.f;
             }
-            qhr[r] = design_q_v(a, hr_f[r], hr_p[r], i, j, k + 1, n, pm, 1.0f, dz_kh);
+            qhr[r] = design_q_v(a, hr_f[r], hr_p[r], i, j, k + 1, n, pm, 1.0f, dx_i, dyr[r], dz_kh);
         }
         float q_up_halo = 0.f;
-        if (kin && j0 + R < ny) q_up_halo = design_q_v(a, up_f, up_p, i, j0 + R, k, n, pm, 1.0f, dz_k);
+        if (kin && j0 + R < ny) q_up_halo = design_q_v(a, up_f, up_p, i, j0 + R, k, n, pm, 1.0f, dx_i, dy_up, dz_k);
         float wyprev = 0.f;
         if (kin && j0 > 0) {
-            const float w = a.dec[1][j0 - 1] * dn_vy + design_q_v(a, dn_f, dn_p, i, j0 - 1, k, n, pm, 1.0f, dz_k) - qc[0];
+            const float w = dy_dn * dn_vy + design_q_v(a, dn_f, dn_p, i, j0 - 1, k, n, pm, 1.0f, dx_i, dy_dn, dz_k) - qc[0];
             wyprev = (dn_f & FDTD_FLAG_OPEN_Y) ? w : 0.f;
         }
 #pragma unroll
@@ -683,7 +695,7 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
             const bool air = f & FDTD_FLAG_AIR;
             float acc = air ? pbc[r] : 0.f;
             float wx, wy, wz;
-            const float dy_j = a.dec[1][j];
+            const float dy_j = dyr[r];
             if (nxt) {
                 wx = (f & FDTD_FLAG_OPEN_X) ? dx_i * vx[r] + qc[r] - qn[r] : 0.f;
                 acc -= cv * wx;
@@ -709,7 +721,8 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
             if (lane == 0) {
                 wzm = 0.f;
                 if (k0 > 0 && kin) {
-                    const float w = dz_kh * hl_vz[r] + design_q_v(a, hl_f[r], hl_p[r], i, j, k - 1, n, pm, 1.0f, dz_kh) -
+                    const float w = dz_kh * hl_vz[r] +
+                                    design_q_v(a, hl_f[r], hl_p[r], i, j, k - 1, n, pm, 1.0f, dx_i, dyr[r], dz_kh) -
                                     qc[r];
                     wzm = (hl_f[r] & FDTD_FLAG_OPEN_Z) ? w : 0.f;
                 }
@@ -796,9 +809,16 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
             }
         }
     }
+    // decays of the strip's rows and its two halo rows, and per plane of i and i + 1
+    float dyr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) dyr[r] = rvld[r] ? a.dec[1][j0 + r] : 1.0f;
+    const float dy_up = (j0 + R < ny) ? a.dec[1][j0 + R] : 1.0f;
+    const float dy_dn = (j0 > 0) ? a.dec[1][j0 - 1] : 1.0f;
     for (int i = i0; i < i1; ++i) {
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
+        const float dx_n = nxt ? a.dec[0][i + 1] : 1.0f;
         const Ax di = dax(i, a.lo[0], bx), din = dax(i + 1, a.lo[0], bx);
         // loads only (the halos' arithmetic follows once they are all issued)
         float pbn[R], qn[R], vx[R], vy[R], vz[R], rn[R], kn[R], rkh[R], kkh[R], hl_p[R], hl_vz[R], hr_p[R];
@@ -853,16 +873,16 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
             if (!(rvld[r] && kin)) continue;
             const int j = j0 + r;
             if (nxt) {
-                pbn[r] = design_pb_v(a, pbn[r], i + 1, j, k, n, pm, dz_k);
+                pbn[r] = design_pb_v(a, pbn[r], i + 1, j, k, n, pm, dx_n, dyr[r], dz_k);
                 qn[r] = (fln[r] & FDTD_FLAG_AIR) ? a.cp0 * kn[r] * pbn[r] : 0.f;
             }
-            qhr[r] = design_q_v(a, hr_f[r], hr_p[r], i, j, k + 1, n, pm, kkh[r], dz_kh);
+            qhr[r] = design_q_v(a, hr_f[r], hr_p[r], i, j, k + 1, n, pm, kkh[r], dx_i, dyr[r], dz_kh);
         }
         float q_up_halo = 0.f;
-        if (kin && j0 + R < ny) q_up_halo = design_q_v(a, up_f, up_p, i, j0 + R, k, n, pm, v_up.y, dz_k);
+        if (kin && j0 + R < ny) q_up_halo = design_q_v(a, up_f, up_p, i, j0 + R, k, n, pm, v_up.y, dx_i, dy_up, dz_k);
         float wyprev = 0.f;
         if (kin && j0 > 0) {
-            const float w = a.dec[1][j0 - 1] * dn_vy + design_q_v(a, dn_f, dn_p, i, j0 - 1, k, n, pm, v_dn.y, dz_k) -
+            const float w = dy_dn * dn_vy + design_q_v(a, dn_f, dn_p, i, j0 - 1, k, n, pm, v_dn.y, dx_i, dy_dn, dz_k) -
                             qc[0];
             wyprev = (dn_f & FDTD_FLAG_OPEN_Y) ? w : 0.f;
         }
@@ -882,7 +902,7 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
             // own faces (c = lo)
             float wx, wy, wz;
             float cvx = 0.f, cvy = 0.f, cvz = 0.f;
-            const float dy_j = a.dec[1][j];
+            const float dy_j = dyr[r];
             if (nxt) {
                 cvx = cvf(a, rc, rn[r]);
                 wx = (f & FDTD_FLAG_OPEN_X) ? dx_i * vx[r] + qc[r] - qn[r] : 0.f;
@@ -915,7 +935,8 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
                 cvzm = cvf(a, rkh[r], rc);
                 if (k0 > 0 && kin) {
                     const float w =
-                        dz_kh * hl_vz[r] + design_q_v(a, hl_f[r], hl_p[r], i, j, k - 1, n, pm, kkh[r], dz_kh) - qc[r];
+                        dz_kh * hl_vz[r] + design_q_v(a, hl_f[r], hl_p[r], i, j, k - 1, n, pm, kkh[r], dx_i, dyr[r], dz_kh) -
+                        qc[r];
                     wzm = (hl_f[r] & FDTD_FLAG_OPEN_Z) ? w : 0.f;
                 }
             }
@@ -950,6 +971,36 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
 // in the per-cell kernel's order (own x, y, z faces, then the faces below),
 // with Kahan compensation. Every box voxel has all six faces (the box keeps
 // one cell to every grid face).
+// Layout of face_e: x faces (bx+1) by bz, then y faces bx (by+1) bz, then z
+// faces bx by (bz+1); the face of axis d below box cell (bi, bj, bk) (between
+// it and its -d neighbour) has that cell's index in the axis' array.
+__host__ __device__ inline int64_t face_count(int bx, int by, int bz, int d) {
+    return static_cast<int64_t>(bx + (d == 0)) * (by + (d == 1)) * (bz + (d == 2));
+}
+
+// The gradient factor of every face of the box: the same expression of the
+// same two r values as in the per-cell adjoint kernel.
+__global__ __launch_bounds__(256) void k_design_faces(const DesignArgs a) {
+    const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
+    int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    int d = 0;
+    while (d < 3 && t >= face_count(bx, by, bz, d)) t -= face_count(bx, by, bz, d++);
+    if (d == 3) return;
+    const int ey = by + (d == 1), ez = bz + (d == 2);
+    const int fk = static_cast<int>(t % ez), fj = static_cast<int>((t / ez) % ey), fi = static_cast<int>(t / (static_cast<int64_t>(ey) * ez));
+    const BoxTab RK = box_tab(a.rk);
+    auto RV = [&](int ii, int jj, int kk) {
+        const bool in = ii >= 0 && ii < bx && jj >= 0 && jj < by && kk >= 0 && kk < bz;
+        return box_val(RK, in ? (ii * by + jj) * bz + kk : 0, in).x;
+    };
+    const float rh = RV(fi, fj, fk);
+    const float rl = RV(fi - (d == 0), fj - (d == 1), fk - (d == 2));
+    const float cv = cvf(a, rl, rh);
+    int64_t off = 0;
+    for (int q = 0; q < d; ++q) off += face_count(bx, by, bz, q);
+    a.face_e[off + t] = -cv / (rl + rh);
+}
+
 __global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t n) {
     const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
     const uint32_t b = blockIdx.x * 256u + threadIdx.x;
@@ -968,18 +1019,19 @@ __global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t
     const GMEM float* __restrict__ hp = gm(a.hist_p + (n - a.hist0) * ebox_cells(a));
     const GMEM float* __restrict__ hd = gm(a.hist_div + (n - a.hist0) * dbox_cells(a));
     GMEM f4v* __restrict__ ACC = gm(a.acc);
-    const BoxTab RK = box_tab(a.rk);
     const int ey = by + 2, ez = bz + 2;
     const uint32_t e = static_cast<uint32_t>(((bi + 1) * ey + (bj + 1)) * ez + (bk + 1));
-    // r of the voxel and its six neighbours (1 outside the box)
-    auto RV = [&](int di, int dj, int dk) {
-        const int ii = bi + di, jj = bj + dj, kk = bk + dk;
-        const bool in = ii >= 0 && ii < bx && jj >= 0 && jj < by && kk >= 0 && kk < bz;
-        return box_val(RK, in ? (ii * by + jj) * bz + kk : 0, in).x;
-    };
-    const float rc = box_val(RK, static_cast<int>(b), true).x;
-    const float rxp = RV(1, 0, 0), ryp = RV(0, 1, 0), rzp = RV(0, 0, 1);
-    const float rxm = RV(-1, 0, 0), rym = RV(0, -1, 0), rzm = RV(0, 0, -1);
+    // gradient factors of the six faces (below: the face's own cell; above:
+    // the +d neighbour's index in the axis' array)
+    const GMEM float* __restrict__ EX = gm(a.face_e);
+    const GMEM float* __restrict__ EY = EX + face_count(bx, by, bz, 0);
+    const GMEM float* __restrict__ EZ = EY + face_count(bx, by, bz, 1);
+    const uint32_t fx = static_cast<uint32_t>((bi * by + bj) * bz + bk);
+    const uint32_t fy = static_cast<uint32_t>((bi * (by + 1) + bj) * bz + bk);
+    const uint32_t fz = static_cast<uint32_t>((bi * by + bj) * (bz + 1) + bk);
+    const float exm = ldo(EX, fx), exp_ = ldo(EX, fx + static_cast<uint32_t>(by * bz));
+    const float eym = ldo(EY, fy), eyp = ldo(EY, fy + static_cast<uint32_t>(bz));
+    const float ezm = ldo(EZ, fz), ezp = ldo(EZ, fz + 1u);
     const float hpc = ldo(hp, e);
     const float hxp = ldo(hp, e + static_cast<uint32_t>(ey * ez)), hyp = ldo(hp, e + static_cast<uint32_t>(ez)),
                 hzp = ldo(hp, e + 1u);
@@ -988,18 +1040,12 @@ __global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t
     const float wx = ldo(WX, c), wy = ldo(WY, c), wz = ldo(WZ, c);
     const float wxm = ldo(WX, c - sx), wym = ldo(WY, c - static_cast<uint32_t>(nz)), wzm = ldo(WZ, c - 1u);
     float gr = 0.f;
-    float cv = cvf(a, rc, rxp);
-    gr += wx * (hxp - hpc) * (-cv / (rc + rxp));
-    cv = cvf(a, rc, ryp);
-    gr += wy * (hyp - hpc) * (-cv / (rc + ryp));
-    cv = cvf(a, rc, rzp);
-    gr += wz * (hzp - hpc) * (-cv / (rc + rzp));
-    cv = cvf(a, rxm, rc);
-    gr += wxm * (hpc - hxm) * (-cv / (rxm + rc));
-    cv = cvf(a, rym, rc);
-    gr += wym * (hpc - hym) * (-cv / (rym + rc));
-    cv = cvf(a, rzm, rc);
-    gr += wzm * (hpc - hzm) * (-cv / (rzm + rc));
+    gr += wx * (hxp - hpc) * exp_;
+    gr += wy * (hyp - hpc) * eyp;
+    gr += wz * (hzp - hpc) * ezp;
+    gr += wxm * (hpc - hxm) * exm;
+    gr += wym * (hpc - hym) * eym;
+    gr += wzm * (hpc - hzm) * ezm;
     f4v g = ldo(ACC, b);
     float y = gr - g.y;
     float s = g.x + y;
@@ -1130,6 +1176,11 @@ int fill_design(DesignArgs& a, float* const* fields, const uint8_t* flags, const
 
 extern "C" {
 
+int64_t fdtd_design_face_count(const int32_t* box_lo, const int32_t* box_hi) {
+    const int bx = box_hi[0] - box_lo[0], by = box_hi[1] - box_lo[1], bz = box_hi[2] - box_lo[2];
+    return face_count(bx, by, bz, 0) + face_count(bx, by, bz, 1) + face_count(bx, by, bz, 2);
+}
+
 int64_t fdtd_design_history_bytes(const int32_t* box_lo, const int32_t* box_hi, int64_t steps) {
     int64_t e = 1, b = 1;
     for (int d = 0; d < 3; ++d) {
@@ -1173,13 +1224,13 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
                         float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
                         int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo, int64_t n_hi,
                         const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps, float* acc,
-                        int planes, void* stream) {
+                        float* face_e, int planes, void* stream) {
     DesignArgs a;
     if (int rc = fill_design(a, lam, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, rk, nullptr, nullptr, n_probe,
                              probe_ijk))
         return rc;
     if (n_hi < n_lo || n_lo < 0 || planes < 1 || (n_probe > 0 && !g)) return -2;
-    if (!hist_p || !hist_div || !acc) return -2;
+    if (!hist_p || !hist_div || !acc || !face_e) return -2;
     if (n_lo < hist0 || n_hi > hist0 + hist_steps) return -2;  // every step needs its history
     a.g = g;
     a.hist_p = const_cast<float*>(hist_p);
@@ -1187,11 +1238,18 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
     a.hist0 = hist0;
     a.hist_steps = hist_steps;
     a.acc = reinterpret_cast<f4v*>(acc);
+    a.face_e = face_e;
     const int blocks = design_grid(a, planes, kDesignAdjBoxRows);
     const int64_t box_cells = static_cast<int64_t>(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]) * (a.hi[2] - a.lo[2]);
     const int grad_blocks = static_cast<int>((box_cells + 255) / 256);
     const bool nts = static_cast<int64_t>(nx) * ny * nz > kCacheResidentCells;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    {
+        const int fbx = a.hi[0] - a.lo[0], fby = a.hi[1] - a.lo[1], fbz = a.hi[2] - a.lo[2];
+        const int64_t faces = face_count(fbx, fby, fbz, 0) + face_count(fbx, fby, fbz, 1) + face_count(fbx, fby, fbz, 2);
+        hipLaunchKernelGGL(k_design_faces, dim3(static_cast<unsigned>((faces + 255) / 256)), dim3(256), 0, st, a);
+        if (hipGetLastError() != hipSuccess) return -5;
+    }
     for (int64_t n = n_hi - 1; n >= n_lo; --n) {
         if (nts)
             hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows, true>), dim3(blocks), dim3(kBlock), 0,
