@@ -200,6 +200,9 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
             vxm[r] = (ldo(FL, c) & FDTD_FLAG_OPEN_X) ? v : 0.f;
         }
     }
+    float dyr[R];  // the rows' j decays, read once per strip
+#pragma unroll
+    for (int r = 0; r < R; ++r) dyr[r] = rvld[r] ? a.dec[1][j0 + r] : 1.0f;
     for (int i = i0; i < i1; ++i) {
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
@@ -268,7 +271,7 @@ __device__ __forceinline__ void design_forward_strip(const DesignArgs& a, int64_
             }
             const float div = ((vxn - vxm[r]) + (vyn - vyprev)) + (vzn - vzm);
             float pnew = (f & FDTD_FLAG_AIR) ? pr + a.cp0 * div : 0.f;
-            const float dy_j = a.dec[1][j];
+            const float dy_j = dyr[r];
             pnew = ((pnew * dx_i) * dy_j) * dz_k;
             if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
                 pnew = static_cast<float>(static_cast<double>(pnew) + wsrc);
@@ -354,6 +357,9 @@ __device__ __forceinline__ void design_forward_box_strip(const DesignArgs& a, in
             }
         }
     }
+    float dyr[R];  // the rows' j decays, read once per strip
+#pragma unroll
+    for (int r = 0; r < R; ++r) dyr[r] = rvld[r] ? a.dec[1][j0 + r] : 1.0f;
     for (int i = i0; i < i1; ++i) {
         const bool nxt = i + 1 < nx;
         const float dx_i = a.dec[0][i];
@@ -429,7 +435,7 @@ __device__ __forceinline__ void design_forward_box_strip(const DesignArgs& a, in
             }
             const float div = ((vxn - vxm[r]) + (vyn - vyprev)) + (vzn - vzm);
             float pnew = (f & FDTD_FLAG_AIR) ? pr + (a.cp0 * kcur[r]) * div : 0.f;
-            const float dy_j = a.dec[1][j];
+            const float dy_j = dyr[r];
             pnew = ((pnew * dx_i) * dy_j) * dz_k;
             if (i == a.src_i && j == a.src_j && k == a.src_k && (f & FDTD_FLAG_AIR))
                 pnew = static_cast<float>(static_cast<double>(pnew) + wsrc);

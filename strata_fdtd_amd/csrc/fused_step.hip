@@ -134,6 +134,15 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     bool rv[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) rv[r] = (j0 + r) < ny;
+    // the rows' j tables, read once per strip rather than once per plane
+    float dyr[R], cvyr[R], icyr[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int jr = j0 + r;
+        dyr[r] = rv[r] ? dY[jr] : 1.0f;
+        cvyr[r] = (NU && jr + 1 < ny) ? cv * ifc1[jr] : cv;
+        icyr[r] = (NU && rv[r]) ? icl1[jr] : 1.0f;
+    }
 
     // ---- prologue: p(i0) and the x-face below the strip, vx'(i0-1) -------------
 #pragma unroll
@@ -241,8 +250,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             // y face (j | j+1)
             float vyn = vyo[r];
             if (jr + 1 < ny) {
-                const float cvy = NU ? cv * ifc1[jr] : cv;
-                const float v = vyo[r] + cvy * (p_up - pr);
+                const float v = vyo[r] + cvyr[r] * (p_up - pr);
                 vyn = (f & FDTD_FLAG_OPEN_Y) ? v : 0.f;
             }
             // z face (k | k+1): p(k+1) from the next lane, or the right halo
@@ -270,8 +278,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             const float dvz = vzn - vzm;
             float pnew;
             if (NU) {
-                const float icy = icl1[jr];
-                const float ddx = icx_i * dvx, ddy = icy * dvy, ddz = icz_k * dvz;
+                const float ddx = icx_i * dvx, ddy = icyr[r] * dvy, ddz = icz_k * dvz;
                 pnew = pr + cp * ((ddx + ddy) + ddz);
             } else {
                 pnew = pr + cp * ((dvx + dvy) + dvz);
@@ -279,7 +286,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             if (!(f & FDTD_FLAG_AIR)) pnew = 0.f;
 
             // PML (interior decay is exactly 1.0f, so the multiply is exact there)
-            const float dy_j = dY[jr];
+            const float dy_j = dyr[r];
             float ps = ((pnew * dx_i) * dy_j) * dz_k;
             if (src_mask && sources_ok && (f & FDTD_FLAG_AIR)) {
                 for (int q = 0; q < a.n_src; ++q) {
