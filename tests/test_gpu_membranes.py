@@ -92,3 +92,37 @@ def test_membrane_weights_follow_geometry_changes():
     s.set_geometry(air)
     s.reset()
     _oracle_check(s, 40, 0)
+
+
+def test_membrane_tables_are_validated_at_the_c_abi():
+    """fdtd_ctx_set_membranes refuses tables that would index outside the grid,
+    the entries, the sources or the probes (no kernel runs with them)."""
+    import ctypes as C
+    from strata_fdtd_amd import _native
+    s = make_solver(_scene())
+    s._prepare(4)
+    ctx = s._ctx
+    n = int(np.prod(s.shape))
+
+    def call(cells, estart, src, field, weight, pstart, pidx, n_src=1):
+        cells = np.asarray(cells, np.int64)
+        estart = np.asarray(estart, np.int32)
+        src = np.asarray(src, np.int32)
+        field = np.asarray(field, np.int32)
+        weight = np.asarray(weight, np.float64)
+        pstart = np.asarray(pstart, np.int32)
+        pidx = np.asarray(pidx or [0], np.int32)
+        wave = s._trace  # any device buffer stands in for the waveform table
+        return _native.load().fdtd_ctx_set_membranes(
+            C.c_void_p(ctx), len(cells), cells.ctypes.data, estart.ctypes.data, len(src), src.ctypes.data,
+            field.ctypes.data, weight.ctypes.data, pstart.ctypes.data, int(pstart[-1]), pidx.ctypes.data, n_src,
+            C.c_void_p(wave.data_ptr()), 4)
+
+    assert call([5], [0, 1], [0], [0], [0.5], [0, 0], []) == 0
+    assert call([n], [0, 1], [0], [0], [0.5], [0, 0], []) < 0          # cell outside the grid
+    assert call([7, 5], [0, 1, 2], [0, 0], [0, 0], [1, 1], [0, 0, 0], []) < 0   # cells not increasing
+    assert call([5], [0, 2], [0], [0], [0.5], [0, 0], []) < 0          # entry range beyond the entries
+    assert call([5], [0, 1], [1], [0], [0.5], [0, 0], []) < 0          # source row beyond n_src
+    assert call([5], [0, 1], [0], [4], [0.5], [0, 0], []) < 0          # field out of range
+    assert call([5], [0, 1], [0], [0], [0.5], [0, 1], [99]) < 0        # probe not set
+    assert call([], [0], [], [], [], [0], []) == 0                     # removes the tables
