@@ -54,6 +54,22 @@ def build_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+# solves up to this many cells (33 B each) fit the 256 MiB Infinity Cache and
+# are swept with ordinary stores; larger ones with non-temporal stores
+CACHE_RESIDENT_CELLS = (256 << 20) // BYTES_PER_CELL
+
+
+def kernel_name(kname: str, rows: int, stack: int, uniform: bool, cells: int) -> str:
+    """Template instance of the sweep that runs for this solve, as rocprofv3
+    names it."""
+    nu = "false" if uniform else "true"
+    if stack != 1:
+        return f"{kname}<{rows},{nu},{stack}>"
+    ix = "uint32_t" if cells < (1 << 30) else "long"
+    nts = "false" if cells <= CACHE_RESIDENT_CELLS else "true"
+    return f"{kname}<{rows},{nu},{ix},{nts}>"
+
+
 def traffic_from_profiles(workload: str):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
@@ -333,8 +349,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": traffic_from_profiles(wl.name),
-        "kernel": (f"{kname}<{rows},{'false' if solver.grid.is_uniform else 'true'},"
-                   f"{'uint32_t' if stack == 1 else stack}>"),
+        "kernel": kernel_name(kname, rows, stack, solver.grid.is_uniform, cells),
         "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells,
         "bytes_per_cell_update": BYTES_PER_CELL,
         "avg_launch_us": round(ms_sweep * 1e3, 1),
