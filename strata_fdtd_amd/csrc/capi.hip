@@ -352,7 +352,8 @@ int fdtd_ctx_set_membranes(fdtd_ctx* c, int n_cells, const int64_t* cell, const 
             return fail(-2, "membrane CSR tables do not span their entries");
         for (int e = 0; e < n_cells; ++e) {
             if (cell[e] < 0 || cell[e] >= cells || cell[e] > INT32_MAX)
-                return fail(-2, "membrane cell %d: flat index %lld outside the grid", e, (long long)cell[e]);
+                return fail(-2, "source table cell %d: flat index %lld outside the grid or above 2^31-1", e,
+                            (long long)cell[e]);
             if (e > 0 && cell[e] <= cell[e - 1]) return fail(-2, "membrane cells must be increasing");
             if (entry_start[e + 1] <= entry_start[e] || probe_start[e + 1] < probe_start[e])
                 return fail(-2, "membrane cell %d: bad entry or probe range", e);

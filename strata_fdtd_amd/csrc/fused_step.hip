@@ -40,6 +40,31 @@ namespace fdtd_amd {
 __device__ __forceinline__ float shfl_down1(float v) { return __shfl_down(v, 1, kWave); }
 __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWave); }
 
+// Source q's add at cell (i, j, k) of step s: numpy adds the float64 sample
+// to the float32 element in float64 and stores float32.
+__device__ __forceinline__ float add_source(const FusedArgs& a, const GMEM double* __restrict__ WAVE, int q, int i,
+                                            int j, int k, int64_t s, float ps) {
+    bool on;
+    if (a.src_kind[q] == 0) {
+        on = a.src_i[q] == i && a.src_j[q] == j && a.src_k[q] == k;
+    } else {
+        const int ax = a.src_i[q], id = a.src_j[q];
+        on = ax == 0 ? id == i : (ax == 1 ? id == j : id == k);
+    }
+    if (on) ps = static_cast<float>(static_cast<double>(ps) + WAVE[q * a.wave_len + s]);
+    return ps;
+}
+
+// Probe q records the cell's pressure if it sits at (i, j, k).
+__device__ __forceinline__ void record_probe(const FusedArgs& a, const GMEM int32_t* __restrict__ PIJK,
+                                             GMEM float* __restrict__ TRACE, int q, int i, int j, int k, int64_t s,
+                                             float ps) {
+    if (PIJK[3 * q] == i && PIJK[3 * q + 1] == j && PIJK[3 * q + 2] == k) {
+        TRACE[s * a.n_probe + q] = ps;
+        if (!isfinite(ps)) atomicOr(a.health, 1u);
+    }
+}
+
 // One wave's work: the strip (segment seg, j-strip sj, i-chunk si) of the
 // solve described by `a` (kernel argument, or a row of a device batch table).
 //
@@ -58,8 +83,10 @@ __device__ __forceinline__ float shfl_up1(float v) { return __shfl_up(v, 1, kWav
 // with ordinary stores instead, so the next step finds its inputs on chip.
 //
 // IX: cell offset type, uint32_t for grids below 2^30 cells (scalar base +
-// 32-bit offset addressing), int64_t beyond.
-template <int R, bool NU, int W, bool NTS, typename IX>
+// 32-bit offset addressing), int64_t beyond. LP: the solo sweeps' probe and
+// source lookups (see below); the batch kernel, at its register cap, keeps the
+// plain loops.
+template <int R, bool NU, int W, bool NTS, typename IX, bool LP>
 __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj, int si, int lane,
                                             float* lds, int wi) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -112,11 +139,24 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
         }
         if (hit) src_mask |= 1u << q;
     }
+    // Probes in the strip. LP: lane q tests probe q (at most 64), so the
+    // coordinates arrive in one round of vector loads rather than a chain of
+    // scalar loads every wave waits on. The batch kernel, at its register
+    // cap, keeps the scalar loop (its solves have few probes).
     uint64_t probe_mask = 0;
-    for (int q = 0; q < a.n_probe; ++q) {
-        const int pi = PIJK[3 * q], pj = PIJK[3 * q + 1], pk = PIJK[3 * q + 2];
-        if (pi >= i0 && pi < i1 && pj >= j0 && pj < j0 + R && pk >= k0 && pk < k0 + kWave)
-            probe_mask |= 1ull << q;
+    if constexpr (LP) {
+        bool probe_hit = false;
+        if (lane < a.n_probe) {
+            const int pi = PIJK[3 * lane], pj = PIJK[3 * lane + 1], pk = PIJK[3 * lane + 2];
+            probe_hit = pi >= i0 && pi < i1 && pj >= j0 && pj < j0 + R && pk >= k0 && pk < k0 + kWave;
+        }
+        probe_mask = __builtin_amdgcn_ballot_w64(probe_hit);
+    } else {
+        for (int q = 0; q < a.n_probe; ++q) {
+            const int pi = PIJK[3 * q], pj = PIJK[3 * q + 1], pk = PIJK[3 * q + 2];
+            if (pi >= i0 && pi < i1 && pj >= j0 && pj < j0 + R && pk >= k0 && pk < k0 + kWave)
+                probe_mask |= 1ull << q;
+        }
     }
     const bool sources_ok = s < a.wave_len;
     const bool trace_ok = s < a.trace_len;
@@ -289,16 +329,13 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
             const float dy_j = dyr[r];
             float ps = ((pnew * dx_i) * dy_j) * dz_k;
             if (src_mask && sources_ok && (f & FDTD_FLAG_AIR)) {
-                for (int q = 0; q < a.n_src; ++q) {
-                    if (!((src_mask >> q) & 1u)) continue;
-                    bool on;
-                    if (a.src_kind[q] == 0) {
-                        on = a.src_i[q] == i && a.src_j[q] == jr && a.src_k[q] == k;
-                    } else {
-                        const int ax = a.src_i[q], id = a.src_j[q];
-                        on = ax == 0 ? id == i : (ax == 1 ? id == jr : id == k);
-                    }
-                    if (on) ps = static_cast<float>(static_cast<double>(ps) + WAVE[q * a.wave_len + s]);
+                // in list order: the set bits (LP), or every source with a test
+                if constexpr (LP) {
+                    for (uint32_t sm = src_mask; sm; sm &= sm - 1)
+                        ps = add_source(a, WAVE, __builtin_ctz(sm), i, jr, k, s, ps);
+                } else {
+                    for (int q = 0; q < a.n_src; ++q)
+                        if ((src_mask >> q) & 1u) ps = add_source(a, WAVE, q, i, jr, k, s, ps);
                 }
             }
             if (kv) {
@@ -315,13 +352,12 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
                 }
             }
             if (probe_mask && trace_ok) {
-                for (int q = 0; q < a.n_probe; ++q) {
-                    if (!((probe_mask >> q) & 1ull)) continue;
-                    if (PIJK[3 * q] == i && PIJK[3 * q + 1] == jr &&
-                        PIJK[3 * q + 2] == k) {
-                        TRACE[s * a.n_probe + q] = ps;
-                        if (!isfinite(ps)) atomicOr(a.health, 1u);
-                    }
+                if constexpr (LP) {
+                    for (uint64_t pm = probe_mask; pm; pm &= pm - 1)
+                        record_probe(a, PIJK, TRACE, __builtin_ctzll(pm), i, jr, k, s, ps);
+                } else {
+                    for (int q = 0; q < a.n_probe; ++q)
+                        if ((probe_mask >> q) & 1ull) record_probe(a, PIJK, TRACE, q, i, jr, k, s, ps);
                 }
             }
             vxm[r] = vxn;
@@ -345,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
     if (*a.step_ptr + a.koff >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, NU, 1, NTS, IX>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, NU, 1, NTS, IX, true>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 // Stacked variant: workgroup = W waves over W consecutive j-strips of one
@@ -364,7 +400,7 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
     const int nstack_j = (a.nstrip_j + W - 1) / W;
     const int seg = lb % a.nseg;
     const int t = lb / a.nseg;
-    fused_strip<R, NU, W, true, uint32_t>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
+    fused_strip<R, NU, W, true, uint32_t, true>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
 }
 
 // Batched step: one launch advances every solve in `table` by one step. Waves
@@ -400,7 +436,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
     if (*a.step_ptr >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, false, 1, NTS, uint32_t>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, false, 1, NTS, uint32_t, false>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 __global__ void k_batch_advance(const FusedArgs* __restrict__ table, const int* __restrict__ n_ptr) {

@@ -59,6 +59,18 @@ def _fast_waveform(source, times: NDArray[np.float64], dt: float) -> NDArray[np.
     return waveform_table(source, times, dt)
 
 
+def _table_source(source, cells: int) -> bool:
+    """Whether a source is added from per-cell tables after the sweep rather
+    than inside it: membranes, and plane sources normal to z. A z plane meets
+    every strip of one 64-wide k segment, where the sweep's per-cell source
+    test costs more than a pass over the plane's cells. The tables hold 32-bit
+    cell indices, so on grids of 2^31 cells or more z planes stay in the
+    sweep."""
+    if source.source_type == "membrane":
+        return True
+    return source.source_type == "plane" and int(source.position["axis"]) == 2 and cells < 2**31
+
+
 class FDTDSolver:
     """3D acoustic FDTD solver (staggered Yee grid) on one MI355X GPU.
 
@@ -288,7 +300,7 @@ class FDTDSolver:
                 raise ValueError(f"Geometry shape {mask.shape} doesn't match solver shape {self.shape}")
             self.geometry = mask.astype(bool)
         self._dirty_geometry = True
-        self._dirty_sources = True   # membrane cells are the air cells of their patch
+        self._dirty_sources = True   # table sources cover the air cells of their patch, plane or point
 
     def _voxelize(self, geometry):
         """(air mask bool, material ids uint8 or None) of an SDF tree or a
@@ -518,17 +530,25 @@ class FDTDSolver:
         cap = self._cap
         if total_steps > cap:
             cap = max(total_steps, 2 * cap, 1024)
-        srcs = [s for s in self._sources if s.source_type != "membrane"]
-        membranes = [s for s in self._sources if s.source_type == "membrane"]
+        # The sweep adds the sources listed before the first table source; that
+        # one and every later source are added after the sweep from per-cell
+        # tables, each cell's entries in list order. Every cell thus receives
+        # its sources in the order of the list (core/solver.py:2386-2433).
+        n_cells = int(np.prod(self.shape))
+        first = next((q for q, s in enumerate(self._sources) if _table_source(s, n_cells)), len(self._sources))
+        srcs = self._sources[:first]
+        table = self._sources[first:]
         times = accumulated_times(self.dt, cap) if self._sources else None
         if srcs:
             wave = np.stack([_fast_waveform(s, times, self.dt) for s in srcs]).astype(np.float64)
             self._wave = torch.from_numpy(np.ascontiguousarray(wave)).to(self._device)
         else:
             self._wave = None
-        if membranes:
-            # each membrane samples its own waveform object
-            wave = np.stack([_fast_waveform(m.waveform, times, self.dt) for m in membranes]).astype(np.float64)
+        if table:
+            # a membrane samples its own waveform object; point and plane
+            # sources are their own waveform
+            wave = np.stack([_fast_waveform(m.waveform if m.source_type == "membrane" else m, times, self.dt)
+                             for m in table]).astype(np.float64)
             self._mwave = torch.from_numpy(np.ascontiguousarray(wave)).to(self._device)
         else:
             self._mwave = None
@@ -582,41 +602,62 @@ class FDTDSolver:
                         dtype=np.int64)
         _native.call("fdtd_ctx_set_probes", self._ctx, len(names), flat.ctypes.data,
                      _native.ptr(self._trace), cap)
-        self._set_membranes(membranes, names, cap)
+        self._set_table_sources(table, names, cap)
         self._cap = cap
         self._dirty_sources = False
 
-    def _set_membranes(self, membranes: list, probe_names: list, cap: int) -> None:
-        """Per-cell tables of the membrane sources: the cells (air, weight > 0)
-        of every membrane, each cell once, with its entries (membrane, field,
-        weight) in source order and the probes recorded on it after them."""
-        if not membranes:
+    def _table_cells(self, src, air: NDArray[np.bool_]) -> tuple[NDArray, NDArray, int]:
+        """(flat cells, float64 weights, field) of one table source: the air
+        cells of a membrane's patch with their mode weights, of a plane, or
+        the point's cell, the last two with weight 1.0 (the product w * 1.0 is
+        w exactly, so the add equals the sweep's)."""
+        if src.source_type == "membrane":
+            if src._cached_weights is None:
+                src._cached_weights = src.get_injection_weights(self._grid)
+                src._cached_mask = src._cached_weights > 0
+            idx = np.flatnonzero(src._cached_mask.ravel() & air.ravel())
+            return idx, src._cached_weights.ravel()[idx], src.injection_field()
+        if src.source_type == "plane":
+            ax, index = int(src.position["axis"]), int(src.position["index"])
+            if not (0 <= ax <= 2 and 0 <= index < self.shape[ax]):
+                raise ValueError(f"plane source: axis {ax} index {index} outside the grid {self.shape}")
+            plane = np.zeros(self.shape, dtype=bool)
+            sl = [slice(None)] * 3
+            sl[ax] = index
+            plane[tuple(sl)] = True
+            idx = np.flatnonzero(plane.ravel() & air.ravel())
+        else:
+            flat = int(np.ravel_multi_index(tuple(int(v) for v in src.position), self.shape))
+            idx = np.array([flat] if air.ravel()[flat] else [], dtype=np.int64)
+        return idx, np.ones(idx.size, dtype=np.float64), 0
+
+    def _set_table_sources(self, table: list, probe_names: list, cap: int) -> None:
+        """Per-cell tables of the sources added after the sweep (membranes,
+        z-normal planes, and every source listed after the first of them):
+        their air cells, each cell once, with its entries (source, field,
+        weight) in list order and the probes recorded on it after them."""
+        air = np.asarray(self.geometry, dtype=bool)
+        cells, order, rows, fields, weights = [], [], [], [], []
+        for q, src in enumerate(table):
+            idx, w, field = self._table_cells(src, air)
+            cells.append(idx.astype(np.int64))
+            order.append(np.full(idx.size, q, dtype=np.int64))
+            rows.append(np.full(idx.size, q, dtype=np.int32))
+            fields.append(np.full(idx.size, field, dtype=np.int32))
+            weights.append(w)
+        cells = np.concatenate(cells) if cells else np.zeros(0, np.int64)
+        if cells.size == 0:
             _native.call("fdtd_ctx_set_membranes", self._ctx, 0, None, None, 0, None, None, None, None, 0, None,
                          0, None, 0)
             return
-        air = np.asarray(self.geometry, dtype=bool).ravel()
-        cells, order, rows, fields, weights = [], [], [], [], []
-        for q, m in enumerate(membranes):
-            if m._cached_weights is None:
-                m._cached_weights = m.get_injection_weights(self._grid)
-                m._cached_mask = m._cached_weights > 0
-            w = m._cached_weights.ravel()
-            idx = np.flatnonzero(m._cached_mask.ravel() & air)
-            cells.append(idx)
-            order.append(np.full(idx.size, q, dtype=np.int64))
-            rows.append(np.full(idx.size, q, dtype=np.int32))
-            fields.append(np.full(idx.size, m.injection_field(), dtype=np.int32))
-            weights.append(w[idx])
-        cells = np.concatenate(cells)
         order = np.concatenate(order)
-        perm = np.lexsort((order, cells))          # by cell, then source order
+        perm = np.lexsort((order, cells))          # by cell, then list order
         cells = cells[perm]
         rows = np.concatenate(rows)[perm]
         fields = np.concatenate(fields)[perm]
         weights = np.ascontiguousarray(np.concatenate(weights)[perm], dtype=np.float64)
         ucells, first = np.unique(cells, return_index=True)
         estart = np.append(first, cells.size).astype(np.int32)
-        self._check_membrane_order(membranes)
         # probes re-recorded after the injection on cells whose pressure changes
         pflat = [(np.ravel_multi_index(self._probes[n].position, self.shape), q) for q, n in enumerate(probe_names)]
         pstart, pidx = [0], []
@@ -631,31 +672,8 @@ class FDTDSolver:
         pidx = np.array(pidx or [0], dtype=np.int32)
         _native.call("fdtd_ctx_set_membranes", self._ctx, int(ucells.size), ucells.ctypes.data, estart.ctypes.data,
                      int(cells.size), rows.ctypes.data, fields.ctypes.data, weights.ctypes.data,
-                     pstart.ctypes.data, int(pstart[-1]), pidx.ctypes.data, len(membranes),
+                     pstart.ctypes.data, int(pstart[-1]), pidx.ctypes.data, len(table),
                      _native.ptr(self._mwave), cap)
-
-    def _check_membrane_order(self, membranes) -> None:
-        """The step adds point and plane sources before membranes. That is the
-        reference's order unless a point or plane source listed after a
-        pressure membrane hits one of its cells; refuse that case."""
-        air = np.asarray(self.geometry, dtype=bool)
-        for k, m in enumerate(self._sources):
-            if m.source_type != "membrane" or m.injection_type != "pressure":
-                continue
-            cells = m._cached_mask & air
-            for s in self._sources[k + 1:]:
-                if s.source_type == "point":
-                    hit = bool(cells[tuple(int(v) for v in s.position)])
-                elif s.source_type == "plane":
-                    sl = [slice(None)] * 3
-                    sl[int(s.position["axis"])] = int(s.position["index"])
-                    hit = bool(np.any(cells[tuple(sl)]))
-                else:
-                    continue
-                if hit:
-                    raise NotImplementedError(
-                        "a point or plane source listed after a membrane source on the same cell is not "
-                        "supported (the step adds point and plane sources first); add it before the membrane")
 
     def _prepare(self, n_steps: int) -> None:
         if self._dirty_geometry:

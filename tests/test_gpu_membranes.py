@@ -1,7 +1,9 @@
-"""Membrane sources on the device vs the oracle (bit for bit): circular and
-rectangular patches, pressure and normal-velocity injection, overlapping
-membranes, solid cells and an ADE material under a patch, probes on membrane
-cells, graph-captured chunks; and the one source order the step refuses."""
+"""Membrane sources and per-cell source tables on the device vs the oracle
+(bit for bit): circular and rectangular patches, pressure and normal-velocity
+injection, overlapping membranes, solid cells and an ADE material under a
+patch, probes on membrane cells, graph-captured chunks; and every source order
+(point, x/y/z plane and membrane sources interleaved on shared cells), which
+the sweep and the tables together apply in list order."""
 
 import numpy as np
 import pytest
@@ -73,14 +75,43 @@ def test_membranes_equal_oracle(uniform, chunk):
     _oracle_check(s, 70, chunk)
 
 
-def test_membrane_then_point_source_on_its_cell_is_refused():
+def test_membrane_then_point_source_on_its_cell_equals_oracle():
+    """A point source listed after a pressure membrane on one of its cells is
+    added after the membrane's weight on that cell, as in the reference."""
     from strata_fdtd_amd import GaussianPulse
     s = make_solver(_scene())
     m = _membranes(s)[0]
     s.add_source(m)
     s.add_source(GaussianPulse(position=(15, 17, 13), frequency=9000.0))
-    with pytest.raises(NotImplementedError):
-        s.step()
+    _oracle_check(s, 50, 0)
+
+
+@pytest.mark.parametrize("uniform,chunk", [(True, 0), (True, 12), (False, 0)])
+def test_interleaved_source_kinds_equal_oracle(uniform, chunk):
+    """Point and x/y plane sources before the first z plane are added in the
+    sweep, everything from the z plane on from the per-cell tables. The list
+    puts sources of both groups on shared cells (a point on the z plane, a
+    y plane crossing it and a membrane, probes on those cells), so any
+    reordering changes float32 roundings the oracle sees."""
+    from strata_fdtd_amd import GaussianPulse
+    sc = _scene(uniform)
+    sc.sources = []
+    s = make_solver(sc)
+    mem = _membranes(s)
+    plane = lambda ax, idx, f: GaussianPulse(position={"axis": ax, "index": idx}, frequency=f, source_type="plane")
+    for src in (GaussianPulse(position=(15, 17, 20), frequency=11000.0),   # sweep
+                plane(0, 7, 8000.0),                                      # sweep
+                plane(2, 20, 7000.0),                                     # table from here on
+                GaussianPulse(position=(15, 17, 20), frequency=5000.0),   # same cell as the first point
+                mem[0],
+                plane(1, 17, 13000.0),                                    # crosses the z plane and the membrane
+                GaussianPulse(position=(15, 17, 13), frequency=9000.0),   # on the membrane
+                plane(2, 13, 6000.0)):                                    # the membrane's plane
+        s.add_source(src)
+    for q, p in enumerate(((15, 17, 20), (15, 17, 13), (7, 5, 20), (3, 17, 2))):
+        s.add_probe(f"extra{q}", p)
+    print(f"[sources] interleaved uniform={uniform} chunk={chunk}", flush=True)
+    _oracle_check(s, 60, chunk)
 
 
 def test_membrane_weights_follow_geometry_changes():
