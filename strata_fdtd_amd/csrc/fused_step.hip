@@ -368,8 +368,12 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     }
 }
 
+// The nonuniform 4-row sweep with 32-bit offsets is held to 72 VGPRs (7 waves
+// per SIMD instead of 6, no spills): 1.1% faster on a 320x320x480 grid.
 template <int R, bool NU, typename IX, bool NTS>
-__global__ __launch_bounds__(kBlock) void k_fused_step(const FusedArgs a) {
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu((NU && R == 4 && sizeof(IX) == 4) ? 7 : 1, 8))) void k_fused_step(
+    const FusedArgs a) {
     const int lane = threadIdx.x & (kWave - 1);
     // XCD-aware remap: hardware block b runs on XCD b % 8; give each XCD a
     // contiguous range of logical blocks so strips that share halo rows share
