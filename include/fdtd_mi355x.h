@@ -158,6 +158,15 @@ int fdtd_time_steps(fdtd_ctx* ctx, int n_steps, int use_graph, float* ms_total);
  * window (milliseconds). Timing instrumentation; no reference counterpart. */
 int fdtd_time_sweeps(fdtd_ctx* ctx, int n_steps, float* ms_sweep, float* ms_total);
 
+/* Practical bandwidth ceiling of the fused sweep's access mix: a streaming
+ * kernel that reads p, vx, vy, vz and the flag byte of every cell from the
+ * current buffer set and writes the four fields to the other set (the next
+ * step's output, so the solve's state is unchanged), with the sweep's store
+ * kind. One warm-up pass, then `reps` (1..1000) timed passes; returns the
+ * average milliseconds per pass. Field buffers must be 16-byte aligned.
+ * Measurement instrumentation; no reference counterpart. */
+int fdtd_stream_ceiling(fdtd_ctx* ctx, int reps, float* ms_per_pass);
+
 /* ---- batched serving (no counterpart in the reference) ------------------- */
 /* A batch advances many independent solves (uniform grids, no ADE) by one step
  * per launch: waves are dealt out over the solves' strips, each solve keeps its

@@ -18,6 +18,7 @@ int fused_grid(FusedArgs& a, int rows, int planes, int stack);
 hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool ade, int blocks,
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
+hipError_t launch_stream_ceiling(const FusedArgs& a, hipStream_t st);
 hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
                         int cap_waves, int rows, bool nts, hipStream_t st);
 hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
@@ -530,6 +531,28 @@ int fdtd_time_steps(fdtd_ctx* c, int n_steps, int use_graph, float* ms_total) {
     if (int r2 = check(hipEventRecord(c->ev1, c->stream), "event record")) return r2;
     if (int r3 = check(hipEventSynchronize(c->ev1), "event sync")) return r3;
     return check(hipEventElapsedTime(ms_total, c->ev0, c->ev1), "elapsed time");
+}
+
+int fdtd_stream_ceiling(fdtd_ctx* c, int reps, float* ms_per_pass) {
+    if (int rc = ready(c)) return rc;
+    if (!ms_per_pass) return fail(-2, "null output");
+    if (reps < 1 || reps > 1000) return fail(-2, "reps must be in [1, 1000]");
+    for (int set = 0; set < 2; ++set)
+        for (int g = 0; g < 4; ++g)
+            if (reinterpret_cast<uintptr_t>(c->a.f[set][g]) % 16)
+                return fail(-2, "field buffers must be 16-byte aligned for the streaming kernel");
+    if (reinterpret_cast<uintptr_t>(c->a.flags) % 4) return fail(-2, "flag buffer must be 4-byte aligned");
+    DeviceGuard g(c->device);
+    if (int rc = check(launch_stream_ceiling(c->a, c->stream), "stream launch")) return rc;  // warm-up
+    if (int rc = check(hipEventRecord(c->ev0, c->stream), "event record")) return rc;
+    for (int r = 0; r < reps; ++r)
+        if (int rc = check(launch_stream_ceiling(c->a, c->stream), "stream launch")) return rc;
+    if (int rc = check(hipEventRecord(c->ev1, c->stream), "event record")) return rc;
+    if (int rc = check(hipEventSynchronize(c->ev1), "event sync")) return rc;
+    float ms = 0.f;
+    if (int rc = check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "elapsed time")) return rc;
+    *ms_per_pass = ms / reps;
+    return 0;
 }
 
 int fdtd_time_sweeps(fdtd_ctx* c, int n_steps, float* ms_sweep, float* ms_total) {

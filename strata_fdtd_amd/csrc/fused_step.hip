@@ -813,6 +813,51 @@ hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n
     return hipGetLastError();
 }
 
+// ---- streaming ceiling ---------------------------------------------------------
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// The fused sweep's bytes with none of its stencil: per cell p, vx, vy, vz and
+// the flag byte read once from the current set, the four fields written once
+// to the other set, four cells per thread in 16-byte accesses. Its rate is the
+// practical bandwidth ceiling of the sweep's access mix on the device. The
+// written set is the next step's output, so the solve's state is unchanged.
+template <bool NT>
+__global__ __launch_bounds__(256) void k_stream_ceiling(const FusedArgs a, int64_t n4, int tail) {
+    const int par = static_cast<int>(*a.step_ptr & 1);
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (t < n4) {
+        const uint32_t f4 = reinterpret_cast<const uint32_t*>(a.flags)[t];
+        const float b0 = static_cast<float>(f4 & 0xffu), b1 = static_cast<float>((f4 >> 8) & 0xffu);
+        const float b2 = static_cast<float>((f4 >> 16) & 0xffu), b3 = static_cast<float>(f4 >> 24);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const f4v v = reinterpret_cast<const f4v*>(a.f[par][g])[t];
+            const f4v o = {v.x + b0, v.y + b1, v.z + b2, v.w + b3};
+            f4v* dst = reinterpret_cast<f4v*>(a.f[par ^ 1][g]) + t;
+            if (NT) __builtin_nontemporal_store(o, dst);
+            else *dst = o;
+        }
+    } else if (t - n4 < tail) {
+        const int64_t c = 4 * n4 + (t - n4);
+        const float b = static_cast<float>(a.flags[c]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) a.f[par ^ 1][g][c] = a.f[par][g][c] + b;
+    }
+}
+
+// Enqueue one pass of the streaming kernel, with the sweep's store kind.
+hipError_t launch_stream_ceiling(const FusedArgs& a, hipStream_t st) {
+    const int64_t cells = static_cast<int64_t>(a.nx) * a.ny * a.nz;
+    const int64_t n4 = cells / 4;
+    const int tail = static_cast<int>(cells % 4);
+    const int64_t blocks = (n4 + tail + 255) / 256;
+    if (cells > kCacheResidentCells)
+        hipLaunchKernelGGL(k_stream_ceiling<true>, dim3(blocks), dim3(256), 0, st, a, n4, tail);
+    else
+        hipLaunchKernelGGL(k_stream_ceiling<false>, dim3(blocks), dim3(256), 0, st, a, n4, tail);
+    return hipGetLastError();
+}
+
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step, n);
     return hipGetLastError();

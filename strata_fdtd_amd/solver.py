@@ -718,6 +718,16 @@ class FDTDSolver:
         self._advance_host(n)
         return float(ms_sweep.value), float(ms_total.value)
 
+    def stream_ceiling(self, reps: int = 20) -> float:
+        """Milliseconds per pass of a streaming kernel with the sweep's bytes
+        per cell and no stencil, on this solve's buffers (state unchanged):
+        the practical bandwidth ceiling the sweep is measured against."""
+        import ctypes as C
+        self._prepare(1)
+        ms = C.c_float()
+        _native.call("fdtd_stream_ceiling", self._ctx, reps, C.byref(ms))
+        return float(ms.value)
+
     def step_n(self, n: int, graph_chunk: int = 64) -> None:
         """Advance n steps on the device without host round trips."""
         if n <= 0:

@@ -193,3 +193,27 @@ def test_reset_restarts(oracle_lib):
     assert s.step_count == 0 and s.time == 0.0
     s.step_n(10)
     assert np.array_equal(s.get_probe_data()["probe3"], first)
+
+
+def test_stream_ceiling_pass_leaves_the_solve_unchanged(oracle_lib):
+    """The bandwidth-ceiling pass writes p, vx, vy, vz + flag byte of every
+    cell into the idle buffer set (an odd cell count exercises its 4-cell
+    tail) and leaves the solve's state, and so every later step, as it was."""
+    torch = _torch()
+    sc = seeded_scenario(3, shape=(13, 11, 9), pml=3)
+    a, b = make_solver(sc), make_solver(sc)
+    a.step_n(7)
+    b.step_n(7)
+    ms = a.stream_ceiling(3)
+    torch.cuda.synchronize()
+    assert ms > 0
+    cur = a._cur
+    for g in range(4):
+        assert torch.equal(a._f[cur ^ 1][g], a._f[cur][g] + a._flags.float())
+        assert torch.equal(a._f[cur][g], b._f[cur][g])
+    a.step_n(20)
+    b.step_n(20)
+    for name, tr in b.get_probe_data().items():
+        assert np.array_equal(a.get_probe_data()[name], tr), name
+    for k in ("p", "vx", "vy", "vz"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
