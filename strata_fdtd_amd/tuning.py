@@ -4,8 +4,8 @@ A launch is described by (rows, planes, stack): a wave owns a 64-wide k
 segment of ``rows`` consecutive j rows and marches ``planes`` planes along i;
 ``stack`` waves of one workgroup own consecutive j-strips and share their halo
 rows through LDS (1: independent waves). The
-heuristic below gives each of the 256 CUs several resident waves while keeping
-the per-strip prologue (one extra plane) and the j-halo rows small. The
+heuristic below picks from measurements: short 2-row strips (many waves) for
+solves that fit the Infinity Cache, 4 rows x 4 planes beyond it. The
 autotuner (``strata_fdtd_amd.autotune``) measures candidates on the device and
 caches the winner per grid shape; ``pick_tiling`` consults that cache first.
 """
@@ -15,7 +15,9 @@ from __future__ import annotations
 import json
 import os
 
-NUM_CU = 256
+# solves up to this many cells (33 B each: four float32 fields and a flag byte)
+# fit the 256 MiB Infinity Cache
+CACHE_RESIDENT_CELLS = (256 << 20) // 33
 ROW_CHOICES = (2, 4, 8, 16)
 STACK_CHOICES = (1, 2, 4, 8)
 STACK_DEFAULT = 1
@@ -50,17 +52,25 @@ def normalize_tiling(t) -> tuple[int, int, int]:
 
 
 def heuristic_tiling(shape) -> tuple[int, int, int]:
-    """rows=4; the most planes per strip (<= 4) that still leaves >= 8 waves
-    per CU. Measured at 512^3 (plain and enclosure): planes 4 is 1% faster
-    than 8 and 3% faster than 16; rows 2 and LDS stacks are slower."""
+    """Measured on MI355X (air boxes, device time per step, best of rows
+    2/4 x planes 1/2/4/8):
+
+    * a solve whose fields and flags (33 B/cell) fit the 256 MiB Infinity
+      Cache runs fastest with 2-row strips 2 planes long, which give the chip
+      several times more waves: 192^3 34.8 us per step (4 rows, 4 planes:
+      43.1), 176^3 29.4 (34.2), 128^3 15.0 (20.2), 100^3 9.5 (11.4), 80^3 7.4
+      (9.9); grids too small for 2048 such waves take 1 plane (64^3: 5.4 us
+      against 9.0);
+    * beyond the cache, 4 rows and 4 planes: at 512^3 planes 4 is 1% faster
+      than 8 and 3% faster than 16, and rows 2 and LDS stacks are slower;
+      224^3 and 256^3 are within 1% of their best there."""
     nx, ny, nz = shape
-    rows = 4
     segs = (nz + 63) // 64
-    strips_ji = segs * ((ny + rows - 1) // rows)
-    planes = 4
-    while planes > 2 and strips_ji * ((nx + planes - 1) // planes) < 8 * NUM_CU:
-        planes //= 2
-    return rows, planes, STACK_DEFAULT
+    if nx * ny * nz <= CACHE_RESIDENT_CELLS:
+        rows = 2
+        planes = 2 if segs * ((ny + 1) // 2) * ((nx + 1) // 2) >= 2048 else 1
+        return rows, planes, STACK_DEFAULT
+    return 4, 4, STACK_DEFAULT
 
 
 def pick_tiling(shape, nonuniform: bool = False, ade: bool = False) -> tuple[int, int, int]:
