@@ -166,9 +166,10 @@ class _Device:
 
 class _Fast:
     """Float32 march kernels of the design solve (csrc/design.hip): forward
-    sweep with optional box history, adjoint sweep reading that history."""
+    sweep with optional box history, adjoint sweep reading that history.
+    PLANES: planes per strip (2 with the kernels' 2-row strips)."""
 
-    PLANES = 4
+    PLANES = 2
 
     def __init__(self, dv: _Device):
         import torch

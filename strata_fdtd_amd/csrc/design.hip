@@ -32,8 +32,10 @@ typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 // strips: rows per strip of the plain launches and of the forward box launch,
 // rows per strip of the adjoint box launch (its register budget), planes per
-// strip of the box launches at most
-constexpr int kDesignRows = 4;
+// strip of the box launches at most. Two-row strips give a cache-resident
+// design grid many short strips: 192^3 with a 128^3 box, 2000 steps, forward
+// 0.117 -> 0.107 s and gradient 0.320 -> 0.307 s against 4-row strips.
+constexpr int kDesignRows = 2;
 constexpr int kDesignAdjBoxRows = 2;
 constexpr int kDesignBoxPlanes = 4;
 
