@@ -30,12 +30,14 @@ constexpr int kDesignMaxProbes = 8;
 // and stored whole
 typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
-// strips: rows per strip of the plain launches and of the forward box launch,
-// rows per strip of the adjoint box launch (its register budget), planes per
-// strip of the box launches at most. Two-row strips give a cache-resident
-// design grid many short strips: 192^3 with a 128^3 box, 2000 steps, forward
-// 0.117 -> 0.107 s and gradient 0.320 -> 0.307 s against 4-row strips.
+// strips: rows per strip of the forward launch (plain and box strips), rows
+// per strip of the adjoint's plain strips and of its box strips (the box
+// sweep's register budget), planes per strip of the box launches at most.
+// Measured at 192^3 with a 128^3 box: the forward is fastest in 2-row strips
+// (50 us per step against 60 with 4 rows), the adjoint in 4-row strips with
+// 2-row box strips (70.8 us against 77.5 with 2-row strips).
 constexpr int kDesignRows = 2;
+constexpr int kDesignAdjRows = 4;
 constexpr int kDesignAdjBoxRows = 2;
 constexpr int kDesignBoxPlanes = 4;
 
@@ -72,6 +74,7 @@ struct DesignArgs {
     // [b_seg0, b_seg1) x [b_sj0, b_sj1) x [b_si0, b_si1)
     int planes, nseg, nstrip_j, nwaves, blocks_per_xcd;
     int b_seg0, b_seg1, b_sj0, b_sj1, b_si0, b_si1, b_blocks_per_xcd;
+    int rows;  // rows per plain strip; the box strips split them
     // the box launch: planes [b_i0, b_i1) in nb_i strips of box_planes planes
     int box_planes, b_i0, b_i1, nb_i;
 };
@@ -508,16 +511,16 @@ __device__ __forceinline__ void plain_strip(const DesignArgs& a, int w, int& seg
     seg = rem % nseg;
 }
 
-// The box launch covers the rows [b_sj0, b_sj1) x kDesignRows and planes
+// The box launch covers the rows [b_sj0, b_sj1) x a.rows and planes
 // [b_i0, b_i1) of the box sub-grid in strips of `rows` rows (nb_j of them) and
 // box_planes planes (nb_i of them).
 __device__ __forceinline__ bool box_strip(const DesignArgs& a, int rows, int w, int& seg, int& j0, int& i0,
                                           int& i1) {
-    const int nseg = a.b_seg1 - a.b_seg0, nbj = (a.b_sj1 - a.b_sj0) * (kDesignRows / rows);
+    const int nseg = a.b_seg1 - a.b_seg0, nbj = (a.b_sj1 - a.b_sj0) * (a.rows / rows);
     if (w >= nseg * nbj * a.nb_i) return false;
     const int t = w / nseg;
     seg = a.b_seg0 + w % nseg;
-    j0 = a.b_sj0 * kDesignRows + (t % nbj) * rows;
+    j0 = a.b_sj0 * a.rows + (t % nbj) * rows;
     if (j0 >= a.ny) return false;  // a split of the last, partial row strip
     i0 = a.b_i0 + (t / nbj) * a.box_planes;
     i1 = min(i0 + a.box_planes, a.b_i1);
@@ -1109,13 +1112,14 @@ static void meeting_strips(int n, int ext, int lo, int hi, int count, int& c0, i
 // Hardware block b runs on XCD b % 8. Per XCD the launch has
 // b_blocks_per_xcd blocks of box strips and blocks_per_xcd of the others;
 // returns the number of blocks.
-static int design_grid(DesignArgs& a, int planes, int box_rows) {
+static int design_grid(DesignArgs& a, int planes, int rows, int box_rows) {
+    a.rows = rows;
     a.planes = planes;
     a.nseg = (a.nz + kWave - 1) / kWave;
-    a.nstrip_j = (a.ny + kDesignRows - 1) / kDesignRows;
+    a.nstrip_j = (a.ny + rows - 1) / rows;
     const int nstrip_i = (a.nx + planes - 1) / planes;
     meeting_strips(a.nz, kWave, a.lo[2], a.hi[2], a.nseg, a.b_seg0, a.b_seg1);
-    meeting_strips(a.ny, kDesignRows, a.lo[1], a.hi[1], a.nstrip_j, a.b_sj0, a.b_sj1);
+    meeting_strips(a.ny, rows, a.lo[1], a.hi[1], a.nstrip_j, a.b_sj0, a.b_sj1);
     meeting_strips(a.nx, planes, a.lo[0], a.hi[0], nstrip_i, a.b_si0, a.b_si1);
     const int bseg = a.b_seg1 - a.b_seg0, bsj = a.b_sj1 - a.b_sj0, bsi = a.b_si1 - a.b_si0;
     a.nwaves = a.nseg * a.nstrip_j * nstrip_i - bseg * bsj * bsi;
@@ -1124,7 +1128,7 @@ static int design_grid(DesignArgs& a, int planes, int box_rows) {
     a.b_i0 = a.b_si0 * planes;
     a.b_i1 = std::min(a.b_si1 * planes, a.nx);
     a.nb_i = bsi ? (a.b_i1 - a.b_i0 + a.box_planes - 1) / a.box_planes : 0;
-    const int bw = bseg * bsj * (kDesignRows / box_rows) * a.nb_i;
+    const int bw = bseg * bsj * (rows / box_rows) * a.nb_i;
     a.b_blocks_per_xcd = ((bw + kBlockWaves - 1) / kBlockWaves + 7) / 8;
     return 8 * (a.blocks_per_xcd + a.b_blocks_per_xcd);
 }
@@ -1214,7 +1218,7 @@ int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float*
     a.hist_div = hist_div;
     a.hist0 = hist0;
     a.hist_steps = hist_p ? hist_steps : 0;
-    const int blocks = design_grid(a, planes, kDesignRows);
+    const int blocks = design_grid(a, planes, kDesignRows, kDesignRows);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const bool nts = static_cast<int64_t>(nx) * ny * nz > kCacheResidentCells;
     for (int m = 0; m < n_steps; ++m) {
@@ -1247,7 +1251,7 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
     a.hist_steps = hist_steps;
     a.acc = reinterpret_cast<f4v*>(acc);
     a.face_e = face_e;
-    const int blocks = design_grid(a, planes, kDesignAdjBoxRows);
+    const int blocks = design_grid(a, planes, kDesignAdjRows, kDesignAdjBoxRows);
     const int64_t box_cells = static_cast<int64_t>(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]) * (a.hi[2] - a.lo[2]);
     const int grad_blocks = static_cast<int>((box_cells + 255) / 256);
     const bool nts = static_cast<int64_t>(nx) * ny * nz > kCacheResidentCells;
@@ -1260,11 +1264,11 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
     }
     for (int64_t n = n_hi - 1; n >= n_lo; --n) {
         if (nts)
-            hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows, true>), dim3(blocks), dim3(kBlock), 0,
-                               st, a, n);
-        else
-            hipLaunchKernelGGL((k_design_adjoint<kDesignRows, kDesignAdjBoxRows, false>), dim3(blocks), dim3(kBlock),
+            hipLaunchKernelGGL((k_design_adjoint<kDesignAdjRows, kDesignAdjBoxRows, true>), dim3(blocks), dim3(kBlock),
                                0, st, a, n);
+        else
+            hipLaunchKernelGGL((k_design_adjoint<kDesignAdjRows, kDesignAdjBoxRows, false>), dim3(blocks),
+                               dim3(kBlock), 0, st, a, n);
         hipLaunchKernelGGL(k_design_grad, dim3(grad_blocks), dim3(256), 0, st, a, n);
         if (hipGetLastError() != hipSuccess) return -5;
     }
