@@ -96,7 +96,7 @@ struct fdtd_ctx {
     struct DevBuf {
         void* p = nullptr;
         size_t bytes = 0;
-    } m_cell, m_estart, m_esrc, m_efield, m_eweight, m_pstart, m_pidx;
+    } m_cell, m_estart, m_esrc, m_efield, m_eweight, m_pstart, m_pidx, ade_cells;
     int64_t* step_counter = nullptr;  // same as a.step_ptr, writable
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -139,7 +139,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
     if (c->mic_w_dev) (void)hipFree(c->mic_w_dev);
     if (c->mic_f64_dev) (void)hipFree(c->mic_f64_dev);
-    for (fdtd_ctx::DevBuf* b : {&c->m_cell, &c->m_estart, &c->m_esrc, &c->m_efield, &c->m_eweight, &c->m_pstart,
+    for (fdtd_ctx::DevBuf* b : {&c->ade_cells, &c->m_cell, &c->m_estart, &c->m_esrc, &c->m_efield, &c->m_eweight, &c->m_pstart,
                                 &c->m_pidx})
         if (b->p) (void)hipFree(b->p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -369,6 +369,8 @@ int fdtd_ctx_set_membranes(fdtd_ctx* c, int n_cells, const int64_t* cell, const 
     }
     DeviceGuard g(c->device);
     drop_graph(c);
+    // steps already on the stream may still read the old tables
+    if (int rc = check(hipStreamSynchronize(c->stream), "stream sync")) return rc;
     c->a.n_mcell = 0;
     if (n_cells == 0) return 0;
     std::vector<int32_t> cell32(cell, cell + n_cells);
@@ -436,6 +438,41 @@ int fdtd_ctx_set_ade(fdtd_ctx* c, int n_pole, const int32_t* mat_id, const int32
     c->a.J = J;
     c->a.Jprev = Jprev;
     c->a.ade_inv_dx = inv_dx;
+    // The box cells whose material has poles, as box indices in box order: the
+    // ADE kernels then run one thread per such cell rather than per box cell
+    // (a sphere fills about half its box). Read from the device material ids
+    // one box plane at a time, after the work already on the stream.
+    c->a.ade_cells = nullptr;
+    c->a.n_ade_cells = 0;
+    const int bx = hi[0] - lo[0], by = hi[1] - lo[1], bz = hi[2] - lo[2];
+    const int64_t box_n = static_cast<int64_t>(bx) * by * bz;
+    if (box_n < INT32_MAX) {
+        DeviceGuard g(c->device);
+        // the material ids may come from any stream; steps in flight may read the old list
+        if (int rc = check(hipDeviceSynchronize(), "device sync")) return rc;
+        bool has_pole[256] = {};
+        for (int q = 0; q < n_pole; ++q) has_pole[mat_id[q]] = true;
+        const size_t plane = static_cast<size_t>(by) * c->a.nz;
+        std::vector<uint8_t> mat(plane);
+        std::vector<int32_t> list;
+        for (int i = lo[0]; i < hi[0]; ++i) {
+            const uint8_t* src = material + (static_cast<int64_t>(i) * c->a.ny + lo[1]) * c->a.nz;
+            if (int rc = check(hipMemcpy(mat.data(), src, plane, hipMemcpyDeviceToHost), "hipMemcpy")) return rc;
+            for (int j = 0; j < by; ++j)
+                for (int k = lo[2]; k < hi[2]; ++k) {
+                    const uint8_t m = mat[static_cast<size_t>(j) * c->a.nz + k];
+                    if (m && has_pole[m])
+                        list.push_back(static_cast<int32_t>(((static_cast<int64_t>(i - lo[0]) * by) + j) * bz +
+                                                            (k - lo[2])));
+                }
+        }
+        const int32_t none = 0;
+        if (int rc = upload(c->ade_cells, list.empty() ? &none : list.data(),
+                            sizeof(int32_t) * std::max<size_t>(list.size(), 1)))
+            return rc;
+        c->a.ade_cells = static_cast<const int32_t*>(c->ade_cells.p);
+        c->a.n_ade_cells = static_cast<int64_t>(list.size());
+    }
     return 0;
 }
 

@@ -154,6 +154,8 @@ struct FusedArgs {
     float* J;                       // [n_pole][box cells]
     float* Jprev;                   // [n_pole][box cells] (Lorentz)
     float ade_inv_dx;               // uniform grids: 1/dx of the divergence source
+    const int32_t* ade_cells;       // device: box indices of the cells whose material has poles
+    int64_t n_ade_cells;            //   (null: the ADE kernels visit every box cell)
 };
 
 }  // namespace fdtd_amd

@@ -469,8 +469,10 @@ __global__ void k_ade_density(const FusedArgs a) {
     const int64_t s = *a.step_ptr + a.koff;
     if (s >= a.step_limit) return;
     const float* __restrict__ P = (s & 1) ? a.f[1][0] : a.f[0][0];
-    for (int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; b < box_n;
-         b += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t n_visit = a.ade_cells ? a.n_ade_cells : box_n;
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n_visit;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t b = a.ade_cells ? static_cast<int64_t>(a.ade_cells[e]) : e;
         const int kk = static_cast<int>(b % bz) + a.box_lo[2];
         const int jj = static_cast<int>((b / bz) % by) + a.box_lo[1];
         const int ii = static_cast<int>(b / (static_cast<int64_t>(by) * bz)) + a.box_lo[0];
@@ -541,8 +543,10 @@ __global__ void k_ade_fixup(const FusedArgs a) {
     float* __restrict__ VYo = par ? a.f[0][2] : a.f[1][2];
     float* __restrict__ VZo = par ? a.f[0][3] : a.f[1][3];
     const int64_t sy = nz, sx = static_cast<int64_t>(ny) * nz;
-    for (int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; b < box_n;
-         b += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t n_visit = a.ade_cells ? a.n_ade_cells : box_n;
+    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n_visit;
+         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const int64_t b = a.ade_cells ? static_cast<int64_t>(a.ade_cells[e]) : e;
         const int k = static_cast<int>(b % bz) + a.box_lo[2];
         const int j = static_cast<int>((b / bz) % by) + a.box_lo[1];
         const int i = static_cast<int>(b / (static_cast<int64_t>(by) * bz)) + a.box_lo[0];
@@ -750,8 +754,9 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after) {
     int box_blocks = 1;
     if (ade) {
-        const int64_t bn = static_cast<int64_t>(a.box_hi[0] - a.box_lo[0]) *
-                           (a.box_hi[1] - a.box_lo[1]) * (a.box_hi[2] - a.box_lo[2]);
+        const int64_t bn = a.ade_cells ? a.n_ade_cells
+                                       : static_cast<int64_t>(a.box_hi[0] - a.box_lo[0]) *
+                                             (a.box_hi[1] - a.box_lo[1]) * (a.box_hi[2] - a.box_lo[2]);
         box_blocks = static_cast<int>(std::min<int64_t>(std::max<int64_t>((bn + 255) / 256, 1), 1 << 16));
         hipLaunchKernelGGL(k_ade_density, dim3(box_blocks), dim3(256), 0, st, a);
         hipError_t e = hipGetLastError();
