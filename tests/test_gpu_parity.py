@@ -217,3 +217,28 @@ def test_stream_ceiling_pass_leaves_the_solve_unchanged(oracle_lib):
         assert np.array_equal(a.get_probe_data()[name], tr), name
     for k in ("p", "vx", "vy", "vz"):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
+
+
+def test_material_change_rebuilds_ade_cell_list(oracle_lib):
+    """The ADE kernels visit a list of the pole-material cells built when the
+    materials are set up; moving the materials after that set-up must rebuild
+    it: the run equals the oracle of the moved scene."""
+    import dataclasses
+    sc = seeded_scenario(21, shape=(24, 20, 28), ade=True)
+    moved = np.roll(sc.material, (3, -2, 4), axis=(0, 1, 2))
+    if sc.air is not None:
+        moved[~sc.air] = 0
+    sc2 = dataclasses.replace(sc, material=moved)
+    n = 40
+    st_o, tr_o = cpu.run(oracle_problem(sc2, n), n)
+    s = make_solver(sc)
+    s._prepare(1)                      # builds the list for the first placement
+    s.set_material_region(np.ones(sc.shape, bool), 0)
+    for mid in np.unique(moved[moved > 0]):
+        s.set_material_region(moved == mid, int(mid))
+    s.step_n(n)
+    tr = s.get_probe_data()
+    for q in range(len(sc.probes)):
+        _same(tr[f"probe{q}"], tr_o[:, q], f"probe{q}")
+    for k in FIELDS:
+        _same(getattr(s, k), st_o[k], k)
