@@ -102,15 +102,19 @@ int fdtd_ctx_set_probes(fdtd_ctx* ctx, int n, const int64_t* flat_idx, float* tr
 int fdtd_ctx_set_microphones(fdtd_ctx* ctx, int n, const int32_t* idx, const double* w, const int32_t* f64,
                              double* out, int64_t out_len);
 
-/* Membrane (weighted-mask) sources, applied after each step's sweep, ADE
- * fix-up and point/plane sources: for every listed cell (flat index, cells
- * increasing), entries entry_start[e] .. entry_start[e+1]-1 in source order
+/* Per-cell source tables (membranes, and any point or plane source a caller
+ * routes here with weight 1.0), applied after each step's sweep, ADE fix-up
+ * and the sweep's own point/plane sources: for every listed cell (flat index
+ * below 2^31, cells increasing), entries entry_start[e] .. entry_start[e+1]-1
+ * in source order
  * do field[c] = float(double(field[c]) + wave[src][s] * weight), field 0 p,
  * 1 vx, 2 vy, 3 vz; then the probes probe_idx[probe_start[e] ..
  * probe_start[e+1]-1] record p at the cell again. Host arrays except wave
  * (device float64 [n_src][wave_len], value at step s in column s); the
- * tables are copied to the device. n_cells = 0 removes them. Replaces the
- * membrane branch of _inject_sources (core/solver.py:2389-2412). */
+ * tables are copied to the device after the steps already enqueued on the
+ * context stream finish. n_cells = 0 removes them. Replaces the membrane
+ * branch of _inject_sources (core/solver.py:2389-2412), and the point/plane
+ * branches for the sources a caller lists here. */
 int fdtd_ctx_set_membranes(fdtd_ctx* ctx, int n_cells, const int64_t* cell, const int32_t* entry_start,
                            int n_entries, const int32_t* entry_src, const int32_t* entry_field,
                            const double* entry_weight, const int32_t* probe_start, int n_probe_refs,
@@ -127,7 +131,9 @@ int fdtd_ctx_set_counters(fdtd_ctx* ctx, int64_t* step_counter, uint32_t* health
  * corr (density: -dt/rho_inf (times 1/dx uniform); modulus: -K_inf dt). The
  * auxiliary fields live on a dense box [lo, hi) (the material bounding box):
  * J and Jprev are device arrays [n_pole][box cells]. material: device uint8
- * per grid cell. */
+ * per grid cell; the call waits for the device and reads the box's ids once to
+ * list the cells whose material has poles (the ADE kernels visit only those),
+ * so call it again after changing the ids. */
 int fdtd_ctx_set_ade(fdtd_ctx* ctx, int n_pole, const int32_t* mat_id, const int32_t* target,
                      const int32_t* lorentz, const float* a, const float* b, const float* d,
                      const float* corr, const uint8_t* material, const int32_t* box_lo,
