@@ -116,6 +116,18 @@ class _Device:
         import torch
         return [torch.zeros(self.pb.shape, dtype=self.dtype, device=self.dev) for _ in range(4)]
 
+    def zeros_state_pair(self):
+        """Both ping-pong sets {p, vx, vy, vz} as views of one zeroed slab.
+        Eight separate allocations land wherever the allocator finds room, and
+        at 192^3 (fields plus parameters just inside the 256 MiB Infinity
+        Cache) that moved the fast forward between 91 and 111 ms for 2000
+        steps; one slab ran it in 90-92 ms every time."""
+        import torch
+        n = int(np.prod(self.pb.shape))
+        slab = torch.zeros(8 * n, dtype=self.dtype, device=self.dev)
+        views = [slab[q * n:(q + 1) * n].view(self.pb.shape) for q in range(8)]
+        return [views[0:4], views[4:8]]
+
     def step(self, src_state, dst_state, n, r, kappa, trace_row):
         pb = self.pb
         rc = _lib().fdtd_param_step(self.code, *[_ptr(t) for t in src_state], *[_ptr(t) for t in dst_state],
@@ -249,7 +261,7 @@ def _forward_fast(pb, r, kappa, dv, history: str):
     import torch
     fast = _Fast(dv)
     rk = fast.pack(r, kappa)
-    sets = [dv.zeros_state(), dv.zeros_state()]
+    sets = dv.zeros_state_pair()
     traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
     ckpts, hist = [], None
     if history == "all":
@@ -269,7 +281,7 @@ def _adjoint_fast(saved, grad_traces):
     import torch
     _, dv, rk, fast, history, hist, ckpts = saved
     pb = dv.pb
-    lam = [dv.zeros_state(), dv.zeros_state()]
+    lam = dv.zeros_state_pair()
     acc = torch.zeros(tuple(pb.box_shape) + (4,), dtype=torch.float32, device=dv.dev)
     g = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
     g[:, :dv.n_probe] = grad_traces.to(torch.float32)
@@ -277,7 +289,7 @@ def _adjoint_fast(saved, grad_traces):
         fast.adjoint(lam, rk, g, 0, pb.steps, hist, 0, acc)
     elif history == "checkpoint":
         K = pb.checkpoint
-        rec = [dv.zeros_state(), dv.zeros_state()]
+        rec = dv.zeros_state_pair()
         seg_hist = fast.history(K)
         scratch = torch.zeros_like(g)
         for seg in reversed(range(len(ckpts))):
