@@ -274,14 +274,19 @@ def _forward_fast(pb, r, kappa, dv, history: str):
             fast.forward(sets, rk, traces, n0, min(K, pb.steps - n0))
     else:
         fast.forward(sets, rk, traces, 0, pb.steps)
-    return traces[:, :dv.n_probe], ("fast", dv, rk, fast, history, hist, ckpts)
+    return traces[:, :dv.n_probe], ("fast", dv, rk, fast, history, hist, ckpts, sets)
 
 
 def _adjoint_fast(saved, grad_traces):
     import torch
-    _, dv, rk, fast, history, hist, ckpts = saved
+    _, dv, rk, fast, history, hist, ckpts, sets = saved
     pb = dv.pb
-    lam = dv.zeros_state_pair()
+    # The forward's field slab is free once its history (or checkpoints) is
+    # kept: reused, the adjoint and the next forward see the same buffer
+    # placement every call (which the cache-resident sweeps are sensitive to).
+    for t in sets[0] + sets[1]:
+        t.zero_()
+    lam = sets
     acc = torch.zeros(tuple(pb.box_shape) + (4,), dtype=torch.float32, device=dv.dev)
     g = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
     g[:, :dv.n_probe] = grad_traces.to(torch.float32)
