@@ -253,3 +253,45 @@ def test_voxelize_entry_point_validates_programs():
     cyl = [0, 0, 0, 0.6, 0.8, 0, 1.0, 0.1, 0]
     assert "signed unit axis" in run([(4, 0)], [0, 1], [0], cyl)[1]
     assert "material ids" in run([(2, 0), (2, 0)], [0, 1, 2], [0, 0], box)[1]
+
+
+def test_tiling_heuristic_follows_the_cache_boundary():
+    """2-row strips for solves whose 33 B/cell fit the 256 MiB Infinity Cache
+    (2 planes, 1 when the grid is too small for 2048 such waves); 4 rows x 4
+    planes beyond."""
+    from strata_fdtd_amd.tuning import CACHE_RESIDENT_CELLS, heuristic_tiling
+    assert CACHE_RESIDENT_CELLS == (256 << 20) // 33
+    assert heuristic_tiling((64, 64, 64)) == (2, 1, 1)
+    assert heuristic_tiling((100, 100, 100)) == (2, 2, 1)
+    assert heuristic_tiling((192, 192, 192)) == (2, 2, 1)
+    assert heuristic_tiling((224, 224, 224)) == (4, 4, 1)
+    assert heuristic_tiling((512, 512, 512)) == (4, 4, 1)
+    assert heuristic_tiling((320, 320, 480)) == (4, 4, 1)
+    # the boundary itself: one cell more leaves the cache
+    n = CACHE_RESIDENT_CELLS
+    assert heuristic_tiling((1, 1, n))[0] == 2 and heuristic_tiling((1, 1, n + 1))[0] == 4
+
+
+def test_table_source_routing():
+    """Membranes and z-normal planes are table sources (z planes stay in the
+    sweep on grids of 2^31 cells or more); points and x/y planes are not."""
+    from strata_fdtd_amd.solver import _table_source
+    from strata_fdtd_amd import CircularMembraneSource
+    plane = lambda ax: GaussianPulse(position={"axis": ax, "index": 3}, frequency=1e3, source_type="plane")
+    point = GaussianPulse(position=(1, 2, 3), frequency=1e3)
+    mem = CircularMembraneSource(center=(0.01, 0.01, 0.01), radius=3e-3, normal_axis="z",
+                                 waveform=GaussianPulse(position=(0, 0, 0), frequency=1e3))
+    small, huge = 10 ** 6, 2 ** 31
+    assert not _table_source(point, small)
+    assert not _table_source(plane(0), small) and not _table_source(plane(1), small)
+    assert _table_source(plane(2), small) and not _table_source(plane(2), huge)
+    assert _table_source(mem, small) and _table_source(mem, huge)
+
+
+def test_bench_names_the_sweep_instance_rocprof_reports():
+    import bench
+    cr = (256 << 20) // 33
+    assert bench.kernel_name("k_fused_step", 4, 1, True, 512 ** 3) == "k_fused_step<4,false,uint32_t,true>"
+    assert bench.kernel_name("k_fused_step", 2, 1, True, cr) == "k_fused_step<2,false,uint32_t,false>"
+    assert bench.kernel_name("k_fused_step", 4, 1, False, 1 << 30) == "k_fused_step<4,true,long,true>"
+    assert bench.kernel_name("k_fused_stack", 4, 2, True, 512 ** 3) == "k_fused_stack<4,false,2>"
