@@ -273,7 +273,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="enclosure_512")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--graph-chunk", type=int, default=50)
+    ap.add_argument("--graph-chunk", type=int, default=0,
+                    help="steps per captured hipGraph replay (0: kernels enqueued one by one, the faster)")
     ap.add_argument("--tiling", default=None, help="rows,planes[,stack] override")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")

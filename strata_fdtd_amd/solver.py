@@ -728,8 +728,15 @@ class FDTDSolver:
         _native.call("fdtd_stream_ceiling", self._ctx, reps, C.byref(ms))
         return float(ms.value)
 
-    def step_n(self, n: int, graph_chunk: int = 64) -> None:
-        """Advance n steps on the device without host round trips."""
+    def step_n(self, n: int, graph_chunk: int = 0) -> None:
+        """Advance n steps on the device without host round trips.
+
+        graph_chunk = 0 (the default) enqueues the steps' kernels one by one;
+        graph_chunk = m replays a captured hipGraph of m steps. Measured on
+        MI355X, the graph replay costs about 3.5 us more per step than the
+        same kernels enqueued back to back (100^3: 12.9 against 9.6 us per
+        step; 512^3: 901 against 863 us), so graphs are only worth it where
+        the host cannot keep up with the launches."""
         if n <= 0:
             return
         self._prepare(n)
