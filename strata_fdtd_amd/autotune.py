@@ -16,7 +16,7 @@ from . import _native
 from .tuning import ROW_CHOICES, STACK_CHOICES, normalize_tiling, store_tiling
 
 
-def candidate_tilings(shape, rows=ROW_CHOICES, planes=(4, 8, 12, 16, 24, 32, 48, 64), stacks=STACK_CHOICES):
+def candidate_tilings(shape, rows=ROW_CHOICES, planes=(1, 2, 4, 8, 12, 16, 24, 32, 48, 64), stacks=STACK_CHOICES):
     nx, ny = shape[0], shape[1]
     out = []
     for r, p, w in itertools.product(rows, planes, stacks):

@@ -270,6 +270,10 @@ def test_tiling_heuristic_follows_the_cache_boundary():
     # the boundary itself: one cell more leaves the cache
     n = CACHE_RESIDENT_CELLS
     assert heuristic_tiling((1, 1, n))[0] == 2 and heuristic_tiling((1, 1, n + 1))[0] == 4
+    # the autotuner's candidates include the heuristic's choice
+    from strata_fdtd_amd.autotune import candidate_tilings
+    for shape in ((64, 64, 64), (100, 100, 100), (192, 192, 192), (512, 512, 512)):
+        assert heuristic_tiling(shape) in candidate_tilings(shape)
 
 
 def test_table_source_routing():
