@@ -242,3 +242,26 @@ def test_material_change_rebuilds_ade_cell_list(oracle_lib):
         _same(tr[f"probe{q}"], tr_o[:, q], f"probe{q}")
     for k in FIELDS:
         _same(getattr(s, k), st_o[k], k)
+
+
+def test_autotune_caches_the_fastest_and_pick_tiling_reads_it(oracle_lib, tmp_path, monkeypatch):
+    """The autotuner times its candidates on the device, stores the fastest
+    for the grid shape, and pick_tiling returns it; the solve stays bit-exact
+    under the chosen launch shape."""
+    from strata_fdtd_amd import autotune, tuning
+    monkeypatch.setenv("STRATA_FDTD_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    sc = seeded_scenario(4, shape=(26, 37, 70), pml=3)
+    s = make_solver(sc)
+    cands = [(2, 1, 1), (2, 2, 1), (4, 4, 1), (4, 7, 2)]
+    best, table = autotune.autotune(s, candidates=cands, steps=5)
+    assert len(table) == len(cands) and all(t[3] > 0 for t in table)
+    assert min(table, key=lambda t: t[3])[:3] == best
+    assert tuning.pick_tiling(sc.shape, False, False) == best
+    n = 20
+    st_o, tr_o = cpu.run(oracle_problem(sc, n), n)
+    t = make_solver(sc, tiling=best)
+    t.step_n(n)
+    for q in range(len(sc.probes)):
+        _same(t.get_probe_data()[f"probe{q}"], tr_o[:, q], f"probe{q}")
+    for k in FIELDS:
+        _same(getattr(t, k), st_o[k], k)
