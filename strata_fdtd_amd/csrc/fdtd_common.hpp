@@ -97,7 +97,9 @@ struct FusedArgs {
     int nstrip_j;                   // ceil(ny / R)
     int nstrip_i;                   // ceil(nx / planes)
     int nwaves;                     // nseg * nstrip_j * nstrip_i
-    int blocks_per_xcd;             // gridDim.x / 8
+    int blocks_per_xcd;             // sweep blocks / 8
+    int sweep_blocks;               // blocks of the sweep; blocks beyond run
+    int dens_blocks;                //   the ADE density update (0: none)
 
     // time
     const int64_t* step_ptr;        // device counter: index of step koff==0

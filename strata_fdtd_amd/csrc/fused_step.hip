@@ -368,12 +368,22 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     }
 }
 
+__device__ __forceinline__ void ade_density_part(const FusedArgs& a, int64_t first, int64_t stride);
+
 // The nonuniform 4-row sweep with 32-bit offsets is held to 72 VGPRs (7 waves
 // per SIMD instead of 6, no spills): 1.1% faster on a 320x320x480 grid.
+// Blocks past a.sweep_blocks run the step's ADE density update: it reads the
+// input pressure and writes only J, which the sweep never touches, so it
+// fills the sweep's tail instead of taking a launch of its own.
 template <int R, bool NU, typename IX, bool NTS>
 __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu((NU && R == 4 && sizeof(IX) == 4) ? 7 : 1, 8))) void k_fused_step(
     const FusedArgs a) {
+    if (a.dens_blocks > 0 && blockIdx.x >= static_cast<unsigned>(a.sweep_blocks)) {
+        ade_density_part(a, static_cast<int64_t>(blockIdx.x - a.sweep_blocks) * kBlock + threadIdx.x,
+                         static_cast<int64_t>(a.dens_blocks) * kBlock);
+        return;
+    }
     const int lane = threadIdx.x & (kWave - 1);
     // XCD-aware remap: hardware block b runs on XCD b % 8; give each XCD a
     // contiguous range of logical blocks so strips that share halo rows share
@@ -463,15 +473,14 @@ __device__ __forceinline__ int64_t box_cells(const FusedArgs& a) {
 
 // Density-pole auxiliary update over the ADE box, run before the sweep of the
 // same time step (J <- a J + b p, Lorentz with history), ade.cpp:25-166.
-__global__ void k_ade_density(const FusedArgs a) {
+__device__ __forceinline__ void ade_density_part(const FusedArgs& a, int64_t first, int64_t stride) {
     const int by = a.box_hi[1] - a.box_lo[1], bz = a.box_hi[2] - a.box_lo[2];
     const int64_t box_n = box_cells(a);
     const int64_t s = *a.step_ptr + a.koff;
     if (s >= a.step_limit) return;
     const float* __restrict__ P = (s & 1) ? a.f[1][0] : a.f[0][0];
     const int64_t n_visit = a.ade_cells ? a.n_ade_cells : box_n;
-    for (int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; e < n_visit;
-         e += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    for (int64_t e = first; e < n_visit; e += stride) {
         const int64_t b = a.ade_cells ? static_cast<int64_t>(a.ade_cells[e]) : e;
         const int kk = static_cast<int>(b % bz) + a.box_lo[2];
         const int jj = static_cast<int>((b / bz) % by) + a.box_lo[1];
@@ -494,6 +503,11 @@ __global__ void k_ade_density(const FusedArgs a) {
             }
         }
     }
+}
+
+__global__ void k_ade_density(const FusedArgs a) {
+    ade_density_part(a, blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x,
+                     static_cast<int64_t>(gridDim.x) * blockDim.x);
 }
 
 // v' of the face between cells lo (flat c) and hi (c + step along `axis`), with
@@ -750,25 +764,33 @@ int fused_grid(FusedArgs& a, int rows, int planes, int stack) {
 // One full step: [ADE density] -> fused sweep -> [ADE fix-up] -> [membrane
 // sources] -> [microphones]. ev_before /
 // ev_after (optional) are recorded around the sweep alone, for timing it.
-hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool ade, int blocks,
+hipError_t launch_fused(const FusedArgs& a_in, int rows, int stack, bool nu, bool ade, int blocks,
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after) {
+    FusedArgs a = a_in;
+    a.sweep_blocks = blocks;
+    a.dens_blocks = 0;
     int box_blocks = 1;
     if (ade) {
         const int64_t bn = a.ade_cells ? a.n_ade_cells
                                        : static_cast<int64_t>(a.box_hi[0] - a.box_lo[0]) *
                                              (a.box_hi[1] - a.box_lo[1]) * (a.box_hi[2] - a.box_lo[2]);
         box_blocks = static_cast<int>(std::min<int64_t>(std::max<int64_t>((bn + 255) / 256, 1), 1 << 16));
-        hipLaunchKernelGGL(k_ade_density, dim3(box_blocks), dim3(256), 0, st, a);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        if (stack == 1) {
+            a.dens_blocks = box_blocks;  // run inside the sweep launch (k_fused_step)
+        } else {
+            hipLaunchKernelGGL(k_ade_density, dim3(box_blocks), dim3(256), 0, st, a);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
     }
     hipError_t e;
     if (ev_before && (e = hipEventRecord(ev_before, st)) != hipSuccess) return e;
+    const int grid = blocks + a.dens_blocks;
     switch (rows) {
-        case 2: e = launch_rows<2>(a, nu, stack, blocks, st); break;
-        case 4: e = launch_rows<4>(a, nu, stack, blocks, st); break;
-        case 8: e = launch_rows<8>(a, nu, stack, blocks, st); break;
-        case 16: e = launch_rows<16>(a, nu, stack, blocks, st); break;
+        case 2: e = launch_rows<2>(a, nu, stack, grid, st); break;
+        case 4: e = launch_rows<4>(a, nu, stack, grid, st); break;
+        case 8: e = launch_rows<8>(a, nu, stack, grid, st); break;
+        case 16: e = launch_rows<16>(a, nu, stack, grid, st); break;
         default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess) return e;
