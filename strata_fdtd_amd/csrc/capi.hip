@@ -390,6 +390,7 @@ int fdtd_ctx_set_membranes(fdtd_ctx* c, int n_cells, const int64_t* cell, const 
     c->a.m_pidx = static_cast<const int32_t*>(c->m_pidx.p);
     c->a.m_wave = wave;
     c->a.m_wave_len = wave_len;
+    c->a.m_single = n_entries == n_cells ? 1 : 0;  // every cell has at least one entry
     c->a.n_mcell = n_cells;
     return 0;
 }

@@ -143,6 +143,7 @@ struct FusedArgs {
     const int32_t* m_pidx;          // device probe numbers
     const double* m_wave;           // device [n_msrc][m_wave_len] waveform per step
     int64_t m_wave_len;
+    int m_single;                   // every listed cell has exactly one entry (entry e of cell e)
 
     // ADE (materials with poles)
     int n_pole;

@@ -677,15 +677,20 @@ __global__ void k_mic_gather(const FusedArgs a) {
 __global__ void k_membrane(const FusedArgs a) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= a.n_mcell) return;
+    // The loads that do not depend on one another are issued together: with
+    // one entry per cell (m_single) entry e belongs to cell e, so the entry
+    // needs no lookup through m_estart.
     const int64_t s = *a.step_ptr + a.koff;
+    const int64_t c = a.mcell[e];
+    const int t0 = a.m_single ? e : a.m_estart[e];
+    const int t1 = a.m_single ? e + 1 : a.m_estart[e + 1];
     if (s >= a.step_limit) return;
     if (s >= a.m_wave_len) {
         atomicOr(a.health, 2u);
         return;
     }
     const int out = static_cast<int>((s + 1) & 1);
-    const int64_t c = a.mcell[e];
-    for (int t = a.m_estart[e]; t < a.m_estart[e + 1]; ++t) {
+    for (int t = t0; t < t1; ++t) {
         float* F = a.f[out][a.m_efield[t]];
         const double add = a.m_wave[static_cast<int64_t>(a.m_esrc[t]) * a.m_wave_len + s] * a.m_eweight[t];
         F[c] = static_cast<float>(static_cast<double>(F[c]) + add);
