@@ -128,7 +128,9 @@ int fdtd_ctx_set_counters(fdtd_ctx* ctx, int64_t* step_counter, uint32_t* health
 /* ADE (frequency-dependent materials). Replaces ADEMaterialData + the
  * update_ade_* / apply_ade_* kernels (ade.cpp:25-473). Poles are host arrays:
  * mat_id, target (0 density, 1 modulus), lorentz (0/1), coefficients a,b,d and
- * corr (density: -dt/rho_inf (times 1/dx uniform); modulus: -K_inf dt). The
+ * corr (density: -dt/rho_inf * inv_dx, with inv_dx = 1/min spacing on every
+ * grid as the native correction is called, core/solver.py:3295-3305; modulus:
+ * -K_inf dt). The
  * auxiliary fields live on a dense box [lo, hi) (the material bounding box):
  * J and Jprev are device arrays [n_pole][box cells]. material: device uint8
  * per grid cell; the call waits for the device and reads the box's ids once to

@@ -14,14 +14,17 @@ Contents
 ``tables``       independent numpy restatement of the host-side precompute:
                  time step, coefficients, PML profiles/decays, nonuniform
                  spacing arrays, Gaussian-pulse waveform and grid-index rules.
-``numpy_step``   whole-array numpy restatement of one step (python ordering)
-                 for tiny grids, in float32 or float64.
-``voxel``        brute-force per-point SDF evaluation used to pin the host
-                 voxelizer on small grids.
+``from_solver``  describes a configured product solver as an oracle problem
+                 (full-size checks and the CPU baseline).
 ``microphones``  the reference's microphone recording (trilinear gathers,
                  polar patterns).
 ``membranes``    membrane-source weights over the whole grid (circular Bessel
                  and rectangular sine modes).
+
+The float64 twin of the step (``cpu.run(..., impl="oracle64")``, the same
+C++ template) is the "error vs FP64" reference. Voxel masks are pinned
+directly by masks the reference's own sdf.py wrote (tests/golden/voxel_masks.npz),
+so the oracle has no voxeliser of its own.
 
 Parity status of each function is recorded in its docstring ("pinned by ...").
 """

@@ -26,6 +26,14 @@
 //
 // A double-precision twin of the whole step (template parameter T) provides the
 // "error vs FP64" figure.
+//
+// Pinned by (tests/test_oracle.py): bit-exact equality with the reference's own
+// kernels compiled from source (oracle/_ref) phase by phase and over whole runs
+// on seeded scenes (uniform and nonuniform grids, rigid walls, PML, plane
+// sources, Debye and Lorentz density and modulus poles, ADE on a nonuniform
+// grid); golden vectors written from those kernels (tests/golden/*.npz); a 1D
+// plane wave against a textbook leapfrog; wave speed, cavity mode, p = 0 in
+// solids, closed-box energy; Debye fixed point and Lorentz ringing frequency.
 
 #include <cmath>
 #include <cstdint>
@@ -196,7 +204,7 @@ struct OrcPole {
     int32_t lorentz;       // 0 Debye, 1 Lorentz
     int32_t pad;
     double a, b, d;        // Debye: alpha=a, beta=b.  Lorentz: a, b, d
-    double corr;           // density: -dt/rho_inf (times 1/dx on uniform grids)
+    double corr;           // density: -dt/rho_inf * (1/min spacing)
                            // modulus: -K_inf*dt
 };
 
@@ -335,9 +343,10 @@ struct Runner {
         }
     }
 
-    // v[face] += coeff * (J[c+e] - J[c]) where both cells hold the pole's
-    // material, ade.cpp:228-401. Uniform: coeff = corr (already times 1/dx).
-    // Nonuniform: coeff = corr * inv_face[index] (core/solver.py:3349-3385).
+    // v[face] += corr * (J[c+e] - J[c]) where both cells hold the pole's
+    // material, ade.cpp:228-401. corr = -dt/rho_inf * inv_dx with the scalar
+    // inv_dx = 1/min spacing the solver passes on every grid, nonuniform ones
+    // included (core/solver.py:3295-3305): no per-face spacing here.
     void velocity_correction(int q, T* vx, T* vy, T* vz) {
         const OrcPole& pl = pb.poles[q];
         const T* Jq = J[q].data();
@@ -345,7 +354,6 @@ struct Runner {
         const uint8_t* m = pb.material;
         const T corr = static_cast<T>(pl.corr);
         const int64_t sy = nz, sx = static_cast<int64_t>(ny) * nz;
-        const bool nu = pb.nonuniform != 0;
 #pragma omp parallel for collapse(2) schedule(static)
         for (int i = 0; i < nx; ++i)
             for (int j = 0; j < ny; ++j) {
@@ -353,18 +361,9 @@ struct Runner {
                 for (int k = 0; k < nz; ++k) {
                     const int64_t c = b + k;
                     if (m[c] != mid) continue;
-                    if (i < nx - 1 && m[c + sx] == mid) {
-                        const T cf = nu ? T(corr * ifx[i]) : corr;
-                        vx[c] += cf * (Jq[c + sx] - Jq[c]);
-                    }
-                    if (j < ny - 1 && m[c + sy] == mid) {
-                        const T cf = nu ? T(corr * ify[j]) : corr;
-                        vy[c] += cf * (Jq[c + sy] - Jq[c]);
-                    }
-                    if (k < nz - 1 && m[c + 1] == mid) {
-                        const T cf = nu ? T(corr * ifz[k]) : corr;
-                        vz[c] += cf * (Jq[c + 1] - Jq[c]);
-                    }
+                    if (i < nx - 1 && m[c + sx] == mid) vx[c] += corr * (Jq[c + sx] - Jq[c]);
+                    if (j < ny - 1 && m[c + sy] == mid) vy[c] += corr * (Jq[c + sy] - Jq[c]);
+                    if (k < nz - 1 && m[c + 1] == mid) vz[c] += corr * (Jq[c + 1] - Jq[c]);
                 }
             }
     }

@@ -51,8 +51,7 @@ def problem_from_solver(solver, n_steps: int) -> cpu.Problem:
         for pole in mat.poles:
             tgt = 0 if pole.target == "density" else 1
             if tgt == 0:
-                corr = (T.native_density_correction(dt, mat.rho_inf, solver.dx) if grid.is_uniform
-                        else float(np.float32(-dt / mat.rho_inf)))
+                corr = T.native_density_correction(dt, mat.rho_inf, solver.dx)
             else:
                 corr = T.native_modulus_correction(dt, mat.K_inf)
             if pole.is_debye:

@@ -530,8 +530,7 @@ __device__ __forceinline__ float ade_face(const FusedArgs& a, const float* __res
             for (int q = 0; q < a.n_pole; ++q) {
                 if (a.pole_target[q] != 0 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m_lo) continue;
                 const float* Jq = a.J + q * box_n;
-                const float corr = NU ? a.pole_corr[q] * a.ifc[axis][idx] : a.pole_corr[q];
-                v += corr * (Jq[box_index(a, ih, jh, kh)] - Jq[box_index(a, i, j, k)]);
+                v += a.pole_corr[q] * (Jq[box_index(a, ih, jh, kh)] - Jq[box_index(a, i, j, k)]);
             }
     }
     const uint32_t open = axis == 0 ? FDTD_FLAG_OPEN_X : (axis == 1 ? FDTD_FLAG_OPEN_Y : FDTD_FLAG_OPEN_Z);

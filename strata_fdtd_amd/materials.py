@@ -120,14 +120,13 @@ AIR_15C = SimpleMaterial(name="air_15C", _rho=1.225, _c=340.3)
 WATER_20C = SimpleMaterial(name="water_20C", _rho=998.2, _c=1482.0)
 
 
-def density_correction_coeff(dt: float, rho_inf: float, inv_dx: float | None) -> np.float32:
+def density_correction_coeff(dt: float, rho_inf: float, inv_dx: float) -> np.float32:
     """Velocity-correction coefficient as the native kernel forms it in float32:
-    ``-dt / rho_inf * inv_dx`` (_kernels/src/ade.cpp:247). On nonuniform grids
-    (inv_dx None) the base ``-dt / rho_inf`` is returned and the kernel
-    multiplies by the per-face inverse spacing (core/solver.py:3349-3385)."""
+    ``-dt / rho_inf * inv_dx`` (_kernels/src/ade.cpp:247). The solver passes
+    ``1.0 / self.dx`` (the minimum spacing) on uniform and nonuniform grids
+    alike (core/solver.py:3295-3305), so nonuniform grids get no per-face
+    spacing in this correction."""
     f = np.float32
-    if inv_dx is None:
-        return f(-dt / rho_inf)
     return f((f(-f(dt)) / f(rho_inf)) * f(inv_dx))
 
 

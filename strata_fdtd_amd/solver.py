@@ -491,7 +491,9 @@ class FDTDSolver:
         hi = np.array([int(a.max()) + 1 for a in idx], dtype=np.int32)
         box_n = int(np.prod(hi - lo))
         n = len(poles)
-        inv_dx = (1.0 / self.dx) if self._grid.is_uniform else None
+        # the native correction is called with the scalar 1/dx (min spacing) on
+        # every grid, nonuniform ones included (core/solver.py:3295-3305)
+        inv_dx = 1.0 / self.dx
         mat_id = np.array([m for m, _, _ in poles], dtype=np.int32)
         target = np.array([0 if p.target == "density" else 1 for _, _, p in poles], dtype=np.int32)
         lorentz = np.array([0 if p.is_debye else 1 for _, _, p in poles], dtype=np.int32)

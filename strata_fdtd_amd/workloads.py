@@ -112,35 +112,45 @@ def enclosure_512(n: int = 512, steps: int = 4000, **solver_kw) -> Workload:
                     "port r12 L60 + ADE sphere r40 (2 Debye poles), 500 Hz pulse, 8 probes")
 
 
-def resonators_nu_scene(shape=(320, 320, 480), core=200, seed=7):
+def resonators_nu_scene(shape=(320, 320, 480), core=200, seed=7, n_side=6, pitch=30, plate=40,
+                        cavity_radius=10, cavity_depth=30, neck_radius=4, neck_length=4):
+    """Grid, air geometry, probes and plane-source index of the resonator-plate
+    workload. The defaults are the 320x320x480 workload; the keyword sizes (in
+    cells of the 1 mm core) scale the plate and its n_side x n_side array of
+    resonators down for tests."""
     rng = np.random.default_rng(seed)
     xs = [explicit_core_stretch_coords(n, min(core, n - (n - core) % 2), 1e-3, 1.05) for n in shape]
     grid = NonuniformGrid(*xs)
     zc = xs[2]
     kc = shape[2] // 2
-    plate_lo = zc[kc] - 0.5e-3          # plate occupies cells [kc, kc+40) along z
-    plate_hi = zc[kc + 39] + 0.5e-3
-    plate = Box(min_corner=(-1.0, -1.0, plate_lo), max_corner=(2.0, 2.0, plate_hi))
+    mm = lambda cells: cells / 1000.0  # noqa: E731  (cells of 1 mm, rounded as a literal 4e-3 is)
+    plate_lo = zc[kc] - 0.5e-3          # plate occupies cells [kc, kc+plate) along z
+    plate_hi = zc[kc + plate - 1] + 0.5e-3
+    slab = Box(min_corner=(-1.0, -1.0, plate_lo), max_corner=(2.0, 2.0, plate_hi))
     holes = []
     xc, yc = xs[0], xs[1]
-    i0, j0 = shape[0] // 2 - 90, shape[1] // 2 - 90
-    for a in range(6):
-        for b in range(6):
-            cx = xc[i0 + 30 * a + 15]
-            cy = yc[j0 + 30 * b + 15]
-            neck = Cylinder(p1=(cx, cy, plate_lo - 1e-3), p2=(cx, cy, plate_lo + 4e-3), radius=4e-3)
-            cav = Cylinder(p1=(cx, cy, plate_lo + 4e-3), p2=(cx, cy, plate_lo + 34e-3), radius=10e-3)
+    half = n_side * pitch // 2
+    i0, j0 = shape[0] // 2 - half, shape[1] // 2 - half
+    for a in range(n_side):
+        for b in range(n_side):
+            cx = xc[i0 + pitch * a + pitch // 2]
+            cy = yc[j0 + pitch * b + pitch // 2]
+            neck = Cylinder(p1=(cx, cy, plate_lo - 1e-3), p2=(cx, cy, plate_lo + mm(neck_length)),
+                            radius=mm(neck_radius))
+            cav = Cylinder(p1=(cx, cy, plate_lo + mm(neck_length)),
+                           p2=(cx, cy, plate_lo + mm(neck_length + cavity_depth)), radius=mm(cavity_radius))
             holes += [cav, neck]
-    solid = Difference(plate, *holes)
+    solid = Difference(slab, *holes)
     domain = Box(min_corner=(-1.0, -1.0, -1.0), max_corner=(2.0, 2.0, 50.0))
     air = Difference(domain, solid)
     probes = []
+    k_lo = max(1, kc - 60)
     for _ in range(16):
-        i = int(rng.integers(shape[0] // 2 - 90, shape[0] // 2 + 90))
-        j = int(rng.integers(shape[1] // 2 - 90, shape[1] // 2 + 90))
-        k = int(rng.integers(kc - 60, kc - 5))
+        i = int(rng.integers(shape[0] // 2 - half, shape[0] // 2 + half))
+        j = int(rng.integers(shape[1] // 2 - half, shape[1] // 2 + half))
+        k = int(rng.integers(k_lo, kc - 5))
         probes.append((i, j, k))
-    src_index = kc - 80
+    src_index = max(1, kc - 80)
     return grid, air, probes, src_index
 
 

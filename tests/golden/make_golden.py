@@ -29,6 +29,7 @@ CASES = {
     "rigid_plane_source": (22, (16, 18, 20), dict(geometry=True, pml=3, plane=True), 40),
     "ade_debye_lorentz": (23, (18, 20, 22), dict(geometry=True, pml=3, ade=True), 40),
     "nonuniform_rigid": (24, (16, 18, 20), dict(geometry=True, pml=3, nonuniform=True), 40),
+    "nonuniform_ade": (25, (18, 20, 22), dict(geometry=True, pml=3, ade=True, nonuniform=True), 40),
 }
 
 

@@ -1,0 +1,183 @@
+"""Write voxel-mask fixtures from the reference's own SDF code.
+
+    python tests/golden/make_voxel_golden.py
+
+Run in a container that holds the reference checkout (at /root/reference, or
+STRATA_REF_SRC). It imports the reference's ``strata_fdtd.geometry.sdf``
+(SDFPrimitive.voxelize :99-127 and the Box/Sphere/Cylinder/CSG formulas),
+``geometry/resonator.py`` (HelmholtzResonator) and
+``geometry/material_assignment.py`` (MaterializedGeometry.voxelize_with_materials
+:133-168), with ``h5py`` (absent here, needed only by the reference's HDF5
+writer) replaced by an empty module. Each scene is built once with the
+product's classes, written down as a data tree (tests/golden/scene_tree.py),
+rebuilt from that tree with the reference's classes and voxelised by the
+reference. ``voxel_masks.npz`` then holds, per case: the tree (JSON text), the
+grid (shape and spacing, or the cell-centre coordinates of a nonuniform grid),
+the packed air mask and, for material scenes, the material ids. Data only; the
+reference is not needed to read it.
+
+Cases:
+  enclosure_64       the 512^3 enclosure workload scaled to 64^3 (shell, port,
+                     ADE sphere region)
+  plate_nu           the nonuniform resonator-plate workload scaled down (96x96x128,
+                     3x3 resonators, half of them in the stretched cells)
+  request_{0,1,2}    three serving requests with axes drawn from 64 and 80
+  tree_u{NN}         random CSG trees with millimetre-rounded parameters on a
+                     1 mm grid; the even ones centred on cell centres with
+                     whole-millimetre sizes (many cell centres exactly on a surface)
+  tree_nu{NN}        random CSG trees on a stretched grid
+  materials_{N}      random geometry plus three random material regions
+  helmholtz_{N}      Helmholtz resonators (box, sphere, cylinder cavities)
+Cylinders here are axis-aligned: their np.dot projection sums exact zeros, so
+the reference's values do not depend on the BLAS build.
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+import types
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+import scene_tree  # noqa: E402
+
+REF_SRC = os.environ.get("STRATA_REF_SRC", "/root/reference/src")
+OUT = os.path.join(HERE, "voxel_masks.npz")
+
+
+def reference_namespace() -> dict:
+    sys.modules.setdefault("h5py", types.ModuleType("h5py"))
+    sys.path.insert(0, REF_SRC)
+    from strata_fdtd.geometry import sdf
+    from strata_fdtd.geometry.material_assignment import MaterializedGeometry
+    from strata_fdtd.geometry.resonator import HelmholtzResonator
+    from strata_fdtd.core.grid import NonuniformGrid, UniformGrid
+    ns = dict(Box=sdf.Box, Sphere=sdf.Sphere, Cylinder=sdf.Cylinder, Union=sdf.Union,
+              Intersection=sdf.Intersection, Difference=sdf.Difference,
+              HelmholtzResonator=HelmholtzResonator, MaterializedGeometry=MaterializedGeometry,
+              material=object(), UniformGrid=UniformGrid, NonuniformGrid=NonuniformGrid)
+    assert sdf.__file__.startswith(REF_SRC), sdf.__file__
+    return ns
+
+
+def random_tree(rng, depth=0, ties=False):
+    """Product-class CSG tree with axis-aligned, millimetre-rounded parameters.
+    With ``ties`` the centres sit on cell centres of a 1 mm grid (computed as
+    the grid computes them, i*1e-3 + 0.5e-3) and sizes and radii are whole
+    millimetres, so faces and spheres pass exactly through cell centres."""
+    from strata_fdtd_amd import Box, Cylinder, Difference, Intersection, Sphere, Union
+
+    def r(a, b, n=None, d=3):
+        return np.round(rng.uniform(a, b, n), d)
+
+    def centre():
+        if ties:
+            return rng.integers(8, 44, 3) * 1e-3 + 1e-3 / 2
+        return r(0.01, 0.08, 3, 4)
+    t = rng.integers(0, 6 if depth < 2 else 3)
+    c = centre()
+    if t == 0:
+        return Box(center=tuple(c), size=tuple(r(0.005, 0.05, 3)))
+    if t == 1:
+        return Sphere(center=tuple(c), radius=float(r(0.005, 0.03)))
+    if t == 2:
+        d = np.zeros(3)
+        d[rng.integers(0, 3)] = r(0.01, 0.05) * (1 if rng.random() < 0.5 else -1)
+        return Cylinder(p1=tuple(c), p2=tuple(c + d), radius=float(r(0.003, 0.02)))
+    kids = [random_tree(rng, depth + 1, ties) for _ in range(rng.integers(1, 4))]
+    return (Union, Intersection, lambda *k: Difference(k[0], *k[1:]))[t - 3](*kids)
+
+
+def nontrivial_tree(rng, grid, ties=False, tries=200):
+    """A random tree whose mask is neither empty nor full on ``grid`` (the
+    product's voxeliser only picks the cases; the fixture holds the
+    reference's masks)."""
+    for _ in range(tries):
+        node = random_tree(rng, ties=ties)
+        frac = node.voxelize(grid).mean()
+        if 0.002 < frac < 0.998:
+            return node
+    raise RuntimeError("no non-trivial tree drawn")
+
+
+def cases():
+    """(name, product geometry object, grid spec) triples."""
+    from strata_fdtd_amd import HelmholtzResonator, MaterializedGeometry, SimpleMaterial
+    from strata_fdtd_amd.serving import sweep_requests
+    from strata_fdtd_amd.workloads import enclosure_512_scene, resonators_nu_scene
+    out = []
+    geo, *_ = enclosure_512_scene(64)
+    out.append(("enclosure_64", geo, dict(shape=(64, 64, 64), dx=1e-3)))
+    grid, air, *_ = resonators_nu_scene(shape=(96, 96, 128), core=60, n_side=3, pitch=24, plate=24,
+                                        cavity_radius=8, cavity_depth=14, neck_radius=3, neck_length=4)
+    out.append(("plate_nu", air, dict(coords=(grid.x_coords, grid.y_coords, grid.z_coords))))
+    for q, req in enumerate(sweep_requests(n=3, seed=77, dims=(64, 80))):
+        out.append((f"request_{q}", req.geometry, dict(shape=req.shape, dx=req.resolution)))
+    from strata_fdtd_amd import NonuniformGrid, UniformGrid
+    rng = np.random.default_rng(2025)
+    ug = UniformGrid((64, 56, 48), 1e-3)
+    for q in range(24):
+        out.append((f"tree_u{q:02d}", nontrivial_tree(rng, ug, ties=q % 2 == 0), dict(shape=(64, 56, 48), dx=1e-3)))
+    nug = NonuniformGrid.from_stretch((64, 56, 48), 1e-3, stretch_z=1.03, stretch_x=1.02)
+    for q in range(8):
+        out.append((f"tree_nu{q:02d}", nontrivial_tree(rng, nug),
+                    dict(coords=(nug.x_coords, nug.y_coords, nug.z_coords))))
+    mg = UniformGrid((48, 40, 52), 1e-3)
+    for q in range(4):
+        geo = MaterializedGeometry(nontrivial_tree(rng, mg))
+        for m in range(3):
+            geo.set_material(nontrivial_tree(rng, mg, ties=True), SimpleMaterial(name=f"m{m}", _rho=1.2 + m, _c=340.0))
+        out.append((f"materials_{q}", geo, dict(shape=(48, 40, 52), dx=1e-3)))
+    for q, (shape, size) in enumerate((("box", (0.012, 0.01, 0.014)), ("sphere", (0.008,)),
+                                       ("cylinder", (0.006, 0.016)))):
+        res = HelmholtzResonator(position=(0.02, 0.021, 0.018), cavity_shape=shape, cavity_size=size,
+                                 neck_length=0.006, neck_radius=0.003,
+                                 neck_direction=((0.0, 0.0, 1.0), (1.0, 0.0, 0.0), (0.0, -1.0, 0.0))[q])
+        out.append((f"helmholtz_{q}", res, dict(shape=(40, 44, 48), dx=1e-3)))
+    return out
+
+
+def ref_grid(ns, spec):
+    if "coords" in spec:
+        return ns["NonuniformGrid"](*[np.asarray(c, np.float64) for c in spec["coords"]])
+    return ns["UniformGrid"](tuple(spec["shape"]), spec["dx"])
+
+
+def main() -> None:
+    ns = reference_namespace()
+    arrays = {}
+    names = []
+    for name, obj, spec in cases():
+        tree = scene_tree.to_tree(obj)
+        ref_obj = scene_tree.build(tree, ns)
+        grid = ref_grid(ns, spec)
+        if tree[0] == "materialized":
+            mask, ids = ref_obj.voxelize_with_materials(grid)
+            arrays[f"{name}__ids"] = np.asarray(ids, np.uint8)
+        else:
+            mask = ref_obj.voxelize(grid)
+        mask = np.asarray(mask, bool)
+        arrays[f"{name}__tree"] = np.array(scene_tree.dumps(tree))
+        arrays[f"{name}__shape"] = np.array(mask.shape, np.int64)
+        arrays[f"{name}__mask"] = np.packbits(mask.ravel())
+        if "coords" in spec:
+            for a, c in zip("xyz", spec["coords"]):
+                arrays[f"{name}__{a}"] = np.asarray(c, np.float64)
+        else:
+            arrays[f"{name}__dx"] = np.array(spec["dx"], np.float64)
+        names.append(name)
+        extra = f", material cells {int((arrays[name + '__ids'] > 0).sum())}" if tree[0] == "materialized" else ""
+        print(f"{name}: {mask.shape}, air {int(mask.sum())}/{mask.size}{extra}", flush=True)
+    arrays["names"] = np.array(names)
+    np.savez_compressed(OUT, **arrays)
+    print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes)")
+
+
+if __name__ == "__main__":
+    main()

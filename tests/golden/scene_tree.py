@@ -1,0 +1,89 @@
+"""Geometry scenes as plain data, so one scene can be built with two SDF class
+sets (the product's and, when writing fixtures, the reference's).
+
+A tree is a JSON-able list:
+
+  ["box", center, size]              Box(center=..., size=...)
+  ["sphere", center, radius]
+  ["cylinder", p1, p2, radius]
+  ["union", [child, ...]]
+  ["intersection", [child, ...]]
+  ["difference", base, [child, ...]]
+  ["helmholtz", position, cavity_shape, cavity_size, neck_length, neck_radius, neck_direction]
+  ["materialized", geometry, [region, ...]]   MaterializedGeometry + set_material per region
+
+Floats go through ``float`` and JSON's shortest repr, which round-trips every
+float64 exactly, so both class sets see bit-identical parameters. Boxes are
+recorded by the centre and size the product stored; a box built from corners
+computes them as ``(lo + hi) / 2`` and ``hi - lo`` in either implementation.
+"""
+
+from __future__ import annotations
+
+import json
+
+import numpy as np
+
+
+def _v(a) -> list:
+    return [float(x) for x in np.asarray(a, dtype=np.float64).ravel()]
+
+
+def to_tree(obj) -> list:
+    """Describe a product geometry object (strata_fdtd_amd.geometry)."""
+    from strata_fdtd_amd import geometry as G
+    if isinstance(obj, G.MaterializedGeometry):
+        return ["materialized", to_tree(obj.geometry), [to_tree(r) for r, _m, _id in obj._material_regions]]
+    if isinstance(obj, G.HelmholtzResonator):
+        return ["helmholtz", _v(obj.position), obj.cavity_shape, [float(s) for s in obj.cavity_size],
+                float(obj.neck_length), float(obj.neck_radius), _v(obj.neck_direction)]
+    if isinstance(obj, G.Box):
+        return ["box", _v(obj.center), _v(obj.size)]
+    if isinstance(obj, G.Sphere):
+        return ["sphere", _v(obj.center), float(obj.radius)]
+    if isinstance(obj, G.Cylinder):
+        return ["cylinder", _v(obj.p1), _v(obj.p2), float(obj.radius)]
+    if isinstance(obj, G.Union):
+        return ["union", [to_tree(c) for c in obj.children]]
+    if isinstance(obj, G.Intersection):
+        return ["intersection", [to_tree(c) for c in obj.children]]
+    if isinstance(obj, G.Difference):
+        return ["difference", to_tree(obj.base), [to_tree(c) for c in obj.subtracted]]
+    raise TypeError(f"no tree form for {type(obj).__name__}")
+
+
+def build(tree, ns: dict):
+    """Instantiate a tree with the classes in ``ns`` (Box, Sphere, Cylinder,
+    Union, Intersection, Difference, HelmholtzResonator, MaterializedGeometry;
+    ``material`` is the object handed to set_material)."""
+    kind = tree[0]
+    if kind == "box":
+        return ns["Box"](center=tuple(tree[1]), size=tuple(tree[2]))
+    if kind == "sphere":
+        return ns["Sphere"](center=tuple(tree[1]), radius=tree[2])
+    if kind == "cylinder":
+        return ns["Cylinder"](p1=tuple(tree[1]), p2=tuple(tree[2]), radius=tree[3])
+    if kind == "union":
+        return ns["Union"](*[build(c, ns) for c in tree[1]])
+    if kind == "intersection":
+        return ns["Intersection"](*[build(c, ns) for c in tree[1]])
+    if kind == "difference":
+        return ns["Difference"](build(tree[1], ns), *[build(c, ns) for c in tree[2]])
+    if kind == "helmholtz":
+        return ns["HelmholtzResonator"](position=tuple(tree[1]), cavity_shape=tree[2],
+                                        cavity_size=tuple(tree[3]), neck_length=tree[4],
+                                        neck_radius=tree[5], neck_direction=tuple(tree[6]))
+    if kind == "materialized":
+        geo = ns["MaterializedGeometry"](build(tree[1], ns))
+        for r in tree[2]:
+            geo.set_material(build(r, ns), ns["material"])
+        return geo
+    raise ValueError(f"unknown node {kind!r}")
+
+
+def dumps(tree) -> str:
+    return json.dumps(tree)
+
+
+def loads(text: str):
+    return json.loads(text)

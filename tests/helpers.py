@@ -92,8 +92,7 @@ def oracle_problem(sc: Scenario, n_steps: int) -> cpu.Problem:
         for pl in plist:
             tgt = 0 if pl["target"] == "density" else 1
             if tgt == 0:
-                corr = (T.native_density_correction(dt, rho_m, sc.dx) if not sc.nonuniform
-                        else float(np.float32(-dt / rho_m)))
+                corr = T.native_density_correction(dt, rho_m, min_dx)
             else:
                 corr = T.native_modulus_correction(dt, rho_m * c_m**2)
             if pl["kind"] == "debye":
@@ -107,7 +106,7 @@ def oracle_problem(sc: Scenario, n_steps: int) -> cpu.Problem:
         shape=sc.shape, cv=cv, cp=cp, air=air, decay=tuple(decay), nonuniform=sc.nonuniform,
         inv_face=inv_face, inv_cell=inv_cell, sources=srcs, probes=probes, poles=poles,
         material=None if sc.material is None else sc.material.astype(np.uint8),
-        ade_inv_dx=1.0 / sc.dx, rho_inf=rho_inf, K_inf=K_inf, dt=dt)
+        ade_inv_dx=1.0 / min_dx, rho_inf=rho_inf, K_inf=K_inf, dt=dt)
 
 
 def make_solver(sc: Scenario, tiling=None):

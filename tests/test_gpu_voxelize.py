@@ -1,10 +1,11 @@
-"""Device voxeliser vs the host voxeliser, bit for bit.
+"""Device voxeliser vs the reference's masks and the host voxeliser, bit for bit.
 
-The host voxeliser is pinned to the reference's direct ``sdf <= 0`` evaluation
-(tests/test_host.py); here the HIP kernel's air masks and material ids must
-equal it exactly: random CSG trees on uniform and nonuniform grids (surface
-ties included), material regions, and the full-size scenes of the 512^3
-enclosure and the 320x320x480 nonuniform resonator plate.
+The masks and material ids the reference's own sdf.py wrote
+(tests/golden/voxel_masks.npz, also checked against the host voxeliser in
+tests/test_voxel_golden.py) must come out of the HIP kernel exactly. Beyond
+those: random CSG trees on uniform and nonuniform grids (surface ties
+included), material regions, and the full-size scenes of the 512^3 enclosure
+and the 320x320x480 nonuniform resonator plate, against the host voxeliser.
 """
 
 import numpy as np
@@ -108,3 +109,18 @@ def test_solver_uses_device_voxeliser_by_default(monkeypatch):
     assert len(calls) == 1
     assert np.array_equal(a.geometry, b.geometry)
     assert np.array_equal(a._material_id, b._material_id)
+
+
+def test_device_voxeliser_equals_reference_masks():
+    """Every scene of tests/golden/voxel_masks.npz (masks and material ids
+    written by the reference's own sdf.py / material_assignment.py): the
+    device voxeliser reproduces them bit for bit."""
+    from test_voxel_golden import CASES, product_grid, product_namespace, scene_tree
+    ns = product_namespace()
+    print(f"[voxelize] {len(CASES)} reference-mask cases", flush=True)
+    for name, case in sorted(CASES.items()):
+        obj = scene_tree.build(case["tree"], ns)
+        dmask, dids = _device(obj, product_grid(case))
+        assert np.array_equal(dmask, case["mask"]), (name, int((dmask != case["mask"]).sum()))
+        if "ids" in case:
+            assert np.array_equal(dids, case["ids"]), name

@@ -23,7 +23,9 @@ from helpers import Scenario, oracle_problem, rel_l2, seeded_scenario
 from oracle import cpu, tables as T
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-GOLDEN = sorted(glob.glob(os.path.join(HERE, "golden", "*.npz")))
+# step fixtures (the voxel-mask fixture is read by tests/test_voxel_golden.py)
+GOLDEN = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "*.npz"))
+                if not os.path.basename(p).startswith("voxel_"))
 FIELDS = ("p", "vx", "vy", "vz")
 
 
@@ -54,6 +56,7 @@ needs_ref = pytest.mark.skipif(not cpu.have_ref(), reason="oracle/_ref not built
     (102, dict(geometry=True, pml=4, ade=True)),
     (103, dict(geometry=True, pml=4, nonuniform=True)),
     (104, dict(geometry=False, pml=0)),
+    (105, dict(geometry=True, pml=4, ade=True, nonuniform=True)),
 ])
 def test_oracle_equals_reference_build(oracle_lib, seed, kw):
     sc = seeded_scenario(seed, shape=(22, 19, 32), **kw)
