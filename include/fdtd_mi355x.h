@@ -31,10 +31,15 @@ extern "C" {
 #define FDTD_FLAG_OPEN_Z 0x08u
 #define FDTD_FLAG_ADE    0x10u  /* cell carries a material with poles          */
 
+/* Limits. Sources: the point and x/y-plane sources the sweep adds itself per
+ * context; the host routes any further source to the per-cell tables
+ * (fdtd_ctx_set_membranes), so a solve's source list is unbounded. Probes:
+ * the first 64 are recorded inside the sweep, the rest by a small kernel after
+ * the step. Poles: read from a device table. */
 #define FDTD_MAX_SOURCES 8
-#define FDTD_MAX_PROBES  64
+#define FDTD_MAX_PROBES  (1 << 20)
 #define FDTD_MAX_MICS 4096
-#define FDTD_MAX_POLES   8
+#define FDTD_MAX_POLES   1024
 
 typedef struct fdtd_ctx fdtd_ctx;
 
@@ -242,8 +247,9 @@ int fdtd_adjoint_run(int dtype, void* const* lam, void* const* states, int64_t n
  * tables. Grids below 2^30 cells; the design box must keep one cell to every
  * grid face. rk: device float32 [hx][hy][hz][2], (rho/rho0, K/K0) per box
  * voxel, or NULL (air). src_ijk: host int32[3] or NULL; wave: device float64
- * [steps]; probe_ijk: device int32 [n_probe][3]; trace and g: device float32
- * [steps][n_probe].
+ * [steps_len]; probe_ijk: device int32 [n_probe][3]; trace and g: device
+ * float32 [steps_len][n_probe], indexed by absolute step (calls that would
+ * touch a row at or past steps_len return -4).
  * History (device float32): p on the box grown by one cell
  * ((hx+2)(hy+2)(hz+2) per step) and div on the box (hx hy hz per step) for
  * steps [hist0, hist0 + hist_steps); fdtd_design_history_bytes gives the size.
@@ -258,13 +264,13 @@ int64_t fdtd_design_history_bytes(const int32_t* box_lo, const int32_t* box_hi, 
 int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
                         float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
                         const int32_t* src_ijk, const double* wave, int n_probe, const int32_t* probe_ijk,
-                        float* trace, int64_t step0, int n_steps, float* hist_p, float* hist_div, int64_t hist0,
-                        int64_t hist_steps, int planes, void* stream);
+                        float* trace, int64_t steps_len, int64_t step0, int n_steps, float* hist_p, float* hist_div,
+                        int64_t hist0, int64_t hist_steps, int planes, void* stream);
 int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
                         float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
-                        int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo, int64_t n_hi,
-                        const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps, float* acc,
-                        float* face_e, int planes, void* stream);
+                        int n_probe, const int32_t* probe_ijk, const float* g, int64_t steps_len, int64_t n_lo,
+                        int64_t n_hi, const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps,
+                        float* acc, float* face_e, int planes, void* stream);
 
 /* ---- per-phase entry points (one kernel per reference function) --------- */
 /* All in place, on `stream`, float32, uniform unless noted. */
@@ -297,6 +303,45 @@ int fdtd_apply_pml_pressure(float* p, const float* dx, const float* dy, const fl
 int fdtd_compute_divergence(float* div, const float* vx, const float* vy, const float* vz, int nx,
                             int ny, int nz, float inv_dx, const float* icx, const float* icy,
                             const float* icz, void* stream);
+/* ---- per-phase ADE entry points ------------------------------------------
+ * One pole of ADEMaterialData (_kernels/include/ade.hpp:44-111): lorentz 0
+ * puts it in the Debye list, 1 in the Lorentz list; field_index is its row in
+ * the J array of that list (rows of nx*ny*nz floats, J_prev alike); a, b, d
+ * are the Debye (alpha, beta, -) or Lorentz (a, b, d) coefficients. poles is a
+ * host array; each call visits the poles of its list and target in the order
+ * given, as the reference iterates its vectors, one device pass per pole.
+ * material: device uint8 ids per cell. rho_inf / K_inf: host float[256] by
+ * material id. In place on `stream`, float32. The fused step applies the same
+ * updates in its own launches; these exist for phase-by-phase bisection. */
+typedef struct fdtd_ade_pole {
+    int32_t material_id, target, lorentz, field_index;
+    float a, b, d, reserved;
+} fdtd_ade_pole;
+/* Replaces fdtd::update_ade_density_debye (ade.cpp:25-68; kernels.cpp:670-695). */
+int fdtd_update_ade_density_debye(float* J_debye, const float* p, const uint8_t* material,
+                                  const fdtd_ade_pole* poles, int n_poles, int nx, int ny, int nz, void* stream);
+/* Replaces fdtd::update_ade_modulus_debye (ade.cpp:70-112; kernels.cpp:697-720). */
+int fdtd_update_ade_modulus_debye(float* J_debye, const float* div, const uint8_t* material,
+                                  const fdtd_ade_pole* poles, int n_poles, int nx, int ny, int nz, void* stream);
+/* Replaces fdtd::update_ade_density_lorentz (ade.cpp:118-170; kernels.cpp:722-748). */
+int fdtd_update_ade_density_lorentz(float* J_lorentz, float* J_lorentz_prev, const float* p, const uint8_t* material,
+                                    const fdtd_ade_pole* poles, int n_poles, int nx, int ny, int nz, void* stream);
+/* Replaces fdtd::update_ade_modulus_lorentz (ade.cpp:172-222; kernels.cpp:750-776). */
+int fdtd_update_ade_modulus_lorentz(float* J_lorentz, float* J_lorentz_prev, const float* div,
+                                    const uint8_t* material, const fdtd_ade_pole* poles, int n_poles, int nx, int ny,
+                                    int nz, void* stream);
+/* Replaces fdtd::apply_ade_velocity_correction (ade.cpp:228-401; kernels.cpp:778-826):
+ * Debye then Lorentz density poles, coeff = -dt / rho_inf[m] * inv_dx in float. */
+int fdtd_apply_ade_velocity_correction(float* vx, float* vy, float* vz, const float* J_debye, const float* J_lorentz,
+                                       const uint8_t* material, const fdtd_ade_pole* poles, int n_poles,
+                                       const float* rho_inf, float dt, float inv_dx, int nx, int ny, int nz,
+                                       void* stream);
+/* Replaces fdtd::apply_ade_pressure_correction (ade.cpp:403-473; kernels.cpp:828-858):
+ * Debye then Lorentz modulus poles, coeff = -K_inf[m] * dt in float. */
+int fdtd_apply_ade_pressure_correction(float* p, const float* J_debye, const float* J_lorentz, const uint8_t* material,
+                                       const fdtd_ade_pole* poles, int n_poles, const float* K_inf, float dt, int nx,
+                                       int ny, int nz, void* stream);
+
 /* Geometry mask (uint8 0/1) + material ids (nullable) -> cell flags.
  * Replaces precompute_boundary_cells (boundaries.cpp:13-64). */
 int fdtd_build_flags(const uint8_t* air, const uint8_t* material, uint8_t* flags, int nx, int ny,
@@ -319,6 +364,7 @@ int fdtd_voxelize(const int32_t* ops, int n_ops, const int32_t* prog_start, cons
                   int n_prog, const double* params, int n_params, void* dev_scratch, int64_t scratch_bytes,
                   const double* xc, const double* yc, const double* zc, int nx, int ny, int nz,
                   uint8_t* air, uint8_t* ids, void* stream);
+
 
 #ifdef __cplusplus
 }

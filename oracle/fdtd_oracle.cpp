@@ -263,6 +263,96 @@ void orc_divergence_f32(float* div, const float* vx, const float* vy, const floa
     divergence<float>(div, vx, vy, vz, nx, ny, nz, inv_u, icx, icy, icz);
 }
 
+// ---- ADE phases (float), restating ade.cpp:25-473 -----------------------------
+// Pole list: {material_id, target, lorentz, field_index, a, b, d, reserved};
+// J arrays hold one full grid per field_index. Each call visits the poles of
+// its list (Debye / Lorentz) and target in list order, one pass per pole.
+struct OrcAdePole {
+    int32_t material_id, target, lorentz, field_index;
+    float a, b, d, reserved;
+};
+
+static void orc_ade_update(float* J, float* Jp, const float* src, const uint8_t* m, const OrcAdePole* poles, int n,
+                           int lorentz, int target, int64_t cells) {
+    for (int q = 0; q < n; ++q) {
+        const OrcAdePole& pl = poles[q];
+        if (pl.lorentz != lorentz || pl.target != target) continue;
+        float* Jq = J + pl.field_index * cells;
+        float* Jpq = lorentz ? Jp + pl.field_index * cells : nullptr;
+        const uint8_t mid = static_cast<uint8_t>(pl.material_id);
+        for (int64_t c = 0; c < cells; ++c) {
+            if (m[c] != mid) continue;
+            if (!lorentz) {
+                Jq[c] = pl.a * Jq[c] + pl.b * src[c];
+            } else {
+                const float j0 = Jq[c], j1 = Jpq[c];
+                const float jn = pl.a * j0 + pl.b * j1 + pl.d * src[c];
+                Jpq[c] = j0;
+                Jq[c] = jn;
+            }
+        }
+    }
+}
+
+void orc_update_ade_density_debye(float* J, const float* p, const uint8_t* m, const OrcAdePole* poles, int n, int nx,
+                                  int ny, int nz) {
+    orc_ade_update(J, nullptr, p, m, poles, n, 0, 0, static_cast<int64_t>(nx) * ny * nz);
+}
+void orc_update_ade_modulus_debye(float* J, const float* div, const uint8_t* m, const OrcAdePole* poles, int n, int nx,
+                                  int ny, int nz) {
+    orc_ade_update(J, nullptr, div, m, poles, n, 0, 1, static_cast<int64_t>(nx) * ny * nz);
+}
+void orc_update_ade_density_lorentz(float* J, float* Jp, const float* p, const uint8_t* m, const OrcAdePole* poles,
+                                    int n, int nx, int ny, int nz) {
+    orc_ade_update(J, Jp, p, m, poles, n, 1, 0, static_cast<int64_t>(nx) * ny * nz);
+}
+void orc_update_ade_modulus_lorentz(float* J, float* Jp, const float* div, const uint8_t* m, const OrcAdePole* poles,
+                                    int n, int nx, int ny, int nz) {
+    orc_ade_update(J, Jp, div, m, poles, n, 1, 1, static_cast<int64_t>(nx) * ny * nz);
+}
+
+// Density poles, Debye list then Lorentz list: coeff = -dt / rho_inf * inv_dx
+// (float), v[face] += coeff * (J[c+e] - J[c]) where both cells hold the material.
+void orc_apply_ade_velocity_correction(float* vx, float* vy, float* vz, const float* Jd, const float* Jl,
+                                       const uint8_t* m, const OrcAdePole* poles, int n, const float* rho_inf,
+                                       float dt, float inv_dx, int nx, int ny, int nz) {
+    const int64_t cells = static_cast<int64_t>(nx) * ny * nz, sy = nz, sx = static_cast<int64_t>(ny) * nz;
+    for (int lorentz = 0; lorentz < 2; ++lorentz)
+        for (int q = 0; q < n; ++q) {
+            const OrcAdePole& pl = poles[q];
+            if (pl.lorentz != lorentz || pl.target != 0) continue;
+            const float* J = (lorentz ? Jl : Jd) + pl.field_index * cells;
+            const uint8_t mid = static_cast<uint8_t>(pl.material_id);
+            const float coeff = -dt / rho_inf[mid] * inv_dx;
+            for (int i = 0; i < nx; ++i)
+                for (int j = 0; j < ny; ++j)
+                    for (int k = 0; k < nz; ++k) {
+                        const int64_t c = lin(i, j, k, ny, nz);
+                        if (m[c] != mid) continue;
+                        if (i < nx - 1 && m[c + sx] == mid) vx[c] += coeff * (J[c + sx] - J[c]);
+                        if (j < ny - 1 && m[c + sy] == mid) vy[c] += coeff * (J[c + sy] - J[c]);
+                        if (k < nz - 1 && m[c + 1] == mid) vz[c] += coeff * (J[c + 1] - J[c]);
+                    }
+        }
+}
+
+// Modulus poles, Debye list then Lorentz list: p += (-K_inf * dt) * J.
+void orc_apply_ade_pressure_correction(float* p, const float* Jd, const float* Jl, const uint8_t* m,
+                                       const OrcAdePole* poles, int n, const float* K_inf, float dt, int nx, int ny,
+                                       int nz) {
+    const int64_t cells = static_cast<int64_t>(nx) * ny * nz;
+    for (int lorentz = 0; lorentz < 2; ++lorentz)
+        for (int q = 0; q < n; ++q) {
+            const OrcAdePole& pl = poles[q];
+            if (pl.lorentz != lorentz || pl.target != 1) continue;
+            const float* J = (lorentz ? Jl : Jd) + pl.field_index * cells;
+            const uint8_t mid = static_cast<uint8_t>(pl.material_id);
+            const float coeff = -K_inf[mid] * dt;
+            for (int64_t c = 0; c < cells; ++c)
+                if (m[c] == mid) p[c] += coeff * J[c];
+        }
+}
+
 // PML decay as the native initialize_pml forms it: std::exp(-sigma * dt) with
 // float sigma and float dt (_kernels/src/pml.cpp:13-45).
 void orc_pml_decay(const float* sigma, int n, float dt, float* out) {

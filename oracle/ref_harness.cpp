@@ -16,6 +16,7 @@
 #include "ade.hpp"
 #include "microphones.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <vector>
@@ -145,6 +146,78 @@ void ref_initialize_pml(const float* sx, const float* sy, const float* sz, int n
 void ref_compute_divergence(float* div, const float* vx, const float* vy, const float* vz, int nx,
                             int ny, int nz, float inv_dx) {
     fdtd::compute_divergence(div, vx, vy, vz, fdtd::GridShape{nx, ny, nz}, inv_dx);
+}
+
+// ---- ADE phases: the reference's own update / correction kernels ------------
+// Pole list layout-identical to fdtd_ade_pole (include/fdtd_mi355x.h).
+struct RefAdePole {
+    int32_t material_id, target, lorentz, field_index;
+    float a, b, d, reserved;
+};
+
+static fdtd::ADEMaterialData ade_from_list(const RefAdePole* poles, int n, const float* rho_inf,
+                                           const float* K_inf) {
+    fdtd::ADEMaterialData d;
+    d.rho_inf.assign(256, 0.f);
+    d.K_inf.assign(256, 0.f);
+    if (rho_inf) d.rho_inf.assign(rho_inf, rho_inf + 256);
+    if (K_inf) d.K_inf.assign(K_inf, K_inf + 256);
+    for (int q = 0; q < n; ++q) {
+        const RefAdePole& p = poles[q];
+        if (!p.lorentz) {
+            fdtd::ADEMaterialData::DebyePole dp;
+            dp.material_id = p.material_id;
+            dp.target = p.target;
+            dp.coeffs.alpha = p.a;
+            dp.coeffs.beta = p.b;
+            dp.field_index = p.field_index;
+            d.debye_poles.push_back(dp);
+            d.n_debye_fields = std::max(d.n_debye_fields, p.field_index + 1);
+        } else {
+            fdtd::ADEMaterialData::LorentzPole lp;
+            lp.material_id = p.material_id;
+            lp.target = p.target;
+            lp.coeffs.a = p.a;
+            lp.coeffs.b = p.b;
+            lp.coeffs.d = p.d;
+            lp.field_index = p.field_index;
+            d.lorentz_poles.push_back(lp);
+            d.n_lorentz_fields = std::max(d.n_lorentz_fields, p.field_index + 1);
+        }
+    }
+    return d;
+}
+
+void ref_update_ade_density_debye(float* J, const float* p, const uint8_t* mat, const RefAdePole* poles, int n,
+                                  int nx, int ny, int nz) {
+    fdtd::update_ade_density_debye(J, p, mat, ade_from_list(poles, n, nullptr, nullptr), fdtd::GridShape{nx, ny, nz});
+}
+void ref_update_ade_modulus_debye(float* J, const float* div, const uint8_t* mat, const RefAdePole* poles, int n,
+                                  int nx, int ny, int nz) {
+    fdtd::update_ade_modulus_debye(J, div, mat, ade_from_list(poles, n, nullptr, nullptr),
+                                   fdtd::GridShape{nx, ny, nz});
+}
+void ref_update_ade_density_lorentz(float* J, float* Jp, const float* p, const uint8_t* mat, const RefAdePole* poles,
+                                    int n, int nx, int ny, int nz) {
+    fdtd::update_ade_density_lorentz(J, Jp, p, mat, ade_from_list(poles, n, nullptr, nullptr),
+                                     fdtd::GridShape{nx, ny, nz});
+}
+void ref_update_ade_modulus_lorentz(float* J, float* Jp, const float* div, const uint8_t* mat,
+                                    const RefAdePole* poles, int n, int nx, int ny, int nz) {
+    fdtd::update_ade_modulus_lorentz(J, Jp, div, mat, ade_from_list(poles, n, nullptr, nullptr),
+                                     fdtd::GridShape{nx, ny, nz});
+}
+void ref_apply_ade_velocity_correction(float* vx, float* vy, float* vz, const float* Jd, const float* Jl,
+                                       const uint8_t* mat, const RefAdePole* poles, int n, const float* rho_inf,
+                                       float dt, float inv_dx, int nx, int ny, int nz) {
+    fdtd::apply_ade_velocity_correction(vx, vy, vz, Jd, Jl, mat, ade_from_list(poles, n, rho_inf, nullptr),
+                                        fdtd::GridShape{nx, ny, nz}, dt, inv_dx);
+}
+void ref_apply_ade_pressure_correction(float* p, const float* Jd, const float* Jl, const uint8_t* mat,
+                                       const RefAdePole* poles, int n, const float* K_inf, float dt, int nx, int ny,
+                                       int nz) {
+    fdtd::apply_ade_pressure_correction(p, Jd, Jl, mat, ade_from_list(poles, n, nullptr, K_inf),
+                                        fdtd::GridShape{nx, ny, nz}, dt);
 }
 
 // ---- full run in the python step ordering --------------------------------

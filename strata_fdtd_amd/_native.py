@@ -55,8 +55,8 @@ _SIGS = {
                          + [C.c_double, C.c_double, _i] + [_vp] * 7),
     "fdtd_design_history_bytes": (_i64, [_vp, _vp, _i64]),
     "fdtd_design_forward": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp,
-                                 _i64, _i, _vp, _vp, _i64, _i64, _i, _vp]),
-    "fdtd_design_adjoint": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _i, _vp, _vp, _i64, _i64,
+                                 _i64, _i64, _i, _vp, _vp, _i64, _i64, _i, _vp]),
+    "fdtd_design_adjoint": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _i, _vp, _vp, _i64, _i64, _i64,
                                  _vp, _vp, _i64, _i64, _vp, _vp, _i, _vp]),
     "fdtd_design_face_count": (_i64, [_vp, _vp]),
     "fdtd_update_velocity": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
@@ -68,6 +68,13 @@ _SIGS = {
     "fdtd_apply_pml_pressure": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]),
     "fdtd_compute_divergence": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),
     "fdtd_build_flags": (_i, [_vp, _vp, _vp, _i, _i, _i, _vp]),
+    "fdtd_update_ade_density_debye": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "fdtd_update_ade_modulus_debye": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "fdtd_update_ade_density_lorentz": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "fdtd_update_ade_modulus_lorentz": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _i, _i, _i, _vp]),
+    "fdtd_apply_ade_velocity_correction": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _f, _f, _i, _i, _i,
+                                                _vp]),
+    "fdtd_apply_ade_pressure_correction": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp, _f, _i, _i, _i, _vp]),
     "fdtd_voxelize_scratch_bytes": (_i64, [_i, _i, _i]),
     "fdtd_voxelize": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _i64, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
 }

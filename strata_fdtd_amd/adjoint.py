@@ -226,9 +226,11 @@ class _Fast:
     def forward(self, sets, rk, trace, step0, n_steps, hist=None, hist0=0):
         hp, hd = hist if hist is not None else (None, None)
         hist_steps = (hp.numel() // self.ecells) if hp is not None else 0
+        rows = trace.shape[0] if self.dv.n_probe > 0 else self.wave_dev.numel()
         _native.call("fdtd_design_forward", self._sets(sets), *self._common(rk), self.src,
-                     _ptr(self.wave_dev), self.dv.n_probe, _ptr(self.probe_ijk), _ptr(trace), step0, n_steps,
-                     _ptr(hp), _ptr(hd), hist0, hist_steps, self.PLANES, self.dv.stream)
+                     _ptr(self.wave_dev), self.dv.n_probe, _ptr(self.probe_ijk), _ptr(trace),
+                     min(rows, self.wave_dev.numel()), step0, n_steps, _ptr(hp), _ptr(hd), hist0, hist_steps,
+                     self.PLANES, self.dv.stream)
 
     def adjoint(self, lam, rk, g, n_lo, n_hi, hist, hist0, acc):
         """acc: float32 [box][4], Kahan (sum, compensation) of dL/dr, then dL/dkappa."""
@@ -238,7 +240,7 @@ class _Fast:
             n = _native.load().fdtd_design_face_count(self.dv.lo.ctypes.data, self.dv.hi.ctypes.data)
             self._faces = torch.empty(n, dtype=torch.float32, device=self.dv.dev)
         _native.call("fdtd_design_adjoint", self._sets(lam), *self._common(rk), self.dv.n_probe,
-                     _ptr(self.probe_ijk), _ptr(g), n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
+                     _ptr(self.probe_ijk), _ptr(g), g.shape[0], n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
                      hp.numel() // self.ecells, _ptr(acc), _ptr(self._faces), self.ADJ_PLANES, self.dv.stream)
 
 

@@ -33,6 +33,12 @@ hipError_t phase_pml_pressure(float*, const float*, const float*, const float*, 
 hipError_t phase_divergence(float*, const float*, const float*, const float*, int, int, int, float,
                             const float*, const float*, const float*, hipStream_t);
 hipError_t phase_build_flags(const uint8_t*, const uint8_t*, uint8_t*, int, int, int, hipStream_t);
+hipError_t phase_ade_update(float* J, float* Jp, const float* src, const uint8_t* mat, int mat_id, int lorentz,
+                            float a, float b, float d, int64_t n, hipStream_t st);
+hipError_t phase_ade_vcorr(float* vx, float* vy, float* vz, const float* J, const uint8_t* mat, int mat_id,
+                           float coeff, int nx, int ny, int nz, hipStream_t st);
+hipError_t phase_ade_pcorr(float* p, const float* J, const uint8_t* mat, int mat_id, float coeff, int64_t n,
+                           hipStream_t st);
 struct VoxProgram {
     const int32_t* ops;
     const int32_t* start;
@@ -87,7 +93,6 @@ struct fdtd_ctx {
     bool have_grid = false, have_fields = false, have_flags = false, have_pml = false;
     bool have_counters = false;
     int rows = 4, planes = 8, stack = 1;
-    int32_t* probe_dev = nullptr;     // owned: [n_probe][3]
     int32_t* mic_idx_dev = nullptr;   // owned: [mic_cap][4][8]
     double* mic_w_dev = nullptr;      // owned: [mic_cap][4][8]
     int32_t* mic_f64_dev = nullptr;   // owned: [mic_cap][4]
@@ -96,7 +101,7 @@ struct fdtd_ctx {
     struct DevBuf {
         void* p = nullptr;
         size_t bytes = 0;
-    } m_cell, m_estart, m_esrc, m_efield, m_eweight, m_pstart, m_pidx, ade_cells;
+    } m_cell, m_estart, m_esrc, m_efield, m_eweight, m_pstart, m_pidx, ade_cells, ade_poles, probes;
     int64_t* step_counter = nullptr;  // same as a.step_ptr, writable
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -135,12 +140,11 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (!c) return 0;
     DeviceGuard g(c->device);
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
-    if (c->probe_dev) (void)hipFree(c->probe_dev);
     if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
     if (c->mic_w_dev) (void)hipFree(c->mic_w_dev);
     if (c->mic_f64_dev) (void)hipFree(c->mic_f64_dev);
     for (fdtd_ctx::DevBuf* b : {&c->ade_cells, &c->m_cell, &c->m_estart, &c->m_esrc, &c->m_efield, &c->m_eweight, &c->m_pstart,
-                                &c->m_pidx})
+                                &c->m_pidx, &c->ade_poles, &c->probes})
         if (b->p) (void)hipFree(b->p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -242,13 +246,15 @@ int fdtd_ctx_set_sources(fdtd_ctx* c, int n, const int32_t* kind, const int64_t*
     return 0;
 }
 
+static int upload(fdtd_ctx::DevBuf& b, const void* host, size_t bytes);
+
 int fdtd_ctx_set_probes(fdtd_ctx* c, int n, const int64_t* idx, float* trace, int64_t trace_len) {
     if (!c) return fail(-1, "null context");
     if (!c->have_grid) return fail(-3, "set the grid before the probes");
     if (n < 0 || n > FDTD_MAX_PROBES) return fail(-2, "%d probes (max %d)", n, FDTD_MAX_PROBES);
     if (n > 0 && (!idx || !trace || trace_len < 1)) return fail(-2, "bad probe arrays");
     const int nx = c->a.nx, ny = c->a.ny, nz = c->a.nz;
-    int32_t host[3 * FDTD_MAX_PROBES];
+    std::vector<int32_t> host(3 * static_cast<size_t>(std::max(n, 1)), 0);
     for (int q = 0; q < n; ++q) {
         const int64_t f = idx[q];
         if (f < 0 || f >= static_cast<int64_t>(nx) * ny * nz)
@@ -259,18 +265,12 @@ int fdtd_ctx_set_probes(fdtd_ctx* c, int n, const int64_t* idx, float* trace, in
     }
     DeviceGuard g(c->device);
     drop_graph(c);
-    if (!c->probe_dev) {
-        // set-up allocation (never on a step path): the probe coordinate table
-        if (int rc = check(hipMalloc(&c->probe_dev, sizeof(int32_t) * 3 * FDTD_MAX_PROBES), "hipMalloc"))
-            return rc;
-    }
-    if (n > 0) {
-        if (int rc = check(hipMemcpy(c->probe_dev, host, sizeof(int32_t) * 3 * n, hipMemcpyHostToDevice),
-                           "hipMemcpy"))
-            return rc;
-    }
+    // steps already enqueued on the context stream still read the old coordinates
+    if (int rc = check(hipStreamSynchronize(c->stream), "stream sync")) return rc;
+    if (int rc = upload(c->probes, host.data(), sizeof(int32_t) * host.size())) return rc;
     c->a.n_probe = n;
-    c->a.probe_ijk = c->probe_dev;
+    c->a.n_sweep_probe = std::min(n, kSweepProbes);
+    c->a.probe_ijk = static_cast<const int32_t*>(c->probes.p);
     c->a.trace = trace;
     c->a.trace_len = n > 0 ? trace_len : 0;
     return 0;
@@ -288,6 +288,8 @@ int fdtd_ctx_set_microphones(fdtd_ctx* c, int n, const int32_t* idx, const doubl
             return fail(-2, "microphone %lld: flat index %d outside the grid", (long long)(q / 32), idx[q]);
     DeviceGuard g(c->device);
     drop_graph(c);
+    // steps already enqueued on the context stream still read the old tables
+    if (int rc = check(hipStreamSynchronize(c->stream), "stream sync")) return rc;
     if (n > c->mic_cap) {
         // set-up allocation (never on a step path): the gather tables
         if (c->mic_idx_dev) (void)hipFree(c->mic_idx_dev);
@@ -413,55 +415,57 @@ int fdtd_ctx_set_ade(fdtd_ctx* c, int n_pole, const int32_t* mat_id, const int32
     if (!c) return fail(-1, "null context");
     if (n_pole < 0 || n_pole > FDTD_MAX_POLES) return fail(-2, "%d poles (max %d)", n_pole, FDTD_MAX_POLES);
     drop_graph(c);
-    c->a.n_pole = n_pole;
+    // Everything is checked and built in `a` first; the context changes only
+    // when the call succeeds, and a failed call leaves it without poles (never
+    // with a pole count over stale tables).
+    c->a.n_pole = 0;
     if (n_pole == 0) return 0;
     if (!mat_id || !target || !lorentz || !pa || !pb || !pd || !corr || !material || !lo || !hi || !J)
         return fail(-2, "bad ADE arrays");
-    const int dims[3] = {c->a.nx, c->a.ny, c->a.nz};
+    if (!c->have_grid) return fail(-3, "set the grid before the ADE materials");
+    FusedArgs a = c->a;
+    const int dims[3] = {a.nx, a.ny, a.nz};
     for (int ax = 0; ax < 3; ++ax) {
         if (lo[ax] < 0 || hi[ax] > dims[ax] || lo[ax] >= hi[ax])
             return fail(-2, "ADE box axis %d [%d,%d) outside the grid", ax, lo[ax], hi[ax]);
-        c->a.box_lo[ax] = lo[ax];
-        c->a.box_hi[ax] = hi[ax];
+        a.box_lo[ax] = lo[ax];
+        a.box_hi[ax] = hi[ax];
     }
+    std::vector<AdePole> poles(n_pole);
     for (int q = 0; q < n_pole; ++q) {
         if (mat_id[q] < 1 || mat_id[q] > 255) return fail(-2, "pole %d: material id %d", q, mat_id[q]);
+        if (target[q] != 0 && target[q] != 1) return fail(-2, "pole %d: target %d", q, target[q]);
+        if (lorentz[q] != 0 && lorentz[q] != 1) return fail(-2, "pole %d: kind %d", q, lorentz[q]);
         if (lorentz[q] && !Jprev) return fail(-2, "Lorentz pole without a history array");
-        c->a.pole_mat[q] = mat_id[q];
-        c->a.pole_target[q] = target[q];
-        c->a.pole_lorentz[q] = lorentz[q];
-        c->a.pole_a[q] = pa[q];
-        c->a.pole_b[q] = pb[q];
-        c->a.pole_d[q] = pd[q];
-        c->a.pole_corr[q] = corr[q];
+        poles[q] = AdePole{mat_id[q], target[q], lorentz[q], 0, pa[q], pb[q], pd[q], corr[q]};
     }
-    c->a.material = material;
-    c->a.J = J;
-    c->a.Jprev = Jprev;
-    c->a.ade_inv_dx = inv_dx;
+    a.material = material;
+    a.J = J;
+    a.Jprev = Jprev;
+    a.ade_inv_dx = inv_dx;
     // The box cells whose material has poles, as box indices in box order: the
     // ADE kernels then run one thread per such cell rather than per box cell
     // (a sphere fills about half its box). Read from the device material ids
     // one box plane at a time, after the work already on the stream.
-    c->a.ade_cells = nullptr;
-    c->a.n_ade_cells = 0;
+    a.ade_cells = nullptr;
+    a.n_ade_cells = 0;
     const int bx = hi[0] - lo[0], by = hi[1] - lo[1], bz = hi[2] - lo[2];
     const int64_t box_n = static_cast<int64_t>(bx) * by * bz;
+    std::vector<int32_t> list;
+    DeviceGuard g(c->device);
+    // the material ids may come from any stream; steps in flight read the old tables
+    if (int rc = check(hipDeviceSynchronize(), "device sync")) return rc;
     if (box_n < INT32_MAX) {
-        DeviceGuard g(c->device);
-        // the material ids may come from any stream; steps in flight may read the old list
-        if (int rc = check(hipDeviceSynchronize(), "device sync")) return rc;
         bool has_pole[256] = {};
         for (int q = 0; q < n_pole; ++q) has_pole[mat_id[q]] = true;
-        const size_t plane = static_cast<size_t>(by) * c->a.nz;
+        const size_t plane = static_cast<size_t>(by) * a.nz;
         std::vector<uint8_t> mat(plane);
-        std::vector<int32_t> list;
         for (int i = lo[0]; i < hi[0]; ++i) {
-            const uint8_t* src = material + (static_cast<int64_t>(i) * c->a.ny + lo[1]) * c->a.nz;
+            const uint8_t* src = material + (static_cast<int64_t>(i) * a.ny + lo[1]) * a.nz;
             if (int rc = check(hipMemcpy(mat.data(), src, plane, hipMemcpyDeviceToHost), "hipMemcpy")) return rc;
             for (int j = 0; j < by; ++j)
                 for (int k = lo[2]; k < hi[2]; ++k) {
-                    const uint8_t m = mat[static_cast<size_t>(j) * c->a.nz + k];
+                    const uint8_t m = mat[static_cast<size_t>(j) * a.nz + k];
                     if (m && has_pole[m])
                         list.push_back(static_cast<int32_t>(((static_cast<int64_t>(i - lo[0]) * by) + j) * bz +
                                                             (k - lo[2])));
@@ -471,9 +475,13 @@ int fdtd_ctx_set_ade(fdtd_ctx* c, int n_pole, const int32_t* mat_id, const int32
         if (int rc = upload(c->ade_cells, list.empty() ? &none : list.data(),
                             sizeof(int32_t) * std::max<size_t>(list.size(), 1)))
             return rc;
-        c->a.ade_cells = static_cast<const int32_t*>(c->ade_cells.p);
-        c->a.n_ade_cells = static_cast<int64_t>(list.size());
+        a.ade_cells = static_cast<const int32_t*>(c->ade_cells.p);
+        a.n_ade_cells = static_cast<int64_t>(list.size());
     }
+    if (int rc = upload(c->ade_poles, poles.data(), sizeof(AdePole) * poles.size())) return rc;
+    a.poles = static_cast<const AdePole*>(c->ade_poles.p);
+    a.n_pole = n_pole;
+    c->a = a;
     return 0;
 }
 
@@ -835,6 +843,9 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
     for (int q = 0; q < n; ++q) {
         fdtd_ctx* c = ctxs[q];
         if (int rc = ready(c)) return rc;
+        if (c->a.n_probe > kSweepProbes)
+            return fail(-2, "batch entry %d: %d probes (batched solves record at most %d)", q, c->a.n_probe,
+                        kSweepProbes);
         if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0 || c->a.n_mcell > 0)
             return fail(-2, "batch entry %d: nonuniform, ADE, microphone or membrane-source solves run on their "
                             "own context", q);
@@ -941,6 +952,109 @@ int fdtd_batch_info(fdtd_batch* b, int* n, int* total_waves) {
     if (!b) return fail(-1, "null batch");
     if (n) *n = b->n;
     if (total_waves) *total_waves = b->total_waves;
+    return 0;
+}
+
+// ---- per-phase ADE entry points -------------------------------------------------
+static int ade_poles_ok(const char* who, const fdtd_ade_pole* poles, int n, int nx, int ny, int nz) {
+    if (nx < 1 || ny < 1 || nz < 1) return fail(-2, "%s: bad dimensions", who);
+    if (static_cast<int64_t>(nx) * ny * nz >= INT32_MAX) return fail(-2, "%s: grid too large", who);
+    if (n < 0 || (n > 0 && !poles)) return fail(-2, "%s: bad pole list", who);
+    for (int q = 0; q < n; ++q) {
+        const fdtd_ade_pole& pl = poles[q];
+        if (pl.material_id < 0 || pl.material_id > 255 || (pl.target != 0 && pl.target != 1) ||
+            (pl.lorentz != 0 && pl.lorentz != 1) || pl.field_index < 0)
+            return fail(-2, "%s: pole %d is malformed", who, q);
+    }
+    return 0;
+}
+
+// Poles of one list (lorentz) and target, in the order given: one pass each.
+static int ade_update(const char* who, float* J, float* Jp, const float* src, const uint8_t* mat,
+                      const fdtd_ade_pole* poles, int n, int lorentz, int target, int nx, int ny, int nz,
+                      void* stream) {
+    if (int rc = ade_poles_ok(who, poles, n, nx, ny, nz)) return rc;
+    if (!J || !src || !mat || (lorentz && !Jp)) return fail(-1, "%s: null device pointer", who);
+    const int64_t cells = static_cast<int64_t>(nx) * ny * nz;
+    for (int q = 0; q < n; ++q) {
+        const fdtd_ade_pole& pl = poles[q];
+        if (pl.lorentz != lorentz || pl.target != target) continue;
+        const int64_t off = pl.field_index * cells;
+        if (int rc = check(phase_ade_update(J + off, lorentz ? Jp + off : nullptr, src, mat, pl.material_id, lorentz,
+                                            pl.a, pl.b, pl.d, cells, static_cast<hipStream_t>(stream)),
+                           who))
+            return rc;
+    }
+    return 0;
+}
+
+int fdtd_update_ade_density_debye(float* J_debye, const float* p, const uint8_t* material, const fdtd_ade_pole* poles,
+                                  int n_poles, int nx, int ny, int nz, void* stream) {
+    return ade_update("update_ade_density_debye", J_debye, nullptr, p, material, poles, n_poles, 0, 0, nx, ny, nz,
+                      stream);
+}
+
+int fdtd_update_ade_modulus_debye(float* J_debye, const float* div, const uint8_t* material, const fdtd_ade_pole* poles,
+                                  int n_poles, int nx, int ny, int nz, void* stream) {
+    return ade_update("update_ade_modulus_debye", J_debye, nullptr, div, material, poles, n_poles, 0, 1, nx, ny, nz,
+                      stream);
+}
+
+int fdtd_update_ade_density_lorentz(float* J_lorentz, float* J_lorentz_prev, const float* p, const uint8_t* material,
+                                    const fdtd_ade_pole* poles, int n_poles, int nx, int ny, int nz, void* stream) {
+    return ade_update("update_ade_density_lorentz", J_lorentz, J_lorentz_prev, p, material, poles, n_poles, 1, 0, nx,
+                      ny, nz, stream);
+}
+
+int fdtd_update_ade_modulus_lorentz(float* J_lorentz, float* J_lorentz_prev, const float* div, const uint8_t* material,
+                                    const fdtd_ade_pole* poles, int n_poles, int nx, int ny, int nz, void* stream) {
+    return ade_update("update_ade_modulus_lorentz", J_lorentz, J_lorentz_prev, div, material, poles, n_poles, 1, 1, nx,
+                      ny, nz, stream);
+}
+
+int fdtd_apply_ade_velocity_correction(float* vx, float* vy, float* vz, const float* J_debye, const float* J_lorentz,
+                                       const uint8_t* material, const fdtd_ade_pole* poles, int n_poles,
+                                       const float* rho_inf, float dt, float inv_dx, int nx, int ny, int nz,
+                                       void* stream) {
+    const char* who = "apply_ade_velocity_correction";
+    if (int rc = ade_poles_ok(who, poles, n_poles, nx, ny, nz)) return rc;
+    if (!vx || !vy || !vz || !material || (n_poles > 0 && !rho_inf)) return fail(-1, "%s: null pointer", who);
+    const int64_t cells = static_cast<int64_t>(nx) * ny * nz;
+    for (int lorentz = 0; lorentz < 2; ++lorentz)
+        for (int q = 0; q < n_poles; ++q) {
+            const fdtd_ade_pole& pl = poles[q];
+            if (pl.lorentz != lorentz || pl.target != 0) continue;
+            const float* J = lorentz ? J_lorentz : J_debye;
+            if (!J) return fail(-1, "%s: null J array", who);
+            // the reference forms the coefficient in float: -dt / rho_inf * inv_dx
+            const float coeff = -dt / rho_inf[pl.material_id] * inv_dx;
+            if (int rc = check(phase_ade_vcorr(vx, vy, vz, J + pl.field_index * cells, material, pl.material_id, coeff,
+                                               nx, ny, nz, static_cast<hipStream_t>(stream)),
+                               who))
+                return rc;
+        }
+    return 0;
+}
+
+int fdtd_apply_ade_pressure_correction(float* p, const float* J_debye, const float* J_lorentz, const uint8_t* material,
+                                       const fdtd_ade_pole* poles, int n_poles, const float* K_inf, float dt, int nx,
+                                       int ny, int nz, void* stream) {
+    const char* who = "apply_ade_pressure_correction";
+    if (int rc = ade_poles_ok(who, poles, n_poles, nx, ny, nz)) return rc;
+    if (!p || !material || (n_poles > 0 && !K_inf)) return fail(-1, "%s: null pointer", who);
+    const int64_t cells = static_cast<int64_t>(nx) * ny * nz;
+    for (int lorentz = 0; lorentz < 2; ++lorentz)
+        for (int q = 0; q < n_poles; ++q) {
+            const fdtd_ade_pole& pl = poles[q];
+            if (pl.lorentz != lorentz || pl.target != 1) continue;
+            const float* J = lorentz ? J_lorentz : J_debye;
+            if (!J) return fail(-1, "%s: null J array", who);
+            const float coeff = -K_inf[pl.material_id] * dt;
+            if (int rc = check(phase_ade_pcorr(p, J + pl.field_index * cells, material, pl.material_id, coeff, cells,
+                                               static_cast<hipStream_t>(stream)),
+                               who))
+                return rc;
+        }
     return 0;
 }
 

@@ -1205,13 +1205,15 @@ int64_t fdtd_design_history_bytes(const int32_t* box_lo, const int32_t* box_hi, 
 int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
                         float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
                         const int32_t* src_ijk, const double* wave, int n_probe, const int32_t* probe_ijk,
-                        float* trace, int64_t step0, int n_steps, float* hist_p, float* hist_div, int64_t hist0,
-                        int64_t hist_steps, int planes, void* stream) {
+                        float* trace, int64_t steps_len, int64_t step0, int n_steps, float* hist_p, float* hist_div,
+                        int64_t hist0, int64_t hist_steps, int planes, void* stream) {
     DesignArgs a;
     if (int rc = fill_design(a, fields, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, rk, src_ijk, wave,
                              n_probe, probe_ijk))
         return rc;
     if (n_steps < 0 || step0 < 0 || planes < 1 || (n_probe > 0 && !trace) || (src_ijk && !wave)) return -2;
+    // the trace rows and waveform entries of steps [step0, step0 + n_steps)
+    if ((n_probe > 0 || src_ijk) && step0 + n_steps > steps_len) return -4;
     if (hist_p && (!hist_div || hist0 < 0 || hist_steps < 1)) return -2;
     a.trace = trace;
     a.hist_p = hist_p;
@@ -1234,14 +1236,15 @@ int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float*
 
 int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
                         float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
-                        int n_probe, const int32_t* probe_ijk, const float* g, int64_t n_lo, int64_t n_hi,
-                        const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps, float* acc,
-                        float* face_e, int planes, void* stream) {
+                        int n_probe, const int32_t* probe_ijk, const float* g, int64_t steps_len, int64_t n_lo,
+                        int64_t n_hi, const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps,
+                        float* acc, float* face_e, int planes, void* stream) {
     DesignArgs a;
     if (int rc = fill_design(a, lam, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, rk, nullptr, nullptr, n_probe,
                              probe_ijk))
         return rc;
     if (n_hi < n_lo || n_lo < 0 || planes < 1 || (n_probe > 0 && !g)) return -2;
+    if (n_probe > 0 && n_hi > steps_len) return -4;  // rows of g read: [n_lo, n_hi)
     if (!hist_p || !hist_div || !acc || !face_e) return -2;
     if (n_lo < hist0 || n_hi > hist0 + hist_steps) return -2;  // every step needs its history
     a.g = g;

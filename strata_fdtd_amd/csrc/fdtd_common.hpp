@@ -81,6 +81,20 @@ constexpr int64_t kMaxCells32 = int64_t(1) << 30;
 // stores, which keep L2 for halo re-reads (224^3: 136 against 129).
 constexpr int64_t kCacheResidentCells = (int64_t(256) << 20) / 33;
 
+// One ADE pole as the kernels read it (a device table; the list is unbounded).
+struct AdePole {
+    int32_t mat;                    // material id 1..255
+    int32_t target;                 // 0 density, 1 modulus
+    int32_t lorentz;                // 0 Debye, 1 Lorentz
+    int32_t pad;
+    float a, b, d;                  // recurrence coefficients
+    float corr;                     // correction coefficient
+};
+
+// Probes recorded inside the sweep (one wave lane tests one probe); probes
+// beyond these are recorded by a small kernel after the step's last pass.
+constexpr int kSweepProbes = 64;
+
 // Everything one fused step launch needs. Passed by value (kernarg segment).
 struct FusedArgs {
     float* f[2][4];                 // ping-pong sets {p, vx, vy, vz}
@@ -116,7 +130,8 @@ struct FusedArgs {
     int64_t wave_len;
 
     // probes
-    int n_probe;
+    int n_probe;                    // all probes (row stride of the trace)
+    int n_sweep_probe;              // min(n_probe, kSweepProbes): recorded by the sweep
     const int32_t* probe_ijk;       // device [n_probe][3]
     float* trace;                   // [trace_len][n_probe]
     int64_t trace_len;
@@ -147,11 +162,7 @@ struct FusedArgs {
 
     // ADE (materials with poles)
     int n_pole;
-    int pole_mat[FDTD_MAX_POLES];
-    int pole_target[FDTD_MAX_POLES];
-    int pole_lorentz[FDTD_MAX_POLES];
-    float pole_a[FDTD_MAX_POLES], pole_b[FDTD_MAX_POLES], pole_d[FDTD_MAX_POLES];
-    float pole_corr[FDTD_MAX_POLES];
+    const AdePole* poles;           // device [n_pole], registration order
     const uint8_t* material;        // per cell
     int box_lo[3], box_hi[3];       // ADE box [lo, hi)
     float* J;                       // [n_pole][box cells]

@@ -146,13 +146,13 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     uint64_t probe_mask = 0;
     if constexpr (LP) {
         bool probe_hit = false;
-        if (lane < a.n_probe) {
+        if (lane < a.n_sweep_probe) {
             const int pi = PIJK[3 * lane], pj = PIJK[3 * lane + 1], pk = PIJK[3 * lane + 2];
             probe_hit = pi >= i0 && pi < i1 && pj >= j0 && pj < j0 + R && pk >= k0 && pk < k0 + kWave;
         }
         probe_mask = __builtin_amdgcn_ballot_w64(probe_hit);
     } else {
-        for (int q = 0; q < a.n_probe; ++q) {
+        for (int q = 0; q < a.n_sweep_probe; ++q) {
             const int pi = PIJK[3 * q], pj = PIJK[3 * q + 1], pk = PIJK[3 * q + 2];
             if (pi >= i0 && pi < i1 && pj >= j0 && pj < j0 + R && pk >= k0 && pk < k0 + kWave)
                 probe_mask |= 1ull << q;
@@ -490,14 +490,15 @@ __device__ __forceinline__ void ade_density_part(const FusedArgs& a, int64_t fir
         if (!m) continue;
         const float pv = P[c];
         for (int q = 0; q < a.n_pole; ++q) {
-            if (a.pole_target[q] != 0 || a.pole_mat[q] != m) continue;
+            const AdePole pl = a.poles[q];
+            if (pl.target != 0 || pl.mat != m) continue;
             float* Jq = a.J + q * box_n;
-            if (!a.pole_lorentz[q]) {
-                Jq[b] = a.pole_a[q] * Jq[b] + a.pole_b[q] * pv;
+            if (!pl.lorentz) {
+                Jq[b] = pl.a * Jq[b] + pl.b * pv;
             } else {
                 float* Jp = a.Jprev + q * box_n;
                 const float j0v = Jq[b], j1v = Jp[b];
-                const float jn = a.pole_a[q] * j0v + a.pole_b[q] * j1v + a.pole_d[q] * pv;
+                const float jn = pl.a * j0v + pl.b * j1v + pl.d * pv;
                 Jp[b] = j0v;
                 Jq[b] = jn;
             }
@@ -528,9 +529,10 @@ __device__ __forceinline__ float ade_face(const FusedArgs& a, const float* __res
         const int ih = i + (axis == 0), jh = j + (axis == 1), kh = k + (axis == 2);
         for (int pass = 0; pass < 2; ++pass)
             for (int q = 0; q < a.n_pole; ++q) {
-                if (a.pole_target[q] != 0 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m_lo) continue;
+                const AdePole pl = a.poles[q];
+                if (pl.target != 0 || pl.lorentz != pass || pl.mat != m_lo) continue;
                 const float* Jq = a.J + q * box_n;
-                v += a.pole_corr[q] * (Jq[box_index(a, ih, jh, kh)] - Jq[box_index(a, i, j, k)]);
+                v += pl.corr * (Jq[box_index(a, ih, jh, kh)] - Jq[box_index(a, i, j, k)]);
             }
     }
     const uint32_t open = axis == 0 ? FDTD_FLAG_OPEN_X : (axis == 1 ? FDTD_FLAG_OPEN_Y : FDTD_FLAG_OPEN_Z);
@@ -566,7 +568,7 @@ __global__ void k_ade_fixup(const FusedArgs a) {
         const int64_t c = lin(i, j, k, ny, nz);
         const int m = a.material[c];
         bool has_pole = false;
-        for (int q = 0; q < a.n_pole; ++q) has_pole |= a.pole_mat[q] == m;
+        for (int q = 0; q < a.n_pole; ++q) has_pole |= a.poles[q].mat == m;
         if (!m || !has_pole) continue;
         const uint32_t f = a.flags[c];
         const float vxn = ade_face<NU>(a, P, VX, 0, i, j, k, c, sx);
@@ -584,14 +586,15 @@ __global__ void k_ade_fixup(const FusedArgs a) {
         dA += dvz * iz;
         for (int pass = 0; pass < 2; ++pass)
             for (int q = 0; q < a.n_pole; ++q) {
-                if (a.pole_target[q] != 1 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m) continue;
+                const AdePole pl = a.poles[q];
+                if (pl.target != 1 || pl.lorentz != pass || pl.mat != m) continue;
                 float* Jq = a.J + q * box_n;
                 if (!pass) {
-                    Jq[b] = a.pole_a[q] * Jq[b] + a.pole_b[q] * dA;
+                    Jq[b] = pl.a * Jq[b] + pl.b * dA;
                 } else {
                     float* Jp = a.Jprev + q * box_n;
                     const float j0v = Jq[b], j1v = Jp[b];
-                    const float jn = a.pole_a[q] * j0v + a.pole_b[q] * j1v + a.pole_d[q] * dA;
+                    const float jn = pl.a * j0v + pl.b * j1v + pl.d * dA;
                     Jp[b] = j0v;
                     Jq[b] = jn;
                 }
@@ -601,8 +604,9 @@ __global__ void k_ade_fixup(const FusedArgs a) {
         else pnew = P[c] + a.cp * ((dvx + dvy) + dvz);
         for (int pass = 0; pass < 2; ++pass)
             for (int q = 0; q < a.n_pole; ++q) {
-                if (a.pole_target[q] != 1 || a.pole_lorentz[q] != pass || a.pole_mat[q] != m) continue;
-                pnew += a.pole_corr[q] * a.J[q * box_n + b];
+                const AdePole pl = a.poles[q];
+                if (pl.target != 1 || pl.lorentz != pass || pl.mat != m) continue;
+                pnew += pl.corr * a.J[q * box_n + b];
             }
         if (!(f & FDTD_FLAG_AIR)) pnew = 0.f;
         const float dx = a.dec[0][i], dy = a.dec[1][j], dz = a.dec[2][k];
@@ -624,7 +628,7 @@ __global__ void k_ade_fixup(const FusedArgs a) {
         VYo[c] = vyn * dy;
         VZo[c] = vzn * dz;
         if (s < a.trace_len)
-            for (int q = 0; q < a.n_probe; ++q)
+            for (int q = 0; q < a.n_sweep_probe; ++q)
                 if (a.probe_ijk[3 * q] == i && a.probe_ijk[3 * q + 1] == j && a.probe_ijk[3 * q + 2] == k) {
                     a.trace[s * a.n_probe + q] = ps;
                     if (!isfinite(ps)) atomicOr(a.health, 1u);
@@ -664,6 +668,26 @@ __global__ void k_mic_gather(const FusedArgs a) {
         }
         a.mic_out[(s * a.n_mic + m) * 4 + g] = out;
     }
+}
+
+// Probes kSweepProbes.. n_probe-1 of step s (the sweep records the first
+// kSweepProbes): after every pass of the step has written the output set
+// (s+1)&1, one thread per probe copies p at its cell into the trace, as
+// _record_probes does after all sources (core/solver.py:2435-2439).
+__global__ void k_probe_tail(const FusedArgs a) {
+    const int q = kSweepProbes + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (q >= a.n_probe) return;
+    const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit) return;
+    if (s >= a.trace_len) {
+        atomicOr(a.health, 2u);
+        return;
+    }
+    const int par = static_cast<int>((s + 1) & 1);
+    const int64_t c = lin(a.probe_ijk[3 * q], a.probe_ijk[3 * q + 1], a.probe_ijk[3 * q + 2], a.ny, a.nz);
+    const float ps = a.f[par][0][c];
+    a.trace[s * a.n_probe + q] = ps;
+    if (!isfinite(ps)) atomicOr(a.health, 1u);
 }
 
 // Membrane sources of step s (core/solver.py:2389-2412): after the sweep and
@@ -806,6 +830,10 @@ hipError_t launch_fused(const FusedArgs& a_in, int rows, int stack, bool nu, boo
     }
     if (a.n_mcell > 0) {
         hipLaunchKernelGGL(k_membrane, dim3((a.n_mcell + 255) / 256), dim3(256), 0, st, a);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (a.n_probe > kSweepProbes) {
+        hipLaunchKernelGGL(k_probe_tail, dim3((a.n_probe - kSweepProbes + 255) / 256), dim3(256), 0, st, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (a.n_mic > 0) {

@@ -148,6 +148,50 @@ __global__ void k_build_flags(const uint8_t* __restrict__ air, const uint8_t* __
     }
 }
 
+// ADE auxiliary update of one pole on the cells of its material
+// (ade.cpp:25-222): Debye J = a*J + b*src; Lorentz Jn = a*J + b*Jprev + d*src,
+// then Jprev = J, J = Jn. src is p (density poles) or the divergence (modulus).
+__global__ void k_ade_update(float* __restrict__ J, float* __restrict__ Jp, const float* __restrict__ src,
+                             const uint8_t* __restrict__ mat, int mat_id, int lorentz, float a, float b, float d,
+                             int64_t n) {
+    FOR_CELLS(n) {
+        if (mat[c] != mat_id) continue;
+        if (!lorentz) {
+            J[c] = a * J[c] + b * src[c];
+        } else {
+            const float j0 = J[c], j1 = Jp[c];
+            const float jn = a * j0 + b * j1 + d * src[c];
+            Jp[c] = j0;
+            J[c] = jn;
+        }
+    }
+}
+
+// Density-pole velocity correction of one pole (ade.cpp:228-401): on faces
+// whose two cells both hold the pole's material, v += coeff * (J[c+e] - J[c]).
+__global__ void k_ade_vcorr(float* vx, float* vy, float* vz, const float* __restrict__ J,
+                            const uint8_t* __restrict__ mat, int mat_id, float coeff, int nx, int ny, int nz) {
+    const int64_t n = static_cast<int64_t>(nx) * ny * nz, sx = static_cast<int64_t>(ny) * nz;
+    FOR_CELLS(n) {
+        if (mat[c] != mat_id) continue;
+        int i, j, k;
+        decompose(c, ny, nz, i, j, k);
+        const float jc = J[c];
+        if (i < nx - 1 && mat[c + sx] == mat_id) vx[c] += coeff * (J[c + sx] - jc);
+        if (j < ny - 1 && mat[c + nz] == mat_id) vy[c] += coeff * (J[c + nz] - jc);
+        if (k < nz - 1 && mat[c + 1] == mat_id) vz[c] += coeff * (J[c + 1] - jc);
+    }
+}
+
+// Modulus-pole pressure correction of one pole (ade.cpp:403-473):
+// p += coeff * J on the cells of its material.
+__global__ void k_ade_pcorr(float* p, const float* __restrict__ J, const uint8_t* __restrict__ mat, int mat_id,
+                            float coeff, int64_t n) {
+    FOR_CELLS(n) {
+        if (mat[c] == mat_id) p[c] += coeff * J[c];
+    }
+}
+
 static inline int grid_for(int64_t n) {
     int64_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -203,6 +247,26 @@ hipError_t phase_build_flags(const uint8_t* air, const uint8_t* mat, uint8_t* fl
                              int nz, hipStream_t st) {
     hipLaunchKernelGGL(k_build_flags, dim3(grid_for(int64_t(nx) * ny * nz)), dim3(256), 0, st, air,
                        mat, fl, nx, ny, nz);
+    return hipGetLastError();
+}
+
+hipError_t phase_ade_update(float* J, float* Jp, const float* src, const uint8_t* mat, int mat_id, int lorentz,
+                            float a, float b, float d, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_update, dim3(grid_for(n)), dim3(256), 0, st, J, Jp, src, mat, mat_id, lorentz, a, b, d,
+                       n);
+    return hipGetLastError();
+}
+
+hipError_t phase_ade_vcorr(float* vx, float* vy, float* vz, const float* J, const uint8_t* mat, int mat_id,
+                           float coeff, int nx, int ny, int nz, hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_vcorr, dim3(grid_for(int64_t(nx) * ny * nz)), dim3(256), 0, st, vx, vy, vz, J, mat,
+                       mat_id, coeff, nx, ny, nz);
+    return hipGetLastError();
+}
+
+hipError_t phase_ade_pcorr(float* p, const float* J, const uint8_t* mat, int mat_id, float coeff, int64_t n,
+                           hipStream_t st) {
+    hipLaunchKernelGGL(k_ade_pcorr, dim3(grid_for(n)), dim3(256), 0, st, p, J, mat, mat_id, coeff, n);
     return hipGetLastError();
 }
 

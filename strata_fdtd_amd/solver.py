@@ -59,6 +59,10 @@ def _fast_waveform(source, times: NDArray[np.float64], dt: float) -> NDArray[np.
     return waveform_table(source, times, dt)
 
 
+# point and x/y-plane sources the sweep adds itself (FDTD_MAX_SOURCES of the C ABI)
+SWEEP_SOURCES = 8
+
+
 def _table_source(source, cells: int) -> bool:
     """Whether a source is added from per-cell tables after the sweep rather
     than inside it: membranes, and plane sources normal to z. A z plane meets
@@ -538,6 +542,13 @@ class FDTDSolver:
         # its sources in the order of the list (core/solver.py:2386-2433).
         n_cells = int(np.prod(self.shape))
         first = next((q for q, s in enumerate(self._sources) if _table_source(s, n_cells)), len(self._sources))
+        if first > SWEEP_SOURCES:
+            # the sweep adds at most SWEEP_SOURCES sources itself; the rest of
+            # the list goes to the tables (32-bit cell indices)
+            if n_cells >= 2**31:
+                raise ValueError(f"{first} point/plane sources on a grid of 2^31 cells or more "
+                                 f"(at most {SWEEP_SOURCES} there)")
+            first = SWEEP_SOURCES
         srcs = self._sources[:first]
         table = self._sources[first:]
         times = accumulated_times(self.dt, cap) if self._sources else None

@@ -210,3 +210,28 @@ def seeded_scenario(seed: int, shape=(24, 20, 28), pml=4, geometry=True, ade=Fal
     probes.append(src_pos)
     return Scenario(shape=shape, pml_depth=pml, air=air, material=material, materials=materials,
                     sources=sources, probes=probes, coords=coords)
+
+
+ADE_POLE_DTYPE = np.dtype([("material_id", "<i4"), ("target", "<i4"), ("lorentz", "<i4"), ("field_index", "<i4"),
+                           ("a", "<f4"), ("b", "<f4"), ("d", "<f4"), ("reserved", "<f4")])
+
+
+def ade_phase_case(seed: int, shape=(13, 17, 70)):
+    """Random inputs of the per-phase ADE functions: fields, J arrays (two
+    Debye and two Lorentz rows), material ids 0-3 and a pole list that mixes
+    both kinds and targets with out-of-order field indices."""
+    rng = np.random.default_rng(seed)
+    f = {k: rng.standard_normal(shape).astype(np.float32) for k in ("p", "vx", "vy", "vz", "div")}
+    f["Jd"] = rng.standard_normal((2,) + shape).astype(np.float32)
+    f["Jl"] = rng.standard_normal((2,) + shape).astype(np.float32)
+    f["Jlp"] = rng.standard_normal((2,) + shape).astype(np.float32)
+    mat = rng.integers(0, 4, shape).astype(np.uint8)
+    mat[:, :, 20:40] = 2          # a block of one material, so faces with both cells in it are common
+    poles = np.zeros(6, dtype=ADE_POLE_DTYPE)
+    rows = [(1, 0, 0, 1, 0.91, 0.05, 0.0), (2, 1, 0, 0, 0.83, 0.11, 0.0), (2, 0, 1, 1, 1.93, -0.95, 0.013),
+            (3, 1, 1, 0, 1.87, -0.91, 0.021), (2, 0, 0, 0, 0.77, 0.09, 0.0), (1, 1, 1, 1, 1.95, -0.97, 0.008)]
+    for q, r in enumerate(rows):
+        poles[q] = r + (0.0,)
+    rho_inf = (1.0 + rng.random(256)).astype(np.float32)
+    K_inf = (1.0e5 + 1.0e5 * rng.random(256)).astype(np.float32)
+    return f, mat, poles, rho_inf, K_inf

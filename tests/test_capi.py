@@ -40,3 +40,25 @@ def test_version_and_error_string_without_gpu():
     rc = lib.fdtd_step_n(None, 1)
     assert rc < 0
     assert b"context" in lib.fdtd_last_error()
+
+
+def test_design_calls_refuse_rows_past_their_tables_without_gpu():
+    """fdtd_design_forward / fdtd_design_adjoint check the trace, waveform and
+    g rows a call would touch against steps_len on the host, before any
+    launch (the pointers below are never dereferenced)."""
+    from strata_fdtd_amd import _native
+    lib = _native.load()
+    fake = [ctypes.c_void_p(0x10000 + 0x100 * q) for q in range(8)]
+    sets = (ctypes.c_void_p * 8)(*fake)
+    dec = (ctypes.c_void_p * 3)(*fake[:3])
+    lo = (ctypes.c_int32 * 3)(2, 2, 2)
+    hi = (ctypes.c_int32 * 3)(10, 10, 10)
+    src = (ctypes.c_int32 * 3)(5, 5, 5)
+    common = (sets, fake[0], dec, 16, 16, 16, -0.01, -30.0, lo, hi, None)
+    # forward: steps [5, 15) against 10 rows
+    rc = lib.fdtd_design_forward(*common, src, fake[1], 1, fake[2], fake[3], 10, 5, 10, None, None, 0, 0, 2, None)
+    assert rc == -4
+    # adjoint: steps [0, 12) against 10 rows of g
+    rc = lib.fdtd_design_adjoint(*common, 1, fake[2], fake[4], 10, 0, 12, fake[5], fake[6], 0, 12, fake[7],
+                                 fake[1], 4, None)
+    assert rc == -4

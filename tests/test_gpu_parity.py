@@ -265,3 +265,40 @@ def test_autotune_caches_the_fastest_and_pick_tiling_reads_it(oracle_lib, tmp_pa
         _same(t.get_probe_data()[f"probe{q}"], tr_o[:, q], f"probe{q}")
     for k in FIELDS:
         _same(getattr(t, k), st_o[k], k)
+
+
+def test_ade_phase_kernels_bitexact(oracle_lib):
+    """The per-phase ADE entry points (update_ade_*, apply_ade_*) equal the
+    oracle's ADE phases, which equal the reference's ade.cpp kernels
+    (tests/test_oracle.py), on random data with Debye and Lorentz density and
+    modulus poles in mixed order."""
+    from helpers import ade_phase_case
+    from strata_fdtd_amd import _native
+    torch = _torch()
+    f, mat, poles, rho_inf, K_inf = ade_phase_case(9)
+    nx, ny, nz = mat.shape
+    n = len(poles)
+    st = torch.cuda.current_stream().cuda_stream
+    d = {k: _dev(v) for k, v in f.items()}
+    dm = _dev(mat)
+    h = {k: v.copy() for k, v in f.items()}
+    P = lambda k: _native.ptr(d[k])  # noqa: E731
+    pl = poles.ctypes.data
+    _native.call("fdtd_update_ade_density_debye", P("Jd"), P("p"), _native.ptr(dm), pl, n, nx, ny, nz, st)
+    _native.call("fdtd_update_ade_density_lorentz", P("Jl"), P("Jlp"), P("p"), _native.ptr(dm), pl, n, nx, ny, nz, st)
+    _native.call("fdtd_apply_ade_velocity_correction", P("vx"), P("vy"), P("vz"), P("Jd"), P("Jl"), _native.ptr(dm),
+                 pl, n, rho_inf.ctypes.data, 1.6e-6, 1000.0, nx, ny, nz, st)
+    _native.call("fdtd_update_ade_modulus_debye", P("Jd"), P("div"), _native.ptr(dm), pl, n, nx, ny, nz, st)
+    _native.call("fdtd_update_ade_modulus_lorentz", P("Jl"), P("Jlp"), P("div"), _native.ptr(dm), pl, n, nx, ny, nz,
+                 st)
+    _native.call("fdtd_apply_ade_pressure_correction", P("p"), P("Jd"), P("Jl"), _native.ptr(dm), pl, n,
+                 K_inf.ctypes.data, 1.6e-6, nx, ny, nz, st)
+    cpu.phase("update_ade_density_debye", h["Jd"], h["p"], mat, poles, n, nx, ny, nz)
+    cpu.phase("update_ade_density_lorentz", h["Jl"], h["Jlp"], h["p"], mat, poles, n, nx, ny, nz)
+    cpu.phase("apply_ade_velocity_correction", h["vx"], h["vy"], h["vz"], h["Jd"], h["Jl"], mat, poles, n, rho_inf,
+              1.6e-6, 1000.0, nx, ny, nz)
+    cpu.phase("update_ade_modulus_debye", h["Jd"], h["div"], mat, poles, n, nx, ny, nz)
+    cpu.phase("update_ade_modulus_lorentz", h["Jl"], h["Jlp"], h["div"], mat, poles, n, nx, ny, nz)
+    cpu.phase("apply_ade_pressure_correction", h["p"], h["Jd"], h["Jl"], mat, poles, n, K_inf, 1.6e-6, nx, ny, nz)
+    for k in f:
+        _same(d[k].cpu().numpy(), h[k], k)

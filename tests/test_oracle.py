@@ -263,3 +263,44 @@ def test_f32_oracle_close_to_f64(oracle_lib):
     _, t32 = cpu.run(pb, 200)
     _, t64 = cpu.run(pb, 200, impl="oracle64")
     assert rel_l2(t32, t64) < 1e-4
+
+
+@needs_ref
+def test_oracle_equals_reference_with_long_lists(oracle_lib):
+    """12 point sources (two cells with two each), 100 probes, 10 poles in
+    three materials: the oracle equals the reference kernels bit for bit (the
+    scene the GPU limits test runs)."""
+    from test_gpu_limits import _many
+    sc = _many(31)
+    pb = oracle_problem(sc, 25)
+    a, ta = cpu.run(pb, 25)
+    b, tb = cpu.run(pb, 25, impl="ref")
+    assert np.array_equal(ta, tb)
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), k
+
+
+@needs_ref
+def test_ade_phases_equal_reference(oracle_lib):
+    """The oracle's per-phase ADE functions equal the reference's own
+    update_ade_* / apply_ade_* kernels (ade.cpp:25-473) bit for bit, Debye and
+    Lorentz, density and modulus, poles out of field order."""
+    from helpers import ade_phase_case
+    f, mat, poles, rho_inf, K_inf = ade_phase_case(9)
+    nx, ny, nz = mat.shape
+    a = {k: v.copy() for k, v in f.items()}
+    b = {k: v.copy() for k, v in f.items()}
+    n = len(poles)
+    for impl, d in (("oracle", a), ("ref", b)):
+        cpu.phase("update_ade_density_debye", d["Jd"], d["p"], mat, poles, n, nx, ny, nz, impl=impl)
+        cpu.phase("update_ade_density_lorentz", d["Jl"], d["Jlp"], d["p"], mat, poles, n, nx, ny, nz, impl=impl)
+        cpu.phase("apply_ade_velocity_correction", d["vx"], d["vy"], d["vz"], d["Jd"], d["Jl"], mat, poles, n,
+                  rho_inf, 1.6e-6, 1000.0, nx, ny, nz, impl=impl)
+        cpu.phase("update_ade_modulus_debye", d["Jd"], d["div"], mat, poles, n, nx, ny, nz, impl=impl)
+        cpu.phase("update_ade_modulus_lorentz", d["Jl"], d["Jlp"], d["div"], mat, poles, n, nx, ny, nz, impl=impl)
+        cpu.phase("apply_ade_pressure_correction", d["p"], d["Jd"], d["Jl"], mat, poles, n, K_inf, 1.6e-6, nx, ny, nz,
+                  impl=impl)
+    for k in f:
+        assert np.array_equal(a[k], b[k]), k
+        if k != "div":
+            assert not np.array_equal(a[k], f[k]), k   # every output array was touched
