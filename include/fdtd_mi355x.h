@@ -365,6 +365,20 @@ int fdtd_voxelize(const int32_t* ops, int n_ops, const int32_t* prog_start, cons
                   const double* xc, const double* yc, const double* zc, int nx, int ny, int nz,
                   uint8_t* air, uint8_t* ids, void* stream);
 
+/* fdtd_voxelize without the dev_scratch workspace, for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. The
+ * programs are staged in a fixed device array of FDTD_VOXELIZE_STATIC_BYTES
+ * bytes inside the library, one copy per GPU, kept until the process ends;
+ * programs needing more staging (fdtd_voxelize_scratch_bytes) are rejected
+ * with an error. The call allocates nothing. The GPU is the stream's (the
+ * current device for a NULL stream). Every call on one GPU shares its array:
+ * calls that overlap on one GPU (other host threads, or other streams whose
+ * kernels have not finished) race; calls on one stream are ordered. */
+#define FDTD_VOXELIZE_STATIC_BYTES (256 * 1024)
+int fdtd_voxelize_static(const int32_t* ops, int n_ops, const int32_t* prog_start, const int32_t* prog_ids,
+                         int n_prog, const double* params, int n_params, const double* xc, const double* yc,
+                         const double* zc, int nx, int ny, int nz, uint8_t* air, uint8_t* ids, void* stream);
 
 #ifdef __cplusplus
 }

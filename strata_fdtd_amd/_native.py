@@ -77,6 +77,7 @@ _SIGS = {
     "fdtd_apply_ade_pressure_correction": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _vp, _f, _i, _i, _i, _vp]),
     "fdtd_voxelize_scratch_bytes": (_i64, [_i, _i, _i]),
     "fdtd_voxelize": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _i64, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
+    "fdtd_voxelize_static": (_i, [_vp, _i, _vp, _vp, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _i, _vp, _vp, _vp]),
 }
 
 _lib: C.CDLL | None = None

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -643,24 +644,30 @@ int64_t fdtd_voxelize_scratch_bytes(int n_ops, int n_prog, int n_params) {
     return ((ints * 4 + 7) / 8) * 8 + 8 * static_cast<int64_t>(n_params);
 }
 
-int fdtd_voxelize(const int32_t* ops, int n_ops, const int32_t* prog_start, const int32_t* prog_ids, int n_prog,
-                  const double* params, int n_params, void* dev_scratch, int64_t scratch_bytes,
-                  const double* xc, const double* yc, const double* zc, int nx, int ny, int nz,
-                  uint8_t* air, uint8_t* ids, void* stream) {
+}  // extern "C"
+
+namespace {
+// Validates a voxeliser call, stages its programs into `base` (device memory
+// of `bytes` bytes) on `st`, waits for the upload (the host arrays belong to
+// the caller) and launches the kernel. `who` prefixes error messages.
+int voxelize_staged(const char* who, const int32_t* ops, int n_ops, const int32_t* prog_start,
+                    const int32_t* prog_ids, int n_prog, const double* params, int n_params, void* base_v,
+                    int64_t bytes, const double* xc, const double* yc, const double* zc, int nx, int ny, int nz,
+                    uint8_t* air, uint8_t* ids, hipStream_t st) {
     if (!prog_start || !prog_ids || (n_ops > 0 && !ops) || (n_params > 0 && !params))
-        return fail(-1, "fdtd_voxelize: null program array");
-    if (!dev_scratch || !xc || !yc || !zc || !air) return fail(-1, "fdtd_voxelize: null device pointer");
-    if (nx < 1 || ny < 1 || nz < 1) return fail(-2, "fdtd_voxelize: bad dimensions");
+        return fail(-1, "%s: null program array", who);
+    if (!base_v || !xc || !yc || !zc || !air) return fail(-1, "%s: null device pointer", who);
+    if (nx < 1 || ny < 1 || nz < 1) return fail(-2, "%s: bad dimensions", who);
     if (n_ops < 0 || n_ops > (1 << 20) || n_params < 0 || n_params > (1 << 22))
-        return fail(-2, "fdtd_voxelize: program too large");
-    if (n_prog > 1 && !ids) return fail(-1, "fdtd_voxelize: material programs need an ids output");
+        return fail(-2, "%s: program too large", who);
+    if (n_prog > 1 && !ids) return fail(-1, "%s: material programs need an ids output", who);
     if (const char* msg = vox_validate(ops, n_ops, prog_start, n_prog, prog_ids, params, n_params))
-        return fail(-2, "fdtd_voxelize: %s", msg);
+        return fail(-2, "%s: %s", who, msg);
     const int64_t need = fdtd_voxelize_scratch_bytes(n_ops, n_prog, n_params);
-    if (scratch_bytes < need) return fail(-2, "fdtd_voxelize: scratch %lld B < %lld B", (long long)scratch_bytes,
-                                          (long long)need);
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    auto* base = static_cast<char*>(dev_scratch);
+    if (bytes < need)
+        return fail(-2, "%s: programs need %lld B of staging, %lld B available", who, (long long)need,
+                    (long long)bytes);
+    auto* base = static_cast<char*>(base_v);
     int32_t* d_ops = reinterpret_cast<int32_t*>(base);
     int32_t* d_start = d_ops + 2 * static_cast<int64_t>(n_ops);
     int32_t* d_ids = d_start + (n_prog + 1);
@@ -684,6 +691,61 @@ int fdtd_voxelize(const int32_t* ops, int n_ops, const int32_t* prog_start, cons
     if (int rc = check(hipStreamSynchronize(st), "voxelize upload sync")) return rc;
     VoxProgram pg{d_ops, d_start, d_ids, d_params, n_prog};
     return check(launch_voxelize(pg, xc, yc, zc, nx, ny, nz, air, ids, st), "voxelize launch");
+}
+
+// Program staging of fdtd_voxelize_static: a fixed array in device memory, one
+// copy per GPU (each GPU holds it until the process ends). In host code the
+// name `scratch` is only a host-side stand-in; every use goes through the
+// address hipGetSymbolAddress returns for the GPU (scratch_addr[device]).
+__device__ __align__(256) static unsigned char scratch[FDTD_VOXELIZE_STATIC_BYTES];
+constexpr int kMaxDevices = 64;
+void* scratch_addr[kMaxDevices] = {};
+std::mutex scratch_mu;
+
+// Address of this GPU's copy of `scratch`, looked up once per device with the
+// device current.
+int static_scratch(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(-2, "fdtd_voxelize_static: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_addr[dev]) {
+        DeviceGuard g(dev);
+        void* p = nullptr;
+        if (int rc = check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "fdtd_voxelize_static: scratch address"))
+            return rc;
+        if (!p) return fail(-5, "fdtd_voxelize_static: scratch address is null");
+        scratch_addr[dev] = p;
+    }
+    *out = scratch_addr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int fdtd_voxelize(const int32_t* ops, int n_ops, const int32_t* prog_start, const int32_t* prog_ids, int n_prog,
+                  const double* params, int n_params, void* dev_scratch, int64_t scratch_bytes,
+                  const double* xc, const double* yc, const double* zc, int nx, int ny, int nz,
+                  uint8_t* air, uint8_t* ids, void* stream) {
+    return voxelize_staged("fdtd_voxelize", ops, n_ops, prog_start, prog_ids, n_prog, params, n_params,
+                           dev_scratch, scratch_bytes, xc, yc, zc, nx, ny, nz, air, ids,
+                           static_cast<hipStream_t>(stream));
+}
+
+int fdtd_voxelize_static(const int32_t* ops, int n_ops, const int32_t* prog_start, const int32_t* prog_ids,
+                         int n_prog, const double* params, int n_params, const double* xc, const double* yc,
+                         const double* zc, int nx, int ny, int nz, uint8_t* air, uint8_t* ids, void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int dev = -1;
+    if (st) {
+        if (int rc = check(hipStreamGetDevice(st, &dev), "fdtd_voxelize_static: stream device")) return rc;
+    } else if (int rc = check(hipGetDevice(&dev), "fdtd_voxelize_static: current device")) {
+        return rc;
+    }
+    void* base = nullptr;
+    if (int rc = static_scratch(dev, &base)) return rc;
+    DeviceGuard g(dev);
+    return voxelize_staged("fdtd_voxelize_static", ops, n_ops, prog_start, prog_ids, n_prog, params, n_params,
+                           base, FDTD_VOXELIZE_STATIC_BYTES, xc, yc, zc, nx, ny, nz, air, ids, st);
 }
 
 // ---- per-phase entry points ---------------------------------------------------
