@@ -70,16 +70,22 @@ def kernel_name(kname: str, rows: int, stack: int, uniform: bool, cells: int) ->
     return f"{kname}<{rows},{nu},{ix},{nts}>"
 
 
-def traffic_from_profiles(workload: str):
+def traffic_from_profiles(workload: str, kernel: str):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
+    PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), and a note.
+    The bytes are returned only when the summary was taken on this workload
+    and on the kernel instance this run launches; otherwise None and why."""
     path = os.path.join(ROOT, "profiles", f"fused_step_{workload}_traffic.json")
     try:
         with open(path) as fh:
             d = json.load(fh)
-        return float(d["bytes_per_launch"])
+        nbytes = float(d["bytes_per_launch"])
     except (OSError, ValueError, KeyError):
-        return None
+        return None, f"no PMC summary for {workload} under profiles/"
+    if d.get("workload") != workload or d.get("kernel") != kernel:
+        return None, (f"PMC summary is of {d.get('kernel')} on {d.get('workload')}, "
+                      f"this run launches {kernel} on {workload}")
+    return nbytes, f"rocprofv3 FETCH_SIZE + WRITE_SIZE, {os.path.relpath(path, ROOT)}"
 
 
 def cpu_baseline(solver, seconds_budget: float = 20.0):
@@ -128,6 +134,65 @@ def cpu_baseline(solver, seconds_budget: float = 20.0):
     }
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+FULL_LENGTH = ("plain_100", "resonators_nu", "enclosure_512")
+
+
+def _rel_l2(a, b) -> float:
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+
+def full_length_parity(name: str) -> dict:
+    """Run a workload over its stated step count on the GPU and compare with
+    its committed full-length fixture (tests/golden/full_<name>.npz, data
+    written once by the CPU oracle in float32 and float64): probe traces bit
+    for bit, SHA-256 of the final fields' bytes, and the error against the
+    float64 run (probe traces, final pressure on a 16-cell sub-lattice)."""
+    import hashlib
+    from strata_fdtd_amd import workloads as W
+    g = np.load(os.path.join(GOLDEN, f"full_{name}.npz"), allow_pickle=False)
+    wl = {"plain_100": W.plain_100, "enclosure_512": W.enclosure_512, "resonators_nu": W.resonators_nu}[name]()
+    steps = int(g["steps"])
+    if steps != wl.steps:
+        raise ValueError(f"{name}: fixture has {steps} steps, the workload {wl.steps}")
+    s = wl.build()
+    s.step_n(steps)
+    tr = np.stack(list(s.get_probe_data().values()), axis=1)
+    out = {"steps": steps, "trace_bitexact": bool(np.array_equal(tr, g["trace_f32"])),
+           "trace_rel_l2_vs_oracle_f32": _rel_l2(tr, g["trace_f32"]),
+           "trace_rel_l2_vs_f64": _rel_l2(tr, g["trace_f64"])}
+    same = True
+    for k in ("p", "vx", "vy", "vz"):
+        a = np.ascontiguousarray(getattr(s, k))
+        same &= hashlib.sha256(a.tobytes()).hexdigest() == str(g[f"sha256_{k}"])
+        if k == "p":
+            sub = int(g["sub"])
+            out["p_sub_rel_l2_vs_f64"] = _rel_l2(a[::sub, ::sub, ::sub], g["p_sub_f64"])
+    out["fields_bitexact"] = bool(same)
+    return out
+
+
+def parity_bench():
+    """Full-length parity of the three solve workloads: each runs its stated
+    step count and is compared with committed oracle fixtures
+    (tests/golden/full_*.npz: probe traces and field hashes of the float32
+    oracle, probe traces and a pressure sub-lattice of the float64 oracle).
+    Reports bit-exactness and the error against float64."""
+    out = {}
+    for name in FULL_LENGTH:
+        if not os.path.exists(os.path.join(GOLDEN, f"full_{name}.npz")):
+            out[name] = {"error": "no fixture"}
+            continue
+        t = time.perf_counter()
+        out[name] = full_length_parity(name)
+        out[name]["seconds"] = round(time.perf_counter() - t, 1)
+        log(f"parity {name}: {out[name]}")
+    return out
+
+
 def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
     """Continuous batching vs one-at-a-time on the seeded Poisson design sweep."""
     import torch
@@ -146,6 +211,12 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
     srv.close()
     log(f"serving batched: {batched.summary()}")
     torch.cuda.empty_cache()
+    # least work first with an age credit
+    srv = SweepServer(chunk=chunk, policy="aged")
+    aged = srv.serve(reqs, realtime=True)
+    srv.close()
+    log(f"serving batched, aged least-work-first: {aged.summary()}")
+    torch.cuda.empty_cache()
     # every arrival admitted at once, in arrival order
     srv = SweepServer(chunk=chunk, max_cells=None, policy="fifo")
     fifo = srv.serve(reqs, realtime=True)
@@ -162,6 +233,7 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
         "requests": len(reqs), "arrival_rate_per_s": rate, "chunk_steps": chunk,
         "gcell_updates_total": round(sum(r.cells * r.steps for r in reqs) / 1e9, 2),
         "batched": batched.summary(), "batched_policy": "<= 6M active cells (cache-resident), least work first",
+        "batched_aged": aged.summary(),
         "batched_all_admitted_fifo": fifo.summary(),
         "one_at_a_time": single.summary(),
         "speedup_solves_per_s": round(batched.solves_per_s / single.solves_per_s, 3),
@@ -278,6 +350,8 @@ def main() -> int:
     ap.add_argument("--tiling", default=None, help="rows,planes[,stack] override")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched-serving sweep")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the full-length parity runs against the oracle fixtures")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference check on N voxels")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
@@ -346,14 +420,17 @@ def main() -> int:
     value = cells * args.steps / (dev_ms / 1e3) / 1e9
     ms_sweep = sweep_ms / args.steps
     achieved = BYTES_PER_CELL * cells / (ms_sweep / 1e3) / 1e9
+    kinst = kernel_name(kname, rows, stack, solver.grid.is_uniform, cells)
+    traffic, traffic_note = traffic_from_profiles(wl.name, kinst)
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic_from_profiles(wl.name),
-        "kernel": kernel_name(kname, rows, stack, solver.grid.is_uniform, cells),
+        "traffic": traffic,
+        "traffic_source": traffic_note,
+        "kernel": kinst,
         "algorithmic_bytes_per_launch": BYTES_PER_CELL * cells,
         "bytes_per_cell_update": BYTES_PER_CELL,
         "avg_launch_us": round(ms_sweep * 1e3, 1),
@@ -388,6 +465,12 @@ def main() -> int:
     }
     del solver
     torch.cuda.empty_cache()
+    if not args.no_parity:
+        try:
+            line["parity"] = parity_bench()
+        except Exception as exc:
+            line["parity"] = {"error": f"{type(exc).__name__}: {exc}"}
+        torch.cuda.empty_cache()
     if not args.no_serving:
         try:
             line["serving"] = serving_bench()

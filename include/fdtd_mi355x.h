@@ -258,7 +258,9 @@ int fdtd_adjoint_run(int dtype, void* const* lam, void* const* states, int64_t n
  * float32 [hx][hy][hz][4]: (sum, compensation) of dL/d(rho/rho0), then of
  * dL/d(K/K0); the gradient is sum - compensation. face_e: device float32
  * scratch of fdtd_design_face_count entries (every box face's gradient
- * factor, formed once per call). */
+ * factor, formed once per call); pb_box: device float32 scratch of
+ * 2 hx hy hz entries (each step's adjoint pressure on the box, kept for the
+ * gradient pass, which takes two steps at a time). */
 int64_t fdtd_design_face_count(const int32_t* box_lo, const int32_t* box_hi);
 int64_t fdtd_design_history_bytes(const int32_t* box_lo, const int32_t* box_hi, int64_t steps);
 int fdtd_design_forward(float* const* fields, const uint8_t* flags, const float* const* dec, int nx, int ny, int nz,
@@ -270,7 +272,7 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
                         float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
                         int n_probe, const int32_t* probe_ijk, const float* g, int64_t steps_len, int64_t n_lo,
                         int64_t n_hi, const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps,
-                        float* acc, float* face_e, int planes, void* stream);
+                        float* acc, float* face_e, float* pb_box, int planes, void* stream);
 
 /* ---- per-phase entry points (one kernel per reference function) --------- */
 /* All in place, on `stream`, float32, uniform unless noted. */

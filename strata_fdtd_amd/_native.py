@@ -57,7 +57,7 @@ _SIGS = {
     "fdtd_design_forward": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp,
                                  _i64, _i64, _i, _vp, _vp, _i64, _i64, _i, _vp]),
     "fdtd_design_adjoint": (_i, [_vp, _vp, _vp, _i, _i, _i, _f, _f, _vp, _vp, _vp, _i, _vp, _vp, _i64, _i64, _i64,
-                                 _vp, _vp, _i64, _i64, _vp, _vp, _i, _vp]),
+                                 _vp, _vp, _i64, _i64, _vp, _vp, _vp, _i, _vp]),
     "fdtd_design_face_count": (_i64, [_vp, _vp]),
     "fdtd_update_velocity": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp]),
     "fdtd_update_velocity_nonuniform": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i, _f, _vp, _vp, _vp, _vp]),

@@ -239,9 +239,11 @@ class _Fast:
         if getattr(self, "_faces", None) is None:
             n = _native.load().fdtd_design_face_count(self.dv.lo.ctypes.data, self.dv.hi.ctypes.data)
             self._faces = torch.empty(n, dtype=torch.float32, device=self.dv.dev)
+            self._pb_box = torch.empty(2 * self.bcells, dtype=torch.float32, device=self.dv.dev)
         _native.call("fdtd_design_adjoint", self._sets(lam), *self._common(rk), self.dv.n_probe,
                      _ptr(self.probe_ijk), _ptr(g), g.shape[0], n_lo, n_hi, _ptr(hp), _ptr(hd), hist0,
-                     hp.numel() // self.ecells, _ptr(acc), _ptr(self._faces), self.ADJ_PLANES, self.dv.stream)
+                     hp.numel() // self.ecells, _ptr(acc), _ptr(self._faces), _ptr(self._pb_box), self.ADJ_PLANES,
+                     self.dv.stream)
 
 
 def history_bytes(pb: DesignProblem, steps: int | None = None) -> int:

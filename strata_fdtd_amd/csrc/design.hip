@@ -66,6 +66,9 @@ struct DesignArgs {
     // gradient accumulators on the box: Kahan (sum, compensation) of dL/dr,
     // then of dL/dkappa, per voxel
     f4v* acc;
+    // pb of every box voxel (the adjoint sweep of step n writes pbb[n & 1]),
+    // read by the gradient pass of step n after the next step's sweep
+    float* pbb[2];
     // gradient factor -cv / (r_lo + r_hi) of every face of the box, formed once
     // per adjoint call: x faces of cells [lo0, hi0] x [lo1, hi1) x [lo2, hi2)
     // (the face below each cell), likewise y and z; see face_offsets()
@@ -762,7 +765,8 @@ __device__ __forceinline__ void design_adjoint_strip(const DesignArgs& a, int64_
 // coefficient cv is formed once, where the face is the cell's own (upper)
 // face, and passed on to the cell above it: along i in registers from plane
 // to plane, along j from row to row, along k from lane to lane. The gradient
-// terms are accumulated afterwards by k_design_grad from this step's output.
+// terms are accumulated afterwards by k_design_grad from this step's output
+// and the pb of every box voxel, which the strip stores.
 template <int R, bool NTS>
 __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, int64_t n, int seg, int j0, int i0,
                                                          int i1, int lane) {
@@ -788,6 +792,7 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
 
     const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
     const BoxTab RK = box_tab(a.rk);
+    GMEM float* __restrict__ PBB = gm(a.pbb[n & 1]);
     const Ax dk = dax(k, a.lo[2], bz), dkh = dax(kh, a.lo[2], bz);
     Ax dj[R];
 #pragma unroll
@@ -954,6 +959,8 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
             if (i > 0) acc += cvxm[r] * wxm[r];
             if (j > 0) acc += cvyp * wyprev;
             if (k > 0) acc += cvzm * wzm;
+            if (di.in && dj[r].in && dk.in)
+                sto(PBB, static_cast<uint32_t>((di.off * by + dj[r].off) * bz + dk.off), pbc[r]);
             if (kin) {
                 stout<NTS>(Po, c, acc);
                 stout<NTS>(VXo, c, wx);
@@ -974,14 +981,17 @@ __device__ __forceinline__ void design_adjoint_box_strip(const DesignArgs& a, in
     }
 }
 
-// Gradient terms of adjoint step n, one thread per design-box voxel, from the
-// step's output lambda_n (the w of the cell's six faces), the input
-// lambda_{n+1} (its pressure) and the forward history of step n:
+// Gradient terms of adjoint steps n and, with count 2, n - 1 (in that order),
+// one thread per design-box voxel, from each step's output lambda (the w of
+// the cell's six faces), its pb (stored by that step's sweep) and the forward
+// history of that step:
 //   dL/dr     += sum over the six faces of w * (p_hi - p_lo) * (-cv / (r_lo + r_hi))
 //   dL/dkappa += air * cp0 * pb * div
 // in the per-cell kernel's order (own x, y, z faces, then the faces below),
-// with Kahan compensation. Every box voxel has all six faces (the box keeps
-// one cell to every grid face).
+// with Kahan compensation. Two steps per pass read and write the
+// accumulators and the face factors once for both (lambda_n stays in its set
+// until the sweep of step n - 2). Every box voxel has all six faces (the box
+// keeps one cell to every grid face).
 // Layout of face_e: x faces (bx+1) by bz, then y faces bx (by+1) bz, then z
 // faces bx by (bz+1); the face of axis d below box cell (bi, bj, bk) (between
 // it and its -d neighbour) has that cell's index in the axis' array.
@@ -1012,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_design_faces(const DesignArgs a) {
     a.face_e[off + t] = -cv / (rl + rh);
 }
 
-__global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t n) {
+__global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t n, int count) {
     const int bx = a.hi[0] - a.lo[0], by = a.hi[1] - a.lo[1], bz = a.hi[2] - a.lo[2];
     const uint32_t b = blockIdx.x * 256u + threadIdx.x;
     if (b >= static_cast<uint32_t>(bx) * by * bz) return;
@@ -1022,13 +1032,6 @@ __global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t
     const int ny = a.ny, nz = a.nz;
     const uint32_t sx = static_cast<uint32_t>(ny) * static_cast<uint32_t>(nz);
     const uint32_t c = cix(i, j, k, ny, nz);
-    const int src = static_cast<int>((n + 1) & 1);
-    const GMEM float* __restrict__ P = gm(a.f[src][0]);
-    const GMEM float* __restrict__ WX = gm(a.f[src ^ 1][1]);
-    const GMEM float* __restrict__ WY = gm(a.f[src ^ 1][2]);
-    const GMEM float* __restrict__ WZ = gm(a.f[src ^ 1][3]);
-    const GMEM float* __restrict__ hp = gm(a.hist_p + (n - a.hist0) * ebox_cells(a));
-    const GMEM float* __restrict__ hd = gm(a.hist_div + (n - a.hist0) * dbox_cells(a));
     GMEM f4v* __restrict__ ACC = gm(a.acc);
     const int ey = by + 2, ez = bz + 2;
     const uint32_t e = static_cast<uint32_t>(((bi + 1) * ey + (bj + 1)) * ez + (bk + 1));
@@ -1043,34 +1046,42 @@ __global__ __launch_bounds__(256) void k_design_grad(const DesignArgs a, int64_t
     const float exm = ldo(EX, fx), exp_ = ldo(EX, fx + static_cast<uint32_t>(by * bz));
     const float eym = ldo(EY, fy), eyp = ldo(EY, fy + static_cast<uint32_t>(bz));
     const float ezm = ldo(EZ, fz), ezp = ldo(EZ, fz + 1u);
-    const float hpc = ldo(hp, e);
-    const float hxp = ldo(hp, e + static_cast<uint32_t>(ey * ez)), hyp = ldo(hp, e + static_cast<uint32_t>(ez)),
-                hzp = ldo(hp, e + 1u);
-    const float hxm = ldo(hp, e - static_cast<uint32_t>(ey * ez)), hym = ldo(hp, e - static_cast<uint32_t>(ez)),
-                hzm = ldo(hp, e - 1u);
-    const float wx = ldo(WX, c), wy = ldo(WY, c), wz = ldo(WZ, c);
-    const float wxm = ldo(WX, c - sx), wym = ldo(WY, c - static_cast<uint32_t>(nz)), wzm = ldo(WZ, c - 1u);
-    float gr = 0.f;
-    gr += wx * (hxp - hpc) * exp_;
-    gr += wy * (hyp - hpc) * eyp;
-    gr += wz * (hzp - hpc) * ezp;
-    gr += wxm * (hpc - hxm) * exm;
-    gr += wym * (hpc - hym) * eym;
-    gr += wzm * (hpc - hzm) * ezm;
+    const bool air = ldo(gm(a.flags), c) & FDTD_FLAG_AIR;
     f4v g = ldo(ACC, b);
-    float y = gr - g.y;
-    float s = g.x + y;
-    g.y = (s - g.x) - y;
-    g.x = s;
-    if (ldo(gm(a.flags), c) & FDTD_FLAG_AIR) {
-        bool pm = false;
-        for (int q = 0; q < a.n_probe; ++q)
-            pm |= a.probe[3 * q] == i && a.probe[3 * q + 1] == j && a.probe[3 * q + 2] == k;
-        const float pb = design_pb(a, P, i, j, k, c, n, pm, a.dec[2][k]);
-        y = pb * a.cp0 * ldo(hd, b) - g.w;
-        s = g.z + y;
-        g.w = (s - g.z) - y;
-        g.z = s;
+    for (int q = 0; q < count; ++q) {
+        const int64_t m = n - q;
+        const int par = static_cast<int>(m & 1);  // lambda_m lives in set m & 1
+        const GMEM float* __restrict__ WX = gm(a.f[par][1]);
+        const GMEM float* __restrict__ WY = gm(a.f[par][2]);
+        const GMEM float* __restrict__ WZ = gm(a.f[par][3]);
+        const GMEM float* __restrict__ hp = gm(a.hist_p + (m - a.hist0) * ebox_cells(a));
+        const GMEM float* __restrict__ hd = gm(a.hist_div + (m - a.hist0) * dbox_cells(a));
+        const float hpc = ldo(hp, e);
+        const float hxp = ldo(hp, e + static_cast<uint32_t>(ey * ez)), hyp = ldo(hp, e + static_cast<uint32_t>(ez)),
+                    hzp = ldo(hp, e + 1u);
+        const float hxm = ldo(hp, e - static_cast<uint32_t>(ey * ez)), hym = ldo(hp, e - static_cast<uint32_t>(ez)),
+                    hzm = ldo(hp, e - 1u);
+        const float wx = ldo(WX, c), wy = ldo(WY, c), wz = ldo(WZ, c);
+        const float wxm = ldo(WX, c - sx), wym = ldo(WY, c - static_cast<uint32_t>(nz)), wzm = ldo(WZ, c - 1u);
+        const float pb = ldo(gm(a.pbb[par]), b);
+        const float dv = ldo(hd, b);
+        float gr = 0.f;
+        gr += wx * (hxp - hpc) * exp_;
+        gr += wy * (hyp - hpc) * eyp;
+        gr += wz * (hzp - hpc) * ezp;
+        gr += wxm * (hpc - hxm) * exm;
+        gr += wym * (hpc - hym) * eym;
+        gr += wzm * (hpc - hzm) * ezm;
+        float y = gr - g.y;
+        float s = g.x + y;
+        g.y = (s - g.x) - y;
+        g.x = s;
+        if (air) {
+            y = pb * a.cp0 * dv - g.w;
+            s = g.z + y;
+            g.w = (s - g.z) - y;
+            g.z = s;
+        }
     }
     sto(ACC, b, g);
 }
@@ -1238,14 +1249,14 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
                         float cv0, float cp0, const int32_t* box_lo, const int32_t* box_hi, const float* rk,
                         int n_probe, const int32_t* probe_ijk, const float* g, int64_t steps_len, int64_t n_lo,
                         int64_t n_hi, const float* hist_p, const float* hist_div, int64_t hist0, int64_t hist_steps,
-                        float* acc, float* face_e, int planes, void* stream) {
+                        float* acc, float* face_e, float* pb_box, int planes, void* stream) {
     DesignArgs a;
     if (int rc = fill_design(a, lam, flags, dec, nx, ny, nz, cv0, cp0, box_lo, box_hi, rk, nullptr, nullptr, n_probe,
                              probe_ijk))
         return rc;
     if (n_hi < n_lo || n_lo < 0 || planes < 1 || (n_probe > 0 && !g)) return -2;
     if (n_probe > 0 && n_hi > steps_len) return -4;  // rows of g read: [n_lo, n_hi)
-    if (!hist_p || !hist_div || !acc || !face_e) return -2;
+    if (!hist_p || !hist_div || !acc || !face_e || !pb_box) return -2;
     if (n_lo < hist0 || n_hi > hist0 + hist_steps) return -2;  // every step needs its history
     a.g = g;
     a.hist_p = const_cast<float*>(hist_p);
@@ -1256,6 +1267,8 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
     a.face_e = face_e;
     const int blocks = design_grid(a, planes, kDesignAdjRows, kDesignAdjBoxRows);
     const int64_t box_cells = static_cast<int64_t>(a.hi[0] - a.lo[0]) * (a.hi[1] - a.lo[1]) * (a.hi[2] - a.lo[2]);
+    a.pbb[0] = pb_box;
+    a.pbb[1] = pb_box + box_cells;
     const int grad_blocks = static_cast<int>((box_cells + 255) / 256);
     const bool nts = static_cast<int64_t>(nx) * ny * nz > kCacheResidentCells;
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -1272,7 +1285,13 @@ int fdtd_design_adjoint(float* const* lam, const uint8_t* flags, const float* co
         else
             hipLaunchKernelGGL((k_design_adjoint<kDesignAdjRows, kDesignAdjBoxRows, false>), dim3(blocks),
                                dim3(kBlock), 0, st, a, n);
-        hipLaunchKernelGGL(k_design_grad, dim3(grad_blocks), dim3(256), 0, st, a, n);
+        // gradient terms two steps per pass: (n + 1, n) after the sweep of n,
+        // and a single last step when the count is odd
+        const int64_t swept = n_hi - n;
+        if (swept % 2 == 0)
+            hipLaunchKernelGGL(k_design_grad, dim3(grad_blocks), dim3(256), 0, st, a, n + 1, 2);
+        else if (n == n_lo)
+            hipLaunchKernelGGL(k_design_grad, dim3(grad_blocks), dim3(256), 0, st, a, n, 1);
         if (hipGetLastError() != hipSuccess) return -5;
     }
     return 0;

@@ -1,10 +1,11 @@
-"""Full-length oracle runs of the two large solve workloads (fixtures).
+"""Full-length oracle runs of the three solve workloads (fixtures).
 
-    python tests/golden/make_full_length.py [enclosure_512] [resonators_nu]
+    python tests/golden/make_full_length.py [plain_100] [resonators_nu] [enclosure_512]
 
 Runs the CPU oracle (oracle/fdtd_oracle.cpp, pinned bit for bit to the
 reference's own native kernels) in float32 and in float64 over the stated
 step counts of
+  plain_100      100^3, PML 10, 1 kHz pulse at the centre, one probe, 1000 steps;
   enclosure_512  512^3, PML 16, rigid SDF enclosure + port + two-pole ADE
                  sphere, 500 Hz pulse, 8 probes, 4000 steps;
   resonators_nu  320x320x480 nonuniform plate with 6x6 Helmholtz resonators,
@@ -63,6 +64,10 @@ def scenario(name: str):
                       materials={mid: _poles(material)}, sources=[("point", tuple(src), 500.0)],
                       probes=[tuple(p) for p in probes])
         return sc, 4000
+    if name == "plain_100":
+        sc = Scenario(shape=(100, 100, 100), pml_depth=10, sources=[("point", (50, 50, 50), 1000.0)],
+                      probes=[(75, 50, 50)])
+        return sc, 1000
     if name == "resonators_nu":
         grid, air_geo, probes, src_index = W.resonators_nu_scene()
         air = air_geo.voxelize(grid)
@@ -112,4 +117,4 @@ def main(names) -> None:
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["resonators_nu", "enclosure_512"])
+    main(sys.argv[1:] or ["plain_100", "resonators_nu", "enclosure_512"])

@@ -60,5 +60,5 @@ def test_design_calls_refuse_rows_past_their_tables_without_gpu():
     assert rc == -4
     # adjoint: steps [0, 12) against 10 rows of g
     rc = lib.fdtd_design_adjoint(*common, 1, fake[2], fake[4], 10, 0, 12, fake[5], fake[6], 0, 12, fake[7],
-                                 fake[1], 4, None)
+                                 fake[1], fake[3], 4, None)
     assert rc == -4

@@ -230,3 +230,58 @@ def test_fast_equals_cell_kernels_on_many_strips():
     for x, y in zip(out["cell"], out["fast"]):
         assert torch.equal(x, y)
     assert out["fast"][0].abs().max() > 0 and out["fast"][1].abs().max() > 0
+
+
+@pytest.mark.slow
+def test_adjoint_workload_192_vs_central_differences():
+    """The adjoint workload at its own size: 192^3 at 1 mm, PML 12, a 128^3
+    design box of rho and K = air * (1 + 0.05 N(0,1)), 1 kHz pulse, 4 probes,
+    2000 steps, L2 loss to seeded targets, gradient with a checkpoint every
+    100 steps and recomputation (float32 fast kernels, Kahan accumulation).
+    Checked against float64 central differences with h = 1e-4 relative on 32
+    seeded voxels (16 of rho, 16 of K), in the ratio the kernels read.
+
+    Bounds: every voxel |g - fd| <= 5e-6 ||g|| (||g|| the L2 norm of the whole
+    rho or K gradient; the float32 adjoint's rounding floor, measured at about
+    4e-7 ||g||), and |g - fd| <= 0.02 |fd| + 5e-6 ||g|| per voxel."""
+    import torch
+    from strata_fdtd_amd.adjoint import design_problem_192, design_ratio, forward_traces, probe_traces
+    pb, rho, K, target = design_problem_192()
+    assert pb.checkpoint == 100
+    print("[adjoint] 192^3 gradient (checkpoint every 100 steps)", flush=True)
+    r = rho.clone().requires_grad_(True)
+    k = K.clone().requires_grad_(True)
+    tr = probe_traces(r, k, pb, history="checkpoint")
+    ((tr - target) ** 2).sum().backward()
+    g_rho, g_K = r.grad.double(), k.grad.double()
+    norms = {"rho": float(g_rho.norm()), "K": float(g_K.norm())}
+    assert norms["rho"] > 0 and norms["K"] > 0
+    rng = np.random.default_rng(17)
+    t64 = target.double()
+
+    def loss64(rr, kk):
+        out, _ = forward_traces(pb, rr, kk, dtype=torch.float64)
+        return float(((out - t64) ** 2).sum())
+
+    rel = []
+    for v in range(32):
+        which = "rho" if v % 2 == 0 else "K"
+        base, grad, ref = (rho, g_rho, pb.rho) if which == "rho" else (K, g_K, pb.K0)
+        idx = tuple(int(rng.integers(0, s)) for s in pb.box_shape)
+        b64 = base.double()
+        plus, minus = b64.clone(), b64.clone()
+        plus[idx] *= 1 + 1e-4
+        minus[idx] *= 1 - 1e-4
+        # the step that reaches the kernels: the difference of the float32 ratios
+        dr = float(design_ratio(plus, ref)[idx].double() - design_ratio(minus, ref)[idx].double())
+        lp = loss64(plus if which == "rho" else rho.double(), plus if which == "K" else K.double())
+        lm = loss64(minus if which == "rho" else rho.double(), minus if which == "K" else K.double())
+        fd = (lp - lm) / (dr * ref)
+        g = float(grad[idx])
+        err = abs(g - fd)
+        rel.append(err / abs(fd))
+        print(f"{which}{idx}: adjoint {g:.6e} fd {fd:.6e} |g-fd|/|fd| {err / abs(fd):.2e} "
+              f"|g-fd|/||g|| {err / norms[which]:.2e}", flush=True)
+        assert err <= 5e-6 * norms[which], (which, idx)
+        assert err <= 0.02 * abs(fd) + 5e-6 * norms[which], (which, idx)
+    print(f"per-voxel relative error: median {np.median(rel):.2e}, max {np.max(rel):.2e}")

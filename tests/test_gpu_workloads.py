@@ -9,6 +9,12 @@
 * 320x320x480 nonuniform resonator plate (PML 16, plane source, 16 probes):
   bit-exact vs the oracle after 64 steps at full size, plus the same
   properties.
+* Full length at full size (tests/golden/full_*.npz, written by the CPU
+  oracle in float32 and float64 by tests/golden/make_full_length.py): the
+  100^3 solve over 1000 steps, the 512^3 enclosure over 4000 steps and the
+  plate over 3000 steps give the oracle's probe traces bit for bit and final
+  fields whose bytes hash to the oracle's; the error against the float64 run
+  is printed and bounded.
 
 Amplitude scaling is exact in float32 only while no value is subnormal: once
 gradual underflow rounds a doubled value differently, the two runs carry
@@ -16,12 +22,15 @@ independent rounding noise (measured on the CPU oracle at 96^3, 1200 steps:
 1.3e-6 to 1.9e-6 relative L2 per probe). The bound used is 1e-4.
 """
 
+import os
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("p", "vx", "vy", "vz")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def _oracle_check(solver, n):
@@ -96,3 +105,20 @@ def test_resonators_nonuniform_full_size():
     p_solid, health = _properties(wl.build, 600)
     assert p_solid == 0.0
     assert health == 0
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("name", ["plain_100", "resonators_nu", "enclosure_512"])
+def test_full_length_equals_oracle(name):
+    if not os.path.exists(os.path.join(GOLDEN, f"full_{name}.npz")):
+        pytest.skip(f"no full-length fixture for {name}")
+    import bench
+    print(f"[workload] {name} full length", flush=True)
+    r = bench.full_length_parity(name)
+    print(f"{name}: {r}", flush=True)
+    assert r["trace_bitexact"] and r["fields_bitexact"], r
+    # float32 against the float64 run of the same oracle over the whole run:
+    # the traces (the pulse and its reflections) within 1e-4; the final
+    # pressure, by then mostly absorbed and small against the rounding the
+    # run accumulated (1.2e-3 at 100^3 after 1000 steps), within 1e-2
+    assert r["trace_rel_l2_vs_f64"] < 1e-4 and r["p_sub_rel_l2_vs_f64"] < 1e-2, r

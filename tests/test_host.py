@@ -299,3 +299,15 @@ def test_bench_names_the_sweep_instance_rocprof_reports():
     assert bench.kernel_name("k_fused_step", 2, 1, True, cr) == "k_fused_step<2,false,uint32_t,false>"
     assert bench.kernel_name("k_fused_step", 4, 1, False, 1 << 30) == "k_fused_step<4,true,long,true>"
     assert bench.kernel_name("k_fused_stack", 4, 2, True, 512 ** 3) == "k_fused_stack<4,false,2>"
+
+
+def test_bench_traffic_only_for_the_profiled_instance():
+    """roofline.traffic comes from the committed PMC summary only when that
+    summary names the workload and the kernel instance the run launches."""
+    import bench
+    k = bench.kernel_name("k_fused_step", 4, 1, True, 512 ** 3)
+    nbytes, note = bench.traffic_from_profiles("enclosure_512", k)
+    assert nbytes and nbytes > 4.4e9 and "FETCH_SIZE" in note
+    nbytes, note = bench.traffic_from_profiles("enclosure_512", bench.kernel_name("k_fused_step", 2, 1, True, 512 ** 3))
+    assert nbytes is None and "this run launches" in note
+    assert bench.traffic_from_profiles("no_such_workload", k)[0] is None
