@@ -206,16 +206,17 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
     log(f"serving: {len(reqs)} requests, Poisson rate {rate}/s, "
         f"{sum(r.cells * r.steps for r in reqs) / 1e9:.1f} Gcell-updates in total")
     # default server: cache-resident active set (6M cells), least work first
+    # with an age credit
     srv = SweepServer(chunk=chunk)
     batched = srv.serve(reqs, realtime=True)
     srv.close()
     log(f"serving batched: {batched.summary()}")
     torch.cuda.empty_cache()
-    # least work first with an age credit
-    srv = SweepServer(chunk=chunk, policy="aged")
-    aged = srv.serve(reqs, realtime=True)
+    # least work first without the age credit
+    srv = SweepServer(chunk=chunk, policy="sjf")
+    sjf = srv.serve(reqs, realtime=True)
     srv.close()
-    log(f"serving batched, aged least-work-first: {aged.summary()}")
+    log(f"serving batched, least work first: {sjf.summary()}")
     torch.cuda.empty_cache()
     # every arrival admitted at once, in arrival order
     srv = SweepServer(chunk=chunk, max_cells=None, policy="fifo")
@@ -232,8 +233,9 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
     return {
         "requests": len(reqs), "arrival_rate_per_s": rate, "chunk_steps": chunk,
         "gcell_updates_total": round(sum(r.cells * r.steps for r in reqs) / 1e9, 2),
-        "batched": batched.summary(), "batched_policy": "<= 6M active cells (cache-resident), least work first",
-        "batched_aged": aged.summary(),
+        "batched": batched.summary(),
+        "batched_policy": "<= 6M active cells (cache-resident), least work first with an age credit",
+        "batched_least_work_first": sjf.summary(),
         "batched_all_admitted_fifo": fifo.summary(),
         "one_at_a_time": single.summary(),
         "speedup_solves_per_s": round(batched.solves_per_s / single.solves_per_s, 3),
@@ -353,6 +355,8 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true",
                     help="skip the full-length parity runs against the oracle fixtures")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference check on N voxels")
+    ap.add_argument("--adjoint-steps", type=int, default=2000,
+                    help="steps of the adjoint workload (2000; fewer for short profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 256 MiB device copy first (reference kernel for rocprof byte counters)")
     args = ap.parse_args()
@@ -478,7 +482,7 @@ def main() -> int:
             line["serving"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_adjoint:
         try:
-            line["adjoint"] = adjoint_bench(fd_voxels=args.adjoint_fd)
+            line["adjoint"] = adjoint_bench(steps=args.adjoint_steps, fd_voxels=args.adjoint_fd)
             log(f"adjoint: {json.dumps({k: v for k, v in line['adjoint'].items() if k != 'finite_differences'})}")
         except Exception as exc:
             line["adjoint"] = {"error": f"{type(exc).__name__}: {exc}"}
