@@ -302,3 +302,27 @@ def test_ade_phase_kernels_bitexact(oracle_lib):
     cpu.phase("apply_ade_pressure_correction", h["p"], h["Jd"], h["Jl"], mat, poles, n, K_inf, 1.6e-6, nx, ny, nz)
     for k in f:
         _same(d[k].cpu().numpy(), h[k], k)
+
+
+@pytest.mark.slow
+def test_wide_offsets_beyond_2_30_cells(oracle_lib):
+    """A grid above 2^30 cells (1024 x 1040 x 1009 = 1,074,544,640) takes the
+    sweep's 64-bit-offset instantiation; nz = 1009 leaves a partial last
+    k segment. Bit-exact against the oracle over 4 steps: traces and fields."""
+    shape = (1024, 1040, 1009)
+    assert np.prod(shape, dtype=np.int64) > (1 << 30)
+    print(f"[wide] {shape}: oracle", flush=True)
+    sc = seeded_scenario(13, shape=shape, pml=4, geometry=True)
+    sc.sources = [("point", (512, 520, 504), 20000.0)]
+    sc.probes = [(512, 520, 505), (511, 519, 504), (513, 521, 503), (1000, 1030, 1000)]
+    n = 4
+    st_o, tr_o = cpu.run(oracle_problem(sc, n), n)
+    print(f"[wide] {shape}: device", flush=True)
+    s = make_solver(sc)
+    s.step_n(n)
+    tr = s.get_probe_data()
+    for q in range(len(sc.probes)):
+        _same(tr[f"probe{q}"], tr_o[:, q], f"probe{q}")
+    for k in FIELDS:
+        _same(getattr(s, k), st_o[k], k)
+    assert np.abs(tr_o[:, 0]).max() > 0

@@ -23,9 +23,10 @@ from helpers import Scenario, oracle_problem, rel_l2, seeded_scenario
 from oracle import cpu, tables as T
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# step fixtures (the voxel-mask fixture is read by tests/test_voxel_golden.py)
+# step fixtures (the voxel-mask fixture is read by tests/test_voxel_golden.py,
+# the full-length ones by the GPU workload tests)
 GOLDEN = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "*.npz"))
-                if not os.path.basename(p).startswith("voxel_"))
+                if not os.path.basename(p).startswith(("voxel_", "full_")))
 FIELDS = ("p", "vx", "vy", "vz")
 
 
