@@ -191,7 +191,14 @@ def new_state(shape, dtype=np.float32):
 
 def run(pb: Problem, n_steps: int, state: dict | None = None, impl: str = "oracle",
         step0: int = 0) -> tuple[dict, np.ndarray]:
-    """Advance ``state`` (created zeroed if None) by ``n_steps``; returns (state, trace)."""
+    """Advance ``state`` (created zeroed if None) by ``n_steps``; returns (state, trace).
+
+    The ADE pole accumulators live only inside one call (they start at zero),
+    so a problem with poles must run from step 0 in a single call: continuing
+    it with step0 > 0 would restart its poles and is refused."""
+    if pb.poles and step0 != 0:
+        raise ValueError("a problem with ADE poles runs from step 0 in one call "
+                         "(pole state does not carry across calls)")
     dtype = np.float64 if impl == "oracle64" else np.float32
     if state is None:
         state = new_state(pb.shape, dtype)

@@ -39,7 +39,7 @@ from helpers import Scenario, oracle_problem  # noqa: E402
 from oracle import cpu  # noqa: E402
 
 SUB = 16          # sub-lattice stride of the stored final pressure
-CHUNK = 250       # steps per oracle call (progress lines)
+CHUNK = 250       # steps per oracle call (progress lines) of scenes without ADE poles
 
 
 def _poles(material) -> list:
@@ -82,8 +82,11 @@ def run(pb, steps, impl, label):
     st = cpu.new_state(pb.shape, dtype)
     traces = []
     t0 = time.time()
-    for s0 in range(0, steps, CHUNK):
-        n = min(CHUNK, steps - s0)
+    # the oracle keeps ADE pole state within one call only: scenes with
+    # poles run all their steps in a single call
+    chunk = steps if pb.poles else CHUNK
+    for s0 in range(0, steps, chunk):
+        n = min(chunk, steps - s0)
         st, tr = cpu.run(pb, n, state=st, impl=impl, step0=s0)
         traces.append(tr.copy())
         print(f"{label} {impl}: {s0 + n}/{steps} steps, {time.time() - t0:.0f} s", flush=True)

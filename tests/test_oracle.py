@@ -266,6 +266,27 @@ def test_f32_oracle_close_to_f64(oracle_lib):
     assert rel_l2(t32, t64) < 1e-4
 
 
+def test_ade_run_refuses_continuation(oracle_lib):
+    """The ADE pole accumulators start at zero in every oracle call, so
+    splitting an ADE run over calls changes its result: the oracle refuses a
+    continuation (step0 > 0) of a problem with poles and accepts one without."""
+    sc = seeded_scenario(12, shape=(20, 18, 24), pml=3, geometry=True, ade=True)
+    pb = oracle_problem(sc, 40)
+    assert pb.poles
+    whole, tw = cpu.run(pb, 40)
+    st, t1 = cpu.run(pb, 20)
+    with pytest.raises(ValueError, match="ADE poles"):
+        cpu.run(pb, 20, state=st, step0=20)
+    # what a continuation would give: the poles restarted at step 20
+    pb_plain = oracle_problem(seeded_scenario(12, shape=(20, 18, 24), pml=3, geometry=True), 40)
+    a, _ = cpu.run(pb_plain, 20)
+    a, _ = cpu.run(pb_plain, 20, state=a, step0=20)
+    b, _ = cpu.run(pb_plain, 40)
+    for k in FIELDS:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.abs(whole["p"]).max() > 0 and np.array_equal(tw[:20], t1)
+
+
 @needs_ref
 def test_oracle_equals_reference_with_long_lists(oracle_lib):
     """12 point sources (two cells with two each), 100 probes, 10 poles in
