@@ -91,6 +91,17 @@ struct AdePole {
     float corr;                     // correction coefficient
 };
 
+// Pole q of a table that no kernel writes while it runs: read through the
+// constant address space, so a wave-uniform q becomes one scalar load (as the
+// kernel arguments are) instead of a vector load per lane and a wait.
+__device__ __forceinline__ AdePole pole_at(const AdePole* t, int q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) AdePole*)t)[q];
+#else
+    return t[q];  // the host pass only parses device functions
+#endif
+}
+
 // Probes recorded inside the sweep (one wave lane tests one probe); probes
 // beyond these are recorded by a small kernel after the step's last pass.
 constexpr int kSweepProbes = 64;

@@ -490,7 +490,7 @@ __device__ __forceinline__ void ade_density_part(const FusedArgs& a, int64_t fir
         if (!m) continue;
         const float pv = P[c];
         for (int q = 0; q < a.n_pole; ++q) {
-            const AdePole pl = a.poles[q];
+            const AdePole pl = pole_at(a.poles, q);
             if (pl.target != 0 || pl.mat != m) continue;
             float* Jq = a.J + q * box_n;
             if (!pl.lorentz) {
@@ -529,7 +529,7 @@ __device__ __forceinline__ float ade_face(const FusedArgs& a, const float* __res
         const int ih = i + (axis == 0), jh = j + (axis == 1), kh = k + (axis == 2);
         for (int pass = 0; pass < 2; ++pass)
             for (int q = 0; q < a.n_pole; ++q) {
-                const AdePole pl = a.poles[q];
+                const AdePole pl = pole_at(a.poles, q);
                 if (pl.target != 0 || pl.lorentz != pass || pl.mat != m_lo) continue;
                 const float* Jq = a.J + q * box_n;
                 v += pl.corr * (Jq[box_index(a, ih, jh, kh)] - Jq[box_index(a, i, j, k)]);
@@ -568,7 +568,7 @@ __global__ void k_ade_fixup(const FusedArgs a) {
         const int64_t c = lin(i, j, k, ny, nz);
         const int m = a.material[c];
         bool has_pole = false;
-        for (int q = 0; q < a.n_pole; ++q) has_pole |= a.poles[q].mat == m;
+        for (int q = 0; q < a.n_pole; ++q) has_pole |= pole_at(a.poles, q).mat == m;
         if (!m || !has_pole) continue;
         const uint32_t f = a.flags[c];
         const float vxn = ade_face<NU>(a, P, VX, 0, i, j, k, c, sx);
@@ -586,7 +586,7 @@ __global__ void k_ade_fixup(const FusedArgs a) {
         dA += dvz * iz;
         for (int pass = 0; pass < 2; ++pass)
             for (int q = 0; q < a.n_pole; ++q) {
-                const AdePole pl = a.poles[q];
+                const AdePole pl = pole_at(a.poles, q);
                 if (pl.target != 1 || pl.lorentz != pass || pl.mat != m) continue;
                 float* Jq = a.J + q * box_n;
                 if (!pass) {
@@ -604,7 +604,7 @@ __global__ void k_ade_fixup(const FusedArgs a) {
         else pnew = P[c] + a.cp * ((dvx + dvy) + dvz);
         for (int pass = 0; pass < 2; ++pass)
             for (int q = 0; q < a.n_pole; ++q) {
-                const AdePole pl = a.poles[q];
+                const AdePole pl = pole_at(a.poles, q);
                 if (pl.target != 1 || pl.lorentz != pass || pl.mat != m) continue;
                 pnew += pl.corr * a.J[q * box_n + b];
             }
