@@ -180,6 +180,13 @@ int fdtd_time_sweeps(fdtd_ctx* ctx, int n_steps, float* ms_sweep, float* ms_tota
  * Measurement instrumentation; no reference counterpart. */
 int fdtd_stream_ceiling(fdtd_ctx* ctx, int reps, float* ms_per_pass);
 
+/* The same streaming work in the fused sweep's own order: one lane per k
+ * cell, the context's rows x planes strips per wave and the sweep's block
+ * remap, with no stencil and no halo; the sweep's cost beyond it is its
+ * stencil and halo traffic. Grids below 2^30 cells, rows 2, 4 or 8. Timing as
+ * fdtd_stream_ceiling. Measurement instrumentation; no reference counterpart. */
+int fdtd_stream_ceiling_march(fdtd_ctx* ctx, int reps, float* ms_per_pass);
+
 /* ---- batched serving (no counterpart in the reference) ------------------- */
 /* A batch advances many independent solves (uniform grids, no ADE) by one step
  * per launch: waves are dealt out over the solves' strips, each solve keeps its

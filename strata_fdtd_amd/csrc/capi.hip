@@ -20,6 +20,7 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
                         hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
 hipError_t launch_stream_ceiling(const FusedArgs& a, hipStream_t st);
+hipError_t launch_stream_march(const FusedArgs& a, int rows, int planes, hipStream_t st);
 hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
                         int cap_waves, int rows, bool nts, hipStream_t st);
 hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
@@ -594,6 +595,27 @@ int fdtd_stream_ceiling(fdtd_ctx* c, int reps, float* ms_per_pass) {
     if (int rc = check(hipEventRecord(c->ev0, c->stream), "event record")) return rc;
     for (int r = 0; r < reps; ++r)
         if (int rc = check(launch_stream_ceiling(c->a, c->stream), "stream launch")) return rc;
+    if (int rc = check(hipEventRecord(c->ev1, c->stream), "event record")) return rc;
+    if (int rc = check(hipEventSynchronize(c->ev1), "event sync")) return rc;
+    float ms = 0.f;
+    if (int rc = check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "elapsed time")) return rc;
+    *ms_per_pass = ms / reps;
+    return 0;
+}
+
+int fdtd_stream_ceiling_march(fdtd_ctx* c, int reps, float* ms_per_pass) {
+    if (int rc = ready(c)) return rc;
+    if (!ms_per_pass) return fail(-2, "null output");
+    if (reps < 1 || reps > 1000) return fail(-2, "reps must be in [1, 1000]");
+    if (static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz >= kMaxCells32)
+        return fail(-2, "the marching pass takes grids below 2^30 cells");
+    if (c->rows != 2 && c->rows != 4 && c->rows != 8) return fail(-2, "rows must be 2, 4 or 8");
+    DeviceGuard g(c->device);
+    auto pass = [&]() { return launch_stream_march(c->a, c->rows, c->planes, c->stream); };
+    if (int rc = check(pass(), "stream launch")) return rc;  // warm-up
+    if (int rc = check(hipEventRecord(c->ev0, c->stream), "event record")) return rc;
+    for (int r = 0; r < reps; ++r)
+        if (int rc = check(pass(), "stream launch")) return rc;
     if (int rc = check(hipEventRecord(c->ev1, c->stream), "event record")) return rc;
     if (int rc = check(hipEventSynchronize(c->ev1), "event sync")) return rc;
     float ms = 0.f;

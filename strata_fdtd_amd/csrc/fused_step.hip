@@ -917,6 +917,71 @@ hipError_t launch_stream_ceiling(const FusedArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// The same streaming work in the sweep's own order: lane = k, R rows by
+// `planes` planes per wave, the sweep's strip numbering and XCD-aware block
+// remap, and no stencil or halo. What the sweep costs beyond this pass is its
+// stencil and halo traffic. Grids below 2^30 cells.
+template <int R, bool NT>
+__global__ __launch_bounds__(kBlock) void k_stream_march(const FusedArgs a) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
+    const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (w >= a.nwaves) return;
+    const int seg = w % a.nseg;
+    const int t = w / a.nseg;
+    const int sj = t % a.nstrip_j, si = t / a.nstrip_j;
+    const int k = seg * kWave + lane;
+    if (k >= a.nz) return;
+    const int i0 = si * a.planes, i1 = min(i0 + a.planes, a.nx);
+    const int par = static_cast<int>(*a.step_ptr & 1);
+    const GMEM uint8_t* __restrict__ FL = gmem(a.flags);
+    for (int i = i0; i < i1; ++i) {
+        float v[R][4], b[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = sj * R + r;
+            b[r] = 0.f;
+            if (j >= a.ny) continue;
+            const uint32_t c = lidx<uint32_t>(i, j, k, a.ny, a.nz);
+            b[r] = static_cast<float>(ldx(FL, c));
+#pragma unroll
+            for (int g = 0; g < 4; ++g) v[r][g] = ldx(gmem(static_cast<const float*>(a.f[par][g])), c);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int j = sj * R + r;
+            if (j >= a.ny) continue;
+            const uint32_t c = lidx<uint32_t>(i, j, k, a.ny, a.nz);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                if (NT) stx_nt(gmem(a.f[par ^ 1][g]), c, v[r][g] + b[r]);
+                else stx(gmem(a.f[par ^ 1][g]), c, v[r][g] + b[r]);
+            }
+        }
+    }
+}
+
+template <int R>
+static hipError_t launch_march_r(const FusedArgs& a, int blocks, bool nt, hipStream_t st) {
+    if (nt) hipLaunchKernelGGL((k_stream_march<R, true>), dim3(blocks), dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL((k_stream_march<R, false>), dim3(blocks), dim3(kBlock), 0, st, a);
+    return hipGetLastError();
+}
+
+// Enqueue one marching pass with the sweep's rows (2, 4 or 8), planes and
+// store kind.
+hipError_t launch_stream_march(const FusedArgs& a_in, int rows, int planes, hipStream_t st) {
+    FusedArgs a = a_in;
+    const int blocks = fused_grid(a, rows, planes, 1);
+    const bool nt = static_cast<int64_t>(a.nx) * a.ny * a.nz > kCacheResidentCells;
+    switch (rows) {
+        case 2: return launch_march_r<2>(a, blocks, nt, st);
+        case 4: return launch_march_r<4>(a, blocks, nt, st);
+        case 8: return launch_march_r<8>(a, blocks, nt, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st) {
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step, n);
     return hipGetLastError();

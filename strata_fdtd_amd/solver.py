@@ -731,14 +731,17 @@ class FDTDSolver:
         self._advance_host(n)
         return float(ms_sweep.value), float(ms_total.value)
 
-    def stream_ceiling(self, reps: int = 20) -> float:
+    def stream_ceiling(self, reps: int = 20, order: str = "linear") -> float:
         """Milliseconds per pass of a streaming kernel with the sweep's bytes
         per cell and no stencil, on this solve's buffers (state unchanged):
-        the practical bandwidth ceiling the sweep is measured against."""
+        the practical bandwidth ceiling the sweep is measured against.
+        order "linear": 16-byte accesses in memory order; "march": the
+        sweep's own strips (one lane per k cell, its rows x planes)."""
         import ctypes as C
         self._prepare(1)
         ms = C.c_float()
-        _native.call("fdtd_stream_ceiling", self._ctx, reps, C.byref(ms))
+        fn = {"linear": "fdtd_stream_ceiling", "march": "fdtd_stream_ceiling_march"}[order]
+        _native.call(fn, self._ctx, reps, C.byref(ms))
         return float(ms.value)
 
     def step_n(self, n: int, graph_chunk: int = 0) -> None:

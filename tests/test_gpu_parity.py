@@ -195,16 +195,19 @@ def test_reset_restarts(oracle_lib):
     assert np.array_equal(s.get_probe_data()["probe3"], first)
 
 
-def test_stream_ceiling_pass_leaves_the_solve_unchanged(oracle_lib):
-    """The bandwidth-ceiling pass writes p, vx, vy, vz + flag byte of every
-    cell into the idle buffer set (an odd cell count exercises its 4-cell
-    tail) and leaves the solve's state, and so every later step, as it was."""
+@pytest.mark.parametrize("order", ["linear", "march"])
+def test_stream_ceiling_pass_leaves_the_solve_unchanged(oracle_lib, order):
+    """The bandwidth-ceiling pass (16-byte accesses in memory order, or the
+    sweep's own strips) writes p, vx, vy, vz + flag byte of every cell into
+    the idle buffer set (an odd cell count exercises the 4-cell tail of the
+    first, ragged rows and k segments the strips of the second) and leaves
+    the solve's state, and so every later step, as it was."""
     torch = _torch()
     sc = seeded_scenario(3, shape=(13, 11, 9), pml=3)
     a, b = make_solver(sc), make_solver(sc)
     a.step_n(7)
     b.step_n(7)
-    ms = a.stream_ceiling(3)
+    ms = a.stream_ceiling(3, order=order)
     torch.cuda.synchronize()
     assert ms > 0
     cur = a._cur
