@@ -193,11 +193,12 @@ def parity_bench():
     return out
 
 
-def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
-    """Continuous batching vs one-at-a-time on the seeded Poisson design sweep."""
+def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100, steps=(2000, 8000)):
+    """Continuous batching vs one-at-a-time on the seeded Poisson design sweep
+    (step counts drawn from `steps`; fewer only for short profiling runs)."""
     import torch
     from strata_fdtd_amd.serving import SweepServer, serve_one_at_a_time, sweep_requests
-    reqs = sweep_requests(n=n_requests, rate=rate)
+    reqs = sweep_requests(n=n_requests, rate=rate, steps=tuple(steps))
     # warm both paths (code objects, allocator) on a tiny stream first
     warm = sweep_requests(n=3, rate=1e6, seed=99, steps=(20, 40), dims=(64,))
     SweepServer(chunk=10).serve(warm, realtime=False)
@@ -231,7 +232,7 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100):
         all(np.array_equal(a.traces[k], b.traces[k]) for k in a.traces)
         for a, b in zip(sorted(batched.results, key=lambda r: r.rid), sorted(single.results, key=lambda r: r.rid)))
     return {
-        "requests": len(reqs), "arrival_rate_per_s": rate, "chunk_steps": chunk,
+        "requests": len(reqs), "arrival_rate_per_s": rate, "chunk_steps": chunk, "request_steps": list(steps),
         "gcell_updates_total": round(sum(r.cells * r.steps for r in reqs) / 1e9, 2),
         "batched": batched.summary(),
         "batched_policy": "<= 6M active cells (cache-resident), least work first with an age credit",
@@ -355,6 +356,8 @@ def main() -> int:
     ap.add_argument("--no-parity", action="store_true",
                     help="skip the full-length parity runs against the oracle fixtures")
     ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference check on N voxels")
+    ap.add_argument("--serving-steps", default="2000,8000",
+                    help="step-count range of the served requests (2000,8000; fewer for short profiling runs)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the adjoint workload (2000; fewer for short profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -488,7 +491,7 @@ def main() -> int:
         torch.cuda.empty_cache()
     if not args.no_serving:
         try:
-            line["serving"] = serving_bench()
+            line["serving"] = serving_bench(steps=tuple(int(v) for v in args.serving_steps.split(",")))
         except Exception as exc:
             line["serving"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_adjoint:
