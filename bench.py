@@ -193,9 +193,11 @@ def parity_bench():
     return out
 
 
-def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100, steps=(2000, 8000)):
+def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100, steps=(2000, 8000),
+                  batched_only: bool = False):
     """Continuous batching vs one-at-a-time on the seeded Poisson design sweep
-    (step counts drawn from `steps`; fewer only for short profiling runs)."""
+    (step counts drawn from `steps`; fewer only for short profiling runs).
+    batched_only: run the default batched server alone (profiling its kernel)."""
     import torch
     from strata_fdtd_amd.serving import SweepServer, serve_one_at_a_time, sweep_requests
     reqs = sweep_requests(n=n_requests, rate=rate, steps=tuple(steps))
@@ -213,6 +215,11 @@ def serving_bench(n_requests: int = 64, rate: float = 40.0, chunk: int = 100, st
     srv.close()
     log(f"serving batched: {batched.summary()}")
     torch.cuda.empty_cache()
+    if batched_only:
+        return {"requests": len(reqs), "arrival_rate_per_s": rate, "chunk_steps": chunk,
+                "request_steps": list(steps),
+                "gcell_updates_total": round(sum(r.cells * r.steps for r in reqs) / 1e9, 2),
+                "batched": batched.summary(), "note": "default batched server only (profiling run)"}
     # least work first without the age credit
     srv = SweepServer(chunk=chunk, policy="sjf")
     sjf = srv.serve(reqs, realtime=True)
@@ -358,6 +365,8 @@ def main() -> int:
     ap.add_argument("--adjoint-fd", type=int, default=0, help="finite-difference check on N voxels")
     ap.add_argument("--serving-steps", default="2000,8000",
                     help="step-count range of the served requests (2000,8000; fewer for short profiling runs)")
+    ap.add_argument("--serving-batched-only", action="store_true",
+                    help="serve with the default batched server only (no comparison legs; profiling)")
     ap.add_argument("--adjoint-steps", type=int, default=2000,
                     help="steps of the adjoint workload (2000; fewer for short profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -491,7 +500,8 @@ def main() -> int:
         torch.cuda.empty_cache()
     if not args.no_serving:
         try:
-            line["serving"] = serving_bench(steps=tuple(int(v) for v in args.serving_steps.split(",")))
+            line["serving"] = serving_bench(steps=tuple(int(v) for v in args.serving_steps.split(",")),
+                                            batched_only=args.serving_batched_only)
         except Exception as exc:
             line["serving"] = {"error": f"{type(exc).__name__}: {exc}"}
     if not args.no_adjoint:
