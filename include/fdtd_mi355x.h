@@ -199,7 +199,9 @@ int fdtd_batch_destroy(fdtd_batch* b);
 /* (Re)build the device table from n ready contexts; step_limits[q] is the
  * total step count of solve q. Set-up call (may allocate, synchronises). */
 int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_limits, int n);
-/* Enqueue n batched steps (use_graph: capture once per size, then replay). */
+/* Enqueue n batched steps (use_graph: capture once per size, then replay):
+ * n launches, step k of each solve's chunk counted from its counter, then one
+ * small launch that moves every counter by n, up to its limit. */
 int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph);
 int fdtd_batch_info(fdtd_batch* b, int* n, int* total_waves);
 

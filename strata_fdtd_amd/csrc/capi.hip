@@ -21,8 +21,9 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
 hipError_t launch_stream_ceiling(const FusedArgs& a, hipStream_t st);
 hipError_t launch_stream_march(const FusedArgs& a, int rows, int planes, hipStream_t st);
-hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
-                        int cap_waves, int rows, bool nts, hipStream_t st);
+hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_waves, int rows,
+                        bool nts, int koff, hipStream_t st);
+hipError_t launch_batch_advance(const FusedArgs* table, const int* n_ptr, int cap_solves, int n, hipStream_t st);
 hipError_t phase_velocity(const float*, float*, float*, float*, int, int, int, float, const float*,
                           const float*, const float*, hipStream_t);
 hipError_t phase_rigid(float*, float*, float*, const uint8_t*, int, int, int, hipStream_t);
@@ -989,13 +990,14 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
     return 0;
 }
 
+// n_steps launches (step koff = 0 .. n_steps - 1 of the chunk, each counted
+// from the solves' counters), then one launch that moves the counters.
 static int batch_enqueue(fdtd_batch* b, hipStream_t st, int n_steps, int cap_waves) {
     for (int s = 0; s < n_steps; ++s)
-        if (int rc = check(launch_batch(b->table_dev, b->meta_dev + 1, b->meta_dev, b->cap, cap_waves, b->rows,
-                                        b->nts, st),
+        if (int rc = check(launch_batch(b->table_dev, b->meta_dev + 1, b->meta_dev, cap_waves, b->rows, b->nts, s, st),
                            "batch step launch"))
             return rc;
-    return 0;
+    return check(launch_batch_advance(b->table_dev, b->meta_dev, b->cap, n_steps, st), "batch advance launch");
 }
 
 int fdtd_batch_step_n(fdtd_batch* b, int n_steps, int use_graph) {

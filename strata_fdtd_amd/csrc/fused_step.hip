@@ -87,7 +87,7 @@ __device__ __forceinline__ void record_probe(const FusedArgs& a, const GMEM int3
 // source lookups (see below); the batch kernel, at its register cap, keeps the
 // plain loops.
 template <int R, bool NU, int W, bool NTS, typename IX, bool LP>
-__device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj, int si, int lane,
+__device__ __forceinline__ void fused_strip(const FusedArgs& a, int64_t s, int seg, int sj, int si, int lane,
                                             float* lds, int wi) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int k0 = seg * kWave;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void fused_strip(const FusedArgs& a, int seg, int sj,
     const int i1 = min(i0 + a.planes, nx);
     const IX sx = static_cast<IX>(ny) * static_cast<IX>(nz);
 
-    const int64_t s = *a.step_ptr + a.koff;
+    // s: index of the step this launch takes
     const int par = static_cast<int>(s & 1);
     // Every buffer is device (global) memory. Saying so lets the compiler use
     // global loads with a scalar base even when the arguments come from a
@@ -392,10 +392,11 @@ __attribute__((amdgpu_waves_per_eu((NU && R == 4 && sizeof(IX) == 4) ? 7 : 1, 8)
     // wave index within the block, made provably wave-uniform (scalar registers)
     const int w = lb * kBlockWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (w >= a.nwaves) return;
-    if (*a.step_ptr + a.koff >= a.step_limit) return;
+    const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, NU, 1, NTS, IX, true>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, NU, 1, NTS, IX, true>(a, s, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
 // Stacked variant: workgroup = W waves over W consecutive j-strips of one
@@ -410,17 +411,19 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
     const int wi = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lb = (blockIdx.x & 7) * a.blocks_per_xcd + (blockIdx.x >> 3);
     if (lb >= a.nwaves) return;                                  // whole workgroup
-    if (*a.step_ptr + a.koff >= a.step_limit) return;            // whole workgroup
+    const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit) return;                               // whole workgroup
     const int nstack_j = (a.nstrip_j + W - 1) / W;
     const int seg = lb % a.nseg;
     const int t = lb / a.nseg;
-    fused_strip<R, NU, W, true, uint32_t, true>(a, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
+    fused_strip<R, NU, W, true, uint32_t, true>(a, s, seg, (t % nstack_j) * W + wi, t / nstack_j, lane, lds, wi);
 }
 
-// Batched step: one launch advances every solve in `table` by one step. Waves
-// are dealt out by solve through the prefix sums `offsets` (n + 1 entries);
-// a solve whose counter reached its limit is left untouched. Solves below 2^30
-// cells.
+// Batched step: one launch advances every solve in `table` by one step, step
+// `koff` of a chunk whose first step is each solve's counter (the counters
+// move once per chunk, by k_batch_advance). Waves are dealt out by solve
+// through the prefix sums `offsets` (n + 1 entries); a solve whose step
+// reached its limit is left untouched. Solves below 2^30 cells.
 // The number of solves is read from device memory (n_ptr), so a captured
 // graph stays valid while the host rewrites the table between chunks; the
 // grid is sized for the table's wave capacity and surplus waves return.
@@ -428,7 +431,7 @@ __global__ __launch_bounds__(W * kWave) void k_fused_stack(const FusedArgs a) {
 // compiler picks unguided; measured about 8% faster on the 64-request sweep.
 template <int R, bool NTS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_batch_step(
-    const FusedArgs* __restrict__ table, const int* __restrict__ offsets, const int* __restrict__ n_ptr) {
+    const FusedArgs* __restrict__ table, const int* __restrict__ offsets, const int* __restrict__ n_ptr, int koff) {
     const int lane = threadIdx.x & (kWave - 1);
     // XCD-aware remap over the blocks the live table needs (the grid may be
     // larger): XCD x takes logical blocks [x * bpx, (x + 1) * bpx)
@@ -447,17 +450,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8))) 
     }
     const FusedArgs& a = table[lo];
     const int w = gw - offsets[lo];
-    if (*a.step_ptr >= a.step_limit) return;
+    const int64_t s = *a.step_ptr + koff;
+    if (s >= a.step_limit) return;
     const int seg = w % a.nseg;
     const int t = w / a.nseg;
-    fused_strip<R, false, 1, NTS, uint32_t, false>(a, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
+    fused_strip<R, false, 1, NTS, uint32_t, false>(a, s, seg, t % a.nstrip_j, t / a.nstrip_j, lane, nullptr, 0);
 }
 
-__global__ void k_batch_advance(const FusedArgs* __restrict__ table, const int* __restrict__ n_ptr) {
+// End of a chunk of n batched steps: every solve's counter moves by n, up to
+// its limit.
+__global__ void k_batch_advance(const FusedArgs* __restrict__ table, const int* __restrict__ n_ptr, int n) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= *n_ptr) return;
     int64_t* step = const_cast<int64_t*>(table[e].step_ptr);
-    if (step[0] < table[e].step_limit) step[0] = step[0] + 1;
+    const int64_t v = step[0], lim = table[e].step_limit;
+    if (v < lim) step[0] = v + n < lim ? v + n : lim;
 }
 
 // ---- ADE --------------------------------------------------------------------
@@ -847,28 +854,31 @@ hipError_t launch_fused(const FusedArgs& a_in, int rows, int stack, bool nu, boo
 // `cap_waves` waves (the grid covers the capacity; n is read on the device).
 template <bool NTS>
 static hipError_t launch_batch_t(const FusedArgs* table, const int* offsets, const int* n_ptr, int blocks, int rows,
-                                 hipStream_t st) {
+                                 int koff, hipStream_t st) {
     switch (rows) {
-        case 2: hipLaunchKernelGGL((k_batch_step<2, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
-        case 4: hipLaunchKernelGGL((k_batch_step<4, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
-        case 8: hipLaunchKernelGGL((k_batch_step<8, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr); break;
+        case 2: hipLaunchKernelGGL((k_batch_step<2, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, koff); break;
+        case 4: hipLaunchKernelGGL((k_batch_step<4, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, koff); break;
+        case 8: hipLaunchKernelGGL((k_batch_step<8, NTS>), dim3(blocks), dim3(kBlock), 0, st, table, offsets, n_ptr, koff); break;
         default: return hipErrorInvalidValue;
     }
     return hipSuccess;
 }
 
-// nts: non-temporal stores (a batch whose solves together exceed the
-// Infinity Cache), else ordinary stores.
-hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_solves,
-                        int cap_waves, int rows, bool nts, hipStream_t st) {
+// Step koff of a chunk. nts: non-temporal stores (a batch whose solves
+// together exceed the Infinity Cache), else ordinary stores.
+hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_waves, int rows,
+                        bool nts, int koff, hipStream_t st) {
     int blocks = (cap_waves + kBlockWaves - 1) / kBlockWaves;
     blocks = std::max(8, (blocks + 7) & ~7);
-    const hipError_t le = nts ? launch_batch_t<true>(table, offsets, n_ptr, blocks, rows, st)
-                              : launch_batch_t<false>(table, offsets, n_ptr, blocks, rows, st);
+    const hipError_t le = nts ? launch_batch_t<true>(table, offsets, n_ptr, blocks, rows, koff, st)
+                              : launch_batch_t<false>(table, offsets, n_ptr, blocks, rows, koff, st);
     if (le != hipSuccess) return le;
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_batch_advance, dim3((cap_solves + 63) / 64), dim3(64), 0, st, table, n_ptr);
+    return hipGetLastError();
+}
+
+// After the n steps of a chunk: the solves' counters move by n.
+hipError_t launch_batch_advance(const FusedArgs* table, const int* n_ptr, int cap_solves, int n, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_advance, dim3((cap_solves + 63) / 64), dim3(64), 0, st, table, n_ptr, n);
     return hipGetLastError();
 }
 

@@ -117,8 +117,13 @@ def test_full_length_equals_oracle(name):
     r = bench.full_length_parity(name)
     print(f"{name}: {r}", flush=True)
     assert r["trace_bitexact"] and r["fields_bitexact"], r
-    # float32 against the float64 run of the same oracle over the whole run:
-    # the traces (the pulse and its reflections) within 1e-4; the final
-    # pressure, by then mostly absorbed and small against the rounding the
-    # run accumulated (1.2e-3 at 100^3 after 1000 steps), within 1e-2
-    assert r["trace_rel_l2_vs_f64"] < 1e-4 and r["p_sub_rel_l2_vs_f64"] < 1e-2, r
+    # float32 against the float64 run of the same oracle over the whole run
+    # (bit-exact above, so these are the float32 oracle's own errors): the
+    # traces (the pulse and its reflections) within 1e-4 in the open and
+    # absorbing scenes, 1e-3 in the enclosure, a resonant rigid cavity that
+    # keeps its energy for the 4000 steps and accumulates rounding in its
+    # ringing (2.6e-4 measured); the final pressure, in the open scenes mostly
+    # absorbed and small against the rounding the run accumulated (1.2e-3 at
+    # 100^3 after 1000 steps), within 1e-2
+    trace_bound = {"enclosure_512": 1e-3}.get(name, 1e-4)
+    assert r["trace_rel_l2_vs_f64"] < trace_bound and r["p_sub_rel_l2_vs_f64"] < 1e-2, r
