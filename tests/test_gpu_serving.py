@@ -72,3 +72,53 @@ def test_batch_beyond_the_cache_equals_solo_runs():
         s.step_n(r.steps)
         for name, tr in s.get_probe_data().items():
             assert np.array_equal(res.traces[name], tr)
+
+
+@pytest.mark.parametrize("rows", [2, 4])
+def test_batch_fields_equal_solo_runs(rows):
+    """Solves whose k extent leaves a partial last 64-lane segment of 16, 32
+    or 48 cells (nz = 80, 96, 112, 144), every cell starting nonzero: after
+    steps in chunks of 7 (the last chunk partial for every solve), all four
+    final fields and the probe traces of every solve equal its solo run bit
+    for bit, and the step counters the chunks moved stop at each limit."""
+    import torch
+    from strata_fdtd_amd import _native
+    from strata_fdtd_amd.serving import build_solver, sweep_requests
+    # nz = 96, 80, 144 and 112
+    reqs = sweep_requests(n=4, rate=1e6, seed=5, steps=(30, 50), dims=(80, 96, 112, 144))
+    print(f"[serving] batch fields, rows={rows}: " + ", ".join(f"{r.shape} x{r.steps}" for r in reqs), flush=True)
+
+    def seeded(r, q):
+        # every cell starts nonzero, so every cell's arithmetic is compared
+        s = build_solver(r, tiling=(rows, 2, 1))
+        g = np.random.default_rng(100 + q)
+        for k in ("p", "vx", "vy", "vz"):
+            getattr(s, k)[...] = g.standard_normal(r.shape).astype(np.float32)
+        return s
+
+    batch = [seeded(r, q) for q, r in enumerate(reqs)]
+    for s, r in zip(batch, reqs):
+        s._prepare(r.steps)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    h = _native._vp()
+    _native.call("fdtd_batch_create", 0, stream.cuda_stream, rows, _native.C.byref(h))
+    try:
+        ctxs = (_native._vp * len(batch))(*[s._ctx for s in batch])
+        limits = np.array([r.steps for r in reqs], dtype=np.int64)
+        _native.call("fdtd_batch_set", h.value, ctxs, limits.ctypes.data, len(batch))
+        for _ in range(0, max(r.steps for r in reqs), 7):
+            _native.call("fdtd_batch_step_n", h.value, 7, 0)
+        torch.cuda.synchronize()
+    finally:
+        _native.load().fdtd_batch_destroy(h.value)
+    for s, r in zip(batch, reqs):
+        s._advance_host(r.steps)
+        solo = seeded(r, reqs.index(r))
+        solo.step_n(r.steps)
+        assert int(s._step_dev.item()) == r.steps
+        for k in ("p", "vx", "vy", "vz"):
+            a, b = getattr(s, k), getattr(solo, k)
+            assert np.array_equal(a, b), (r.shape, k, int((a != b).sum()))
+        for name, tr in solo.get_probe_data().items():
+            assert np.array_equal(s.get_probe_data()[name], tr), (r.shape, name)
