@@ -24,9 +24,10 @@ from oracle import cpu, tables as T
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # step fixtures (the voxel-mask fixture is read by tests/test_voxel_golden.py,
-# the full-length ones by the GPU workload tests)
+# the full-length ones by the GPU workload tests, the organ pipes by
+# tests/test_organ_pipes.py and tests/test_gpu_organ_pipes.py)
 GOLDEN = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "*.npz"))
-                if not os.path.basename(p).startswith(("voxel_", "full_")))
+                if not os.path.basename(p).startswith(("voxel_", "full_", "organ_")))
 FIELDS = ("p", "vx", "vy", "vz")
 
 
