@@ -24,10 +24,11 @@ from oracle import cpu, tables as T
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # step fixtures (the voxel-mask fixture is read by tests/test_voxel_golden.py,
-# the full-length ones by the GPU workload tests, the organ pipes by
-# tests/test_organ_pipes.py and tests/test_gpu_organ_pipes.py)
+# the full-length ones by the GPU workload tests, the reference's recorded
+# runs by tests/test_organ_pipes.py, tests/test_simple_pulse.py and their GPU
+# counterparts)
 GOLDEN = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "*.npz"))
-                if not os.path.basename(p).startswith(("voxel_", "full_", "organ_")))
+                if not os.path.basename(p).startswith(("voxel_", "full_", "organ_", "simple_pulse")))
 FIELDS = ("p", "vx", "vy", "vz")
 
 
