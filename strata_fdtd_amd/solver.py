@@ -769,22 +769,41 @@ class FDTDSolver:
             energy_sample_interval: int = 1, output_file: str | None = None,
             script_content: str | None = None, callback: Callable[[int], None] | None = None,
             snapshot_interval: int | None = None) -> None:
-        """Run ceil(duration/dt) steps (core/solver.py:2520-2606)."""
-        if output_file:
-            raise NotImplementedError("HDF5 output is not implemented on the mi355x backend")
+        """Run ceil(duration/dt) steps (core/solver.py:2520-2606).
+
+        output_file: an HDF5 file in the reference's result layout
+        (io/hdf5.py), written when the run ends (also when it ends with an
+        exception), with the pressure field after every step whose index in
+        this run is a multiple of snapshot_interval."""
+        import time as _time_mod
         n_steps = int(np.ceil(duration / self.dt))
         self._track_energy = track_energy
         self._energy_sample_interval = energy_sample_interval
         if track_energy and not self._energy_history:
             self._energy_history.append((self._step_count, self._time, self.compute_energy()))
-        per_step = bool(callback or track_energy or self._snapshot_interval or progress)
-        if per_step:
-            for _ in range(n_steps):
-                self.step()
-                if callback:
-                    callback(self._step_count - 1)
-        else:
-            self.step_n(n_steps)
+        h5_snaps = [] if output_file and snapshot_interval else None
+        started = _time_mod.time()
+        per_step = bool(callback or track_energy or self._snapshot_interval or progress or h5_snaps is not None)
+        try:
+            if per_step:
+                for step in range(n_steps):
+                    t_before = self._time
+                    self.step()
+                    if h5_snaps is not None and step % snapshot_interval == 0:
+                        h5_snaps.append((t_before, self._host_field(0).copy()))
+                        self._exported.discard(0)
+                    if callback:
+                        callback(self._step_count - 1)
+            else:
+                self.step_n(n_steps)
+        finally:
+            if output_file:
+                from datetime import datetime, timezone
+
+                from . import hdf5_out
+                tree = hdf5_out.solver_tree(self, h5_snaps, snapshot_interval, _time_mod.time() - started,
+                                            script_content, datetime.now(timezone.utc).isoformat())
+                hdf5_out.write(str(output_file), tree)
         self.check_health()
         if self._warn_energy_drift and len(self._energy_history) >= 2:
             rep = self.energy_report()

@@ -15,12 +15,16 @@ the cadence of a snapshot every 5 steps (FDTDSolver.enable_snapshots(5)).
 The file is read with the minimal parser beside this script (h5py is not
 installed here). Stored in ``simple_pulse.npz``: the run's parameters, the
 recorded probe traces, the 13 snapshots on the sub-lattice of every second
-cell per axis, and a SHA-256 of the oracle's float32 probe traces.
+cell per axis, a SHA-256 of the oracle's float32 probe traces, and the file's
+layout (objects, shapes, attribute names) with the attributes that describe
+the run (grid, simulation, source), for comparison with a file the product
+writes.
 """
 
 from __future__ import annotations
 
 import hashlib
+import json
 import os
 import sys
 
@@ -78,6 +82,19 @@ def main() -> None:
         sub=np.array(SUB, np.int64), snapshots_sub=np.ascontiguousarray(snaps[(slice(None),) + sub]),
         snapshot0=snaps[0], backend=np.array(str(meta["backend"])),
         oracle_traces_sha256=np.array(hashlib.sha256(np.ascontiguousarray(tr).tobytes()).hexdigest()))
+    # the file's layout (every object's kind, shape and attribute names) and
+    # the run-describing attributes, for a product-written file to match
+    layout = {}
+    for opath, node in H.walk(r):
+        kind = "dataset" if isinstance(node, H.Dataset) else "group"
+        layout[opath] = {"kind": kind, "attrs": sorted(node.attrs),
+                         "shape": list(node.shape) if kind == "dataset" else None}
+    values = {}
+    for g in ("grid", "simulation", "sources/source_0"):
+        values[g] = {k: (v.tolist() if isinstance(v, np.ndarray) else (v.item() if hasattr(v, "item") else v))
+                     for k, v in r[g].attrs.items()}
+    out["layout_json"] = np.array(json.dumps(layout, sort_keys=True))
+    out["attrs_json"] = np.array(json.dumps(values, sort_keys=True))
     path = os.path.join(HERE, "simple_pulse.npz")
     np.savez_compressed(path, **out)
     print(f"{shape}, {steps} steps, probes {names}, {snaps.shape[0]} snapshots; wrote {path} "

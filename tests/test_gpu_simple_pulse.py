@@ -53,3 +53,57 @@ def test_product_reproduces_recorded_simple_pulse():
         if t == 0:
             assert np.array_equal(p, d["snapshot0"])
         assert rel_l2(p[sub], d["snapshots_sub"][t]) < 1e-6, t
+
+
+def test_run_writes_reference_result_file(tmp_path):
+    """run(duration, output_file=..., snapshot_interval=5) on the same run
+    writes an HDF5 file in the reference's result layout: read back with the
+    parser that reads the reference's own sample, it has the sample's
+    objects, shapes and attribute names, the sample's grid, simulation and
+    source attributes, the probe traces the solver returns, the pressure
+    snapshots (the first equal to the recorded one bit for bit, all within
+    1e-6 of the recorded ones) and the air mask."""
+    import json
+    import os
+    import sys
+    import strata_fdtd_amd as sfa
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import hdf5_read as H
+    d = load()
+    shape = tuple(int(v) for v in d["shape"])
+    steps = int(d["steps"])
+    s = sfa.FDTDSolver(grid=sfa.UniformGrid(shape, float(d["dx"])), c=float(d["c"]), rho=float(d["rho"]),
+                       backend="mi355x")
+    s.set_geometry(air_mask(d))
+    s.add_source(sfa.GaussianPulse(position=tuple(int(v) for v in d["source_position"]),
+                                   frequency=float(d["source_frequency"]),
+                                   bandwidth=float(d["source_bandwidth"])))
+    names = [str(n) for n in d["probe_names"]]
+    for name, pos in zip(names, d["probe_positions"]):
+        s.add_probe(name, tuple(int(v) for v in pos))
+    path = tmp_path / "result.h5"
+    s.run(duration=steps * s.dt * (1 - 1e-9), output_file=str(path), snapshot_interval=int(d["snapshot_every"]))
+    f = H.File(str(path))
+    layout = {}
+    for opath, node in H.walk(f.root):
+        kind = "dataset" if isinstance(node, H.Dataset) else "group"
+        layout[opath] = {"kind": kind, "attrs": sorted(node.attrs),
+                         "shape": list(node.shape) if kind == "dataset" else None}
+    assert layout == json.loads(str(d["layout_json"]))
+    want = json.loads(str(d["attrs_json"]))
+    for g, attrs in want.items():
+        got = f.root[g].attrs
+        for k, v in attrs.items():
+            gv = got[k]
+            gv = gv.tolist() if isinstance(gv, np.ndarray) else (gv.item() if hasattr(gv, "item") else gv)
+            assert gv == v, (g, k, gv, v)
+    probes = s.get_probe_data()
+    for n in names:
+        assert np.array_equal(f.root[f"probes/{n}"].read(), probes[n].astype(np.float32))
+    p = f.root["fields/pressure"].read()
+    assert np.array_equal(p[0], d["snapshot0"])
+    sub = (slice(None),) + (slice(None, None, int(d["sub"])),) * 3
+    for t in range(p.shape[0]):
+        assert rel_l2(p[sub][t], d["snapshots_sub"][t]) < 1e-6, t
+    assert np.array_equal(f.root["materials/geometry"].read(), air_mask(d).astype(np.uint8))
+    assert f.root["metadata"].attrs["backend"] == "mi355x"

@@ -1,0 +1,82 @@
+"""The product's HDF5 writer (strata_fdtd_amd/hdf5_out.py) read back by the
+minimal HDF5 parser of tests/golden/hdf5_read.py, which reads the
+reference's own h5py-written sample file (tests/test_simple_pulse.py pins
+what it extracts from that file against the oracle). Every attribute type
+the result layout uses, nested and empty groups, a group with more members
+than the default symbol-table node holds, and datasets of float32, float64,
+int64 and uint8."""
+
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+
+import hdf5_read as H  # noqa: E402
+from strata_fdtd_amd import hdf5_out as W  # noqa: E402
+
+
+def test_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    root = W.Group()
+    meta = root.group("metadata")
+    meta.attrs.update(created_at="2026-01-01T00:00:00+00:00", solver_version="0.1.0", backend="mi355x",
+                      num_threads=np.int64(1), total_runtime_seconds=0.25, note="ünïcode")
+    grid = root.group("grid")
+    grid.attrs.update(shape=np.array([5, 6, 7], np.int64), is_uniform=True, resolution=0.002,
+                      extent=np.array([0.01, 0.012, 0.014]))
+    root.group("empty")
+    f32 = rng.standard_normal((3, 5, 6, 7)).astype(np.float32)
+    root.group("fields").dataset("pressure", f32, units="Pa", snapshot_interval=np.int64(5))
+    geo = (rng.random((5, 6, 7)) > 0.3).astype(np.uint8)
+    root.group("materials").dataset("geometry", geo)
+    probes = root.group("probes")
+    traces = {}
+    for q in range(40):     # five symbol-table nodes of 8 names
+        traces[f"probe{q:02d}"] = rng.standard_normal(17).astype(np.float32)
+        probes.dataset(f"probe{q:02d}", traces[f"probe{q:02d}"], position=np.array([q, 1, 2], np.int64),
+                       units="Pa")
+    root.group("grid").dataset("x_coords", np.linspace(0, 1, 5))
+    root.group("deep").group("er").attrs["flag"] = False
+    path = tmp_path / "out.h5"
+    W.write(str(path), root)
+
+    data = path.read_bytes()
+    assert data[:8] == b"\x89HDF\r\n\x1a\n"
+    assert int.from_bytes(data[40:48], "little") == len(data)      # end-of-file address
+    f = H.File(str(path))
+    r = f.root
+    assert sorted(r.keys()) == sorted(["metadata", "grid", "empty", "fields", "materials", "probes", "deep"])
+    a = r["metadata"].attrs
+    assert a["created_at"] == "2026-01-01T00:00:00+00:00" and a["note"] == "ünïcode"
+    assert a["num_threads"] == 1 and a["total_runtime_seconds"] == 0.25 and a["backend"] == "mi355x"
+    g = r["grid"].attrs
+    assert np.array_equal(g["shape"], [5, 6, 7]) and g["shape"].dtype == np.int64
+    assert g["is_uniform"] is np.True_ or g["is_uniform"] == True  # noqa: E712
+    assert g["resolution"] == 0.002 and np.array_equal(g["extent"], [0.01, 0.012, 0.014])
+    assert r["empty"].keys() == []
+    p = r["fields/pressure"]
+    assert p.shape == f32.shape and np.array_equal(p.read(), f32) and p.read().dtype == np.float32
+    assert p.attrs["units"] == "Pa" and p.attrs["snapshot_interval"] == 5
+    assert np.array_equal(r["materials/geometry"].read(), geo)
+    assert sorted(r["probes"].keys()) == sorted(traces)
+    for name, tr in traces.items():
+        d = r[f"probes/{name}"]
+        assert np.array_equal(d.read(), tr)
+        assert np.array_equal(d.attrs["position"], [int(name[5:]), 1, 2])
+    assert np.array_equal(r["grid/x_coords"].read(), np.linspace(0, 1, 5))
+    assert bool(r["deep/er"].attrs["flag"]) is False
+
+
+def test_group_size_limit(tmp_path):
+    root = W.Group()
+    g = root.group("many")
+    for q in range(4 * W.LEAF_K * W.INTERNAL_K + 1):
+        g.dataset(f"d{q}", np.zeros(1, np.float32))
+    try:
+        W.write(str(tmp_path / "x.h5"), root)
+    except ValueError as e:
+        assert "at most" in str(e)
+    else:
+        raise AssertionError("a group beyond one B-tree leaf of symbol-table nodes must be refused")
