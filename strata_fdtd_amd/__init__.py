@@ -19,13 +19,13 @@ from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
 from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, Pole, PoleType, SimpleMaterial
 from .membranes import CircularMembraneSource, MembraneSource, RectangularMembraneSource
 from .microphones import POLAR_PATTERNS, Microphone
-from .solver import FDTDSolver, has_mi355x_backend
+from .solver import FDTDSolver, has_gpu_support, has_mi355x_backend, has_native_kernels
 from .sources import GaussianPulse, Probe
 
 __version__ = "0.1.0"
 
 __all__ = [
-    "FDTDSolver", "has_mi355x_backend", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
+    "FDTDSolver", "has_mi355x_backend", "has_native_kernels", "has_gpu_support", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
     "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder",
     "Union", "Intersection", "Difference", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",

@@ -49,6 +49,22 @@ def has_mi355x_backend() -> bool:
         return False
 
 
+def has_native_kernels() -> bool:
+    """The reference's availability check (core/solver.py:95-101) for this
+    backend: True if the compiled HIP kernel library loads."""
+    try:
+        _native.load()
+        return True
+    except OSError:
+        return False
+
+
+def has_gpu_support() -> bool:
+    """The reference's GPU check (core/solver_gpu.py:101-107) for this
+    backend: True if the kernel library loads and a GPU is visible."""
+    return has_mi355x_backend()
+
+
 def _fast_waveform(source, times: NDArray[np.float64], dt: float) -> NDArray[np.float64]:
     """Vectorised sampling when it reproduces the per-step calls exactly."""
     if isinstance(source, GaussianPulse):

@@ -42,6 +42,16 @@ def test_version_and_error_string_without_gpu():
     assert b"context" in lib.fdtd_last_error()
 
 
+def test_availability_checks():
+    """The reference's has_native_kernels / has_gpu_support: the built library
+    loads; a GPU is reported only where torch sees one."""
+    import torch
+
+    import strata_fdtd_amd as S
+    assert S.has_native_kernels() is True
+    assert S.has_gpu_support() == torch.cuda.is_available()
+
+
 def test_design_calls_refuse_rows_past_their_tables_without_gpu():
     """fdtd_design_forward / fdtd_design_adjoint check the trace, waveform and
     g rows a call would touch against steps_len on the host, before any
