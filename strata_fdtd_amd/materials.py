@@ -108,6 +108,12 @@ class SimpleMaterial:
     def c_inf(self) -> float:
         return float(np.sqrt(self.K_inf / self.rho_inf))
 
+    @classmethod
+    def from_impedance(cls, name: str, rho: float, Z: float) -> "SimpleMaterial":
+        """A material of density rho and acoustic impedance Z (c = Z / rho),
+        materials/base.py:543-555."""
+        return cls(name=name, _rho=rho, _c=Z / rho)
+
     def n_poles(self) -> int:
         return len(self.poles)
 

@@ -88,6 +88,13 @@ def test_waveform_table_matches_oracle():
         assert np.array_equal(waveform_table(src, t, dt), T.waveform_table(t, f))
 
 
+def test_material_from_impedance():
+    from strata_fdtd_amd import SimpleMaterial
+    m = SimpleMaterial.from_impedance("water-like", rho=998.2, Z=998.2 * 1482.0)
+    assert m.name == "water-like" and m.rho_inf == 998.2
+    assert abs(m.c_inf - 1482.0) < 1e-9 and abs(m.K_inf - 998.2 * 1482.0**2) < 1e-3
+
+
 def test_pole_and_correction_coefficients_match_oracle():
     dt = T.time_step(1e-3, 343.0)
     p = Pole(PoleType.DEBYE, 0.3, "density", tau=2e-5)
