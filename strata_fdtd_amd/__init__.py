@@ -5,7 +5,7 @@ Drop-in surface (the reference's names and semantics, ``backend="mi355x"``):
 ``FDTDSolver``, ``UniformGrid``, ``NonuniformGrid``, ``PML``, ``GaussianPulse``,
 membrane sources (``CircularMembraneSource``, ``RectangularMembraneSource``),
 ``Microphone``,
-SDF geometry (``Box``, ``Sphere``, ``Cylinder``, ``Union``, ``Intersection``,
+SDF geometry (``Box``, ``Sphere``, ``Cylinder``, ``Cone``, ``Union``, ``Intersection``,
 ``Difference``, ``HelmholtzResonator``, ``MaterializedGeometry``) and ADE
 materials (``Pole``, ``PoleType``, ``SimpleMaterial``, ``AIR_20C``).
 Beyond the reference: ``serving`` (continuous batching of many solves) and
@@ -13,7 +13,7 @@ Beyond the reference: ``serving`` (continuous batching of many solves) and
 """
 
 from .boundaries import PML, RigidBoundary
-from .geometry import (Box, Cylinder, Difference, HelmholtzResonator, Intersection,
+from .geometry import (Box, Cone, Cylinder, Difference, HelmholtzResonator, Intersection,
                        MaterializedGeometry, SDFPrimitive, Sphere, Union)
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
 from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, Pole, PoleType, SimpleMaterial
@@ -26,7 +26,7 @@ __version__ = "0.1.0"
 
 __all__ = [
     "FDTDSolver", "has_mi355x_backend", "has_native_kernels", "has_gpu_support", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
-    "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder",
+    "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder", "Cone",
     "Union", "Intersection", "Difference", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
     "POLAR_PATTERNS", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",

@@ -1,18 +1,19 @@
-// Device voxeliser: CSG trees of boxes, spheres and axis-aligned cylinders
+// Device voxeliser: CSG trees of boxes, spheres and axis-aligned cylinders and cones
 // evaluated in float64 at every cell centre, giving the air mask (sdf <= 0)
 // and the material ids of MaterializedGeometry, bit-identical to the
 // reference's numpy evaluation.
 //
 // Reference formulas (geometry/sdf.py): Box :274-294, Sphere :320-333,
-// Cylinder :382-425, Union/Intersection/Difference :566-722, voxelize :99-127;
+// Cylinder :382-425, Cone :436-543, Union/Intersection/Difference :566-722,
+// voxelize :99-127;
 // material ids geometry/material_assignment.py:133-168 (later regions
 // override earlier ones, only inside air).
 //
 // Exactness. Every float64 value a test depends on is formed with the same
 // IEEE operations in the same order as numpy forms it: np.linalg.norm over
-// axis 1 is sqrt((x0*x0 + x1*x1) + x2*x2); `** 2` is x*x; the cylinder's
-// np.dot with an axis-aligned unit vector is exactly +-v_axis (every other
-// product is a signed zero); sqrt is correctly rounded in both. The file is
+// axis 1 is sqrt((x0*x0 + x1*x1) + x2*x2); `** 2` is x*x; the cylinder's and
+// cone's np.dot with an axis-aligned unit vector is exactly +-v_axis (every
+// other product is a signed zero); sqrt is correctly rounded in both. The file is
 // built with -ffp-contract=off, so no multiply-add is fused. CSG nodes
 // combine the per-primitive tests as booleans: min(a, b) <= 0 iff a <= 0 or
 // b <= 0; max(a, b) <= 0 iff both; max(b, -s) <= 0 iff b <= 0 and not s < 0
@@ -24,6 +25,7 @@
 //   VOX_BOX  arg = offset of {cx, cy, cz, hx, hy, hz} in params   push test
 //   VOX_SPH  arg = offset of {cx, cy, cz, r}                      push test
 //   VOX_CYL  arg = offset of {p1x, p1y, p1z, nx, ny, nz, len, r, axis}
+//   VOX_CONE arg = offset of {p1x, p1y, p1z, nx, ny, nz, len, r1, r2, axis}
 //   the strict variants (+ VOX_STRICT) test sdf < 0 instead of sdf <= 0
 //   VOX_OR / VOX_AND / VOX_ANDNOT   pop b, pop a, push a|b, a&b, a&!b
 //   VOX_TRUE / VOX_FALSE            push a constant
@@ -47,6 +49,7 @@ enum VoxOp : int32_t {
     VOX_OR = 5,
     VOX_AND = 6,
     VOX_ANDNOT = 7,
+    VOX_CONE = 8,
     VOX_STRICT = 16,
 };
 
@@ -109,6 +112,30 @@ __device__ bool eval_program(const VoxProgram& pg, int m, double x, double y, do
             const double rr = radial - r;
             const double d = body ? rr : sqrt(rr * rr + axial * axial);
             bit = cmp_le(d, strict);
+        } else if (code == VOX_CONE) {
+            const double* P = pg.params + arg;
+            const int ax = static_cast<int>(P[9]);
+            const double v0 = x - P[0], v1 = y - P[1], v2 = z - P[2];
+            const double va = ax == 0 ? v0 : (ax == 1 ? v1 : v2);
+            const double na = ax == 0 ? P[3] : (ax == 1 ? P[4] : P[5]);  // +-1
+            const double len = P[6], r1 = P[7], r2 = P[8];
+            // t = np.dot(v, n) / length; np.clip(t, 0, 1)
+            const double t = (na > 0.0 ? va : -va) / len;
+            double tc = t > 0.0 ? t : 0.0;
+            tc = tc < 1.0 ? tc : 1.0;
+            const double rt = r1 + tc * (r2 - r1);
+            // closest = p1 + (tc * length) * n: only the axis component moves
+            const double s = tc * len;
+            const double c0 = ax == 0 ? P[0] + s * na : P[0];
+            const double c1 = ax == 1 ? P[1] + s * na : P[1];
+            const double c2 = ax == 2 ? P[2] + s * na : P[2];
+            const double radial = norm3(x - c0, y - c1, z - c2);
+            const double ad = fabs(t - 0.5) - 0.5;
+            const double axial = (0.0 >= ad ? 0.0 : ad) * len;  // np.maximum(0, ad) * length
+            const bool body = (t >= 0.0) && (t <= 1.0);
+            const double rr = radial - rt;
+            const double d = body ? rr : sqrt(rr * rr + axial * axial);
+            bit = cmp_le(d, strict);
         } else if (code == VOX_TRUE || code == VOX_FALSE) {
             bit = code == VOX_TRUE;
         } else {
@@ -155,22 +182,24 @@ const char* vox_validate(const int32_t* ops, int n_ops, const int32_t* start, in
         for (int q = start[m]; q < start[m + 1]; ++q) {
             const int32_t op = ops[2 * q], arg = ops[2 * q + 1];
             const int32_t code = op & ~VOX_STRICT;
-            if ((op & ~(VOX_STRICT | 7)) != 0) return "unknown opcode";
+            if ((op & ~(VOX_STRICT | 15)) != 0 || code > VOX_CONE) return "unknown opcode";
             int need = 0;
             if (code == VOX_BOX) need = 6;
             else if (code == VOX_SPH) need = 4;
             else if (code == VOX_CYL) need = 9;
+            else if (code == VOX_CONE) need = 10;
             if (need) {
                 if (arg < 0 || arg + need > n_params) return "parameter offset out of range";
-                if (code == VOX_CYL) {
-                    const double axd = params[arg + 8];
-                    if (axd != 0.0 && axd != 1.0 && axd != 2.0) return "cylinder axis must be 0, 1 or 2";
+                if (code == VOX_CYL || code == VOX_CONE) {
+                    const double axd = params[arg + need - 1];
+                    if (axd != 0.0 && axd != 1.0 && axd != 2.0) return "cylinder or cone axis must be 0, 1 or 2";
                     const int ax = static_cast<int>(axd);
                     for (int d = 0; d < 3; ++d) {
                         const double nd = params[arg + 3 + d];
                         if (d == ax ? (nd != 1.0 && nd != -1.0) : nd != 0.0)
-                            return "cylinder direction must be a signed unit axis";
+                            return "cylinder or cone direction must be a signed unit axis";
                     }
+                    if (!(params[arg + 6] > 0.0)) return "cylinder or cone length must be positive";
                 }
                 ++depth;
             } else if (code == VOX_TRUE || code == VOX_FALSE) {

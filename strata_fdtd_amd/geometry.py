@@ -2,7 +2,8 @@
 
 The signed-distance formulas, CSG rules and voxelisation rule follow the
 reference's ``geometry/sdf.py`` (SDFPrimitive.voxelize :99-127, Box :274-294,
-Sphere :320-333, Cylinder :382-425, Union/Intersection/Difference :566-722),
+Sphere :320-333, Cylinder :382-425, Cone :436-543,
+Union/Intersection/Difference :566-722),
 ``geometry/resonator.py`` (HelmholtzResonator :137-207) and
 ``geometry/material_assignment.py`` (MaterializedGeometry :48-205). The masks
 must match the reference bit for bit, so every formula uses the same float64
@@ -94,8 +95,11 @@ def _mask(node, grid, strict: bool) -> NDArray[np.bool_]:
             q = np.abs(c - node.center[ax]) - half[ax]
             ms.append((q < 0) if strict else (q <= 0))
         return ms[0][:, None, None] & ms[1][None, :, None] & ms[2][None, None, :]
-    if isinstance(node, (Sphere, Cylinder)):
-        if isinstance(node, Cylinder) and np.count_nonzero(node.axis_normalized) != 1:
+    if isinstance(node, (Sphere, Cylinder, Cone)):
+        # an oblique axis goes through np.dot's BLAS kernel, whose arithmetic
+        # may depend on a point's place in the array: evaluate every cell at
+        # once, as the reference's voxelize does
+        if isinstance(node, (Cylinder, Cone)) and np.count_nonzero(node.axis_normalized) != 1:
             return _direct(node, xc, yc, zc, None if np.prod(shape) <= (1 << 25) else VOXEL_SLAB_POINTS,
                            strict)
         lo, hi = node.bounding_box
@@ -213,6 +217,46 @@ class Cylinder(SDFPrimitive):
     @property
     def bounding_box(self):
         r = np.array([self.radius] * 3)
+        return np.minimum(self.p1, self.p2) - r, np.maximum(self.p1, self.p2) + r
+
+
+class Cone(SDFPrimitive):
+    """Truncated cone (frustum) between p1 (radius r1) and p2 (radius r2),
+    geometry/sdf.py:436-543: the radius interpolated linearly along the axis,
+    the radial distance inside the body and, beyond the caps, the radial
+    and axial excesses combined as sqrt(a^2 + b^2)."""
+
+    def __init__(self, p1, p2, r1: float, r2: float):
+        self.p1 = np.array(p1, dtype=np.float64)
+        self.p2 = np.array(p2, dtype=np.float64)
+        self.r1 = float(r1)
+        self.r2 = float(r2)
+        if self.r1 < 0 or self.r2 < 0:
+            raise ValueError(f"Cone radii must be non-negative, got r1={r1}, r2={r2}")
+        if self.r1 == 0 and self.r2 == 0:
+            raise ValueError("Cone cannot have both radii zero")
+        self.axis = self.p2 - self.p1
+        self.length = np.linalg.norm(self.axis)
+        if self.length == 0:
+            raise ValueError("Cone endpoints must be distinct")
+        self.axis_normalized = self.axis / self.length
+
+    def sdf(self, points):
+        points = _points(points)
+        v = points - self.p1
+        t = np.dot(v, self.axis_normalized) / self.length
+        t_clamped = np.clip(t, 0, 1)
+        radius_at_t = self.r1 + t_clamped * (self.r2 - self.r1)
+        closest = self.p1 + (t_clamped * self.length)[:, np.newaxis] * self.axis_normalized
+        radial = np.linalg.norm(points - closest, axis=1)
+        axial = np.maximum(0, np.abs(t - 0.5) - 0.5) * self.length
+        inside_body = (t >= 0) & (t <= 1)
+        return np.where(inside_body, radial - radius_at_t,
+                        np.sqrt((radial - radius_at_t) ** 2 + axial**2))
+
+    @property
+    def bounding_box(self):
+        r = np.array([max(self.r1, self.r2)] * 3)
         return np.minimum(self.p1, self.p2) - r, np.maximum(self.p1, self.p2) + r
 
 

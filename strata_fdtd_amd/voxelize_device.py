@@ -2,7 +2,8 @@
 (``csrc/voxelize.hip``, C ABI ``fdtd_voxelize``).
 
 The kernel evaluates every primitive with the reference's float64 formulas
-(geometry/sdf.py: Box :274-294, Sphere :320-333, Cylinder :382-425) at each
+(geometry/sdf.py: Box :274-294, Sphere :320-333, Cylinder :382-425, Cone
+:436-543) at each
 cell centre and combines them as booleans (Union :566-585 -> or,
 Intersection :629-648 -> and, Difference :704-722 -> and-not with the
 subtrahend's test made strict, since ``max(b, -s) <= 0`` iff ``b <= 0`` and not
@@ -10,11 +11,11 @@ subtrahend's test made strict, since ``max(b, -s) <= 0`` iff ``b <= 0`` and not
 (geometry/material_assignment.py:133-168): later regions override earlier ones,
 only inside air.
 
-Supported: Box, Sphere, Cylinder whose axis is parallel to a grid axis, Union,
-Intersection, Difference and HelmholtzResonator built from those. For an
-oblique cylinder the reference's ``np.dot`` goes through BLAS, whose summation
-order is not fixed, so such trees raise ``Unsupported`` and the caller uses the
-host voxeliser.
+Supported: Box, Sphere, Cylinder and Cone whose axis is parallel to a grid
+axis, Union, Intersection, Difference and HelmholtzResonator built from those.
+For an oblique cylinder or cone the reference's ``np.dot`` goes through BLAS,
+whose summation order is not fixed, so such trees raise ``Unsupported`` and
+the caller uses the host voxeliser.
 """
 
 from __future__ import annotations
@@ -24,9 +25,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
-from .geometry import Box, Cylinder, Difference, HelmholtzResonator, Intersection, Sphere, Union
+from .geometry import Box, Cone, Cylinder, Difference, HelmholtzResonator, Intersection, Sphere, Union
 
-VOX_FALSE, VOX_TRUE, VOX_BOX, VOX_SPH, VOX_CYL, VOX_OR, VOX_AND, VOX_ANDNOT = range(8)
+VOX_FALSE, VOX_TRUE, VOX_BOX, VOX_SPH, VOX_CYL, VOX_OR, VOX_AND, VOX_ANDNOT, VOX_CONE = range(9)
 VOX_STRICT = 16
 MAX_DEPTH = 32
 
@@ -47,11 +48,12 @@ class ProgramSet:
         return len(self.start) - 1
 
 
-def _cyl_axis(node: Cylinder) -> int:
+def _cyl_axis(node) -> int:
     n = node.axis_normalized
     nz = np.flatnonzero(n)
     if len(nz) != 1 or abs(float(n[nz[0]])) != 1.0:
-        raise Unsupported("device voxeliser: cylinder axis must be parallel to a grid axis")
+        kind = type(node).__name__.lower()
+        raise Unsupported(f"device voxeliser: {kind} axis must be parallel to a grid axis")
     return int(nz[0])
 
 
@@ -73,6 +75,12 @@ def _emit(node, strict: bool, ops: list, params: list) -> int:
         ops.append((VOX_CYL | s, len(params)))
         params.extend([*map(float, node.p1), *map(float, node.axis_normalized), float(node.length),
                        float(node.radius), float(ax)])
+        return 1
+    if isinstance(node, Cone):
+        ax = _cyl_axis(node)
+        ops.append((VOX_CONE | s, len(params)))
+        params.extend([*map(float, node.p1), *map(float, node.axis_normalized), float(node.length),
+                       float(node.r1), float(node.r2), float(ax)])
         return 1
     if isinstance(node, HelmholtzResonator):
         return _emit(node._combined, strict, ops, params)

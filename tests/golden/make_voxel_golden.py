@@ -28,8 +28,15 @@ Cases:
   tree_nu{NN}        random CSG trees on a stretched grid
   materials_{N}      random geometry plus three random material regions
   helmholtz_{N}      Helmholtz resonators (box, sphere, cylinder cavities)
-Cylinders here are axis-aligned: their np.dot projection sums exact zeros, so
-the reference's values do not depend on the BLAS build.
+  cone_{NN}          CSG trees of cones (frustums, full cones) along grid axes,
+                     with boxes and spheres; the even ones with apexes on cell
+                     centres and whole-millimetre lengths and radii
+  cone_nu{NN}        cone trees on the stretched grid
+  cone_oblique{NN}   cone trees with tilted axes (host voxeliser only)
+Cylinders and the cones other than the oblique ones are axis-aligned: their
+np.dot projection sums exact zeros, so the reference's values do not depend on
+the BLAS build. The oblique cones' masks do, and hold for the numpy/BLAS build
+of this image, the one the tests run with.
 """
 
 from __future__ import annotations
@@ -58,7 +65,7 @@ def reference_namespace() -> dict:
     from strata_fdtd.geometry.material_assignment import MaterializedGeometry
     from strata_fdtd.geometry.resonator import HelmholtzResonator
     from strata_fdtd.core.grid import NonuniformGrid, UniformGrid
-    ns = dict(Box=sdf.Box, Sphere=sdf.Sphere, Cylinder=sdf.Cylinder, Union=sdf.Union,
+    ns = dict(Box=sdf.Box, Sphere=sdf.Sphere, Cylinder=sdf.Cylinder, Cone=sdf.Cone, Union=sdf.Union,
               Intersection=sdf.Intersection, Difference=sdf.Difference,
               HelmholtzResonator=HelmholtzResonator, MaterializedGeometry=MaterializedGeometry,
               material=object(), UniformGrid=UniformGrid, NonuniformGrid=NonuniformGrid)
@@ -94,12 +101,56 @@ def random_tree(rng, depth=0, ties=False):
     return (Union, Intersection, lambda *k: Difference(k[0], *k[1:]))[t - 3](*kids)
 
 
-def nontrivial_tree(rng, grid, ties=False, tries=200):
+def cone_tree(rng, ties=False, oblique=False, depth=0):
+    """Product-class tree around cones (geometry/sdf.py Cone :436-543):
+    frustums and full cones (one radius 0) along a grid axis, either sign,
+    combined with boxes, spheres and other cones. With ``ties`` the apex
+    ends sit on cell centres of a 1 mm grid and lengths and radii are whole
+    millimetres, so cap planes and the radius interpolation land exactly on
+    cell centres. ``oblique`` tilts the cones' axes."""
+    from strata_fdtd_amd import Box, Cone, Difference, Intersection, Sphere, Union
+
+    def cone():
+        if ties:
+            p1 = rng.integers(10, 40, 3) * 1e-3 + 1e-3 / 2
+        else:
+            p1 = np.round(rng.uniform(0.012, 0.04, 3), 4)
+        if oblique:
+            d = np.round(rng.uniform(-0.02, 0.02, 3), 3)
+            d[rng.integers(0, 3)] = np.round(rng.uniform(0.012, 0.025), 3)
+        else:
+            d = np.zeros(3)
+            ln = (rng.integers(8, 24) * 1e-3) if ties else np.round(rng.uniform(0.008, 0.024), 4)
+            d[rng.integers(0, 3)] = ln * (1 if rng.random() < 0.5 else -1)
+        if ties:
+            r1, r2 = (float(v) * 1e-3 for v in rng.integers(0, 12, 2))
+        else:
+            r1, r2 = (float(v) for v in np.round(rng.uniform(0.0, 0.012, 2), 4))
+        if rng.random() < 0.3:
+            (r1, r2) = (r1, 0.0) if rng.random() < 0.5 else (0.0, r2)
+        if r1 == 0 and r2 == 0:
+            r1 = 0.006
+        return Cone(p1=tuple(p1), p2=tuple(p1 + d), r1=r1, r2=r2)
+
+    t = rng.integers(0, 6 if depth < 1 else 3)
+    if t in (0, 1, 2):
+        if t == 2 and depth > 0:
+            return Sphere(center=tuple(np.round(rng.uniform(0.015, 0.035, 3), 4)),
+                          radius=float(np.round(rng.uniform(0.004, 0.01), 4)))
+        return cone()
+    kids = [cone_tree(rng, ties, oblique, depth + 1) for _ in range(rng.integers(2, 4))]
+    if t == 5:
+        outer = Box(center=(0.025, 0.025, 0.025), size=(0.03, 0.028, 0.032))
+        return Difference(outer, *kids)
+    return (Union, Intersection)[t - 3](*kids)
+
+
+def nontrivial_tree(rng, grid, ties=False, tries=200, make=None):
     """A random tree whose mask is neither empty nor full on ``grid`` (the
     product's voxeliser only picks the cases; the fixture holds the
     reference's masks)."""
     for _ in range(tries):
-        node = random_tree(rng, ties=ties)
+        node = make(rng) if make else random_tree(rng, ties=ties)
         frac = node.voxelize(grid).mean()
         if 0.002 < frac < 0.998:
             return node
@@ -140,6 +191,18 @@ def cases():
                                  neck_length=0.006, neck_radius=0.003,
                                  neck_direction=((0.0, 0.0, 1.0), (1.0, 0.0, 0.0), (0.0, -1.0, 0.0))[q])
         out.append((f"helmholtz_{q}", res, dict(shape=(40, 44, 48), dx=1e-3)))
+    # cones (own random stream, so the cases above stay as they were)
+    crng = np.random.default_rng(4242)
+    cg = UniformGrid((50, 52, 48), 1e-3)
+    for q in range(10):
+        out.append((f"cone_{q:02d}", nontrivial_tree(crng, cg, make=lambda g, t=(q % 2 == 0): cone_tree(g, ties=t)),
+                    dict(shape=(50, 52, 48), dx=1e-3)))
+    for q in range(3):
+        out.append((f"cone_nu{q:02d}", nontrivial_tree(crng, nug, make=lambda g: cone_tree(g)),
+                    dict(coords=(nug.x_coords, nug.y_coords, nug.z_coords))))
+    for q in range(3):
+        out.append((f"cone_oblique{q:02d}", nontrivial_tree(crng, cg, make=lambda g: cone_tree(g, oblique=True)),
+                    dict(shape=(50, 52, 48), dx=1e-3)))
     return out
 
 

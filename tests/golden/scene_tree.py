@@ -6,6 +6,7 @@ A tree is a JSON-able list:
   ["box", center, size]              Box(center=..., size=...)
   ["sphere", center, radius]
   ["cylinder", p1, p2, radius]
+  ["cone", p1, p2, r1, r2]
   ["union", [child, ...]]
   ["intersection", [child, ...]]
   ["difference", base, [child, ...]]
@@ -43,6 +44,8 @@ def to_tree(obj) -> list:
         return ["sphere", _v(obj.center), float(obj.radius)]
     if isinstance(obj, G.Cylinder):
         return ["cylinder", _v(obj.p1), _v(obj.p2), float(obj.radius)]
+    if isinstance(obj, G.Cone):
+        return ["cone", _v(obj.p1), _v(obj.p2), float(obj.r1), float(obj.r2)]
     if isinstance(obj, G.Union):
         return ["union", [to_tree(c) for c in obj.children]]
     if isinstance(obj, G.Intersection):
@@ -53,7 +56,7 @@ def to_tree(obj) -> list:
 
 
 def build(tree, ns: dict):
-    """Instantiate a tree with the classes in ``ns`` (Box, Sphere, Cylinder,
+    """Instantiate a tree with the classes in ``ns`` (Box, Sphere, Cylinder, Cone,
     Union, Intersection, Difference, HelmholtzResonator, MaterializedGeometry;
     ``material`` is the object handed to set_material)."""
     kind = tree[0]
@@ -63,6 +66,8 @@ def build(tree, ns: dict):
         return ns["Sphere"](center=tuple(tree[1]), radius=tree[2])
     if kind == "cylinder":
         return ns["Cylinder"](p1=tuple(tree[1]), p2=tuple(tree[2]), radius=tree[3])
+    if kind == "cone":
+        return ns["Cone"](p1=tuple(tree[1]), p2=tuple(tree[2]), r1=tree[3], r2=tree[4])
     if kind == "union":
         return ns["Union"](*[build(c, ns) for c in tree[1]])
     if kind == "intersection":

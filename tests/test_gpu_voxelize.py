@@ -115,12 +115,22 @@ def test_device_voxeliser_equals_reference_masks():
     """Every scene of tests/golden/voxel_masks.npz (masks and material ids
     written by the reference's own sdf.py / material_assignment.py): the
     device voxeliser reproduces them bit for bit."""
+    from strata_fdtd_amd.voxelize_device import Unsupported
     from test_voxel_golden import CASES, product_grid, product_namespace, scene_tree
     ns = product_namespace()
     print(f"[voxelize] {len(CASES)} reference-mask cases", flush=True)
+    host_only = 0
     for name, case in sorted(CASES.items()):
         obj = scene_tree.build(case["tree"], ns)
+        if name.startswith("cone_oblique"):
+            # tilted axes go through BLAS in the reference: the device
+            # voxeliser refuses them and a solver falls back to the host
+            with pytest.raises(Unsupported):
+                _device(obj, product_grid(case))
+            host_only += 1
+            continue
         dmask, dids = _device(obj, product_grid(case))
         assert np.array_equal(dmask, case["mask"]), (name, int((dmask != case["mask"]).sum()))
         if "ids" in case:
             assert np.array_equal(dids, case["ids"]), name
+    assert host_only == 3

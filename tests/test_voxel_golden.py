@@ -49,7 +49,7 @@ CASES = load_cases()
 
 def product_namespace():
     import strata_fdtd_amd as sfa
-    return dict(Box=sfa.Box, Sphere=sfa.Sphere, Cylinder=sfa.Cylinder, Union=sfa.Union,
+    return dict(Box=sfa.Box, Sphere=sfa.Sphere, Cylinder=sfa.Cylinder, Cone=sfa.Cone, Union=sfa.Union,
                 Intersection=sfa.Intersection, Difference=sfa.Difference,
                 HelmholtzResonator=sfa.HelmholtzResonator, MaterializedGeometry=sfa.MaterializedGeometry,
                 material=sfa.SimpleMaterial(name="m", _rho=1.2, _c=340.0))
