@@ -187,6 +187,25 @@ int fdtd_stream_ceiling(fdtd_ctx* ctx, int reps, float* ms_per_pass);
  * fdtd_stream_ceiling. Measurement instrumentation; no reference counterpart. */
 int fdtd_stream_ceiling_march(fdtd_ctx* ctx, int reps, float* ms_per_pass);
 
+/* ---- observation of the state (between steps, on the context's stream) -- */
+/* Both read the buffer set the next step reads (set counter & 1 of the device
+ * step counter), so they order after the steps already enqueued and need no
+ * host sync; outputs are device pointers. */
+
+/* Energy sums of the current state: out[0] = sum over air cells of p^2,
+ * out[1] = sum over air cells of vx^2 + vy^2 + vz^2, in float64 (every
+ * float32 square is exact in float64), formed in a fixed order so that a call
+ * repeats to the bit. The host forms 1/2 out[0]/(rho c^2) dV + 1/2 rho out[1]
+ * dV. Field buffers 16-byte and flags 4-byte aligned. Replaces
+ * FDTDSolver.compute_energy (core/solver.py:2689-2706). */
+int fdtd_ctx_energy(fdtd_ctx* ctx, double* out);
+
+/* Cell-centred velocities (three device float32 arrays of nx*ny*nz):
+ * out[c] = 0.5 * (v[c - e_axis] + v[c]), and 0.5 * v[c] on the first plane
+ * of the axis. Replaces FDTDSolver._interpolate_v{x,y,z}_to_centers
+ * (core/solver.py:2647-2687), used by velocity snapshots. */
+int fdtd_ctx_centred_velocity(fdtd_ctx* ctx, float* vx, float* vy, float* vz);
+
 /* ---- batched serving (no counterpart in the reference) ------------------- */
 /* A batch advances many independent solves (uniform grids, no ADE) by one step
  * per launch: waves are dealt out over the solves' strips, each solve keeps its

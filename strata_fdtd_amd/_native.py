@@ -40,6 +40,8 @@ _SIGS = {
     "fdtd_time_steps": (_i, [_vp, _i, _i, C.POINTER(_f)]),
     "fdtd_stream_ceiling": (_i, [_vp, _i, C.POINTER(_f)]),
     "fdtd_stream_ceiling_march": (_i, [_vp, _i, C.POINTER(_f)]),
+    "fdtd_ctx_energy": (_i, [_vp, _vp]),
+    "fdtd_ctx_centred_velocity": (_i, [_vp, _vp, _vp, _vp]),
     "fdtd_time_sweeps": (_i, [_vp, _i, C.POINTER(_f), C.POINTER(_f)]),
     "fdtd_batch_create": (_i, [_i, _vp, _i, C.POINTER(_vp)]),
     "fdtd_batch_destroy": (_i, [_vp]),

@@ -16,7 +16,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_lib", "libfdtd_mi355x.so")
-SOURCES = ["capi.hip", "fused_step.hip", "phase_kernels.hip", "adjoint.hip", "design.hip", "voxelize.hip"]
+SOURCES = ["capi.hip", "fused_step.hip", "phase_kernels.hip", "adjoint.hip", "design.hip", "voxelize.hip",
+           "observe.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -21,6 +21,9 @@ hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool a
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
 hipError_t launch_stream_ceiling(const FusedArgs& a, hipStream_t st);
 hipError_t launch_stream_march(const FusedArgs& a, int rows, int planes, hipStream_t st);
+int energy_partials();
+hipError_t launch_energy(const FusedArgs& a, double* partial, double* out, hipStream_t st);
+hipError_t launch_centre_velocity(const FusedArgs& a, float* ox, float* oy, float* oz, hipStream_t st);
 hipError_t launch_batch(const FusedArgs* table, const int* offsets, const int* n_ptr, int cap_waves, int rows,
                         bool nts, int koff, hipStream_t st);
 hipError_t launch_batch_advance(const FusedArgs* table, const int* n_ptr, int cap_solves, int n, hipStream_t st);
@@ -110,6 +113,7 @@ struct fdtd_ctx {
     int graph_steps = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> sweep_ev;  // pairs around each sweep (fdtd_time_sweeps)
+    double* energy_partial = nullptr;  // owned: [energy_partials()][2]
 };
 
 extern "C" {
@@ -135,6 +139,10 @@ int fdtd_ctx_create(int device, void* stream, fdtd_ctx** out) {
     DeviceGuard g(device);
     if (int rc = check(hipEventCreate(&c->ev0), "hipEventCreate")) { delete c; return rc; }
     if (int rc = check(hipEventCreate(&c->ev1), "hipEventCreate")) { delete c; return rc; }
+    if (int rc = check(hipMalloc(&c->energy_partial, sizeof(double) * 2 * energy_partials()), "hipMalloc")) {
+        fdtd_ctx_destroy(c);
+        return rc;
+    }
     *out = c;
     return 0;
 }
@@ -152,6 +160,7 @@ int fdtd_ctx_destroy(fdtd_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (hipEvent_t e : c->sweep_ev) (void)hipEventDestroy(e);
+    if (c->energy_partial) (void)hipFree(c->energy_partial);
     delete c;
     return 0;
 }
@@ -623,6 +632,25 @@ int fdtd_stream_ceiling_march(fdtd_ctx* c, int reps, float* ms_per_pass) {
     if (int rc = check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "elapsed time")) return rc;
     *ms_per_pass = ms / reps;
     return 0;
+}
+
+int fdtd_ctx_energy(fdtd_ctx* c, double* out) {
+    if (int rc = ready(c)) return rc;
+    if (!out) return fail(-2, "null output");
+    for (int set = 0; set < 2; ++set)
+        for (int g = 0; g < 4; ++g)
+            if (reinterpret_cast<uintptr_t>(c->a.f[set][g]) % 16)
+                return fail(-2, "field buffers must be 16-byte aligned for the energy sums");
+    if (reinterpret_cast<uintptr_t>(c->a.flags) % 4) return fail(-2, "flag buffer must be 4-byte aligned");
+    DeviceGuard g(c->device);
+    return check(launch_energy(c->a, c->energy_partial, out, c->stream), "energy launch");
+}
+
+int fdtd_ctx_centred_velocity(fdtd_ctx* c, float* vx, float* vy, float* vz) {
+    if (int rc = ready(c)) return rc;
+    if (!vx || !vy || !vz) return fail(-2, "null output");
+    DeviceGuard g(c->device);
+    return check(launch_centre_velocity(c->a, vx, vy, vz, c->stream), "centring launch");
 }
 
 int fdtd_time_sweeps(fdtd_ctx* c, int n_steps, float* ms_sweep, float* ms_total) {

@@ -15,15 +15,24 @@ groups and datasets of the reference's HDF5ResultWriter (io/hdf5.py:24-233):
 The file uses the classic format h5py writes by default, which every HDF5
 reader accepts: superblock version 0 with 8-byte offsets, version-1 object
 headers, groups as symbol tables (a version-1 B-tree leaf over symbol-table
-nodes, names in a local heap), contiguous datasets (uncompressed), attributes
-as version-1 messages. Booleans are stored as h5py stores them (an int8
-enumeration FALSE/TRUE) and strings as fixed-length UTF-8. The whole file is
-built in memory and written once, when the run ends.
+nodes, names in a local heap), attributes as version-1 messages. Datasets are
+stored as the reference's writer asks h5py to store them: pressure, probes and
+geometry chunked and deflate-compressed at level 4, pressure in chunks of one
+snapshot (1, nx, ny, nz) and extensible along time, probes extensible with
+h5py's automatic chunk shape, geometry with h5py's automatic chunk shape;
+the coordinate tables contiguous. Chunks are indexed by version-1 B-trees
+(HDF5's default of 2K = 64 chunks per node, more levels as needed). Booleans
+are stored as h5py stores them (an int8 enumeration FALSE/TRUE) and strings
+as fixed-length UTF-8. The whole file is built in memory and written once,
+when the run ends.
 """
 
 from __future__ import annotations
 
+import itertools
+import math
 import struct
+import zlib
 
 import numpy as np
 
@@ -31,6 +40,8 @@ UNDEF = 0xFFFFFFFFFFFFFFFF
 HEAP_FREE_NULL = 1  # a local heap's free-list offset when it has no free block
 LEAF_K = 4          # symbol-table nodes of 2K = 8 names (HDF5's default)
 INTERNAL_K = 16     # B-tree nodes of 2K = 32 children (HDF5's default)
+CHUNK_K = 32        # chunk B-tree nodes of 2K = 64 children (HDF5's default)
+GZIP_LEVEL = 4      # the reference writer's compression_level default
 
 
 def _pad8(n: int) -> int:
@@ -84,8 +95,36 @@ def _encode(value):
     raise TypeError(f"cannot store {type(value).__name__} ({a.dtype})")
 
 
-def _dataspace(dims) -> bytes:
-    return struct.pack("<BBBB4x", 1, len(dims), 0, 0) + b"".join(struct.pack("<Q", d) for d in dims)
+def _dataspace(dims, maxdims=None) -> bytes:
+    """Version-1 dataspace; maxdims (None entries unlimited) adds the maximum sizes."""
+    out = struct.pack("<BBBB4x", 1, len(dims), 0 if maxdims is None else 1, 0)
+    out += b"".join(struct.pack("<Q", d) for d in dims)
+    if maxdims is not None:
+        out += b"".join(struct.pack("<Q", UNDEF if m is None else m) for m in maxdims)
+    return out
+
+
+def guess_chunk(shape, typesize: int) -> tuple:
+    """h5py's automatic chunk shape (h5py/_hl/filters.py, guess_chunk), which
+    the reference's datasets get when they ask for compression without a chunk
+    shape: a zero (extensible, empty) extent counts as 1024; the target chunk
+    size is 16 KiB x 2^log10(dataset MiB), clamped to [8 KiB, 1 MiB]; axes are
+    halved (rounding up) in turn, starting with the first, until the chunk is
+    below the target, or within 50% of it, and below 1 MiB."""
+    dims = [float(n if n else 1024) for n in shape]
+    total = math.prod(dims) * typesize
+    target = 16 * 1024 * 2 ** math.log10(total / (1024.0 * 1024))
+    target = min(max(target, 8 * 1024), 1024 * 1024)
+    axis = 0
+    while True:
+        nbytes = math.prod(dims) * typesize
+        if (nbytes < target or abs(nbytes - target) / target < 0.5) and nbytes < 1024 * 1024:
+            break
+        if math.prod(dims) == 1:
+            break
+        dims[axis % len(dims)] = math.ceil(dims[axis % len(dims)] / 2.0)
+        axis += 1
+    return tuple(int(d) for d in dims)
 
 
 def _attribute(name: str, value) -> bytes:
@@ -108,16 +147,30 @@ class Group:
             g = self.children[name] = Group()
         return g
 
-    def dataset(self, name: str, data: np.ndarray, **attrs) -> None:
-        ds = Dataset(np.asarray(data))
+    def dataset(self, name: str, data: np.ndarray, chunks=None, maxshape=None, gzip=None, **attrs) -> None:
+        ds = Dataset(np.asarray(data), chunks, maxshape, gzip)
         ds.attrs.update(attrs)
         self.children[name] = ds
 
 
 class Dataset:
-    def __init__(self, data: np.ndarray):
+    """chunks: chunk shape (None: contiguous); maxshape: maximum extent, None
+    entries unlimited (chunked datasets only); gzip: deflate level or None."""
+
+    def __init__(self, data: np.ndarray, chunks=None, maxshape=None, gzip=None):
         self.data = data
         self.attrs: dict = {}
+        if chunks is not None:
+            chunks = tuple(int(c) for c in chunks)
+            if len(chunks) != data.ndim or any(c < 1 for c in chunks):
+                raise ValueError(f"chunk shape {chunks} for a dataset of shape {data.shape}")
+            if maxshape is None:
+                maxshape = data.shape
+            if len(maxshape) != data.ndim or any(m is not None and m < n for m, n in zip(maxshape, data.shape)):
+                raise ValueError(f"maximum shape {maxshape} for a dataset of shape {data.shape}")
+        elif maxshape is not None or gzip is not None:
+            raise ValueError("maximum shapes and compression need a chunked dataset")
+        self.chunks, self.maxshape, self.gzip = chunks, maxshape, gzip
 
 
 class _Out:
@@ -138,16 +191,93 @@ def _object_header(msgs) -> bytes:
     return struct.pack("<BBHII4x", 1, 0, len(msgs), 1, len(body)) + body
 
 
+def _chunk_btree(out: _Out, entries, rank: int) -> int:
+    """Version-1 B-tree (node type 1) over the chunks; entries are
+    (stored size, chunk offsets, address) in offset order. Each key is
+    (size, filter mask, offsets + [0]); a node's last key is the offset just
+    past its last chunk (its offsets plus the chunk shape). Returns the root."""
+    key_size = 8 + 8 * (rank + 1)
+    node_size = 24 + 2 * CHUNK_K * 8 + (2 * CHUNK_K + 1) * key_size
+
+    def key(size, offs):
+        return struct.pack("<II", size, 0) + b"".join(struct.pack("<Q", o) for o in offs)
+
+    # level 0: (first key, last key, child address) per chunk
+    items = [(key(size, list(offs) + [0]), key(0, list(end) + [esize]), addr) for size, offs, end, esize, addr in entries]
+    level = 0
+    while True:
+        groups = [items[g:g + 2 * CHUNK_K] for g in range(0, len(items), 2 * CHUNK_K)]
+        addrs = [out.alloc(node_size) for _ in groups]
+        parents = []
+        for n, grp in enumerate(groups):
+            left = addrs[n - 1] if n else UNDEF
+            right = addrs[n + 1] if n + 1 < len(addrs) else UNDEF
+            node = bytearray(b"TREE" + struct.pack("<BBHQQ", 1, level, len(grp), left, right))
+            for first, _last, child in grp:
+                node += first + struct.pack("<Q", child)
+            node += grp[-1][1]
+            out.put(addrs[n], bytes(node))
+            parents.append((grp[0][0], grp[-1][1], addrs[n]))
+        if len(parents) == 1:
+            return parents[0][2]
+        items = parents
+        level += 1
+
+
+def _write_chunks(out: _Out, ds: Dataset, raw_dtype: np.dtype) -> int:
+    """Store every chunk that meets the extent (edge chunks padded with
+    zeros to the full chunk shape), deflated when asked; returns the chunk
+    B-tree address (UNDEF for an empty dataset)."""
+    a = np.ascontiguousarray(np.asarray(ds.data).astype(raw_dtype))
+    shape, chunks = a.shape, ds.chunks
+    if a.size == 0:
+        return UNDEF
+    entries = []
+    for idx in itertools.product(*(range(-(-n // c)) for n, c in zip(shape, chunks))):
+        offs = [i * c for i, c in zip(idx, chunks)]
+        block = np.zeros(chunks, dtype=raw_dtype)
+        src = tuple(slice(o, min(o + c, n)) for o, c, n in zip(offs, chunks, shape))
+        block[tuple(slice(0, s.stop - s.start) for s in src)] = a[src]
+        data = block.tobytes()
+        if ds.gzip is not None:
+            data = zlib.compress(data, ds.gzip)
+        addr = out.alloc(len(data))
+        out.put(addr, data)
+        end = [o + c for o, c in zip(offs, chunks)]
+        entries.append((len(data), offs, end, raw_dtype.itemsize, addr))
+    return _chunk_btree(out, entries, a.ndim)
+
+
 def _write_dataset(out: _Out, ds: Dataset) -> int:
-    dt, dims, raw = _encode(ds.data)
-    data_addr = out.alloc(len(raw))
-    out.put(data_addr, raw)
-    msgs = [
-        (0x0001, 0, _dataspace(dims)),
-        (0x0003, 1, dt),
-        (0x0005, 1, struct.pack("<BBBB", 2, 1, 0, 0)),        # fill value: v2, early, on alloc, undefined
-        (0x0008, 0, struct.pack("<BBQQ", 3, 1, data_addr, len(raw))),   # contiguous layout
-    ]
+    if ds.chunks is not None:
+        arr = np.asarray(ds.data)
+        dt, _, _ = _encode(arr.reshape(-1)[:1])      # the datatype message only
+        dims = arr.shape
+        raw_dtype = np.dtype(np.int8) if arr.dtype == np.bool_ else arr.dtype.newbyteorder("<")
+        btree = _write_chunks(out, ds, raw_dtype)
+        msgs = [
+            (0x0001, 0, _dataspace(dims, ds.maxshape)),
+            (0x0003, 1, dt),
+            # fill value: v2, incremental allocation, written if set, defined as the default (0)
+            (0x0005, 1, struct.pack("<BBBBI", 2, 3, 2, 1, 0)),
+        ]
+        if ds.gzip is not None:
+            # filter pipeline v1: deflate (id 1, optional as h5py marks it), its level
+            msgs.append((0x000B, 1, struct.pack("<BB6xHHHH", 1, 1, 1, 8, 1, 1) + b"deflate\0"
+                         + struct.pack("<I4x", ds.gzip)))
+        layout = struct.pack("<BBBQ", 3, 2, len(dims) + 1, btree)
+        layout += b"".join(struct.pack("<I", c) for c in ds.chunks) + struct.pack("<I", raw_dtype.itemsize)
+        msgs.append((0x0008, 0, layout))
+    else:
+        dt, dims, raw = _encode(ds.data)
+        data_addr = out.alloc(len(raw))
+        out.put(data_addr, raw)
+        msgs = [
+            (0x0001, 0, _dataspace(dims)),
+            (0x0003, 1, dt),
+            (0x0005, 1, struct.pack("<BBBB", 2, 1, 0, 0)),        # fill value: v2, early, on alloc, undefined
+            (0x0008, 0, struct.pack("<BBQQ", 3, 1, data_addr, len(raw))),   # contiguous layout
+        ]
     msgs += [(0x000C, 0, _attribute(k, v)) for k, v in ds.attrs.items()]
     hdr = _object_header(msgs)
     addr = out.alloc(len(hdr))
@@ -264,13 +394,17 @@ def solver_tree(solver, snapshots, snapshot_interval, runtime: float, script_con
             sg.attrs["bandwidth"] = float(bw)
     fields = root.group("fields")
     if snapshots:
-        fields.dataset("pressure", np.stack([p for _t, p in snapshots]).astype(np.float32), units="Pa",
-                       snapshot_interval=np.int64(snapshot_interval))
+        # one chunk per snapshot, extensible along time (io/hdf5.py:186-206)
+        fields.dataset("pressure", np.stack([p for _t, p in snapshots]).astype(np.float32),
+                       chunks=(1,) + tuple(solver.shape), maxshape=(None,) + tuple(solver.shape), gzip=GZIP_LEVEL,
+                       units="Pa", snapshot_interval=np.int64(snapshot_interval))
     probes = root.group("probes")
     for name, data in solver.get_probe_data().items():
-        probes.dataset(name, np.asarray(data, np.float32),
-                       position=np.array(solver._probes[name].position, np.int64), units="Pa")
-    root.group("materials").dataset("geometry", np.asarray(solver.geometry).astype(np.uint8))
+        # created empty and extensible, automatic chunks (io/hdf5.py:138-152)
+        probes.dataset(name, np.asarray(data, np.float32), chunks=guess_chunk((0,), 4), maxshape=(None,),
+                       gzip=GZIP_LEVEL, position=np.array(solver._probes[name].position, np.int64), units="Pa")
+    geo = np.asarray(solver.geometry).astype(np.uint8)
+    root.group("materials").dataset("geometry", geo, chunks=guess_chunk(geo.shape, 1), gzip=GZIP_LEVEL)
     sim.attrs["num_steps"] = np.int64(solver.step_count)
     sim.attrs["total_time"] = float(solver.time)
     meta.attrs["total_runtime_seconds"] = float(runtime)

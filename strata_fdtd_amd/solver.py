@@ -171,6 +171,11 @@ class FDTDSolver:
         self._energy_history: list[tuple[int, float, float]] = []
         self._track_energy = False
         self._energy_sample_interval = 1
+        # energy samples enqueued on the device and not yet read back:
+        # (step count, time) per row of _energy_dev (float64 [cap][2] sums)
+        self._energy_pending: list[tuple[int, float]] = []
+        self._energy_dev = None
+        self._d2h_pending = False  # snapshot copies in flight to pinned host memory
 
         # host mirrors of the current field set
         self._mirror: list[np.ndarray | None] = [None] * 4
@@ -730,12 +735,37 @@ class FDTDSolver:
 
     def _after_step_hooks(self, n_before: int, t_before: float) -> None:
         if self._snapshot_interval and n_before % self._snapshot_interval == 0:
-            self._snapshots.append((t_before, self._host_field(0).copy()))
-            self._exported.discard(0)
+            self._snapshots.append((t_before, self._snapshot_p()))
             if self._snapshot_velocity:
                 self._velocity_snapshots.append((t_before, *self._centred_velocity()))
         if self._track_energy and self._step_count % self._energy_sample_interval == 0:
-            self._energy_history.append((self._step_count, self._time, self.compute_energy()))
+            self._queue_energy()
+
+    def _hooks_fire(self, n_before: int) -> bool:
+        """Whether the step taken at count n_before snapshots or samples energy."""
+        if self._snapshot_interval and n_before % self._snapshot_interval == 0:
+            return True
+        return bool(self._track_energy and (n_before + 1) % self._energy_sample_interval == 0)
+
+    def _to_pinned(self, dev_tensor) -> np.ndarray:
+        """Copy a device tensor to new pinned host memory on the solver's
+        stream without waiting; the array is valid after _sync_d2h()."""
+        torch = _torch()
+        host = torch.empty(dev_tensor.shape, dtype=dev_tensor.dtype, pin_memory=True)
+        with torch.cuda.stream(self._stream):
+            host.copy_(dev_tensor, non_blocking=True)
+        self._d2h_pending = True
+        return host.numpy()
+
+    def _sync_d2h(self) -> None:
+        if self._d2h_pending:
+            self._stream.synchronize()
+            self._d2h_pending = False
+
+    def _snapshot_p(self) -> np.ndarray:
+        """The current pressure, copied asynchronously to pinned host memory."""
+        self._flush_mirrors()
+        return self._to_pinned(self._f[self._cur][0])
 
     def time_sweeps(self, n: int) -> tuple[float, float]:
         """Advance n (1..4096) steps eagerly with HIP events around every fused
@@ -795,24 +825,32 @@ class FDTDSolver:
         n_steps = int(np.ceil(duration / self.dt))
         self._track_energy = track_energy
         self._energy_sample_interval = energy_sample_interval
-        if track_energy and not self._energy_history:
-            self._energy_history.append((self._step_count, self._time, self.compute_energy()))
+        if track_energy and not self._energy_history and not self._energy_pending:
+            self._queue_energy()
         h5_snaps = [] if output_file and snapshot_interval else None
         started = _time_mod.time()
-        per_step = bool(callback or track_energy or self._snapshot_interval or progress or h5_snaps is not None)
         try:
-            if per_step:
-                for step in range(n_steps):
-                    t_before = self._time
-                    self.step()
-                    if h5_snaps is not None and step % snapshot_interval == 0:
-                        h5_snaps.append((t_before, self._host_field(0).copy()))
-                        self._exported.discard(0)
-                    if callback:
-                        callback(self._step_count - 1)
-            else:
-                self.step_n(n_steps)
+            step = 0
+            while step < n_steps:
+                if not (callback or progress):
+                    # steps at which nothing is recorded go to the device as one run
+                    j = step
+                    while j < n_steps and not self._hooks_fire(self._step_count + j - step) and not (
+                            h5_snaps is not None and j % snapshot_interval == 0):
+                        j += 1
+                    if j > step:
+                        self.step_n(j - step)
+                        step = j
+                        continue
+                t_before = self._time
+                self.step()
+                if h5_snaps is not None and step % snapshot_interval == 0:
+                    h5_snaps.append((t_before, self._snapshot_p()))
+                if callback:
+                    callback(self._step_count - 1)
+                step += 1
         finally:
+            self._sync_d2h()
             if output_file:
                 from datetime import datetime, timezone
 
@@ -821,6 +859,7 @@ class FDTDSolver:
                                             script_content, datetime.now(timezone.utc).isoformat())
                 hdf5_out.write(str(output_file), tree)
         self.check_health()
+        self._resolve_energy()
         if self._warn_energy_drift and len(self._energy_history) >= 2:
             rep = self.energy_report()
             if abs(rep["energy_change_percent"]) > self._energy_drift_threshold * 100:
@@ -856,46 +895,79 @@ class FDTDSolver:
         return out
 
     def get_snapshots(self):
+        self._sync_d2h()
         return self._snapshots
 
     def get_velocity_snapshots(self):
+        self._sync_d2h()
         return self._velocity_snapshots
 
     def _centred_velocity(self):
-        out = []
-        for ax, idx in ((0, 1), (1, 2), (2, 3)):
-            v = self._host_field(idx)
-            self._exported.discard(idx)
-            c = np.zeros(self.shape, dtype=np.float32)
-            sl_hi = [slice(None)] * 3
-            sl_lo = [slice(None)] * 3
-            sl_hi[ax] = slice(1, None)
-            sl_lo[ax] = slice(None, -1)
-            c[tuple(sl_hi)] = 0.5 * (v[tuple(sl_lo)] + v[tuple(sl_hi)])
-            first = [slice(None)] * 3
-            first[ax] = 0
-            c[tuple(first)] = 0.5 * v[tuple(first)]
-            out.append(c)
-        return out
-
-    def compute_energy(self) -> float:
-        """1/2 sum_air p^2/(rho c^2) dV + 1/2 rho sum_air |v|^2 dV (core/solver.py:2689-2706),
-        reduced on the device in float64."""
+        """Cell-centred vx, vy, vz of the current state (core/solver.py:
+        2647-2687), formed on the device and copied asynchronously to pinned
+        host memory."""
         torch = _torch()
+        self._flush_mirrors()
+        with torch.cuda.stream(self._stream):
+            outs = [torch.empty(self.shape, dtype=torch.float32, device=self._device) for _ in range(3)]
+        _native.call("fdtd_ctx_centred_velocity", self._ctx, *(_native.ptr(t) for t in outs))
+        return [self._to_pinned(t) for t in outs]
+
+    def _enqueue_energy(self, out) -> None:
+        """Enqueue the energy sums of the current state into the device
+        float64 pair `out` (fdtd_ctx_energy), after the steps enqueued so far."""
         self._flush_mirrors()
         if self._dirty_geometry:
             self._upload_geometry()
-        p, vx, vy, vz = (t.double() for t in self._f[self._cur])
-        air = self._air_dev.bool()
+        _native.call("fdtd_ctx_energy", self._ctx, _native.ptr(out))
+
+    def _energy_from_sums(self, sp: float, sv: float) -> float:
         dV = self.dx**3
-        pe = 0.5 * torch.sum(p[air] ** 2).item() / (self.rho * self.c**2) * dV
-        ke = 0.5 * self.rho * torch.sum((vx**2 + vy**2 + vz**2)[air]).item() * dV
+        pe = 0.5 * sp / (self.rho * self.c**2) * dV
+        ke = 0.5 * self.rho * sv * dV
         return pe + ke
 
+    def compute_energy(self) -> float:
+        """1/2 sum_air p^2/(rho c^2) dV + 1/2 rho sum_air |v|^2 dV (core/solver.py:2689-2706),
+        reduced on the device in float64 by one pass over the fields."""
+        torch = _torch()
+        with torch.cuda.stream(self._stream):
+            out = torch.empty(2, dtype=torch.float64, device=self._device)
+        self._enqueue_energy(out)
+        self._stream.synchronize()
+        sp, sv = out.cpu().tolist()
+        return self._energy_from_sums(sp, sv)
+
+    def _queue_energy(self) -> None:
+        """Record an energy sample of the current state without waiting: the
+        sums go to a device history read back when the history is read."""
+        torch = _torch()
+        n = len(self._energy_pending)
+        with torch.cuda.stream(self._stream):
+            if self._energy_dev is None or self._energy_dev.shape[0] <= n:
+                grown = torch.empty((max(256, 2 * (n + 1)), 2), dtype=torch.float64, device=self._device)
+                if n:
+                    grown[:n].copy_(self._energy_dev[:n])
+                self._energy_dev = grown
+        self._enqueue_energy(self._energy_dev[n])
+        self._energy_pending.append((self._step_count, self._time))
+
+    def _resolve_energy(self) -> None:
+        if not self._energy_pending:
+            return
+        n = len(self._energy_pending)
+        self._stream.synchronize()
+        sums = self._energy_dev[:n].cpu().numpy()
+        for (count, t), (sp, sv) in zip(self._energy_pending, sums):
+            self._energy_history.append((count, t, self._energy_from_sums(float(sp), float(sv))))
+        self._energy_pending.clear()
+
     def get_energy_history(self):
+        self._resolve_energy()
         return self._energy_history.copy()
 
     def energy_report(self) -> dict:
+        self._resolve_energy()
         if not self._energy_history:
             raise ValueError("No energy history recorded. Call run() with track_energy=True first.")
         e = np.array([x for _, _, x in self._energy_history])
@@ -923,9 +995,11 @@ class FDTDSolver:
                 self._ade_tensors[key].zero_()
         self._step_count = 0
         self._time = 0.0
+        self._sync_d2h()
         self._snapshots.clear()
         self._velocity_snapshots.clear()
         self._energy_history.clear()
+        self._energy_pending.clear()
         self._track_energy = False
         self._exported.clear()
         self._mirror_valid = False

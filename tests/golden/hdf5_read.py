@@ -338,6 +338,29 @@ class Dataset(Node):
         return out
 
 
+def storage(root) -> dict:
+    """How each dataset is stored: layout class, chunk shape, maximum extent
+    (None unlimited), and the fill-value and filter-pipeline messages as hex."""
+    out = {}
+    for opath, node in walk(root):
+        if not isinstance(node, Dataset):
+            continue
+        lay, ds = node._msg(0x08), node._msg(0x01)
+        rank, flags = ds[1], ds[2]
+        maxdims = None
+        if flags & 1:
+            maxdims = [None if v == UNDEF else v for v in
+                       (_u(ds, 8 + 8 * rank + 8 * d, 8) for d in range(rank))]
+        info = {"layout": lay[1], "maxdims": maxdims, "fill_hex": node._msg(0x05).hex()}
+        if lay[1] == 2:
+            cd = [_u(lay, 11 + 4 * d, 4) for d in range(lay[2])]
+            info["chunk"], info["element_size"] = cd[:-1], cd[-1]
+        flt = node._msg(0x0B)
+        info["filter_hex"] = None if flt is None else flt.hex()
+        out[opath] = info
+    return out
+
+
 def walk(g: Group, prefix=""):
     """(path, node) for every object under a group."""
     for name in g.keys():
@@ -347,4 +370,4 @@ def walk(g: Group, prefix=""):
             yield from walk(node, prefix + "/" + name)
 
 
-__all__ = ["File", "Group", "Dataset", "walk", "struct"]
+__all__ = ["File", "Group", "Dataset", "walk", "storage", "struct"]

@@ -15,10 +15,11 @@ the cadence of a snapshot every 5 steps (FDTDSolver.enable_snapshots(5)).
 The file is read with the minimal parser beside this script (h5py is not
 installed here). Stored in ``simple_pulse.npz``: the run's parameters, the
 recorded probe traces, the 13 snapshots on the sub-lattice of every second
-cell per axis, a SHA-256 of the oracle's float32 probe traces, and the file's
+cell per axis, a SHA-256 of the oracle's float32 probe traces, the file's
 layout (objects, shapes, attribute names) with the attributes that describe
-the run (grid, simulation, source), for comparison with a file the product
-writes.
+the run (grid, simulation, source), and how each dataset is stored (chunk
+shapes, maximum extents, fill-value and filter messages), for comparison with
+a file the product writes.
 """
 
 from __future__ import annotations
@@ -94,6 +95,7 @@ def main() -> None:
         values[g] = {k: (v.tolist() if isinstance(v, np.ndarray) else (v.item() if hasattr(v, "item") else v))
                      for k, v in r[g].attrs.items()}
     out["layout_json"] = np.array(json.dumps(layout, sort_keys=True))
+    out["storage_json"] = np.array(json.dumps(H.storage(r), sort_keys=True))
     out["attrs_json"] = np.array(json.dumps(values, sort_keys=True))
     path = os.path.join(HERE, "simple_pulse.npz")
     np.savez_compressed(path, **out)

@@ -224,3 +224,21 @@ def test_waveform_objects_bitexact(oracle_lib):
     for k in FIELDS:
         assert np.array_equal(getattr(s, k), st[k]), k
     assert np.abs(tr).max() > 0
+
+
+@pytest.mark.parametrize("shape", [(17, 19, 23), (160, 170, 203)])
+def test_energy_sums_tail_and_many_blocks(shape):
+    """fdtd_ctx_energy over a cell count that is not a multiple of four (the
+    scalar tail) and over more cells than one pass of its fixed grid covers:
+    a call repeats to the bit and equals the float64 sums."""
+    sc = seeded_scenario(45, shape=shape, pml=0, geometry=True)
+    s = make_solver(sc)
+    rng = np.random.default_rng(7)
+    state = {k: rng.standard_normal(shape).astype(np.float32) for k in FIELDS}
+    for k in FIELDS:
+        setattr(s, k, state[k])
+    e1 = s.compute_energy()
+    e2 = s.compute_energy()
+    assert e1 == e2
+    e64 = _energy64(state, sc.air, sc.dx)
+    assert abs(e1 - e64) <= 1e-12 * e64

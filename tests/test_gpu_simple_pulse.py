@@ -59,7 +59,8 @@ def test_run_writes_reference_result_file(tmp_path):
     """run(duration, output_file=..., snapshot_interval=5) on the same run
     writes an HDF5 file in the reference's result layout: read back with the
     parser that reads the reference's own sample, it has the sample's
-    objects, shapes and attribute names, the sample's grid, simulation and
+    objects, shapes and attribute names, its dataset storage (chunk shapes,
+    maximum extents, gzip level 4), the sample's grid, simulation and
     source attributes, the probe traces the solver returns, the pressure
     snapshots (the first equal to the recorded one bit for bit, all within
     1e-6 of the recorded ones) and the air mask."""
@@ -90,6 +91,7 @@ def test_run_writes_reference_result_file(tmp_path):
         layout[opath] = {"kind": kind, "attrs": sorted(node.attrs),
                          "shape": list(node.shape) if kind == "dataset" else None}
     assert layout == json.loads(str(d["layout_json"]))
+    assert H.storage(f.root) == json.loads(str(d["storage_json"]))
     want = json.loads(str(d["attrs_json"]))
     for g, attrs in want.items():
         got = f.root[g].attrs

@@ -80,3 +80,62 @@ def test_group_size_limit(tmp_path):
         assert "at most" in str(e)
     else:
         raise AssertionError("a group beyond one B-tree leaf of symbol-table nodes must be refused")
+
+
+def _sample_storage():
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "simple_pulse.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return json.loads(str(z["storage_json"]))
+
+
+def test_storage_matches_reference_sample(tmp_path):
+    """Datasets stored as the reference's writer stores them (h5py, gzip 4):
+    the writer's chunk shapes, maximum extents, fill-value and filter-pipeline
+    messages equal those of the reference's sample file (simple-pulse.h5,
+    40^3, 13 snapshots, 63-step probes), and guess_chunk gives h5py's
+    automatic chunk shapes found there."""
+    want = _sample_storage()
+    assert W.guess_chunk((40, 40, 40), 1) == tuple(want["/materials/geometry"]["chunk"])
+    assert W.guess_chunk((0,), 4) == tuple(want["/probes/far_field"]["chunk"])
+    root = W.Group()
+    root.group("fields").dataset("pressure", np.zeros((13, 40, 40, 40), np.float32), chunks=(1, 40, 40, 40),
+                                 maxshape=(None, 40, 40, 40), gzip=W.GZIP_LEVEL)
+    root.group("materials").dataset("geometry", np.ones((40, 40, 40), np.uint8),
+                                    chunks=W.guess_chunk((40, 40, 40), 1), gzip=W.GZIP_LEVEL)
+    for name in ("far_field", "near_field"):
+        root.group("probes").dataset(name, np.zeros(63, np.float32), chunks=W.guess_chunk((0,), 4),
+                                     maxshape=(None,), gzip=W.GZIP_LEVEL)
+    path = tmp_path / "s.h5"
+    W.write(str(path), root)
+    assert H.storage(H.File(str(path)).root) == want
+
+
+def test_chunked_round_trip(tmp_path):
+    """Chunked datasets: more chunks than one B-tree node indexes (two
+    levels), edge chunks padded to the chunk shape, an empty extensible
+    dataset, deflated and raw chunks, and booleans."""
+    rng = np.random.default_rng(9)
+    root = W.Group()
+    snaps = rng.standard_normal((150, 4, 5, 6)).astype(np.float32)      # 150 chunks > 64 per node
+    root.dataset("snaps", snaps, chunks=(1, 4, 5, 6), maxshape=(None, 4, 5, 6), gzip=4)
+    geo = rng.random((45, 41, 43)) > 0.5
+    root.dataset("geo", geo.astype(np.uint8), chunks=W.guess_chunk(geo.shape, 1), gzip=4)
+    root.dataset("trace", rng.standard_normal(3000).astype(np.float32), chunks=(1024,), maxshape=(None,), gzip=1)
+    root.dataset("empty", np.zeros(0, np.float32), chunks=(1024,), maxshape=(None,), gzip=4)
+    raw = rng.standard_normal((10, 10)).astype(np.float64)
+    root.dataset("raw", raw, chunks=(3, 4))
+    flags = rng.random(50) > 0.5
+    root.dataset("flags", flags, chunks=(16,), gzip=4)
+    path = tmp_path / "c.h5"
+    W.write(str(path), root)
+    r = H.File(str(path)).root
+    assert np.array_equal(r["snaps"].read(), snaps)
+    assert np.array_equal(r["geo"].read(), geo.astype(np.uint8))
+    assert np.array_equal(r["trace"].read(), root.children["trace"].data)
+    assert r["empty"].read().shape == (0,)
+    assert np.array_equal(r["raw"].read(), raw)
+    assert np.array_equal(r["flags"].read().astype(bool), flags)
+    st = H.storage(r)
+    assert st["/snaps"]["maxdims"] == [None, 4, 5, 6] and st["/raw"]["filter_hex"] is None
+    assert st["/geo"]["chunk"] == list(W.guess_chunk(geo.shape, 1))
