@@ -6,15 +6,17 @@ Drop-in surface (the reference's names and semantics, ``backend="mi355x"``):
 membrane sources (``CircularMembraneSource``, ``RectangularMembraneSource``),
 ``Microphone``,
 SDF geometry (``Box``, ``Sphere``, ``Cylinder``, ``Cone``, ``Union``, ``Intersection``,
-``Difference``, ``HelmholtzResonator``, ``MaterializedGeometry``) and ADE
+``Difference``, their smooth forms, ``Translate``/``Scale``/``Rotate``,
+``HelmholtzResonator``, ``MaterializedGeometry``) and ADE
 materials (``Pole``, ``PoleType``, ``SimpleMaterial``, ``AIR_20C``).
 Beyond the reference: ``serving`` (continuous batching of many solves) and
 ``adjoint`` (gradients w.r.t. per-voxel density and bulk modulus).
 """
 
 from .boundaries import PML, RigidBoundary
-from .geometry import (Box, Cone, Cylinder, Difference, HelmholtzResonator, Intersection,
-                       MaterializedGeometry, SDFPrimitive, Sphere, Union)
+from .geometry import (Box, Cone, CSGNode, Cylinder, Difference, HelmholtzResonator, Intersection,
+                       MaterializedGeometry, Rotate, Scale, SDFPrimitive, SmoothDifference, SmoothIntersection,
+                       SmoothUnion, Sphere, Transform, Translate, Union, rotation_matrix)
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
 from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, Pole, PoleType, SimpleMaterial
 from .membranes import CircularMembraneSource, MembraneSource, RectangularMembraneSource
@@ -27,7 +29,8 @@ __version__ = "0.1.0"
 __all__ = [
     "FDTDSolver", "has_mi355x_backend", "has_native_kernels", "has_gpu_support", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
     "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder", "Cone",
-    "Union", "Intersection", "Difference", "HelmholtzResonator", "MaterializedGeometry", "Pole",
+    "Union", "Intersection", "Difference", "CSGNode", "SmoothUnion", "SmoothIntersection", "SmoothDifference",
+    "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
     "POLAR_PATTERNS", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
 ]

@@ -3,7 +3,8 @@
 The signed-distance formulas, CSG rules and voxelisation rule follow the
 reference's ``geometry/sdf.py`` (SDFPrimitive.voxelize :99-127, Box :274-294,
 Sphere :320-333, Cylinder :382-425, Cone :436-543,
-Union/Intersection/Difference :566-722),
+Union/Intersection/Difference :566-722, the smooth forms :736-936,
+rotation_matrix and Translate/Scale/Rotate :1151-1391),
 ``geometry/resonator.py`` (HelmholtzResonator :137-207) and
 ``geometry/material_assignment.py`` (MaterializedGeometry :48-205). The masks
 must match the reference bit for bit, so every formula uses the same float64
@@ -20,8 +21,17 @@ small enough to underflow the norm). Sphere and cylinder formulas are evaluated
 with the reference's expressions only on the cells inside their bounding box
 plus a two-cell margin; outside it the distance is positive by construction.
 Every value is therefore the reference's own float64 value, so the mask is
-identical; the tests compare it with the direct single-pass evaluation. Other
-SDF types fall back to the direct evaluation (in slabs of whole i-planes).
+identical; the tests compare it with the direct single-pass evaluation. A
+translation is the child's mask on shifted coordinates (p - offset per axis,
+the same values). Scales, rotations and smooth CSG are evaluated directly (in
+slabs of whole i-planes), and so is any other SDF type; trees whose distances
+pass through a BLAS product (rotations, oblique cylinders and cones) over
+every cell at once, as the reference evaluates them.
+
+The reference's smooth intersection and difference blend with its smooth
+minimum's weight h = clip(0.5 + 0.5 (d2 - d1) / k, 0, 1) and add the blend
+term, so away from the blend they return the smaller distance, as a union
+does; they are restated as written, since the masks must be the reference's.
 """
 
 from __future__ import annotations
@@ -62,6 +72,25 @@ class SDFPrimitive(ABC):
         (< 0 when strict). ``slab_points=None`` evaluates all cells at once."""
         nx, ny, nz = grid.shape
         return _direct(self, grid.x_coords, grid.y_coords, grid.z_coords, slab_points, strict)
+
+    # transforms (geometry/sdf.py:128-231)
+    def translate(self, offset) -> "SDFPrimitive":
+        return Translate(self, offset)
+
+    def scale(self, factor) -> "SDFPrimitive":
+        return Scale(self, factor)
+
+    def rotate(self, axis, angle: float) -> "SDFPrimitive":
+        return Rotate(self, axis=axis, angle=angle)
+
+    def rotate_x(self, angle: float) -> "SDFPrimitive":
+        return self.rotate((1, 0, 0), angle)
+
+    def rotate_y(self, angle: float) -> "SDFPrimitive":
+        return self.rotate((0, 1, 0), angle)
+
+    def rotate_z(self, angle: float) -> "SDFPrimitive":
+        return self.rotate((0, 0, 1), angle)
 
 
 def _direct(node, xc, yc, zc, slab_points, strict) -> NDArray[np.bool_]:
@@ -130,7 +159,45 @@ def _mask(node, grid, strict: bool) -> NDArray[np.bool_]:
         return out
     if isinstance(node, HelmholtzResonator):
         return _mask(node._combined, grid, strict)
-    return _direct(node, xc, yc, zc, VOXEL_SLAB_POINTS, strict)
+    if isinstance(node, Translate):
+        # p - offset per axis: the child sees the shifted cell-centre
+        # coordinates, the same float64 values
+        return _mask(node.child, _Coords(xc - node.offset[0], yc - node.offset[1], zc - node.offset[2]), strict)
+    # scales, rotations and smooth CSG: the reference's evaluation of the
+    # distances; trees whose values go through BLAS (rotations, oblique axes)
+    # over every cell at once, as the reference's voxelize does
+    whole = _blas_dependent(node) and np.prod(shape) <= (1 << 25)
+    return _direct(node, xc, yc, zc, None if whole else VOXEL_SLAB_POINTS, strict)
+
+
+class _Coords:
+    """Cell-centre coordinates per axis (the part of a grid _mask reads)."""
+
+    def __init__(self, xc, yc, zc):
+        self.x_coords, self.y_coords, self.z_coords = xc, yc, zc
+
+
+def _blas_dependent(node) -> bool:
+    """Whether a tree's distances pass through a BLAS product whose rounding
+    may depend on a point's place in the array."""
+    if isinstance(node, Rotate):
+        return True
+    if isinstance(node, (Cylinder, Cone)):
+        return np.count_nonzero(node.axis_normalized) != 1
+    kids = []
+    if isinstance(node, (Union, Intersection)):
+        kids = list(node.children)
+    elif isinstance(node, Difference):
+        kids = [node.base, *node.subtracted]
+    elif isinstance(node, (SmoothUnion, SmoothIntersection)):
+        kids = [node.a, node.b]
+    elif isinstance(node, SmoothDifference):
+        kids = [node.base, node.subtracted]
+    elif isinstance(node, Transform):
+        kids = [node.child]
+    elif isinstance(node, HelmholtzResonator):
+        kids = [node._combined]
+    return any(_blas_dependent(k) for k in kids)
 
 
 def _points(points) -> NDArray[np.float64]:
@@ -260,7 +327,11 @@ class Cone(SDFPrimitive):
         return np.minimum(self.p1, self.p2) - r, np.maximum(self.p1, self.p2) + r
 
 
-class Union(SDFPrimitive):
+class CSGNode(SDFPrimitive):
+    """Base class of the CSG operations (geometry/sdf.py:544-547)."""
+
+
+class Union(CSGNode):
     def __init__(self, *children: SDFPrimitive):
         self.children = children
 
@@ -277,7 +348,7 @@ class Union(SDFPrimitive):
                 np.maximum.reduce([c.bounding_box[1] for c in self.children]))
 
 
-class Intersection(SDFPrimitive):
+class Intersection(CSGNode):
     def __init__(self, *children: SDFPrimitive):
         self.children = children
 
@@ -297,7 +368,7 @@ class Intersection(SDFPrimitive):
         return lo, hi
 
 
-class Difference(SDFPrimitive):
+class Difference(CSGNode):
     """base minus subtracted: max(base, -s1, -s2, ...) applied one at a time."""
 
     def __init__(self, base: SDFPrimitive, *subtracted: SDFPrimitive):
@@ -313,6 +384,172 @@ class Difference(SDFPrimitive):
     @property
     def bounding_box(self):
         return self.base.bounding_box
+
+
+# ---- smooth CSG (geometry/sdf.py:736-936) ----------------------------------
+# Polynomial smooth minimum / maximum of two distances with blend radius k:
+# h = clip(0.5 + 0.5 (d2 - d1) / k, 0, 1), then d2 (1 - h) + d1 h -+ k h (1 - h)
+# (minus for the union, plus for the intersection and for the difference,
+# whose second distance is the negated subtracted shape). Evaluated with the
+# reference's expressions, term for term.
+class _SmoothPair(CSGNode):
+    _sign = 1.0
+
+    def _check(self, radius):
+        if radius <= 0:
+            raise ValueError(f"radius must be positive, got {radius}")
+        self.radius = radius
+
+    def _blend(self, d1, d2):
+        h = np.clip(0.5 + 0.5 * (d2 - d1) / self.radius, 0, 1)
+        if self._sign < 0:
+            return d2 * (1 - h) + d1 * h - self.radius * h * (1 - h)
+        return d2 * (1 - h) + d1 * h + self.radius * h * (1 - h)
+
+
+class SmoothUnion(_SmoothPair):
+    _sign = -1.0
+
+    def __init__(self, a: SDFPrimitive, b: SDFPrimitive, radius: float):
+        self._check(radius)
+        self.a, self.b = a, b
+
+    def sdf(self, points):
+        return self._blend(self.a.sdf(points), self.b.sdf(points))
+
+    @property
+    def bounding_box(self):
+        (lo_a, hi_a), (lo_b, hi_b) = self.a.bounding_box, self.b.bounding_box
+        e = np.array([self.radius] * 3)
+        return np.minimum(lo_a, lo_b) - e, np.maximum(hi_a, hi_b) + e
+
+
+class SmoothIntersection(_SmoothPair):
+    def __init__(self, a: SDFPrimitive, b: SDFPrimitive, radius: float):
+        self._check(radius)
+        self.a, self.b = a, b
+
+    def sdf(self, points):
+        return self._blend(self.a.sdf(points), self.b.sdf(points))
+
+    @property
+    def bounding_box(self):
+        (lo_a, hi_a), (lo_b, hi_b) = self.a.bounding_box, self.b.bounding_box
+        e = np.array([self.radius] * 3)
+        lo = np.maximum(lo_a, lo_b) + e
+        hi = np.minimum(hi_a, hi_b) - e
+        if np.any(lo > hi):
+            return np.array([np.inf] * 3), np.array([-np.inf] * 3)
+        return lo, hi
+
+
+class SmoothDifference(_SmoothPair):
+    def __init__(self, base: SDFPrimitive, subtracted: SDFPrimitive, radius: float):
+        self._check(radius)
+        self.base, self.subtracted = base, subtracted
+
+    def sdf(self, points):
+        return self._blend(self.base.sdf(points), -self.subtracted.sdf(points))
+
+    @property
+    def bounding_box(self):
+        lo, hi = self.base.bounding_box
+        e = np.array([self.radius] * 3)
+        return lo - e, hi + e
+
+
+# ---- transforms (geometry/sdf.py:1151-1391) ---------------------------------
+def rotation_matrix(axis, angle: float) -> NDArray[np.floating]:
+    """Rodrigues' rotation matrix I + sin(a) K + (1 - cos(a)) K K for the unit
+    axis (K its cross-product matrix), formed with the reference's numpy
+    expressions (geometry/sdf.py:1151-1185)."""
+    u = np.asarray(axis, dtype=np.float64)
+    u = u / np.linalg.norm(u)
+    K = np.array([[0, -u[2], u[1]], [u[2], 0, -u[0]], [-u[1], u[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1 - np.cos(angle)) * (K @ K)
+
+
+class Transform(SDFPrimitive):
+    """A child primitive evaluated in transformed coordinates."""
+
+    def __init__(self, child: SDFPrimitive):
+        self.child = child
+
+
+class Translate(Transform):
+    """child moved by `offset`: sdf(p) = child.sdf(p - offset)."""
+
+    def __init__(self, child: SDFPrimitive, offset):
+        super().__init__(child)
+        self.offset = np.asarray(offset, dtype=np.float64)
+        if self.offset.shape != (3,):
+            raise ValueError(f"offset must be a 3-element vector, got shape {self.offset.shape}")
+
+    def sdf(self, points):
+        return self.child.sdf(points - self.offset)
+
+    @property
+    def bounding_box(self):
+        lo, hi = self.child.bounding_box
+        return lo + self.offset, hi + self.offset
+
+
+class Scale(Transform):
+    """child scaled by a factor (scalar or per axis): sdf(p) =
+    child.sdf(p / s) times s (uniform) or times min(s) (not a true distance)."""
+
+    def __init__(self, child: SDFPrimitive, scale):
+        super().__init__(child)
+        if np.isscalar(scale):
+            self.scale = np.array([scale, scale, scale], dtype=np.float64)
+            self.uniform = True
+        else:
+            self.scale = np.asarray(scale, dtype=np.float64)
+            self.uniform = np.allclose(self.scale, self.scale[0])
+        if self.scale.shape != (3,):
+            raise ValueError(f"scale must be scalar or 3-element vector, got shape {self.scale.shape}")
+        if np.any(self.scale <= 0):
+            raise ValueError(f"scale factors must be positive, got {self.scale}")
+
+    def sdf(self, points):
+        d = self.child.sdf(points / self.scale)
+        if self.uniform:
+            return d * self.scale[0]
+        return d * np.min(self.scale)
+
+    @property
+    def bounding_box(self):
+        lo, hi = self.child.bounding_box
+        return lo * self.scale, hi * self.scale
+
+
+class Rotate(Transform):
+    """child rotated by `matrix` (or by `angle` about `axis`): sdf(p) =
+    child.sdf(p R), the inverse rotation applied as a product with the
+    transpose of the inverse, as the reference writes it."""
+
+    def __init__(self, child: SDFPrimitive, axis=None, angle: float | None = None, matrix=None):
+        super().__init__(child)
+        self.axis, self.angle = (None, None) if matrix is not None else (axis, angle)
+        if matrix is not None:
+            self.matrix = np.asarray(matrix, dtype=np.float64)
+            if self.matrix.shape != (3, 3):
+                raise ValueError(f"matrix must be 3x3, got shape {self.matrix.shape}")
+        elif axis is not None and angle is not None:
+            self.matrix = rotation_matrix(np.asarray(axis), angle)
+        else:
+            raise ValueError("Must provide either (axis, angle) or matrix")
+        self.inv_matrix = self.matrix.T
+
+    def sdf(self, points):
+        return self.child.sdf(points @ self.inv_matrix.T)
+
+    @property
+    def bounding_box(self):
+        lo, hi = self.child.bounding_box
+        corners = np.array([[x, y, z] for x in (lo[0], hi[0]) for y in (lo[1], hi[1]) for z in (lo[2], hi[2])])
+        rc = corners @ self.matrix.T
+        return rc.min(axis=0), rc.max(axis=0)
 
 
 @dataclass

@@ -829,17 +829,27 @@ class FDTDSolver:
             self._queue_energy()
         h5_snaps = [] if output_file and snapshot_interval else None
         started = _time_mod.time()
+        bar = None
+        if progress:
+            from tqdm import tqdm
+            bar = tqdm(total=n_steps, desc="FDTD simulation")
         try:
             step = 0
             while step < n_steps:
-                if not (callback or progress):
-                    # steps at which nothing is recorded go to the device as one run
+                if not callback:
+                    # steps at which nothing is recorded go to the device as one
+                    # run (with a progress bar, runs of at most 256 steps, each
+                    # waited for before the bar moves)
                     j = step
-                    while j < n_steps and not self._hooks_fire(self._step_count + j - step) and not (
+                    limit = min(n_steps, step + 256) if bar is not None else n_steps
+                    while j < limit and not self._hooks_fire(self._step_count + j - step) and not (
                             h5_snaps is not None and j % snapshot_interval == 0):
                         j += 1
                     if j > step:
                         self.step_n(j - step)
+                        if bar is not None:
+                            self._stream.synchronize()
+                            bar.update(j - step)
                         step = j
                         continue
                 t_before = self._time
@@ -848,8 +858,12 @@ class FDTDSolver:
                     h5_snaps.append((t_before, self._snapshot_p()))
                 if callback:
                     callback(self._step_count - 1)
+                if bar is not None:
+                    bar.update(1)
                 step += 1
         finally:
+            if bar is not None:
+                bar.close()
             self._sync_d2h()
             if output_file:
                 from datetime import datetime, timezone

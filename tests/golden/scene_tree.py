@@ -10,6 +10,11 @@ A tree is a JSON-able list:
   ["union", [child, ...]]
   ["intersection", [child, ...]]
   ["difference", base, [child, ...]]
+  ["smooth_union" | "smooth_intersection" | "smooth_difference", a, b, radius]
+  ["translate", child, offset]
+  ["scale", child, factors]          Scale(child, (sx, sy, sz))
+  ["rotate", child, axis, angle]     Rotate(child, axis=..., angle=...)
+  ["rotate_matrix", child, matrix]   Rotate(child, matrix=...) (row-major 9-list)
   ["helmholtz", position, cavity_shape, cavity_size, neck_length, neck_radius, neck_direction]
   ["materialized", geometry, [region, ...]]   MaterializedGeometry + set_material per region
 
@@ -52,12 +57,27 @@ def to_tree(obj) -> list:
         return ["intersection", [to_tree(c) for c in obj.children]]
     if isinstance(obj, G.Difference):
         return ["difference", to_tree(obj.base), [to_tree(c) for c in obj.subtracted]]
+    if isinstance(obj, G.SmoothUnion):
+        return ["smooth_union", to_tree(obj.a), to_tree(obj.b), float(obj.radius)]
+    if isinstance(obj, G.SmoothIntersection):
+        return ["smooth_intersection", to_tree(obj.a), to_tree(obj.b), float(obj.radius)]
+    if isinstance(obj, G.SmoothDifference):
+        return ["smooth_difference", to_tree(obj.base), to_tree(obj.subtracted), float(obj.radius)]
+    if isinstance(obj, G.Translate):
+        return ["translate", to_tree(obj.child), _v(obj.offset)]
+    if isinstance(obj, G.Scale):
+        return ["scale", to_tree(obj.child), _v(obj.scale)]
+    if isinstance(obj, G.Rotate):
+        if obj.axis is None:
+            return ["rotate_matrix", to_tree(obj.child), _v(obj.matrix)]
+        return ["rotate", to_tree(obj.child), _v(obj.axis), float(obj.angle)]
     raise TypeError(f"no tree form for {type(obj).__name__}")
 
 
 def build(tree, ns: dict):
     """Instantiate a tree with the classes in ``ns`` (Box, Sphere, Cylinder, Cone,
-    Union, Intersection, Difference, HelmholtzResonator, MaterializedGeometry;
+    Union, Intersection, Difference, SmoothUnion, SmoothIntersection,
+    SmoothDifference, Translate, Scale, Rotate, HelmholtzResonator, MaterializedGeometry;
     ``material`` is the object handed to set_material)."""
     kind = tree[0]
     if kind == "box":
@@ -74,6 +94,18 @@ def build(tree, ns: dict):
         return ns["Intersection"](*[build(c, ns) for c in tree[1]])
     if kind == "difference":
         return ns["Difference"](build(tree[1], ns), *[build(c, ns) for c in tree[2]])
+    if kind in ("smooth_union", "smooth_intersection", "smooth_difference"):
+        cls = {"smooth_union": "SmoothUnion", "smooth_intersection": "SmoothIntersection",
+               "smooth_difference": "SmoothDifference"}[kind]
+        return ns[cls](build(tree[1], ns), build(tree[2], ns), tree[3])
+    if kind == "translate":
+        return ns["Translate"](build(tree[1], ns), tuple(tree[2]))
+    if kind == "scale":
+        return ns["Scale"](build(tree[1], ns), tuple(tree[2]))
+    if kind == "rotate":
+        return ns["Rotate"](build(tree[1], ns), axis=tuple(tree[2]), angle=tree[3])
+    if kind == "rotate_matrix":
+        return ns["Rotate"](build(tree[1], ns), matrix=np.array(tree[2], dtype=np.float64).reshape(3, 3))
     if kind == "helmholtz":
         return ns["HelmholtzResonator"](position=tuple(tree[1]), cavity_shape=tree[2],
                                         cavity_size=tuple(tree[3]), neck_length=tree[4],

@@ -242,3 +242,26 @@ def test_energy_sums_tail_and_many_blocks(shape):
     assert e1 == e2
     e64 = _energy64(state, sc.air, sc.dx)
     assert abs(e1 - e64) <= 1e-12 * e64
+
+
+def test_run_progress_and_callback_equal_plain_run():
+    """run(progress=True) (device runs of at most 256 steps between bar
+    updates) and run(callback=...) (one step at a time, the callback after
+    each with the step index) give the probe traces and fields of a plain run
+    bit for bit."""
+    sc = seeded_scenario(46, shape=(24, 26, 40), pml=4, geometry=True)
+    n = 600
+    seen = []
+    results = []
+    for kw in ({}, {"progress": True}, {"callback": seen.append}):
+        s = make_solver(sc)
+        s.run(duration=n * s.dt * (1 - 1e-9), **kw)
+        assert s.step_count == n
+        results.append((s.get_probe_data(), {k: getattr(s, k).copy() for k in FIELDS}))
+    assert seen == list(range(n))
+    base_tr, base_f = results[0]
+    for tr, f in results[1:]:
+        for name in base_tr:
+            assert np.array_equal(tr[name], base_tr[name]), name
+        for k in FIELDS:
+            assert np.array_equal(f[k], base_f[k]), k

@@ -1,0 +1,120 @@
+"""Transforms and smooth CSG (geometry/sdf.py Translate / Scale / Rotate
+:1188-1391, rotation_matrix :1151-1185, SmoothUnion / SmoothIntersection /
+SmoothDifference :736-936) against the known answers of the reference's own
+tests (tests/test_transformations.py, tests/test_sdf_csg.py) and closed forms.
+Their voxel masks are pinned bit for bit to masks the reference wrote
+(tests/test_voxel_golden.py, cases xform_*, smooth_*)."""
+
+import numpy as np
+import pytest
+
+import strata_fdtd_amd as sfa
+
+
+def test_rotation_matrix_known_answers():
+    R = sfa.rotation_matrix(np.array([0, 0, 1]), np.pi / 2)
+    np.testing.assert_allclose(R @ np.array([1, 0, 0]), [0, 1, 0], atol=1e-10)
+    R = sfa.rotation_matrix(np.array([1, 0, 0]), np.pi / 2)
+    np.testing.assert_allclose(R @ np.array([0, 1, 0]), [0, 0, 1], atol=1e-10)
+    R = sfa.rotation_matrix(np.array([1, 1, 1]), np.pi / 3)
+    np.testing.assert_allclose(R @ R.T, np.eye(3), atol=1e-10)
+    assert abs(np.linalg.det(R) - 1.0) < 1e-10
+    # a third of a turn about (1,1,1) permutes the axes: x -> y -> z
+    R2 = sfa.rotation_matrix(np.array([1, 1, 1]), 2 * np.pi / 3)
+    np.testing.assert_allclose(R2 @ np.array([1, 0, 0]), [0, 1, 0], atol=1e-12)
+    np.testing.assert_allclose(R2 @ R2 @ R2, np.eye(3), atol=1e-12)
+    np.testing.assert_allclose(R @ R @ R, sfa.rotation_matrix(np.array([1, 1, 1]), np.pi), atol=1e-12)
+    np.testing.assert_allclose(sfa.rotation_matrix(np.array([2, 0, 0]), np.pi / 2),
+                               sfa.rotation_matrix(np.array([1, 0, 0]), np.pi / 2), atol=1e-10)
+    np.testing.assert_allclose(sfa.rotation_matrix(np.array([1, 0, 0]), 0.0), np.eye(3), atol=1e-10)
+
+
+def test_translate_scale_rotate_known_answers():
+    box = sfa.Box(center=(0, 0, 0), size=(0.02, 0.02, 0.02))
+    off = np.array([0.05, 0.05, 0.05])
+    t = sfa.Translate(box, off)
+    assert box.sdf(np.zeros((1, 3)))[0] < 0 and t.sdf(np.zeros((1, 3)))[0] > 0
+    np.testing.assert_allclose(t.sdf(off[None]), box.sdf(np.zeros((1, 3))), atol=1e-10)
+    lo, hi = t.bounding_box
+    np.testing.assert_allclose(lo, np.array([-0.01] * 3) + off, atol=1e-10)
+    np.testing.assert_allclose(hi, np.array([0.01] * 3) + off, atol=1e-10)
+    pts = np.random.default_rng(1).uniform(-0.05, 0.1, (50, 3))
+    assert np.array_equal(box.translate((0.05, 0.05, 0.05)).sdf(pts), t.sdf(pts))
+    with pytest.raises(ValueError):
+        sfa.Translate(box, (1.0, 2.0))
+
+    s = sfa.Scale(box, 2.0)
+    assert abs(s.sdf(np.array([[0.02, 0, 0]]))[0]) < 1e-10          # surface moved to 2x
+    np.testing.assert_allclose(s.sdf(pts), 2.0 * box.sdf(pts / 2.0), atol=1e-12)
+    lo, hi = sfa.Scale(box, (2.0, 1.0, 0.5)).bounding_box
+    np.testing.assert_allclose(lo, np.array([-0.01] * 3) * [2.0, 1.0, 0.5], atol=1e-10)
+    np.testing.assert_allclose(hi, np.array([0.01] * 3) * [2.0, 1.0, 0.5], atol=1e-10)
+    nu = sfa.Scale(box, (2.0, 1.0, 0.5))
+    np.testing.assert_allclose(nu.sdf(pts), box.sdf(pts / [2.0, 1.0, 0.5]) * 0.5, atol=1e-15)
+    for bad in (-1.0, 0.0):
+        with pytest.raises(ValueError):
+            sfa.Scale(box, bad)
+
+    long = sfa.Box(center=(0, 0, 0), size=(0.04, 0.02, 0.02))
+    r = sfa.Rotate(long, axis=(0, 0, 1), angle=np.pi / 2)
+    assert abs(r.sdf(np.array([[0.0, 0.02, 0.0]]))[0]) < 1e-10
+    lo, hi = r.bounding_box
+    np.testing.assert_allclose(lo, [-0.01, -0.02, -0.01], atol=1e-10)
+    np.testing.assert_allclose(hi, [0.01, 0.02, 0.01], atol=1e-10)
+    np.testing.assert_allclose(sfa.Rotate(long, matrix=sfa.rotation_matrix((0, 0, 1), np.pi / 2)).sdf(pts),
+                               r.sdf(pts), atol=0)
+    for ax, f in ((0, "rotate_x"), (1, "rotate_y"), (2, "rotate_z")):
+        e = np.zeros(3)
+        e[ax] = 1.0
+        assert np.array_equal(getattr(long, f)(0.3).sdf(pts), sfa.Rotate(long, axis=tuple(e), angle=0.3).sdf(pts))
+    with pytest.raises(ValueError):
+        sfa.Rotate(long)
+    with pytest.raises(ValueError):
+        sfa.Rotate(long, matrix=np.eye(2))
+    moved = long.rotate_z(np.pi / 2).translate((0.05, 0.05, 0.0))
+    assert abs(moved.sdf(np.array([[0.05, 0.07, 0.0]]))[0]) < 1e-10
+
+
+def test_smooth_csg_closed_forms():
+    a = sfa.Sphere(center=(0, 0, 0), radius=0.01)
+    b = sfa.Sphere(center=(0.015, 0, 0), radius=0.01)
+    k = 0.004
+    pts = np.random.default_rng(2).uniform(-0.03, 0.04, (200, 3))
+    d1, d2 = a.sdf(pts), b.sdf(pts)
+    su, si, sd = sfa.SmoothUnion(a, b, k), sfa.SmoothIntersection(a, b, k), sfa.SmoothDifference(a, b, k)
+    # The reference blends all three with h = clip(0.5 + 0.5 (d2 - d1) / k):
+    # away from the blend (|d2 - d1| >= k) h is 0 or 1 and each returns the
+    # smaller of its two distances exactly (for the intersection and the
+    # difference too: the blend term is added, but h does not switch to the
+    # larger distance); in the blend the union subtracts k h (1 - h) and the
+    # others add it, at most k / 4 at d1 == d2.
+    far = np.abs(d1 - d2) >= k
+    assert far.sum() > 50 and (~far).sum() > 0
+    assert np.array_equal(su.sdf(pts)[far], np.minimum(d1, d2)[far])
+    assert np.array_equal(si.sdf(pts)[far], np.minimum(d1, d2)[far])
+    far_d = np.abs(-d2 - d1) >= k
+    assert np.array_equal(sd.sdf(pts)[far_d], np.minimum(d1, -d2)[far_d])
+    lo = np.minimum(d1, d2)
+    assert np.all(su.sdf(pts) <= lo + 1e-18) and np.all(su.sdf(pts) >= lo - k / 4 - 1e-18)
+    assert np.all(si.sdf(pts) >= lo - 1e-18) and np.all(si.sdf(pts) <= np.maximum(d1, d2) + k / 4 + 1e-18)
+    mid = np.array([[0.0075, 0.0, 0.0]])                            # d1 == d2
+    np.testing.assert_allclose(su.sdf(mid), a.sdf(mid) - k / 4, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(si.sdf(mid), a.sdf(mid) + k / 4, rtol=0, atol=1e-15)
+    for cls in (sfa.SmoothUnion, sfa.SmoothIntersection, sfa.SmoothDifference):
+        with pytest.raises(ValueError):
+            cls(a, b, 0.0)
+        assert isinstance(cls(a, b, k), sfa.CSGNode)
+    assert isinstance(sfa.Union(a, b), sfa.CSGNode)
+
+
+def test_device_voxeliser_refuses_transforms_and_smooth_trees():
+    """The device voxeliser evaluates boolean CSG programs; transforms and
+    smooth CSG go to the host voxeliser (voxelizer="auto")."""
+    from strata_fdtd_amd.voxelize_device import Unsupported, compile_programs, supports
+    box = sfa.Box(center=(0.02, 0.02, 0.02), size=(0.01, 0.01, 0.01))
+    assert supports(sfa.Union(box))
+    for node in (box.translate((0.001, 0, 0)), box.scale(1.1), box.rotate_z(0.2),
+                 sfa.SmoothUnion(box, box.translate((0.005, 0, 0)), 0.002)):
+        assert not supports(sfa.Union(box, node))
+        with pytest.raises(Unsupported):
+            compile_programs(sfa.Difference(box, node))
