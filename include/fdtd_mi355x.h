@@ -378,8 +378,10 @@ int fdtd_build_flags(const uint8_t* air, const uint8_t* material, uint8_t* flags
                      int nz, void* stream);
 
 /* ---- device voxeliser ------------------------------------------------------
- * Evaluates CSG programs (boxes, spheres, axis-aligned cylinders; union,
- * intersection, difference) in float64 at the cell centres xc/yc/zc (device)
+ * Evaluates CSG programs (boxes, spheres, axis-aligned cylinders and cones;
+ * union, intersection, difference; translations and scales of subtrees, at
+ * most 8 nested, scale factors in [1e-6, 1e6]; geometry/sdf.py:274-722,
+ * 1199-1301) in float64 at the cell centres xc/yc/zc (device)
  * and writes air[c] = (sdf <= 0) and, for material programs, ids[c].
  * Replaces SDFPrimitive.voxelize (geometry/sdf.py:99-127) and
  * MaterializedGeometry.voxelize_with_materials

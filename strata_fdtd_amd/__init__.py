@@ -18,7 +18,7 @@ from .geometry import (Box, Cone, CSGNode, Cylinder, Difference, HelmholtzResona
                        MaterializedGeometry, Rotate, Scale, SDFPrimitive, SmoothDifference, SmoothIntersection,
                        SmoothUnion, Sphere, Transform, Translate, Union, rotation_matrix)
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
-from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, Pole, PoleType, SimpleMaterial
+from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, AcousticMaterial, Pole, PoleType, SimpleMaterial
 from .membranes import CircularMembraneSource, MembraneSource, RectangularMembraneSource
 from .microphones import POLAR_PATTERNS, Microphone
 from .solver import FDTDSolver, has_gpu_support, has_mi355x_backend, has_native_kernels
@@ -31,6 +31,6 @@ __all__ = [
     "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder", "Cone",
     "Union", "Intersection", "Difference", "CSGNode", "SmoothUnion", "SmoothIntersection", "SmoothDifference",
     "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "HelmholtzResonator", "MaterializedGeometry", "Pole",
-    "PoleType", "SimpleMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
+    "PoleType", "SimpleMaterial", "AcousticMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
     "POLAR_PATTERNS", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
 ]

@@ -1,11 +1,11 @@
-// Device voxeliser: CSG trees of boxes, spheres and axis-aligned cylinders and cones
-// evaluated in float64 at every cell centre, giving the air mask (sdf <= 0)
+// Device voxeliser: CSG trees of boxes, spheres and axis-aligned cylinders and cones,
+// translated and scaled, evaluated in float64 at every cell centre, giving the air mask (sdf <= 0)
 // and the material ids of MaterializedGeometry, bit-identical to the
 // reference's numpy evaluation.
 //
 // Reference formulas (geometry/sdf.py): Box :274-294, Sphere :320-333,
 // Cylinder :382-425, Cone :436-543, Union/Intersection/Difference :566-722,
-// voxelize :99-127;
+// Translate :1199-1237, Scale :1239-1301, voxelize :99-127;
 // material ids geometry/material_assignment.py:133-168 (later regions
 // override earlier ones, only inside air).
 //
@@ -19,6 +19,11 @@
 // b <= 0; max(a, b) <= 0 iff both; max(b, -s) <= 0 iff b <= 0 and not s < 0
 // (strict forms dually), so no intermediate float64 value of a CSG node is
 // needed. A box is inside iff every |x - c| - size/2 is <= 0 (< 0 strict).
+// A translation evaluates its subtree at (x - ox, y - oy, z - oz) and a scale
+// at (x / sx, y / sy, z / sz), the coordinates numpy forms (`points - offset`,
+// `points / scale`); a scale multiplies its subtree's distance by a positive
+// factor, which keeps the sign (the factors are limited to [1e-6, 1e6], so
+// the product of a nonzero distance never underflows to zero).
 //
 // Program format (postfix, validated on the host before launch). Each
 // instruction is two int32: {op, arg}.
@@ -29,6 +34,10 @@
 //   the strict variants (+ VOX_STRICT) test sdf < 0 instead of sdf <= 0
 //   VOX_OR / VOX_AND / VOX_ANDNOT   pop b, pop a, push a|b, a&b, a&!b
 //   VOX_TRUE / VOX_FALSE            push a constant
+//   VOX_TRANSLATE arg = offset of {ox, oy, oz}   save the point, then p - o
+//   VOX_SCALE     arg = offset of {sx, sy, sz}   save the point, then p / s
+//   VOX_POP                                      restore the saved point
+//   (transforms nest at most 8 deep and leave the bit stack alone)
 // Program 0 is the air mask; program m >= 1 is material region m, whose id
 // (1..255) is written where the air mask and the region both hold.
 
@@ -50,10 +59,14 @@ enum VoxOp : int32_t {
     VOX_AND = 6,
     VOX_ANDNOT = 7,
     VOX_CONE = 8,
+    VOX_TRANSLATE = 9,
+    VOX_SCALE = 10,
+    VOX_POP = 11,
     VOX_STRICT = 16,
 };
 
 constexpr int kVoxMaxDepth = 32;
+constexpr int kVoxMaxXform = 8;
 constexpr int kVoxMaxProgs = 256;
 
 struct VoxProgram {
@@ -72,12 +85,38 @@ __device__ __forceinline__ double norm3(double a, double b, double c) {
 
 __device__ bool eval_program(const VoxProgram& pg, int m, double x, double y, double z) {
     uint32_t st = 0;  // bit stack, top = bit 0
+    double sx[kVoxMaxXform], sy[kVoxMaxXform], sz[kVoxMaxXform];  // saved points
+    int xd = 0;
     const int b = pg.start[m], e = pg.start[m + 1];
     for (int q = b; q < e; ++q) {
         const int32_t op = pg.ops[2 * q];
         const int32_t arg = pg.ops[2 * q + 1];
         const bool strict = (op & VOX_STRICT) != 0;
         const int32_t code = op & ~VOX_STRICT;
+        if (code == VOX_TRANSLATE || code == VOX_SCALE) {
+            const double* P = pg.params + arg;
+            sx[xd] = x;
+            sy[xd] = y;
+            sz[xd] = z;
+            ++xd;
+            if (code == VOX_TRANSLATE) {
+                x = x - P[0];
+                y = y - P[1];
+                z = z - P[2];
+            } else {
+                x = x / P[0];
+                y = y / P[1];
+                z = z / P[2];
+            }
+            continue;
+        }
+        if (code == VOX_POP) {
+            --xd;
+            x = sx[xd];
+            y = sy[xd];
+            z = sz[xd];
+            continue;
+        }
         bool bit;
         if (code == VOX_BOX) {
             const double* P = pg.params + arg;
@@ -178,11 +217,25 @@ const char* vox_validate(const int32_t* ops, int n_ops, const int32_t* start, in
     for (int m = 0; m < n_prog; ++m) {
         if (start[m + 1] < start[m]) return "program offsets must not decrease";
         if (m >= 1 && (ids[m] < 1 || ids[m] > 255)) return "material ids must be in [1, 255]";
-        int depth = 0;
+        int depth = 0, xdepth = 0;
         for (int q = start[m]; q < start[m + 1]; ++q) {
             const int32_t op = ops[2 * q], arg = ops[2 * q + 1];
             const int32_t code = op & ~VOX_STRICT;
-            if ((op & ~(VOX_STRICT | 15)) != 0 || code > VOX_CONE) return "unknown opcode";
+            if ((op & ~(VOX_STRICT | 15)) != 0 || code > VOX_POP) return "unknown opcode";
+            if (code == VOX_TRANSLATE || code == VOX_SCALE) {
+                if (arg < 0 || arg + 3 > n_params) return "parameter offset out of range";
+                for (int d = 0; d < 3; ++d) {
+                    const double v = params[arg + d];
+                    if (code == VOX_SCALE ? !(v >= 1e-6 && v <= 1e6) : !(v == v && v - v == 0.0))
+                        return "scale factors must be in [1e-6, 1e6] and offsets finite";
+                }
+                if (++xdepth > kVoxMaxXform) return "transforms nested deeper than 8";
+                continue;
+            }
+            if (code == VOX_POP) {
+                if (--xdepth < 0) return "transform stack underflow";
+                continue;
+            }
             int need = 0;
             if (code == VOX_BOX) need = 6;
             else if (code == VOX_SPH) need = 4;
@@ -213,6 +266,7 @@ const char* vox_validate(const int32_t* ops, int n_ops, const int32_t* start, in
             if (depth > kVoxMaxDepth) return "stack deeper than 32";
         }
         if (depth != 1) return "a program must leave exactly one value";
+        if (xdepth != 0) return "a program must restore every transformed point";
     }
     return nullptr;
 }

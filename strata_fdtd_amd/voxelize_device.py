@@ -12,10 +12,14 @@ subtrahend's test made strict, since ``max(b, -s) <= 0`` iff ``b <= 0`` and not
 only inside air.
 
 Supported: Box, Sphere, Cylinder and Cone whose axis is parallel to a grid
-axis, Union, Intersection, Difference and HelmholtzResonator built from those.
-For an oblique cylinder or cone the reference's ``np.dot`` goes through BLAS,
-whose summation order is not fixed, so such trees raise ``Unsupported`` and
-the caller uses the host voxeliser.
+axis, Union, Intersection, Difference and HelmholtzResonator built from those,
+and Translate and Scale (geometry/sdf.py :1199-1301) of any of them: the
+kernel evaluates the subtree at p - offset or p / scale, the coordinates numpy
+forms, and a scale's positive factor (limited to [1e-6, 1e6]) keeps the sign
+of the distance. For an oblique cylinder or cone, and for a rotation (a
+matrix product per point), the reference's values go through BLAS, whose
+summation order is not fixed; smooth CSG needs distances, not inside tests.
+Such trees raise ``Unsupported`` and the caller uses the host voxeliser.
 """
 
 from __future__ import annotations
@@ -25,11 +29,14 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native
-from .geometry import Box, Cone, Cylinder, Difference, HelmholtzResonator, Intersection, Sphere, Union
+from .geometry import Box, Cone, Cylinder, Difference, HelmholtzResonator, Intersection, Scale, Sphere, Translate, Union
 
-VOX_FALSE, VOX_TRUE, VOX_BOX, VOX_SPH, VOX_CYL, VOX_OR, VOX_AND, VOX_ANDNOT, VOX_CONE = range(9)
+(VOX_FALSE, VOX_TRUE, VOX_BOX, VOX_SPH, VOX_CYL, VOX_OR, VOX_AND, VOX_ANDNOT, VOX_CONE, VOX_TRANSLATE, VOX_SCALE,
+ VOX_POP) = range(12)
 VOX_STRICT = 16
 MAX_DEPTH = 32
+MAX_XFORM = 8            # transforms nested per program
+SCALE_RANGE = (1e-6, 1e6)
 
 
 class Unsupported(NotImplementedError):
@@ -84,6 +91,18 @@ def _emit(node, strict: bool, ops: list, params: list) -> int:
         return 1
     if isinstance(node, HelmholtzResonator):
         return _emit(node._combined, strict, ops, params)
+    if isinstance(node, (Translate, Scale)):
+        if isinstance(node, Scale):
+            if not np.all((node.scale >= SCALE_RANGE[0]) & (node.scale <= SCALE_RANGE[1])):
+                raise Unsupported("device voxeliser: scale factors outside [1e-6, 1e6]")
+            ops.append((VOX_SCALE, len(params)))
+            params.extend(map(float, node.scale))
+        else:
+            ops.append((VOX_TRANSLATE, len(params)))
+            params.extend(map(float, node.offset))
+        depth = _emit(node.child, strict, ops, params)
+        ops.append((VOX_POP, 0))
+        return depth
     if isinstance(node, (Union, Intersection)):
         if not node.children:
             # min over nothing is +inf (never inside); max over nothing is -inf (always)
@@ -112,8 +131,14 @@ def compile_programs(geometry, regions=()) -> ProgramSet:
     start = [0]
     ids = []
     for node, mid in [(geometry, 0), *regions]:
+        first = len(ops)
         if _emit(node, False, ops, params) > MAX_DEPTH:
             raise Unsupported("device voxeliser: CSG tree deeper than 32")
+        nest = 0
+        for op, _arg in ops[first:]:
+            nest += 1 if op in (VOX_TRANSLATE, VOX_SCALE) else (-1 if op == VOX_POP else 0)
+            if nest > MAX_XFORM:
+                raise Unsupported("device voxeliser: transforms nested deeper than 8")
         start.append(len(ops))
         ids.append(int(mid))
     return ProgramSet(np.asarray(ops, dtype=np.int32).reshape(-1, 2), np.asarray(start, dtype=np.int32),
@@ -174,12 +199,42 @@ def evaluate_host(ps: ProgramSet, xc, yc, zc):
         return np.sqrt((a * a + b * b) + c * c)
 
     def run(m):
+        nonlocal X, Y, Z
         stack = []
+        saved = []
         for op, arg in ps.ops[ps.start[m]:ps.start[m + 1]]:
             strict = bool(op & VOX_STRICT)
             code = int(op) & ~VOX_STRICT
             le = (lambda d: d < 0) if strict else (lambda d: d <= 0)
-            P = ps.params[arg:arg + 9]
+            P = ps.params[arg:arg + 10]
+            if code in (VOX_TRANSLATE, VOX_SCALE):
+                saved.append((X, Y, Z))
+                if code == VOX_TRANSLATE:
+                    X, Y, Z = X - P[0], Y - P[1], Z - P[2]
+                else:
+                    X, Y, Z = X / P[0], Y / P[1], Z / P[2]
+                continue
+            if code == VOX_POP:
+                X, Y, Z = saved.pop()
+                continue
+            if code == VOX_CONE:
+                ax = int(P[9])
+                v = (X - P[0], Y - P[1], Z - P[2])
+                na = P[3 + ax]
+                L, r1, r2 = P[6], P[7], P[8]
+                t = (v[ax] if na > 0 else -v[ax]) / L
+                tc = np.where(t > 0, t, 0.0)
+                tc = np.where(tc < 1.0, tc, 1.0)
+                rt = r1 + tc * (r2 - r1)
+                sl = tc * L
+                c = [np.full_like(X, P[d]) if d != ax else P[d] + sl * na for d in range(3)]
+                radial = norm3(X - c[0], Y - c[1], Z - c[2])
+                ad = np.abs(t - 0.5) - 0.5
+                axial = np.where(0.0 >= ad, 0.0, ad) * L
+                body = (t >= 0) & (t <= 1.0)
+                rr = radial - rt
+                stack.append(le(np.where(body, rr, np.sqrt(rr * rr + axial * axial))))
+                continue
             if code == VOX_BOX:
                 bit = le(np.abs(X - P[0]) - P[3]) & le(np.abs(Y - P[1]) - P[4]) & le(np.abs(Z - P[2]) - P[5])
             elif code == VOX_SPH:

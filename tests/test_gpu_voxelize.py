@@ -40,6 +40,30 @@ def _device(geometry, grid):
 
 
 @pytest.mark.parametrize("nonuniform", [False, True])
+def test_translated_scaled_trees_equal_host(nonuniform):
+    """Translations and scales run in the device programs: random translated
+    and scaled trees (cones and surface ties included) and material regions
+    give the host voxeliser's masks and ids bit for bit."""
+    import strata_fdtd_amd as sfa
+    from test_transforms import translated_scaled_tree
+    rng = np.random.default_rng(23)
+    grid = (sfa.NonuniformGrid.from_stretch((50, 52, 48), 1e-3, stretch_z=1.03) if nonuniform
+            else sfa.UniformGrid((50, 52, 48), 1e-3))
+    print(f"[voxelize] translated/scaled trees nonuniform={nonuniform}", flush=True)
+    for _ in range(30):
+        node = sfa.Union(translated_scaled_tree(rng), translated_scaled_tree(rng))
+        dev, _ = _device(node, grid)
+        host = node.voxelize(grid)
+        assert np.array_equal(dev, host), int((dev != host).sum())
+    geo = sfa.MaterializedGeometry(translated_scaled_tree(rng))
+    for q in range(3):
+        geo.set_material(translated_scaled_tree(rng), sfa.SimpleMaterial(name=f"m{q}", _rho=1.0 + q, _c=300.0))
+    mask, ids = geo.voxelize_with_materials(grid)
+    dmask, dids = _device(geo, grid)
+    assert np.array_equal(dmask, mask) and np.array_equal(dids, ids)
+
+
+@pytest.mark.parametrize("nonuniform", [False, True])
 def test_random_trees_equal_host(nonuniform):
     from strata_fdtd_amd import NonuniformGrid, UniformGrid
     rng = np.random.default_rng(21)
@@ -122,7 +146,8 @@ def test_device_voxeliser_equals_reference_masks():
     host_only = 0
     for name, case in sorted(CASES.items()):
         obj = scene_tree.build(case["tree"], ns)
-        if name.startswith(("cone_oblique", "xform_", "smooth_")):
+        tree_text = scene_tree.dumps(case["tree"])
+        if name.startswith("cone_oblique") or '"rotate' in tree_text or '"smooth_' in tree_text:
             # tilted axes and rotations go through BLAS in the reference, and
             # smooth CSG needs distances rather than inside tests: the device
             # voxeliser refuses them and a solver falls back to the host
@@ -134,4 +159,7 @@ def test_device_voxeliser_equals_reference_masks():
         assert np.array_equal(dmask, case["mask"]), (name, int((dmask != case["mask"]).sum()))
         if "ids" in case:
             assert np.array_equal(dids, case["ids"]), name
-    assert host_only == 3 + 17 + 9
+    # 3 tilted cone trees, 7 trees with rotations, 9 smooth trees and the 2
+    # transformed material scenes (each with a smooth region); the other 8
+    # transformed trees (translations and scales only) run on the device
+    assert host_only == 3 + 7 + 9 + 2

@@ -95,6 +95,38 @@ def test_material_from_impedance():
     assert abs(m.c_inf - 1482.0) < 1e-9 and abs(m.K_inf - 998.2 * 1482.0**2) < 1e-3
 
 
+def test_material_frequency_response_closed_forms():
+    """AcousticMaterial (materials/base.py:215-505): a lossless material has
+    k = w / c, Z = rho c, no attenuation, reflection (Z - Z') / (Z + Z');
+    a Debye density pole gives rho(w) = rho (1 + dchi / (1 + i w tau)), with
+    the low- and high-frequency limits rho (1 + dchi) and rho, and loss in
+    between (Im k < 0: the poles' susceptibilities use the e^{+iwt}
+    convention, under which a decaying wave has a negative Im k)."""
+    from strata_fdtd_amd import SimpleMaterial
+    from strata_fdtd_amd.materials import AcousticMaterial
+    w = 2 * np.pi * np.array([100.0, 1000.0, 10000.0])
+    air = SimpleMaterial(name="air", _rho=1.2, _c=343.0)
+    assert isinstance(air, AcousticMaterial) and air.n_poles() == 0
+    np.testing.assert_allclose(air.wavenumber(w), w / 343.0, rtol=1e-15)
+    np.testing.assert_allclose(air.phase_velocity(w), 343.0, rtol=1e-15)
+    np.testing.assert_allclose(air.group_velocity(w), 343.0, rtol=1e-9)
+    np.testing.assert_allclose(air.impedance(w), 1.2 * 343.0, rtol=1e-15)
+    assert np.all(air.attenuation(w) == 0) and air.Z_inf == 1.2 * 343.0
+    assert np.allclose(air.reflection_coefficient(w, 1.2 * 343.0), 0) and np.allclose(
+        air.absorption_coefficient(w, 1.2 * 343.0), 1)
+    np.testing.assert_allclose(air.reflection_coefficient(w, 3 * 411.6), (411.6 - 3 * 411.6) / (4 * 411.6))
+    tau, dchi = 1e-4, 0.5
+    m = SimpleMaterial(name="debye", _rho=1.2, _c=343.0, _poles=[Pole(PoleType.DEBYE, dchi, "density", tau=tau)])
+    assert m.n_density_poles() == 1 and m.n_modulus_poles() == 0
+    np.testing.assert_allclose(m.effective_density(w), 1.2 * (1 + dchi / (1 + 1j * w * tau)), rtol=1e-15)
+    np.testing.assert_allclose(m.effective_modulus(w), m.K_inf, rtol=1e-15)
+    np.testing.assert_allclose(m.effective_density(1e-3), 1.2 * (1 + dchi), rtol=1e-6)
+    np.testing.assert_allclose(m.effective_density(1e12), 1.2, rtol=1e-6)
+    assert np.all(m.attenuation(w) < 0)
+    np.testing.assert_allclose(m.attenuation_db_per_meter(w), 20 / np.log(10) * m.attenuation(w), rtol=1e-15)
+    assert "Poles: 1 (1 density, 0 modulus)" in m.summary()
+
+
 def test_pole_and_correction_coefficients_match_oracle():
     dt = T.time_step(1e-3, 343.0)
     p = Pole(PoleType.DEBYE, 0.3, "density", tau=2e-5)
@@ -253,7 +285,13 @@ def test_voxelize_entry_point_validates_programs():
         return rc, lib.fdtd_last_error().decode()
 
     box = [0.0] * 6
-    assert run([(9, 0)], [0, 1], [0], box)[1].endswith("unknown opcode")
+    assert run([(12, 0)], [0, 1], [0], box)[1].endswith("unknown opcode")
+    # transforms: balanced, at most 8 deep, scale factors in [1e-6, 1e6]
+    one = [1.0] * 6
+    assert "restore every transformed point" in run([(9, 0), (2, 0)], [0, 2], [0], one)[1]
+    assert "transform stack underflow" in run([(2, 0), (11, 0)], [0, 2], [0], one)[1]
+    assert "scale factors" in run([(10, 0), (2, 0), (11, 0)], [0, 3], [0], [0.0] * 6)[1]
+    assert "nested deeper than 8" in run([(9, 0)] * 9 + [(2, 0)] + [(11, 0)] * 9, [0, 19], [0], one)[1]
     assert "underflow" in run([(2, 0), (5, 0)], [0, 2], [0], box)[1]
     assert "out of range" in run([(2, 3)], [0, 1], [0], box)[1]
     assert "exactly one value" in run([(2, 0), (2, 0)], [0, 2], [0], box)[1]

@@ -107,14 +107,72 @@ def test_smooth_csg_closed_forms():
     assert isinstance(sfa.Union(a, b), sfa.CSGNode)
 
 
-def test_device_voxeliser_refuses_transforms_and_smooth_trees():
-    """The device voxeliser evaluates boolean CSG programs; transforms and
-    smooth CSG go to the host voxeliser (voxelizer="auto")."""
+def test_device_voxeliser_transform_support():
+    """The device voxeliser evaluates translations and scales (factors in
+    [1e-6, 1e6], nested at most 8 deep); rotations and smooth CSG go to the
+    host voxeliser (voxelizer="auto")."""
     from strata_fdtd_amd.voxelize_device import Unsupported, compile_programs, supports
     box = sfa.Box(center=(0.02, 0.02, 0.02), size=(0.01, 0.01, 0.01))
-    assert supports(sfa.Union(box))
-    for node in (box.translate((0.001, 0, 0)), box.scale(1.1), box.rotate_z(0.2),
-                 sfa.SmoothUnion(box, box.translate((0.005, 0, 0)), 0.002)):
+    assert supports(sfa.Union(box, box.translate((0.001, 0, 0)), box.scale(1.1), box.scale((0.9, 1.0, 1.2))))
+    for node in (box.rotate_z(0.2), sfa.SmoothUnion(box, box.translate((0.005, 0, 0)), 0.002),
+                 box.scale(1e-7), box.translate((0.001, 0, 0)).scale(2.0).rotate_x(0.1)):
         assert not supports(sfa.Union(box, node))
         with pytest.raises(Unsupported):
             compile_programs(sfa.Difference(box, node))
+    deep = box
+    for _ in range(8):
+        deep = deep.translate((1e-4, 0, 0))
+    assert supports(deep)
+    with pytest.raises(Unsupported):
+        compile_programs(deep.scale(1.01))
+
+
+def translated_scaled_tree(rng):
+    """A random tree (cones and surface ties included) under one to three
+    translations (whole millimetres) and uniform or per-axis scales."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_voxel_golden import cone_tree, random_tree
+    child = cone_tree(rng, ties=True) if rng.random() < 0.3 else random_tree(rng, ties=bool(rng.random() < 0.5))
+    for _ in range(rng.integers(1, 4)):
+        if rng.random() < 0.5:
+            child = child.translate(tuple(float(v) for v in rng.integers(-6, 7, 3) * 1e-3))
+        elif rng.random() < 0.5:
+            # Scale(...) rather than .scale(): a Scale's own `scale` attribute
+            # (its factors) hides the method, in the reference as here
+            child = sfa.Scale(child, float(np.round(rng.uniform(0.6, 1.4), 2)))
+        else:
+            child = sfa.Scale(child, tuple(float(v) for v in np.round(rng.uniform(0.7, 1.3, 3), 2)))
+    return child
+
+
+@pytest.mark.parametrize("nonuniform", [False, True])
+def test_device_programs_with_transforms_restate_host_voxeliser(nonuniform):
+    """Programs of translated and scaled trees (cones included), evaluated by
+    the numpy restatement of the kernel, give the host voxeliser's masks
+    (pinned to the reference's masks) on random trees, surface ties
+    included, with material regions."""
+    from strata_fdtd_amd.voxelize_device import compile_programs, evaluate_host
+    rng = np.random.default_rng(17)
+    grid = (sfa.NonuniformGrid.from_stretch((50, 52, 48), 1e-3, stretch_z=1.03) if nonuniform
+            else sfa.UniformGrid((50, 52, 48), 1e-3))
+
+    def xtree():
+        return translated_scaled_tree(rng)
+
+    checked = 0
+    for _ in range(25):
+        node = sfa.Union(xtree(), xtree()) if rng.random() < 0.5 else sfa.Difference(xtree(), xtree())
+        air, _ = evaluate_host(compile_programs(node), grid.x_coords, grid.y_coords, grid.z_coords)
+        want = node.voxelize(grid)
+        assert np.array_equal(air, want)
+        checked += int(0 < want.sum() < want.size)
+    assert checked >= 10
+    geo = sfa.MaterializedGeometry(xtree())
+    for q in range(3):
+        geo.set_material(xtree(), sfa.SimpleMaterial(name=f"m{q}", _rho=1.0 + q, _c=300.0))
+    mask, ids = geo.voxelize_with_materials(grid)
+    ps = compile_programs(geo.geometry, [(r, mid) for r, _m, mid in geo._material_regions])
+    air, ids_p = evaluate_host(ps, grid.x_coords, grid.y_coords, grid.z_coords)
+    assert np.array_equal(air, mask) and np.array_equal(ids_p, ids)
