@@ -14,9 +14,10 @@ Beyond the reference: ``serving`` (continuous batching of many solves) and
 """
 
 from .boundaries import PML, RigidBoundary
-from .geometry import (Box, Cone, CSGNode, Cylinder, Difference, HelmholtzResonator, Intersection,
+from .geometry import (Box, Cone, CSGNode, Cylinder, Difference, HelmholtzResonator, Horn, Intersection,
                        MaterializedGeometry, Rotate, Scale, SDFPrimitive, SmoothDifference, SmoothIntersection,
-                       SmoothUnion, Sphere, Transform, Translate, Union, rotation_matrix)
+                       SmoothUnion, Sphere, Transform, Translate, Union, conical_horn, exponential_horn,
+                       hyperbolic_horn, rotation_matrix, tractrix_horn)
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
 from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, AcousticMaterial, Pole, PoleType, SimpleMaterial
 from .membranes import CircularMembraneSource, MembraneSource, RectangularMembraneSource
@@ -30,7 +31,8 @@ __all__ = [
     "FDTDSolver", "has_mi355x_backend", "has_native_kernels", "has_gpu_support", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
     "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder", "Cone",
     "Union", "Intersection", "Difference", "CSGNode", "SmoothUnion", "SmoothIntersection", "SmoothDifference",
-    "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "HelmholtzResonator", "MaterializedGeometry", "Pole",
+    "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "Horn", "conical_horn", "exponential_horn",
+    "hyperbolic_horn", "tractrix_horn", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AcousticMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
     "POLAR_PATTERNS", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
 ]

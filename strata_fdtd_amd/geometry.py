@@ -3,8 +3,9 @@
 The signed-distance formulas, CSG rules and voxelisation rule follow the
 reference's ``geometry/sdf.py`` (SDFPrimitive.voxelize :99-127, Box :274-294,
 Sphere :320-333, Cylinder :382-425, Cone :436-543,
-Union/Intersection/Difference :566-722, the smooth forms :736-936,
-rotation_matrix and Translate/Scale/Rotate :1151-1391),
+Union/Intersection/Difference :566-722, the smooth forms :736-936, Horn
+:937-1150, rotation_matrix and Translate/Scale/Rotate :1151-1391, the horn
+constructors :1724-1820),
 ``geometry/resonator.py`` (HelmholtzResonator :137-207) and
 ``geometry/material_assignment.py`` (MaterializedGeometry :48-205). The masks
 must match the reference bit for bit, so every formula uses the same float64
@@ -184,6 +185,8 @@ def _blas_dependent(node) -> bool:
         return True
     if isinstance(node, (Cylinder, Cone)):
         return np.count_nonzero(node.axis_normalized) != 1
+    if isinstance(node, Horn):
+        return np.count_nonzero(node._axis) != 1
     kids = []
     if isinstance(node, (Union, Intersection)):
         kids = list(node.children)
@@ -456,6 +459,104 @@ class SmoothDifference(_SmoothPair):
         lo, hi = self.base.bounding_box
         e = np.array([self.radius] * 3)
         return lo - e, hi + e
+
+
+# ---- horns (geometry/sdf.py:937-1150, 1724-1820) -----------------------------
+class Horn(SDFPrimitive):
+    """Surface of revolution from a throat (radius rt) to a mouth (radius rm)
+    whose radius follows a flare profile along the axis, x = t * L:
+    conical rt (1 + m t) with m = sqrt(Am/A0) - 1; exponential rt exp(a x)
+    with a = ln(Am/A0) / 2L; hyperbolic rt cosh(a x) with a =
+    arccosh(sqrt(Am/A0)) / L; tractrix rt / cos(x/R)^2 with R = L /
+    arccos((A0/Am)^(1/4)); A = pi r^2. The distance is the radial excess over
+    the profile radius (at the axial position clamped to the horn) between
+    the end planes and the hypotenuse with the distance past the nearer end
+    plane beyond them."""
+
+    PROFILES = ("conical", "exponential", "hyperbolic", "tractrix")
+
+    def __init__(self, throat_position, mouth_position, throat_radius: float, mouth_radius: float,
+                 profile: str = "exponential"):
+        if profile not in self.PROFILES:
+            raise ValueError(f"profile must be one of {set(self.PROFILES)}, got '{profile}'")
+        self.throat_position = np.array(throat_position, dtype=np.float64)
+        self.mouth_position = np.array(mouth_position, dtype=np.float64)
+        self.throat_radius = float(throat_radius)
+        self.mouth_radius = float(mouth_radius)
+        self.profile = profile
+        if self.throat_radius <= 0 or self.mouth_radius <= 0:
+            raise ValueError(f"Horn radii must be positive, got throat={throat_radius}, mouth={mouth_radius}")
+        axis = self.mouth_position - self.throat_position
+        self._length = np.linalg.norm(axis)
+        if self._length == 0:
+            raise ValueError("Horn throat and mouth positions must be distinct")
+        self._axis = axis / self._length
+        A0 = np.pi * self.throat_radius**2
+        Am = np.pi * self.mouth_radius**2
+        L = self._length
+        if profile == "conical":
+            self._m = np.sqrt(Am / A0) - 1
+        elif profile == "exponential":
+            self._alpha = np.log(Am / A0) / (2 * L)
+        elif profile == "hyperbolic":
+            self._alpha = np.arccosh(np.sqrt(Am / A0)) / L
+        else:
+            self._R = L / np.arccos(np.power(A0 / Am, 0.25))
+
+    def radius_at_position(self, t):
+        t = np.asarray(t)
+        x = t * self._length
+        if self.profile == "conical":
+            return self.throat_radius * (1 + self._m * t)
+        if self.profile == "exponential":
+            return self.throat_radius * np.exp(self._alpha * x)
+        if self.profile == "hyperbolic":
+            return self.throat_radius * np.cosh(self._alpha * x)
+        return self.throat_radius / np.cos(x / self._R) ** 2
+
+    @property
+    def cutoff_frequency(self) -> float:
+        """Flare cutoff for c = 343 m/s: c / (2 pi L) conical, c a / (2 pi)
+        exponential and hyperbolic, c / (4 R) tractrix."""
+        c = 343.0
+        if self.profile == "conical":
+            return c / (2 * np.pi * self._length)
+        if self.profile in ("exponential", "hyperbolic"):
+            return c * self._alpha / (2 * np.pi)
+        return c / (4 * self._R)
+
+    def sdf(self, points):
+        points = _points(points)
+        v = points - self.throat_position
+        t = np.dot(v, self._axis)
+        tn = np.clip(t / self._length, 0, 1)
+        r = np.linalg.norm(v - np.outer(t, self._axis), axis=1)
+        radial = r - self.radius_at_position(tn)
+        beyond = np.maximum(-t, t - self._length)
+        inside_body = (t >= 0) & (t <= self._length)
+        return np.where(inside_body, radial, np.sqrt(radial**2 + beyond**2))
+
+    @property
+    def bounding_box(self):
+        r = np.array([self.mouth_radius] * 3)
+        return (np.minimum(self.throat_position, self.mouth_position) - r,
+                np.maximum(self.throat_position, self.mouth_position) + r)
+
+
+def conical_horn(throat_pos, mouth_pos, throat_r: float, mouth_r: float) -> Horn:
+    return Horn(throat_pos, mouth_pos, throat_r, mouth_r, "conical")
+
+
+def exponential_horn(throat_pos, mouth_pos, throat_r: float, mouth_r: float) -> Horn:
+    return Horn(throat_pos, mouth_pos, throat_r, mouth_r, "exponential")
+
+
+def hyperbolic_horn(throat_pos, mouth_pos, throat_r: float, mouth_r: float) -> Horn:
+    return Horn(throat_pos, mouth_pos, throat_r, mouth_r, "hyperbolic")
+
+
+def tractrix_horn(throat_pos, mouth_pos, throat_r: float, mouth_r: float) -> Horn:
+    return Horn(throat_pos, mouth_pos, throat_r, mouth_r, "tractrix")
 
 
 # ---- transforms (geometry/sdf.py:1151-1391) ---------------------------------

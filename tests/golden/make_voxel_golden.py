@@ -37,12 +37,17 @@ Cases:
                      trees, some nested; the even ones from trees with ties
   xform_nu{NN}       transformed trees on the stretched grid
   smooth_{NN}        smooth unions, intersections and differences
+  horn_{NN}          horns of the four flare profiles along grid axes, alone,
+                     cut out of a box or joined with a sphere; the even ones
+                     with throats on cell centres
+  horn_nu{NN}        horn trees on the stretched grid
+  horn_oblique{NN}   horns with tilted axes
   xform_materials_{N} material regions that are transformed and smooth trees
                      (transforms and smooth CSG: host voxeliser only)
 Cylinders and the cones other than the oblique ones are axis-aligned: their
 np.dot projection sums exact zeros, so the reference's values do not depend on
-the BLAS build. The oblique cones' masks and the rotated trees' masks (a
-matrix product per point) do, and hold for the numpy/BLAS build of this image,
+the BLAS build. The oblique cones' and horns' masks and the rotated trees'
+masks (a matrix product per point) do, and hold for the numpy/BLAS build of this image,
 the one the tests run with.
 """
 
@@ -75,7 +80,7 @@ def reference_namespace() -> dict:
     ns = dict(Box=sdf.Box, Sphere=sdf.Sphere, Cylinder=sdf.Cylinder, Cone=sdf.Cone, Union=sdf.Union,
               Intersection=sdf.Intersection, Difference=sdf.Difference, SmoothUnion=sdf.SmoothUnion,
               SmoothIntersection=sdf.SmoothIntersection, SmoothDifference=sdf.SmoothDifference,
-              Translate=sdf.Translate, Scale=sdf.Scale, Rotate=sdf.Rotate,
+              Translate=sdf.Translate, Scale=sdf.Scale, Rotate=sdf.Rotate, Horn=sdf.Horn,
               HelmholtzResonator=HelmholtzResonator, MaterializedGeometry=MaterializedGeometry,
               material=object(), UniformGrid=UniformGrid, NonuniformGrid=NonuniformGrid)
     assert sdf.__file__.startswith(REF_SRC), sdf.__file__
@@ -180,6 +185,35 @@ def xform_tree(rng, ties=False, depth=0):
     return Union(Rotate(child, axis=tuple(float(a) for a in axis), angle=float(np.round(rng.uniform(-0.6, 0.6), 3))))
 
 
+def horn_tree(rng, ties=False, oblique=False):
+    """Product-class horns (geometry/sdf.py Horn :937-1150) of the four flare
+    profiles along a grid axis (either direction), alone, as a cavity cut out
+    of a box, or joined with a sphere. With ``ties`` the throat sits on a cell
+    centre and the length is whole millimetres; ``oblique`` tilts the axis."""
+    from strata_fdtd_amd import Box, Difference, Sphere, Union
+    from strata_fdtd_amd.geometry import Horn
+    if ties:
+        throat = rng.integers(14, 36, 3) * 1e-3 + 1e-3 / 2
+    else:
+        throat = np.round(rng.uniform(0.015, 0.035, 3), 4)
+    if oblique:
+        d = np.round(rng.uniform(-0.012, 0.012, 3), 3)
+        d[rng.integers(0, 3)] = np.round(rng.uniform(0.01, 0.016), 3)
+    else:
+        d = np.zeros(3)
+        ln = (rng.integers(8, 16) * 1e-3) if ties else np.round(rng.uniform(0.008, 0.016), 4)
+        d[rng.integers(0, 3)] = ln * (1 if rng.random() < 0.5 else -1)
+    rt = float(np.round(rng.uniform(0.002, 0.005), 4))
+    rm = float(np.round(rt * rng.uniform(1.3, 2.5), 4))
+    horn = Horn(tuple(throat), tuple(throat + d), rt, rm, Horn.PROFILES[rng.integers(0, 4)])
+    t = rng.integers(0, 3)
+    if t == 0:
+        return horn
+    if t == 1:
+        return Difference(Box(center=(0.025, 0.026, 0.024), size=(0.04, 0.04, 0.04)), horn)
+    return Union(horn, Sphere(center=tuple(throat), radius=float(np.round(rt * 1.5, 4))))
+
+
 def smooth_tree(rng, ties=False):
     """Smooth union / intersection / difference (geometry/sdf.py :736-936) of
     two random trees, blend radius 1-6 mm."""
@@ -256,6 +290,16 @@ def cases():
                     dict(coords=(nug.x_coords, nug.y_coords, nug.z_coords))))
     for q in range(9):
         out.append((f"smooth_{q:02d}", nontrivial_tree(xrng, cg, make=lambda g, t=(q % 2 == 0): smooth_tree(g, ties=t)),
+                    dict(shape=(50, 52, 48), dx=1e-3)))
+    hrng = np.random.default_rng(6161)
+    for q in range(12):
+        out.append((f"horn_{q:02d}", nontrivial_tree(hrng, cg, make=lambda g, t=(q % 2 == 0): horn_tree(g, ties=t)),
+                    dict(shape=(50, 52, 48), dx=1e-3)))
+    for q in range(2):
+        out.append((f"horn_nu{q:02d}", nontrivial_tree(hrng, nug, make=lambda g: horn_tree(g)),
+                    dict(coords=(nug.x_coords, nug.y_coords, nug.z_coords))))
+    for q in range(2):
+        out.append((f"horn_oblique{q:02d}", nontrivial_tree(hrng, cg, make=lambda g: horn_tree(g, oblique=True)),
                     dict(shape=(50, 52, 48), dx=1e-3)))
     for q in range(2):
         geo = MaterializedGeometry(nontrivial_tree(xrng, cg))

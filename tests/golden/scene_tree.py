@@ -7,6 +7,7 @@ A tree is a JSON-able list:
   ["sphere", center, radius]
   ["cylinder", p1, p2, radius]
   ["cone", p1, p2, r1, r2]
+  ["horn", throat, mouth, throat_radius, mouth_radius, profile]
   ["union", [child, ...]]
   ["intersection", [child, ...]]
   ["difference", base, [child, ...]]
@@ -51,6 +52,9 @@ def to_tree(obj) -> list:
         return ["cylinder", _v(obj.p1), _v(obj.p2), float(obj.radius)]
     if isinstance(obj, G.Cone):
         return ["cone", _v(obj.p1), _v(obj.p2), float(obj.r1), float(obj.r2)]
+    if isinstance(obj, G.Horn):
+        return ["horn", _v(obj.throat_position), _v(obj.mouth_position), float(obj.throat_radius),
+                float(obj.mouth_radius), obj.profile]
     if isinstance(obj, G.Union):
         return ["union", [to_tree(c) for c in obj.children]]
     if isinstance(obj, G.Intersection):
@@ -75,7 +79,7 @@ def to_tree(obj) -> list:
 
 
 def build(tree, ns: dict):
-    """Instantiate a tree with the classes in ``ns`` (Box, Sphere, Cylinder, Cone,
+    """Instantiate a tree with the classes in ``ns`` (Box, Sphere, Cylinder, Cone, Horn,
     Union, Intersection, Difference, SmoothUnion, SmoothIntersection,
     SmoothDifference, Translate, Scale, Rotate, HelmholtzResonator, MaterializedGeometry;
     ``material`` is the object handed to set_material)."""
@@ -88,6 +92,8 @@ def build(tree, ns: dict):
         return ns["Cylinder"](p1=tuple(tree[1]), p2=tuple(tree[2]), radius=tree[3])
     if kind == "cone":
         return ns["Cone"](p1=tuple(tree[1]), p2=tuple(tree[2]), r1=tree[3], r2=tree[4])
+    if kind == "horn":
+        return ns["Horn"](tuple(tree[1]), tuple(tree[2]), tree[3], tree[4], tree[5])
     if kind == "union":
         return ns["Union"](*[build(c, ns) for c in tree[1]])
     if kind == "intersection":

@@ -176,3 +176,37 @@ def test_device_programs_with_transforms_restate_host_voxeliser(nonuniform):
     ps = compile_programs(geo.geometry, [(r, mid) for r, _m, mid in geo._material_regions])
     air, ids_p = evaluate_host(ps, grid.x_coords, grid.y_coords, grid.z_coords)
     assert np.array_equal(air, mask) and np.array_equal(ids_p, ids)
+
+
+@pytest.mark.parametrize("profile", ["conical", "exponential", "hyperbolic", "tractrix"])
+def test_horn_profiles_known_answers(profile):
+    """Horn (geometry/sdf.py:937-1150): every profile runs from the throat
+    radius to the mouth radius, grows monotonically, and the cutoff has its
+    closed form; on the axis inside the horn the distance is minus the
+    profile radius, beyond the mouth it is the distance to the mouth rim's
+    plane combined with the radial excess (reference tests/test_sdf.py
+    TestHorn: radius_at_position, cutoff_frequency)."""
+    h = sfa.Horn((0.0, 0.0, 0.1), (0.0, 0.0, 0.0), 0.01, 0.03, profile)
+    r = h.radius_at_position(np.linspace(0, 1, 11))
+    np.testing.assert_allclose(r[0], 0.01, rtol=1e-12)
+    np.testing.assert_allclose(r[-1], 0.03, rtol=1e-12)
+    assert np.all(np.diff(r) > 0)
+    c, L = 343.0, 0.1
+    want = {"conical": c / (2 * np.pi * L),
+            "exponential": c * np.log(9.0) / (2 * L) / (2 * np.pi),
+            "hyperbolic": c * np.arccosh(3.0) / L / (2 * np.pi),
+            "tractrix": c / (4 * L / np.arccos(np.power(1 / 9.0, 0.25)))}[profile]
+    np.testing.assert_allclose(h.cutoff_frequency, want, rtol=1e-12)
+    d = h.sdf(np.array([[0.0, 0.0, 0.05], [0.0, 0.0, 0.1], [0.0, 0.0, -0.02]]))
+    np.testing.assert_allclose(d[0], -h.radius_at_position(0.5), rtol=1e-12)
+    np.testing.assert_allclose(d[1], -0.01, rtol=1e-12)
+    np.testing.assert_allclose(d[2], np.sqrt(0.03**2 + 0.02**2), rtol=1e-12)
+    ctor = {"conical": sfa.conical_horn, "exponential": sfa.exponential_horn,
+            "hyperbolic": sfa.hyperbolic_horn, "tractrix": sfa.tractrix_horn}[profile]
+    assert ctor((0, 0, 0.1), (0, 0, 0), 0.01, 0.03).profile == profile
+    with pytest.raises(ValueError):
+        sfa.Horn((0, 0, 0), (0, 0, 0), 0.01, 0.03, profile)
+    with pytest.raises(ValueError):
+        sfa.Horn((0, 0, 0), (0, 0, 0.1), 0.0, 0.03, profile)
+    with pytest.raises(ValueError):
+        sfa.Horn((0, 0, 0), (0, 0, 0.1), 0.01, 0.03, "parabolic")
