@@ -125,6 +125,41 @@ int fdtd_ctx_set_membranes(fdtd_ctx* ctx, int n_cells, const int64_t* cell, cons
                            const double* entry_weight, const int32_t* probe_start, int n_probe_refs,
                            const int32_t* probe_idx, int n_src, const double* wave, int64_t wave_len);
 
+/* Boundary planes updated after each step's pressure update (and PML damping),
+ * before its sources and probes, in list order. Replaces apply_pressure of
+ * ABCFirstOrder (first-order Mur, boundaries/_boundaries.py:475-511) and of
+ * RadiationImpedance (:707-776); both exist only in python in the reference.
+ *   kind 0, Mur, both faces of `axis`:
+ *     p[0] = prev0 + coeff (p[1] - p[0]), prev0 = p[1];
+ *     p[N-1] = prev1 + coeff (p[N-2] - p[N-1]), prev1 = p[N-2]
+ *   kind 1, radiation impedance, face `side` (0 low, 1 high), inner plane p_in:
+ *     abc = prev + coeff (p_in - p_b); p_b = refl p_in + omr abc; prev = p_in
+ * with numpy's types: float32 differences, float64 products and sums with the
+ * float64 coeff and omr (= 1 - R), refl rounded to float32 for refl * p_in,
+ * one rounding to float32 per stored value. */
+typedef struct {
+    int32_t kind;        /* 0 Mur ABC, 1 radiation impedance */
+    int32_t axis;        /* 0 x, 1 y, 2 z */
+    int32_t side;        /* kind 1: 0 low face, 1 high face */
+    int32_t pad;
+    double coeff;        /* Mur coefficient (c dt - dx) / (c dt + dx) */
+    double refl;         /* kind 1: reflection coefficient R */
+    double omr;          /* kind 1: 1 - R */
+    int64_t state_off;   /* first element of this entry's previous-step planes in `state` */
+} fdtd_boundary_plane;
+#define FDTD_MAX_BOUNDARY_PLANES 16
+/* planes: host array of n entries (n <= FDTD_MAX_BOUNDARY_PLANES; 0 removes
+ * them). state: device float32 [state_len], zeroed by the caller, holding each
+ * entry's previous-step planes (a Mur entry two cross-sections of its axis,
+ * a radiation entry one) from its state_off on; the library keeps using it.
+ * Every axis a plane acts on needs at least 2 cells. While planes are set,
+ * every probe is recorded after the step's last pass and the step kernels add
+ * no sources themselves: the caller lists every source in the per-cell tables
+ * (fdtd_ctx_set_membranes), which run after the planes, as the reference adds
+ * its sources after its boundaries (core/solver.py:2044-2049). */
+int fdtd_ctx_set_boundary_planes(fdtd_ctx* ctx, int n, const fdtd_boundary_plane* planes, float* state,
+                                 int64_t state_len);
+
 /* Device int64 holding the index of the next step (read by every step kernel,
  * advanced by fdtd_step_n) and a device uint32 health word (bit0: a probe
  * sample was not finite). */

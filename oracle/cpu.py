@@ -40,6 +40,11 @@ class _Pole(C.Structure):
                 ("corr", C.c_double)]
 
 
+class _Boundary(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("axis", C.c_int32), ("side", C.c_int32), ("pad", C.c_int32),
+                ("coeff", C.c_double), ("refl", C.c_double), ("omr", C.c_double), ("state_off", C.c_int64)]
+
+
 class _Problem(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
                 ("nonuniform", C.c_int32), ("cv", C.c_double), ("cp", C.c_double),
@@ -48,7 +53,8 @@ class _Problem(C.Structure):
                 ("src_a", _i64p), ("src_b", _i64p), ("src_wave", _dp), ("wave_len", C.c_int32),
                 ("n_probe", C.c_int32), ("probe_idx", _i64p), ("n_pole", C.c_int32),
                 ("poles", C.POINTER(_Pole)), ("material", _u8p), ("ade_inv_dx", C.c_double),
-                ("src_weight", C.POINTER(_dp))]
+                ("src_weight", C.POINTER(_dp)), ("n_bnd", C.c_int32), ("bnd", C.POINTER(_Boundary)),
+                ("bnd_state", C.c_void_p)]
 
 
 class _Material(C.Structure):
@@ -84,6 +90,24 @@ class Problem:
     rho_inf: np.ndarray | None = None    # per material id (reference harness)
     K_inf: np.ndarray | None = None
     dt: float = 0.0
+    # boundary planes after the PML, before the sources, in order:
+    # (kind 0 Mur ABC | 1 radiation impedance, axis, side, coeff, refl, omr)
+    boundaries: list = field(default_factory=list)
+    # their previous-step planes per run dtype, kept across calls (zeroed by a
+    # call that starts at step 0)
+    bnd_state: dict = field(default_factory=dict)
+
+
+def boundary_layout(shape, boundaries):
+    """Element offset of each boundary entry's planes in the state buffer and
+    the total size: a Mur entry holds two cross-sections of its axis, a
+    radiation entry one."""
+    offs, total = [], 0
+    for kind, axis, *_ in boundaries:
+        cross = int(np.prod(shape)) // shape[axis]
+        offs.append(total)
+        total += cross * (2 if kind == 0 else 1)
+    return offs, total
 
 
 def build(ref: bool = True) -> None:
@@ -174,6 +198,15 @@ class _Packed:
             self.keep.append(mat)
             s.material = _ptr(mat, _u8p)
         s.ade_inv_dx = pb.ade_inv_dx
+        s.n_bnd = len(pb.boundaries)
+        if pb.boundaries:
+            offs, _ = boundary_layout(pb.shape, pb.boundaries)
+            arr = (_Boundary * len(pb.boundaries))()
+            for q, (bd, off) in enumerate(zip(pb.boundaries, offs)):
+                kind, axis, side, coeff, refl, omr = bd
+                arr[q] = _Boundary(kind, axis, side, 0, coeff, refl, omr, off)
+            self.keep.append(arr)
+            s.bnd = C.cast(arr, C.POINTER(_Boundary))
         self.s = s
         m = _Material()
         if pb.rho_inf is not None:
@@ -205,6 +238,14 @@ def run(pb: Problem, n_steps: int, state: dict | None = None, impl: str = "oracl
     for k in ("p", "vx", "vy", "vz"):
         assert state[k].dtype == dtype and state[k].flags.c_contiguous
     pk = _Packed(pb)
+    if pb.boundaries:
+        if impl == "ref":
+            raise ValueError("the reference kernels have no boundary planes (they are python-only there)")
+        _, total = boundary_layout(pb.shape, pb.boundaries)
+        st = pb.bnd_state.get(dtype)
+        if st is None or st.size != max(total, 1) or step0 == 0:
+            st = pb.bnd_state[dtype] = np.zeros(max(total, 1), dtype=dtype)
+        pk.s.bnd_state = st.ctypes.data
     trace = np.zeros((n_steps, max(len(pb.probes), 1)), dtype=np.float32)
     args = [state[k] for k in ("p", "vx", "vy", "vz")]
     if impl == "oracle":
@@ -222,6 +263,19 @@ def run(pb: Problem, n_steps: int, state: dict | None = None, impl: str = "oracl
     if rc != 0:
         raise RuntimeError(f"{impl} run failed: {rc}")
     return state, trace[:, :len(pb.probes)]
+
+
+def boundary_planes(pb: Problem, p: np.ndarray, reset: bool = False) -> None:
+    """Apply the problem's boundary planes once to the float32 field p (in
+    place), continuing their plane state (zeroed first when ``reset``)."""
+    assert p.dtype == np.float32 and p.flags.c_contiguous and p.shape == tuple(pb.shape)
+    pk = _Packed(pb)
+    _, total = boundary_layout(pb.shape, pb.boundaries)
+    st = pb.bnd_state.get(np.float32)
+    if st is None or st.size != max(total, 1) or reset:
+        st = pb.bnd_state[np.float32] = np.zeros(max(total, 1), dtype=np.float32)
+    pk.s.bnd_state = st.ctypes.data
+    _lib("oracle").orc_boundary_planes_f32(C.byref(pk.s), _ptr(p, _fp))
 
 
 # ---- single-phase helpers (float32, in place) --------------------------------

@@ -16,6 +16,8 @@
 //   step ordering        core/solver.py:2003-2077 (python ordering, which is the
 //                        only ordering that runs ADE: core/solver.py:2135-2193)
 //   sources / probes     core/solver.py:2386-2439
+//   boundary planes      first-order Mur ABC and radiation impedance,
+//                        boundaries/_boundaries.py:420-527, 529-792
 //
 // Rounding rules kept on purpose (they decide bit-exact parity):
 //   * every expression is evaluated in float with the same association as the
@@ -234,6 +236,28 @@ struct OrcProblem {
     // membrane sources (src_kind 2, src_a = field 0 p / 1 vx / 2 vy / 3 vz):
     // per source a float64 weight per cell, null for point and plane sources
     const double* const* src_weight;
+    // boundary planes, applied in order after the PML and before the sources
+    int32_t n_bnd;
+    const struct OrcBoundary* bnd;
+    void* bnd_state;               // previous-step planes (float or double, as the run),
+                                   // kept by the caller across calls
+};
+
+// One boundary plane operation on p:
+//   kind 0 (Mur ABC, _boundaries.py:479-511), both faces of `axis`:
+//     p[0] = prev0 + coeff (p[1] - p[0]);      prev0 = p[1]
+//     p[N-1] = prev1 + coeff (p[N-2] - p[N-1]); prev1 = p[N-2]
+//   kind 1 (radiation impedance, :707-776), face `side` (0 low, 1 high):
+//     abc = prev + coeff (p_in - p_b); p_b = refl p_in + omr abc; prev = p_in
+// Types as numpy forms them: the Mur coefficient is a float64 numpy scalar
+// (the solver's dt comes from np.sqrt), so a float32 difference times it is
+// float64 and the sums with it are float64, rounded once when stored; R is a
+// Python float, rounded to the field's type in refl * p_in; omr = 1 - R in
+// double multiplies the float64 abc.
+struct OrcBoundary {
+    int32_t kind, axis, side, pad;
+    double coeff, refl, omr;
+    int64_t state_off;             // first element of this entry's planes in bnd_state
 };
 
 // Per-phase entry points (float) so tests can compare single phases.
@@ -478,6 +502,50 @@ struct Runner {
                 if (pb.poles[q].target == target && pb.poles[q].lorentz == pass) f(q);
     }
 
+    // Boundary plane b on p, one cross-section cell at a time: the cells of a
+    // cross-section column along the axis are independent of the others, so
+    // the reference's whole-plane numpy statements are restated per column in
+    // their order.
+    void boundary_plane(const OrcBoundary& bd, T* p) {
+        const int dims[3] = {nx, ny, nz};
+        const int ax = bd.axis, N = dims[ax];
+        const int nu = ax == 0 ? ny : nx, nw = ax == 2 ? ny : nz;
+        T* st = static_cast<T*>(pb.bnd_state) + bd.state_off;
+        const double coeff = bd.coeff, omr = bd.omr;
+        const T refl = static_cast<T>(bd.refl);
+        auto cell = [&](int a, int u, int w) -> int64_t {
+            if (ax == 0) return lin(a, u, w, ny, nz);
+            if (ax == 1) return lin(u, a, w, ny, nz);
+            return lin(u, w, a, ny, nz);
+        };
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int u = 0; u < nu; ++u)
+            for (int w = 0; w < nw; ++w) {
+                const int64_t t = static_cast<int64_t>(u) * nw + w;
+                if (bd.kind == 0) {
+                    T* prev0 = st;
+                    T* prev1 = st + static_cast<int64_t>(nu) * nw;
+                    const int64_t c0 = cell(0, u, w), c1 = cell(1, u, w);
+                    const int64_t cl = cell(N - 1, u, w), cm = cell(N - 2, u, w);
+                    const T d0 = p[c1] - p[c0];
+                    p[c0] = static_cast<T>(static_cast<double>(prev0[t]) + coeff * static_cast<double>(d0));
+                    prev0[t] = p[c1];
+                    const T d1 = p[cm] - p[cl];
+                    p[cl] = static_cast<T>(static_cast<double>(prev1[t]) + coeff * static_cast<double>(d1));
+                    prev1[t] = p[cm];
+                } else {
+                    const int64_t cb = cell(bd.side ? N - 1 : 0, u, w);
+                    const int64_t ci = cell(bd.side ? N - 2 : 1, u, w);
+                    const T pin = p[ci];
+                    const T d = pin - p[cb];
+                    const double abc = static_cast<double>(st[t]) + coeff * static_cast<double>(d);
+                    const T rig = refl * pin;
+                    p[cb] = static_cast<T>(static_cast<double>(rig) + omr * abc);
+                    st[t] = pin;
+                }
+            }
+    }
+
     void step(T* p, T* vx, T* vy, T* vz, int s, float* trace_row) {
         const bool ade = pb.n_pole > 0;
         if (ade) for_poles(0, [&](int q) { pole_update(q, p); });
@@ -504,6 +572,8 @@ struct Runner {
         }
         pml_velocity<T>(vx, vy, vz, tab(ddx), tab(ddy), tab(ddz), nx, ny, nz);
         pml_pressure<T>(p, tab(ddx), tab(ddy), tab(ddz), nx, ny, nz);
+        // boundary planes in list order (core/solver.py:2044-2047)
+        for (int b = 0; b < pb.n_bnd; ++b) boundary_plane(pb.bnd[b], p);
         // sources in list order (core/solver.py:2386-2433)
         for (int q = 0; q < pb.n_src; ++q) {
             const double w = pb.src_wave[static_cast<int64_t>(q) * pb.wave_len + s];
@@ -547,6 +617,14 @@ extern "C" {
 
 // Run n_steps from the given state (in place). trace: [n_steps][n_probe].
 // step0 selects the waveform column of the first step. Returns 0.
+// The problem's boundary planes alone, once, on p (float32, in place), with
+// the problem's persistent plane state.
+int orc_boundary_planes_f32(const OrcProblem* pb, float* p) {
+    Runner<float> r(*pb);
+    for (int b = 0; b < pb->n_bnd; ++b) r.boundary_plane(pb->bnd[b], p);
+    return 0;
+}
+
 int orc_run_f32(const OrcProblem* pb, int n_steps, int step0, float* p, float* vx, float* vy,
                 float* vz, float* trace) {
     Runner<float> r(*pb);

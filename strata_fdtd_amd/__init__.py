@@ -13,7 +13,7 @@ Beyond the reference: ``serving`` (continuous batching of many solves) and
 ``adjoint`` (gradients w.r.t. per-voxel density and bulk modulus).
 """
 
-from .boundaries import PML, RigidBoundary
+from .boundaries import PML, ABCFirstOrder, RadiationImpedance, RigidBoundary
 from .geometry import (Box, Cone, CSGNode, Cylinder, Difference, HelmholtzResonator, Horn, Intersection,
                        MaterializedGeometry, Rotate, Scale, SDFPrimitive, SmoothDifference, SmoothIntersection,
                        SmoothUnion, Sphere, Transform, Translate, Union, conical_horn, exponential_horn,
@@ -29,7 +29,7 @@ __version__ = "0.1.0"
 
 __all__ = [
     "FDTDSolver", "has_mi355x_backend", "has_native_kernels", "has_gpu_support", "UniformGrid", "NonuniformGrid", "explicit_core_stretch_coords",
-    "PML", "RigidBoundary", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder", "Cone",
+    "PML", "RigidBoundary", "ABCFirstOrder", "RadiationImpedance", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder", "Cone",
     "Union", "Intersection", "Difference", "CSGNode", "SmoothUnion", "SmoothIntersection", "SmoothDifference",
     "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "Horn", "conical_horn", "exponential_horn",
     "hyperbolic_horn", "tractrix_horn", "HelmholtzResonator", "MaterializedGeometry", "Pole",

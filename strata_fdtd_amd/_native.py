@@ -18,6 +18,12 @@ _i = C.c_int
 _f = C.c_float
 _i64 = C.c_int64
 
+class BoundaryPlane(C.Structure):
+    """fdtd_boundary_plane of the C ABI."""
+    _fields_ = [("kind", C.c_int32), ("axis", C.c_int32), ("side", C.c_int32), ("pad", C.c_int32),
+                ("coeff", C.c_double), ("refl", C.c_double), ("omr", C.c_double), ("state_off", C.c_int64)]
+
+
 _SIGS = {
     "fdtd_last_error": (C.c_char_p, []),
     "fdtd_version": (C.c_char_p, []),
@@ -28,6 +34,7 @@ _SIGS = {
     "fdtd_ctx_set_fields": (_i, [_vp] + [_vp] * 8),
     "fdtd_ctx_set_flags": (_i, [_vp, _vp]),
     "fdtd_ctx_set_pml": (_i, [_vp, _vp, _vp, _vp]),
+    "fdtd_ctx_set_boundary_planes": (_i, [_vp, _i, _vp, _vp, _i64]),
     "fdtd_ctx_set_sources": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _i64]),
     "fdtd_ctx_set_probes": (_i, [_vp, _i, _vp, _vp, _i64]),
     "fdtd_ctx_set_membranes": (_i, [_vp, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _i, _vp, _i64]),

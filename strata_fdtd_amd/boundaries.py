@@ -163,3 +163,115 @@ class PML:
     @property
     def is_initialized(self) -> bool:
         return self._initialized
+
+
+def _mur_coefficient(solver) -> float:
+    """(c dt - dx) / (c dt + dx), in float64 from the solver's c, dt and dx."""
+    return (solver.c * solver.dt - solver.dx) / (solver.c * solver.dt + solver.dx)
+
+
+class ABCFirstOrder:
+    """First-order Mur absorbing boundary on both faces of the chosen axes
+    (boundaries/_boundaries.py:420-527): after the step's pressure update,
+    p[0] = prev + k (p[1] - p[0]) and prev = p[1], likewise p[N-1] from
+    p[N-2], with k = (c dt - dx) / (c dt + dx) and prev the inner plane of the
+    previous step; axes in the order x, y, z. On ``backend="mi355x"`` the
+    planes are updated on the device after each sweep (``k_boundary_plane``)."""
+
+    def __init__(self, axis: Literal["all", "x", "y", "z"] | tuple[str, ...] = "all"):
+        if axis == "all":
+            self.axes = ("x", "y", "z")
+        elif isinstance(axis, str):
+            self.axes = (axis,)
+        else:
+            self.axes = tuple(axis)
+        self._solver = None
+        self._coeff = 0.0
+
+    def initialize(self, solver) -> None:
+        self._solver = solver
+        self._coeff = _mur_coefficient(solver)
+
+    def planes(self) -> list[tuple]:
+        """(kind 0, axis, side, coeff, refl, omr) per axis, in x, y, z order."""
+        return [(0, ax, 0, self._coeff, 0.0, 0.0) for ax, name in enumerate("xyz") if name in self.axes]
+
+    def apply_velocity(self, solver) -> None:
+        """No velocity modification (a Mur boundary acts on the pressure)."""
+
+    def apply_pressure(self, solver) -> None:
+        """Applied on the device after each sweep on the MI355X backend."""
+
+    def reset(self) -> None:
+        if self._solver is not None:
+            self._solver._reset_boundary_state()
+
+
+class RadiationImpedance:
+    """Partially reflecting open end on one face (boundaries/_boundaries.py:
+    529-792): after the step's pressure update, with the inner plane p_in, the
+    face plane p_b and the previous step's inner plane prev,
+    abc = prev + m (p_in - p_b), p_b = R p_in + (1 - R) abc, prev = p_in, where
+    m is the Mur coefficient. R is the given reflection coefficient, or for a
+    pipe of radius a: |(z - 1)/(z + 1)| with z = (k a)^2 / 4 at k = 2 pi 1 kHz / c,
+    capped at 0.95."""
+
+    def __init__(self, axis: Literal["x", "y", "z"], side: Literal["low", "high"],
+                 reflection_coeff: float | None = None, pipe_radius: float | None = None):
+        self.axis = axis
+        self.side = side
+        if reflection_coeff is not None:
+            if not 0.0 <= reflection_coeff <= 1.0:
+                raise ValueError("reflection_coeff must be in [0, 1]")
+            self._reflection_coeff = reflection_coeff
+            self._frequency_dependent = False
+        elif pipe_radius is not None:
+            if pipe_radius <= 0:
+                raise ValueError("pipe_radius must be positive")
+            self._pipe_radius = pipe_radius
+            self._frequency_dependent = True
+            self._reflection_coeff = None
+        else:
+            raise ValueError("Must specify either reflection_coeff or pipe_radius")
+        self._solver = None
+        self._initialized = False
+        self._coeff_a = 0.0
+        self._coeff_b = 0.0
+        self._mur = 0.0
+
+    def initialize(self, solver) -> None:
+        self._solver = solver
+        mur = _mur_coefficient(solver)
+        if self._frequency_dependent:
+            ka = (2 * np.pi * 1000 / solver.c) * self._pipe_radius
+            z = ka**2 / 4
+            R = min(abs((z - 1) / (z + 1)), 0.95)
+        else:
+            R = self._reflection_coeff
+        self._coeff_a = (1 - R) * mur
+        self._coeff_b = R
+        self._mur = mur
+        self._initialized = True
+
+    def planes(self) -> list[tuple]:
+        ax = "xyz".index(self.axis)
+        R = self._coeff_b
+        return [(1, ax, 0 if self.side == "low" else 1, self._mur, R, 1 - R)]
+
+    def apply_velocity(self, solver) -> None:
+        """The reference keeps the face's velocity without using it: nothing to do."""
+
+    def apply_pressure(self, solver) -> None:
+        """Applied on the device after each sweep on the MI355X backend."""
+
+    def reset(self) -> None:
+        if self._solver is not None:
+            self._solver._reset_boundary_state()
+
+    @property
+    def reflection_coefficient(self) -> float:
+        return self._coeff_b
+
+    @property
+    def is_initialized(self) -> bool:
+        return self._initialized

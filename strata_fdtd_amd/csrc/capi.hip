@@ -17,7 +17,7 @@
 namespace fdtd_amd {
 int fused_grid(FusedArgs& a, int rows, int planes, int stack);
 hipError_t launch_fused(const FusedArgs& a, int rows, int stack, bool nu, bool ade, int blocks,
-                        hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after);
+                        hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after, const BoundarySet* bnd);
 hipError_t launch_advance(int64_t* step, int n, hipStream_t st);
 hipError_t launch_stream_ceiling(const FusedArgs& a, hipStream_t st);
 hipError_t launch_stream_march(const FusedArgs& a, int rows, int planes, hipStream_t st);
@@ -114,6 +114,7 @@ struct fdtd_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> sweep_ev;  // pairs around each sweep (fdtd_time_sweeps)
     double* energy_partial = nullptr;  // owned: [energy_partials()][2]
+    BoundarySet bnd;                   // boundary planes (state owned by the caller)
 };
 
 extern "C" {
@@ -281,7 +282,8 @@ int fdtd_ctx_set_probes(fdtd_ctx* c, int n, const int64_t* idx, float* trace, in
     if (int rc = check(hipStreamSynchronize(c->stream), "stream sync")) return rc;
     if (int rc = upload(c->probes, host.data(), sizeof(int32_t) * host.size())) return rc;
     c->a.n_probe = n;
-    c->a.n_sweep_probe = std::min(n, kSweepProbes);
+    // with boundary planes every probe is recorded after them
+    c->a.n_sweep_probe = c->bnd.n ? 0 : std::min(n, kSweepProbes);
     c->a.probe_ijk = static_cast<const int32_t*>(c->probes.p);
     c->a.trace = trace;
     c->a.trace_len = n > 0 ? trace_len : 0;
@@ -409,6 +411,35 @@ int fdtd_ctx_set_membranes(fdtd_ctx* c, int n_cells, const int64_t* cell, const 
     return 0;
 }
 
+int fdtd_ctx_set_boundary_planes(fdtd_ctx* c, int n, const fdtd_boundary_plane* planes, float* state,
+                                 int64_t state_len) {
+    if (!c) return fail(-1, "null context");
+    if (n < 0 || n > FDTD_MAX_BOUNDARY_PLANES) return fail(-2, "%d boundary planes (max %d)", n, FDTD_MAX_BOUNDARY_PLANES);
+    if (n > 0 && (!planes || !state)) return fail(-2, "null boundary arrays");
+    if (n > 0 && !c->have_grid) return fail(-3, "set the grid before the boundary planes");
+    const int dims[3] = {c->a.nx, c->a.ny, c->a.nz};
+    BoundarySet b;
+    for (int q = 0; q < n; ++q) {
+        const fdtd_boundary_plane& p = planes[q];
+        if (p.kind != 0 && p.kind != 1) return fail(-2, "boundary plane %d: kind %d", q, p.kind);
+        if (p.axis < 0 || p.axis > 2) return fail(-2, "boundary plane %d: axis %d", q, p.axis);
+        if (p.kind == 1 && p.side != 0 && p.side != 1) return fail(-2, "boundary plane %d: side %d", q, p.side);
+        if (dims[p.axis] < 2) return fail(-2, "boundary plane %d: its axis has %d cell(s), needs 2", q, dims[p.axis]);
+        const int64_t cross = static_cast<int64_t>(dims[0]) * dims[1] * dims[2] / dims[p.axis];
+        const int64_t need = cross * (p.kind == 0 ? 2 : 1);
+        if (p.state_off < 0 || p.state_off + need > state_len)
+            return fail(-2, "boundary plane %d: state [%lld, %lld) outside the %lld-element state", q,
+                        (long long)p.state_off, (long long)(p.state_off + need), (long long)state_len);
+        b.planes[q] = p;
+    }
+    b.n = n;
+    b.state = n ? state : nullptr;
+    drop_graph(c);
+    c->bnd = b;
+    c->a.n_sweep_probe = n ? 0 : std::min(c->a.n_probe, kSweepProbes);
+    return 0;
+}
+
 int fdtd_ctx_set_counters(fdtd_ctx* c, int64_t* step_counter, uint32_t* health) {
     if (!c) return fail(-1, "null context");
     if (!step_counter || !health) return fail(-2, "null counter");
@@ -519,6 +550,9 @@ static int ready(fdtd_ctx* c) {
                     c->have_fields, c->have_flags, c->have_pml, c->have_counters);
     if (c->stack > 1 && static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz >= kMaxCells32)
         return fail(-2, "stacked sweeps need fewer than 2^30 cells; use stack 1");
+    if (c->bnd.n && c->a.n_src)
+        return fail(-2, "with boundary planes every source goes in the per-cell tables (the sweep adds %d)",
+                    c->a.n_src);
     return 0;
 }
 
@@ -528,7 +562,7 @@ static int enqueue_steps(fdtd_ctx* c, int n) {
     for (int s = 0; s < n; ++s) {
         a.koff = s;
         if (int rc = check(launch_fused(a, c->rows, c->stack, c->nonuniform, a.n_pole > 0, blocks, c->stream,
-                                        nullptr, nullptr),
+                                        nullptr, nullptr, c->bnd.n ? &c->bnd : nullptr),
                            "fused step launch"))
             return rc;
     }
@@ -669,7 +703,7 @@ int fdtd_time_sweeps(fdtd_ctx* c, int n_steps, float* ms_sweep, float* ms_total)
     for (int s = 0; s < n_steps; ++s) {
         a.koff = s;
         if (int rc = check(launch_fused(a, c->rows, c->stack, c->nonuniform, a.n_pole > 0, blocks, c->stream,
-                                        c->sweep_ev[2 * s], c->sweep_ev[2 * s + 1]),
+                                        c->sweep_ev[2 * s], c->sweep_ev[2 * s + 1], c->bnd.n ? &c->bnd : nullptr),
                            "fused step launch"))
             return rc;
     }
@@ -959,9 +993,9 @@ int fdtd_batch_set(fdtd_batch* b, fdtd_ctx* const* ctxs, const int64_t* step_lim
         if (c->a.n_probe > kSweepProbes)
             return fail(-2, "batch entry %d: %d probes (batched solves record at most %d)", q, c->a.n_probe,
                         kSweepProbes);
-        if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0 || c->a.n_mcell > 0)
-            return fail(-2, "batch entry %d: nonuniform, ADE, microphone or membrane-source solves run on their "
-                            "own context", q);
+        if (c->nonuniform || c->a.n_pole > 0 || c->a.n_mic > 0 || c->a.n_mcell > 0 || c->bnd.n > 0)
+            return fail(-2, "batch entry %d: nonuniform, ADE, microphone, membrane-source or boundary-plane "
+                            "solves run on their own context", q);
         if (c->device != b->device) return fail(-2, "batch entry %d is on another device", q);
         if (static_cast<int64_t>(c->a.nx) * c->a.ny * c->a.nz >= kMaxCells32)
             return fail(-2, "batch entry %d: batched solves need fewer than 2^30 cells", q);

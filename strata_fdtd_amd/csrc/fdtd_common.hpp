@@ -183,4 +183,11 @@ struct FusedArgs {
     int64_t n_ade_cells;            //   (null: the ADE kernels visit every box cell)
 };
 
+// The boundary planes of a context (host side of a launch).
+struct BoundarySet {
+    int n = 0;
+    fdtd_boundary_plane planes[FDTD_MAX_BOUNDARY_PLANES];
+    float* state = nullptr;
+};
+
 }  // namespace fdtd_amd

@@ -677,12 +677,12 @@ __global__ void k_mic_gather(const FusedArgs a) {
     }
 }
 
-// Probes kSweepProbes.. n_probe-1 of step s (the sweep records the first
-// kSweepProbes): after every pass of the step has written the output set
+// Probes n_sweep_probe.. n_probe-1 of step s (the sweep records the first
+// n_sweep_probe): after every pass of the step has written the output set
 // (s+1)&1, one thread per probe copies p at its cell into the trace, as
 // _record_probes does after all sources (core/solver.py:2435-2439).
 __global__ void k_probe_tail(const FusedArgs a) {
-    const int q = kSweepProbes + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int q = a.n_sweep_probe + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (q >= a.n_probe) return;
     const int64_t s = *a.step_ptr + a.koff;
     if (s >= a.step_limit) return;
@@ -695,6 +695,49 @@ __global__ void k_probe_tail(const FusedArgs a) {
     const float ps = a.f[par][0][c];
     a.trace[s * a.n_probe + q] = ps;
     if (!isfinite(ps)) atomicOr(a.health, 1u);
+}
+
+// Boundary plane `bp` of step s on the output pressure (s+1)&1, after the
+// sweep and the ADE fix-up, before the table sources and the probes: one
+// thread per cell of the plane's cross-section, which owns that column's
+// cells on the faces (and reads its inner planes), in the reference's
+// statement order (boundaries/_boundaries.py:479-511 Mur, :707-776 radiation).
+// Float32 differences, float64 products and sums with coeff / omr, refl in
+// float32, one rounding per stored value: numpy's types for a float64
+// coefficient scalar and a Python-float R.
+__global__ void k_boundary_plane(const FusedArgs a, const fdtd_boundary_plane bp, float* __restrict__ state) {
+    const int64_t s = *a.step_ptr + a.koff;
+    if (s >= a.step_limit) return;
+    const int ax = bp.axis;
+    const int N = ax == 0 ? a.nx : (ax == 1 ? a.ny : a.nz);
+    const int nu = ax == 0 ? a.ny : a.nx, nw = ax == 2 ? a.ny : a.nz;
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(nu) * nw) return;
+    const int u = static_cast<int>(t / nw), w = static_cast<int>(t % nw);
+    float* __restrict__ P = a.f[(s + 1) & 1][0];
+    auto cell = [&](int c) -> int64_t {
+        return ax == 0 ? lin(c, u, w, a.ny, a.nz) : (ax == 1 ? lin(u, c, w, a.ny, a.nz) : lin(u, w, c, a.ny, a.nz));
+    };
+    float* st = state + bp.state_off;
+    if (bp.kind == 0) {
+        float* prev0 = st;
+        float* prev1 = st + static_cast<int64_t>(nu) * nw;
+        const int64_t c0 = cell(0), c1 = cell(1), cl = cell(N - 1), cm = cell(N - 2);
+        const float d0 = P[c1] - P[c0];
+        P[c0] = static_cast<float>(static_cast<double>(prev0[t]) + bp.coeff * static_cast<double>(d0));
+        prev0[t] = P[c1];
+        const float d1 = P[cm] - P[cl];
+        P[cl] = static_cast<float>(static_cast<double>(prev1[t]) + bp.coeff * static_cast<double>(d1));
+        prev1[t] = P[cm];
+    } else {
+        const int64_t cb = cell(bp.side ? N - 1 : 0), ci = cell(bp.side ? N - 2 : 1);
+        const float pin = P[ci];
+        const float d = pin - P[cb];
+        const double abc = static_cast<double>(st[t]) + bp.coeff * static_cast<double>(d);
+        const float rig = static_cast<float>(bp.refl) * pin;
+        P[cb] = static_cast<float>(static_cast<double>(rig) + bp.omr * abc);
+        st[t] = pin;
+    }
 }
 
 // Membrane sources of step s (core/solver.py:2389-2412): after the sweep and
@@ -796,11 +839,12 @@ int fused_grid(FusedArgs& a, int rows, int planes, int stack) {
     return blocks;
 }
 
-// One full step: [ADE density] -> fused sweep -> [ADE fix-up] -> [membrane
-// sources] -> [microphones]. ev_before /
-// ev_after (optional) are recorded around the sweep alone, for timing it.
+// One full step: [ADE density] -> fused sweep -> [ADE fix-up] -> [boundary
+// planes] -> [table sources] -> [probes past the sweep's] -> [microphones].
+// ev_before / ev_after (optional) are recorded around the sweep alone, for
+// timing it. bnd: the context's boundary planes, or null.
 hipError_t launch_fused(const FusedArgs& a_in, int rows, int stack, bool nu, bool ade, int blocks,
-                        hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after) {
+                        hipStream_t st, hipEvent_t ev_before, hipEvent_t ev_after, const BoundarySet* bnd) {
     FusedArgs a = a_in;
     a.sweep_blocks = blocks;
     a.dens_blocks = 0;
@@ -835,12 +879,22 @@ hipError_t launch_fused(const FusedArgs& a_in, int rows, int stack, bool nu, boo
         else hipLaunchKernelGGL(k_ade_fixup<false>, dim3(box_blocks), dim3(256), 0, st, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if (bnd) {
+        for (int b = 0; b < bnd->n; ++b) {
+            const fdtd_boundary_plane& bp = bnd->planes[b];
+            const int64_t cross = static_cast<int64_t>(a.nx) * a.ny * a.nz /
+                                  (bp.axis == 0 ? a.nx : (bp.axis == 1 ? a.ny : a.nz));
+            hipLaunchKernelGGL(k_boundary_plane, dim3(static_cast<unsigned>((cross + 255) / 256)), dim3(256), 0, st,
+                               a, bp, bnd->state);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+    }
     if (a.n_mcell > 0) {
         hipLaunchKernelGGL(k_membrane, dim3((a.n_mcell + 255) / 256), dim3(256), 0, st, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    if (a.n_probe > kSweepProbes) {
-        hipLaunchKernelGGL(k_probe_tail, dim3((a.n_probe - kSweepProbes + 255) / 256), dim3(256), 0, st, a);
+    if (a.n_probe > a.n_sweep_probe) {
+        hipLaunchKernelGGL(k_probe_tail, dim3((a.n_probe - a.n_sweep_probe + 255) / 256), dim3(256), 0, st, a);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (a.n_mic > 0) {

@@ -218,11 +218,38 @@ class Microphone:
             self._solver._sync_microphones()
 
     def get_waveform(self, weighting=None) -> NDArray[np.floating]:
+        """Recorded output, float32; weighting "A" or "C" filters it with the
+        IEC 61672-1 weighting at the solver's sample rate (core/solver.py:
+        1175-1205)."""
         self._sync()
         waveform = np.array(self._data, dtype=np.float32)
         if weighting in ("Z", None) or len(waveform) == 0:
             return waveform
-        raise NotImplementedError("A/C frequency weighting (analysis package) is not part of the mi355x backend")
+        from .levels import apply_weighting
+        return apply_weighting(waveform, self.get_sample_rate(), weighting).astype(np.float32)
+
+    def _levels_input(self, weighting):
+        if len(self) == 0:
+            raise RuntimeError("No data recorded. Run simulation first.")
+        return self.get_waveform(weighting=weighting)
+
+    def get_spl(self, weighting=None, p_ref: float = 2e-5) -> float:
+        """20 log10(p_rms / p_ref) of the (weighted) recording (core/solver.py:1226-1258)."""
+        from .levels import spl
+        return spl(self._levels_input(weighting), p_ref)
+
+    def get_leq(self, weighting=None, duration: float | None = None, p_ref: float = 2e-5) -> float:
+        """Equivalent level of the most recent `duration` seconds, or of the
+        whole recording (core/solver.py:1260-1296)."""
+        from .levels import leq
+        return leq(self._levels_input(weighting), self.get_sample_rate(), duration, p_ref)
+
+    def get_time_weighted_level(self, weighting=None, time_constant: str = "fast",
+                                p_ref: float = 2e-5) -> NDArray[np.floating]:
+        """Fast / slow / impulse exponentially averaged level per sample
+        (core/solver.py:1298-1335)."""
+        from .levels import time_weighted_level
+        return time_weighted_level(self._levels_input(weighting), self.get_sample_rate(), time_constant, p_ref)
 
     def get_time_axis(self) -> NDArray[np.floating]:
         self._sync()

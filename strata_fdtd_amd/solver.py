@@ -26,7 +26,7 @@ import numpy as np
 from numpy.typing import NDArray
 
 from . import _native
-from .boundaries import PML
+from .boundaries import PML, ABCFirstOrder, RadiationImpedance
 from .grid import NonuniformGrid, UniformGrid
 from .materials import density_correction_coeff, modulus_correction_coeff
 from .sources import GaussianPulse, Probe, accumulated_times, waveform_table
@@ -160,6 +160,8 @@ class FDTDSolver:
         self._mic_synced: dict[str, int] = {}  # steps already moved into the Microphone
         self._step_times: list[float] = []     # solver time at which each step was taken
         self._boundaries: list = []
+        self._bnd_state = None                 # device float32 planes of Mur / radiation boundaries
+        self._bnd_planes: list = []
         self._step_count = 0
         self._time = 0.0
         self._snapshots: list = []
@@ -427,16 +429,58 @@ class FDTDSolver:
             self._mic_synced[name] = self._step_count
 
     def add_boundary(self, boundary) -> None:
+        """Boundaries act in the order they are added (core/solver.py:1963-1990,
+        2044-2047). The PML damps inside the step kernel; Mur ABC and radiation
+        impedance planes are updated after it, so a PML must come before them."""
         boundary.initialize(self)
         if isinstance(boundary, PML):
             if any(isinstance(b, PML) for b in self._boundaries):
                 raise NotImplementedError("more than one PML per solver")
+            if self._bnd_planes:
+                raise NotImplementedError("a PML added after a Mur or radiation boundary (its damping runs "
+                                          "inside the step, before the boundary planes)")
             torch = _torch()
             for ax, tab in enumerate(boundary.decay_tables(self.dt, self.shape)):
                 self._decay[ax].copy_(torch.from_numpy(np.ascontiguousarray(tab)))
+        elif isinstance(boundary, (ABCFirstOrder, RadiationImpedance)):
+            self._boundaries.append(boundary)
+            self._setup_boundary_planes()
+            return
         elif type(boundary).__name__ != "RigidBoundary":
             raise NotImplementedError(f"{type(boundary).__name__} is not implemented on the mi355x backend")
         self._boundaries.append(boundary)
+
+    def _setup_boundary_planes(self) -> None:
+        """Hand the Mur / radiation planes to the context, in list order, with
+        a zeroed device state (each entry's previous-step planes); every source
+        moves to the per-cell tables, which run after the planes."""
+        import ctypes as C
+        torch = _torch()
+        planes = [pl for b in self._boundaries if isinstance(b, (ABCFirstOrder, RadiationImpedance))
+                  for pl in b.planes()]
+        if len(planes) > 16:
+            raise NotImplementedError(f"{len(planes)} boundary planes (at most 16)")
+        offs, total = [], 0
+        for kind, axis, *_ in planes:
+            if self.shape[axis] < 2:
+                raise ValueError(f"a boundary plane on axis {'xyz'[axis]} needs at least 2 cells")
+            offs.append(total)
+            total += int(np.prod(self.shape)) // self.shape[axis] * (2 if kind == 0 else 1)
+        self._stream.synchronize()  # steps in flight may still use the previous state
+        with torch.cuda.stream(self._stream):
+            self._bnd_state = torch.zeros(max(total, 1), dtype=torch.float32, device=self._device)
+        arr = (_native.BoundaryPlane * max(len(planes), 1))()
+        for q, ((kind, axis, side, coeff, refl, omr), off) in enumerate(zip(planes, offs)):
+            arr[q] = _native.BoundaryPlane(kind, axis, side, 0, coeff, refl, omr, off)
+        _native.call("fdtd_ctx_set_boundary_planes", self._ctx, len(planes), C.cast(arr, C.c_void_p),
+                     _native.ptr(self._bnd_state), max(total, 1))
+        self._bnd_planes = planes
+        self._dirty_sources = True
+
+    def _reset_boundary_state(self) -> None:
+        if self._bnd_state is not None:
+            with _torch().cuda.stream(self._stream):
+                self._bnd_state.zero_()
 
     def enable_snapshots(self, interval: int, capture_velocity: bool = False) -> None:
         self._snapshot_interval = interval
@@ -563,6 +607,12 @@ class FDTDSolver:
         # its sources in the order of the list (core/solver.py:2386-2433).
         n_cells = int(np.prod(self.shape))
         first = next((q for q, s in enumerate(self._sources) if _table_source(s, n_cells)), len(self._sources))
+        if self._bnd_planes and self._sources:
+            # boundary planes come between the step and its sources: every
+            # source is added after them, from the tables (32-bit cell indices)
+            if n_cells >= 2**31:
+                raise NotImplementedError("sources with Mur / radiation boundaries on 2^31 cells or more")
+            first = 0
         if first > SWEEP_SOURCES:
             # the sweep adds at most SWEEP_SOURCES sources itself; the rest of
             # the list goes to the tables (32-bit cell indices)
