@@ -116,17 +116,19 @@ class _Device:
         import torch
         return [torch.zeros(self.pb.shape, dtype=self.dtype, device=self.dev) for _ in range(4)]
 
-    def zeros_state_pair(self):
-        """Both ping-pong sets {p, vx, vy, vz} as views of one zeroed slab.
-        Eight separate allocations land wherever the allocator finds room, and
-        at 192^3 (fields plus parameters just inside the 256 MiB Infinity
-        Cache) that moved the fast forward between 91 and 111 ms for 2000
-        steps; one slab ran it in 90-92 ms every time."""
+    def zeros_state_pair(self, pairs: int = 1):
+        """Both ping-pong sets {p, vx, vy, vz} as views of one zeroed slab
+        (`pairs` such pairs from the same slab). Eight separate allocations
+        land wherever the allocator finds room, and at 192^3 (fields plus
+        parameters just inside the 256 MiB Infinity Cache) that moved the fast
+        forward between 91 and 111 ms for 2000 steps; one slab ran it in
+        90-92 ms every time."""
         import torch
         n = int(np.prod(self.pb.shape))
-        slab = torch.zeros(8 * n, dtype=self.dtype, device=self.dev)
-        views = [slab[q * n:(q + 1) * n].view(self.pb.shape) for q in range(8)]
-        return [views[0:4], views[4:8]]
+        slab = torch.zeros(8 * pairs * n, dtype=self.dtype, device=self.dev)
+        views = [slab[q * n:(q + 1) * n].view(self.pb.shape) for q in range(8 * pairs)]
+        out = [[views[8 * m:8 * m + 4], views[8 * m + 4:8 * m + 8]] for m in range(pairs)]
+        return out[0] if pairs == 1 else out
 
     def step(self, src_state, dst_state, n, r, kappa, trace_row):
         pb = self.pb
@@ -265,7 +267,13 @@ def _forward_fast(pb, r, kappa, dv, history: str):
     import torch
     fast = _Fast(dv)
     rk = fast.pack(r, kappa)
-    sets = dv.zeros_state_pair()
+    rec = None
+    if history == "checkpoint":
+        # the recomputation's fields come from the same slab as the forward's,
+        # so every call sees the same placement (one slab, not two)
+        sets, rec = dv.zeros_state_pair(pairs=2)
+    else:
+        sets = dv.zeros_state_pair()
     traces = torch.zeros((pb.steps, max(dv.n_probe, 1)), dtype=torch.float32, device=dv.dev)
     ckpts, hist = [], None
     if history == "all":
@@ -278,12 +286,12 @@ def _forward_fast(pb, r, kappa, dv, history: str):
             fast.forward(sets, rk, traces, n0, min(K, pb.steps - n0))
     else:
         fast.forward(sets, rk, traces, 0, pb.steps)
-    return traces[:, :dv.n_probe], ("fast", dv, rk, fast, history, hist, ckpts, sets)
+    return traces[:, :dv.n_probe], ("fast", dv, rk, fast, history, hist, ckpts, (sets, rec))
 
 
 def _adjoint_fast(saved, grad_traces):
     import torch
-    _, dv, rk, fast, history, hist, ckpts, sets = saved
+    _, dv, rk, fast, history, hist, ckpts, (sets, rec) = saved
     pb = dv.pb
     # The forward's field slab is free once its history (or checkpoints) is
     # kept: reused, the adjoint and the next forward see the same buffer
@@ -298,7 +306,6 @@ def _adjoint_fast(saved, grad_traces):
         fast.adjoint(lam, rk, g, 0, pb.steps, hist, 0, acc)
     elif history == "checkpoint":
         K = pb.checkpoint
-        rec = dv.zeros_state_pair()
         seg_hist = fast.history(K)
         scratch = torch.zeros_like(g)
         for seg in reversed(range(len(ckpts))):
