@@ -18,6 +18,7 @@ from .geometry import (Box, Cone, CSGNode, Cylinder, Difference, HelmholtzResona
                        MaterializedGeometry, Rotate, Scale, SDFPrimitive, SmoothDifference, SmoothIntersection,
                        SmoothUnion, Sphere, Transform, Translate, Union, conical_horn, exponential_horn,
                        hyperbolic_horn, rotation_matrix, tractrix_horn)
+from .hdf5_in import HDF5ResultReader
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
 from .materials import AIR_15C, AIR_20C, AIR_25C, WATER_20C, AcousticMaterial, Pole, PoleType, SimpleMaterial
 from .membranes import CircularMembraneSource, MembraneSource, RectangularMembraneSource
@@ -34,5 +35,5 @@ __all__ = [
     "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "Horn", "conical_horn", "exponential_horn",
     "hyperbolic_horn", "tractrix_horn", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AcousticMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
-    "POLAR_PATTERNS", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
+    "POLAR_PATTERNS", "HDF5ResultReader", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
 ]

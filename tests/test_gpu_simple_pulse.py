@@ -109,3 +109,11 @@ def test_run_writes_reference_result_file(tmp_path):
         assert rel_l2(p[sub][t], d["snapshots_sub"][t]) < 1e-6, t
     assert np.array_equal(f.root["materials/geometry"].read(), air_mask(d).astype(np.uint8))
     assert f.root["metadata"].attrs["backend"] == "mi355x"
+    # read back through the reference's reader interface
+    with sfa.HDF5ResultReader(path) as rd:
+        assert rd.get_num_snapshots() == p.shape[0] and sorted(rd.get_probe_names()) == sorted(names)
+        assert np.array_equal(rd.load_timestep(0), d["snapshot0"])
+        for n in names:
+            assert np.array_equal(rd.load_probe(n), probes[n].astype(np.float32))
+        assert np.array_equal(rd.load_geometry(), air_mask(d))
+        assert rd.get_metadata()["simulation"]["num_steps"] == steps

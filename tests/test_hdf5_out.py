@@ -1,5 +1,5 @@
 """The product's HDF5 writer (strata_fdtd_amd/hdf5_out.py) read back by the
-minimal HDF5 parser of tests/golden/hdf5_read.py, which reads the
+minimal HDF5 parser of strata_fdtd_amd/hdf5_in.py, which reads the
 reference's own h5py-written sample file (tests/test_simple_pulse.py pins
 what it extracts from that file against the oracle). Every attribute type
 the result layout uses, nested and empty groups, a group with more members
@@ -10,6 +10,7 @@ import os
 import sys
 
 import numpy as np
+import pytest
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
 
@@ -139,3 +140,43 @@ def test_chunked_round_trip(tmp_path):
     st = H.storage(r)
     assert st["/snaps"]["maxdims"] == [None, 4, 5, 6] and st["/raw"]["filter_hex"] is None
     assert st["/geo"]["chunk"] == list(W.guess_chunk(geo.shape, 1))
+
+
+def test_result_reader(tmp_path):
+    """HDF5ResultReader (io/hdf5.py:243-377) on a file in the result layout."""
+    from strata_fdtd_amd import HDF5ResultReader
+    rng = np.random.default_rng(4)
+    root = W.Group()
+    root.group("metadata").attrs.update(solver_version="0.1.0", backend="mi355x")
+    root.group("grid").attrs.update(shape=np.array([4, 5, 6], np.int64), is_uniform=False)
+    for a, n in zip("xyz", (4, 5, 6)):
+        root.group("grid").dataset(f"{a}_coords", np.linspace(0, 1, n))
+    root.group("simulation").attrs.update(timestep=1e-6, num_steps=np.int64(9))
+    root.group("sources").group("source_0").attrs.update(type="point", frequency=1000.0)
+    snaps = rng.standard_normal((3, 4, 5, 6)).astype(np.float32)
+    root.group("fields").dataset("pressure", snaps, chunks=(1, 4, 5, 6), maxshape=(None, 4, 5, 6), gzip=4,
+                                 units="Pa")
+    tr = rng.standard_normal(9).astype(np.float32)
+    root.group("probes").dataset("mic", tr, chunks=(1024,), maxshape=(None,), gzip=4,
+                                 position=np.array([1, 2, 3], np.int64), units="Pa")
+    geo = rng.random((4, 5, 6)) > 0.5
+    root.group("materials").dataset("geometry", geo.astype(np.uint8), chunks=(4, 5, 6), gzip=4)
+    path = tmp_path / "r.h5"
+    W.write(str(path), root)
+    with HDF5ResultReader(path) as r:
+        md = r.get_metadata()
+        assert md["metadata"]["backend"] == "mi355x" and md["simulation"]["num_steps"] == 9
+        assert np.array_equal(md["grid"]["y_coords"], np.linspace(0, 1, 5))
+        assert md["sources"][0]["type"] == "point" and np.array_equal(md["probes"]["mic"]["position"], [1, 2, 3])
+        assert r.get_probe_names() == ["mic"] and r.get_num_snapshots() == 3
+        assert np.array_equal(r.load_probe("mic"), tr) and np.array_equal(r.load_timestep(2), snaps[2])
+        assert np.array_equal(r.load_geometry(), geo)
+        with pytest.raises(KeyError):
+            r.load_probe("nope")
+    empty = W.Group()
+    empty.group("metadata")
+    W.write(str(tmp_path / "e.h5"), empty)
+    r = HDF5ResultReader(tmp_path / "e.h5")
+    assert r.get_num_snapshots() == 0 and r.get_probe_names() == [] and r.load_geometry() is None
+    with pytest.raises(ValueError):
+        r.load_timestep(0)
