@@ -16,7 +16,7 @@ Beyond the reference: ``serving`` (continuous batching of many solves) and
 from .boundaries import PML, ABCFirstOrder, RadiationImpedance, RigidBoundary
 from .geometry import (Box, Cone, CSGNode, Cylinder, Difference, HelmholtzResonator, Horn, Intersection,
                        MaterializedGeometry, Rotate, Scale, SDFPrimitive, SmoothDifference, SmoothIntersection,
-                       SmoothUnion, Sphere, Transform, Translate, Union, conical_horn, exponential_horn,
+                       SmoothUnion, Sphere, SpiralHorn, HelicalTube, Transform, Translate, Union, conical_horn, exponential_horn,
                        hyperbolic_horn, rotation_matrix, tractrix_horn)
 from .hdf5_in import HDF5ResultReader
 from .grid import NonuniformGrid, UniformGrid, explicit_core_stretch_coords
@@ -33,7 +33,7 @@ __all__ = [
     "PML", "RigidBoundary", "ABCFirstOrder", "RadiationImpedance", "GaussianPulse", "Probe", "SDFPrimitive", "Box", "Sphere", "Cylinder", "Cone",
     "Union", "Intersection", "Difference", "CSGNode", "SmoothUnion", "SmoothIntersection", "SmoothDifference",
     "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "Horn", "conical_horn", "exponential_horn",
-    "hyperbolic_horn", "tractrix_horn", "HelmholtzResonator", "MaterializedGeometry", "Pole",
+    "hyperbolic_horn", "tractrix_horn", "SpiralHorn", "HelicalTube", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AcousticMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
     "POLAR_PATTERNS", "HDF5ResultReader", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
 ]

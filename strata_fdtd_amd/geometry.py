@@ -4,8 +4,9 @@ The signed-distance formulas, CSG rules and voxelisation rule follow the
 reference's ``geometry/sdf.py`` (SDFPrimitive.voxelize :99-127, Box :274-294,
 Sphere :320-333, Cylinder :382-425, Cone :436-543,
 Union/Intersection/Difference :566-722, the smooth forms :736-936, Horn
-:937-1150, rotation_matrix and Translate/Scale/Rotate :1151-1391, the horn
-constructors :1724-1820),
+:937-1150, rotation_matrix and Translate/Scale/Rotate :1151-1391, SpiralHorn
+and HelicalTube :1392-1723 (with geometry/paths.py), the horn constructors
+:1724-1820),
 ``geometry/resonator.py`` (HelmholtzResonator :137-207) and
 ``geometry/material_assignment.py`` (MaterializedGeometry :48-205). The masks
 must match the reference bit for bit, so every formula uses the same float64
@@ -187,6 +188,8 @@ def _blas_dependent(node) -> bool:
         return np.count_nonzero(node.axis_normalized) != 1
     if isinstance(node, Horn):
         return np.count_nonzero(node._axis) != 1
+    if isinstance(node, (SpiralHorn, HelicalTube)):
+        return True         # np.dot against the path's segments
     kids = []
     if isinstance(node, (Union, Intersection)):
         kids = list(node.children)
@@ -557,6 +560,107 @@ def hyperbolic_horn(throat_pos, mouth_pos, throat_r: float, mouth_r: float) -> H
 
 def tractrix_horn(throat_pos, mouth_pos, throat_r: float, mouth_r: float) -> Horn:
     return Horn(throat_pos, mouth_pos, throat_r, mouth_r, "tractrix")
+
+
+# ---- swept volumes (geometry/sdf.py:1392-1723) --------------------------------
+class SpiralHorn(SDFPrimitive):
+    """A horn swept along a spiral in the xy plane and extruded over z in
+    [0, height] (geometry/sdf.py:1392-1578): the spiral sampled at max(200,
+    100 turns) points, the cross-section radius at sample n from the profile
+    (exponential rt (rm/rt)^t, linear, or hyperbolic rt / (1 - t (1 - rt/rm))
+    at t = n/(N-1)); the distance is the planar distance to the sampled spiral
+    minus the radius at the nearest sample, combined with the z excess beyond
+    [0, height]."""
+
+    def __init__(self, center, throat_radius: float, mouth_radius: float, inner_spiral_radius: float,
+                 outer_spiral_radius: float, turns: float, height: float, profile: str = "exponential",
+                 spiral_type: str = "logarithmic"):
+        from .paths import SpiralPath
+        self.center_2d = np.array(center, dtype=np.float64)
+        self.throat_radius = float(throat_radius)
+        self.mouth_radius = float(mouth_radius)
+        self.inner_spiral_radius = float(inner_spiral_radius)
+        self.outer_spiral_radius = float(outer_spiral_radius)
+        self.turns = float(turns)
+        self.height = float(height)
+        self.profile = profile
+        self.spiral_type = spiral_type
+        if self.throat_radius <= 0 or self.mouth_radius <= 0:
+            raise ValueError("Throat and mouth radii must be positive")
+        if self.height <= 0:
+            raise ValueError("Height must be positive")
+        if self.profile not in ("exponential", "linear", "hyperbolic"):
+            raise ValueError("profile must be 'exponential', 'linear', or 'hyperbolic'")
+        self.spiral = SpiralPath(center=self.center_2d, inner_radius=self.inner_spiral_radius,
+                                 outer_radius=self.outer_spiral_radius, turns=self.turns,
+                                 spiral_type=self.spiral_type)
+        n = max(200, int(100 * self.turns))
+        self.path_2d = self.spiral.sample_points(n)
+        t = np.linspace(0, 1, n)
+        if self.profile == "exponential":
+            self.radii = self.throat_radius * (self.mouth_radius / self.throat_radius) ** t
+        elif self.profile == "linear":
+            self.radii = self.throat_radius + (self.mouth_radius - self.throat_radius) * t
+        else:
+            self.radii = self.throat_radius / (1 - t * (1 - self.throat_radius / self.mouth_radius))
+
+    def sdf(self, points):
+        from .paths import distance_to_polyline, nearest_sample
+        points = _points(points)
+        xy = points[:, :2]
+        z = points[:, 2]
+        radial = distance_to_polyline(xy, self.path_2d) - self.radii[nearest_sample(xy, self.path_2d)]
+        z_dist = np.maximum(0, np.maximum(-z, z - self.height))
+        inside = (z >= 0) & (z <= self.height)
+        return np.where(inside, radial, np.sqrt(radial**2 + z_dist**2))
+
+    @property
+    def bounding_box(self):
+        r = self.outer_spiral_radius + max(self.throat_radius, self.mouth_radius)
+        return (np.array([self.center_2d[0] - r, self.center_2d[1] - r, 0.0]),
+                np.array([self.center_2d[0] + r, self.center_2d[1] + r, self.height]))
+
+
+class HelicalTube(SDFPrimitive):
+    """A tube swept along a helix, its radius tapering linearly from start to
+    end (geometry/sdf.py:1579-1723): the helix sampled at max(200, 100 turns)
+    points; the distance is the distance to the sampled helix minus the tube
+    radius at the nearest sample."""
+
+    def __init__(self, center, helix_radius: float, tube_radius_start: float, tube_radius_end: float,
+                 pitch: float, turns: float):
+        from .paths import HelixPath
+        self.center = np.array(center, dtype=np.float64)
+        self.helix_radius = float(helix_radius)
+        self.tube_radius_start = float(tube_radius_start)
+        self.tube_radius_end = float(tube_radius_end)
+        self.pitch = float(pitch)
+        self.turns = float(turns)
+        if self.helix_radius <= 0:
+            raise ValueError("helix_radius must be positive")
+        if self.tube_radius_start <= 0 or self.tube_radius_end <= 0:
+            raise ValueError("tube radii must be positive")
+        if self.pitch <= 0:
+            raise ValueError("pitch must be positive")
+        if self.turns <= 0:
+            raise ValueError("turns must be positive")
+        self.helix = HelixPath(center=self.center, radius=self.helix_radius, pitch=self.pitch, turns=self.turns)
+        n = max(200, int(100 * self.turns))
+        self.path_3d = self.helix.sample_points(n)
+        t = np.linspace(0, 1, n)
+        self.tube_radii = self.tube_radius_start + (self.tube_radius_end - self.tube_radius_start) * t
+
+    def sdf(self, points):
+        from .paths import distance_to_polyline, nearest_sample
+        points = _points(points)
+        return distance_to_polyline(points, self.path_3d) - self.tube_radii[nearest_sample(points, self.path_3d)]
+
+    @property
+    def bounding_box(self):
+        # the reference's box: the helix's height only along z
+        r = self.helix_radius + max(self.tube_radius_start, self.tube_radius_end)
+        return (np.array([self.center[0] - r, self.center[1] - r, self.center[2]]),
+                np.array([self.center[0] + r, self.center[1] + r, self.center[2] + self.helix.total_height]))
 
 
 # ---- transforms (geometry/sdf.py:1151-1391) ---------------------------------

@@ -42,6 +42,9 @@ Cases:
                      with throats on cell centres
   horn_nu{NN}        horn trees on the stretched grid
   horn_oblique{NN}   horns with tilted axes
+  swept_{NN}         spiral horns (logarithmic and Archimedean spirals, three
+                     flare profiles) and helical tubes (tapered and not)
+  swept_union        two of them cut out of a box
   xform_materials_{N} material regions that are transformed and smooth trees
                      (transforms and smooth CSG: host voxeliser only)
 Cylinders and the cones other than the oblique ones are axis-aligned: their
@@ -81,6 +84,7 @@ def reference_namespace() -> dict:
               Intersection=sdf.Intersection, Difference=sdf.Difference, SmoothUnion=sdf.SmoothUnion,
               SmoothIntersection=sdf.SmoothIntersection, SmoothDifference=sdf.SmoothDifference,
               Translate=sdf.Translate, Scale=sdf.Scale, Rotate=sdf.Rotate, Horn=sdf.Horn,
+              SpiralHorn=sdf.SpiralHorn, HelicalTube=sdf.HelicalTube,
               HelmholtzResonator=HelmholtzResonator, MaterializedGeometry=MaterializedGeometry,
               material=object(), UniformGrid=UniformGrid, NonuniformGrid=NonuniformGrid)
     assert sdf.__file__.startswith(REF_SRC), sdf.__file__
@@ -301,6 +305,17 @@ def cases():
     for q in range(2):
         out.append((f"horn_oblique{q:02d}", nontrivial_tree(hrng, cg, make=lambda g: horn_tree(g, oblique=True)),
                     dict(shape=(50, 52, 48), dx=1e-3)))
+    from strata_fdtd_amd.geometry import Box, Difference, HelicalTube, SpiralHorn
+    sg = UniformGrid((40, 42, 24), 1e-3)
+    swept = [SpiralHorn((0.02, 0.021), 0.002, 0.005, 0.006, 0.016, 1.5, 0.012, "exponential", "logarithmic"),
+             SpiralHorn((0.0205, 0.0215), 0.0015, 0.004, 0.005, 0.015, 2.0, 0.02, "linear", "archimedean"),
+             SpiralHorn((0.019, 0.02), 0.002, 0.003, 0.007, 0.016, 1.25, 0.006, "hyperbolic", "logarithmic"),
+             HelicalTube((0.02, 0.021, 0.002), 0.012, 0.002, 0.004, 0.008, 2.5),
+             HelicalTube((0.0205, 0.0205, 0.0005), 0.01, 0.003, 0.003, 0.01, 1.5)]
+    for q, node in enumerate(swept):
+        out.append((f"swept_{q:02d}", node, dict(shape=(40, 42, 24), dx=1e-3)))
+    out.append(("swept_union", Difference(Box(center=(0.02, 0.021, 0.012), size=(0.036, 0.038, 0.02)), swept[0],
+                                          swept[3]), dict(shape=(40, 42, 24), dx=1e-3)))
     for q in range(2):
         geo = MaterializedGeometry(nontrivial_tree(xrng, cg))
         geo.set_material(nontrivial_tree(xrng, cg, make=lambda g: xform_tree(g)), SimpleMaterial(name="x", _rho=1.5, _c=300.0))

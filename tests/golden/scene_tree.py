@@ -8,6 +8,8 @@ A tree is a JSON-able list:
   ["cylinder", p1, p2, radius]
   ["cone", p1, p2, r1, r2]
   ["horn", throat, mouth, throat_radius, mouth_radius, profile]
+  ["spiral_horn", center2, throat_r, mouth_r, inner_r, outer_r, turns, height, profile, spiral_type]
+  ["helical_tube", center, helix_r, tube_r_start, tube_r_end, pitch, turns]
   ["union", [child, ...]]
   ["intersection", [child, ...]]
   ["difference", base, [child, ...]]
@@ -55,6 +57,12 @@ def to_tree(obj) -> list:
     if isinstance(obj, G.Horn):
         return ["horn", _v(obj.throat_position), _v(obj.mouth_position), float(obj.throat_radius),
                 float(obj.mouth_radius), obj.profile]
+    if isinstance(obj, G.SpiralHorn):
+        return ["spiral_horn", _v(obj.center_2d), obj.throat_radius, obj.mouth_radius, obj.inner_spiral_radius,
+                obj.outer_spiral_radius, obj.turns, obj.height, obj.profile, obj.spiral_type]
+    if isinstance(obj, G.HelicalTube):
+        return ["helical_tube", _v(obj.center), obj.helix_radius, obj.tube_radius_start, obj.tube_radius_end,
+                obj.pitch, obj.turns]
     if isinstance(obj, G.Union):
         return ["union", [to_tree(c) for c in obj.children]]
     if isinstance(obj, G.Intersection):
@@ -94,6 +102,10 @@ def build(tree, ns: dict):
         return ns["Cone"](p1=tuple(tree[1]), p2=tuple(tree[2]), r1=tree[3], r2=tree[4])
     if kind == "horn":
         return ns["Horn"](tuple(tree[1]), tuple(tree[2]), tree[3], tree[4], tree[5])
+    if kind == "spiral_horn":
+        return ns["SpiralHorn"](tuple(tree[1]), *tree[2:8], profile=tree[8], spiral_type=tree[9])
+    if kind == "helical_tube":
+        return ns["HelicalTube"](tuple(tree[1]), *tree[2:7])
     if kind == "union":
         return ns["Union"](*[build(c, ns) for c in tree[1]])
     if kind == "intersection":

@@ -148,7 +148,7 @@ def test_device_voxeliser_equals_reference_masks():
         obj = scene_tree.build(case["tree"], ns)
         tree_text = scene_tree.dumps(case["tree"])
         if (name.startswith("cone_oblique") or '"rotate' in tree_text or '"smooth_' in tree_text
-                or '"horn"' in tree_text):
+                or '"horn"' in tree_text or '"spiral_horn"' in tree_text or '"helical_tube"' in tree_text):
             # tilted axes and rotations go through BLAS in the reference,
             # smooth CSG needs distances rather than inside tests, and horns
             # numpy's exp / cosh / cos: the device voxeliser refuses them and
@@ -162,7 +162,7 @@ def test_device_voxeliser_equals_reference_masks():
         if "ids" in case:
             assert np.array_equal(dids, case["ids"]), name
     # 3 tilted cone trees, 7 trees with rotations, 9 smooth trees, the 2
-    # transformed material scenes (each with a smooth region) and 16 horn
-    # trees; the other 8 transformed trees (translations and scales only) run
-    # on the device
-    assert host_only == 3 + 7 + 9 + 2 + 16
+    # transformed material scenes (each with a smooth region), 16 horn trees
+    # and 6 swept-volume scenes; the other 8 transformed trees (translations
+    # and scales only) run on the device
+    assert host_only == 3 + 7 + 9 + 2 + 16 + 6

@@ -210,3 +210,38 @@ def test_horn_profiles_known_answers(profile):
         sfa.Horn((0, 0, 0), (0, 0, 0.1), 0.0, 0.03, profile)
     with pytest.raises(ValueError):
         sfa.Horn((0, 0, 0), (0, 0, 0.1), 0.01, 0.03, "parabolic")
+
+
+def test_paths_closed_forms():
+    """geometry/paths.py restated: spiral and helix samples, their lengths,
+    and polyline distances on cases with closed forms."""
+    from strata_fdtd_amd.paths import HelixPath, SpiralPath, distance_to_helix, distance_to_polyline
+    sp = SpiralPath(center=(0.1, 0.2), inner_radius=0.01, outer_radius=0.04, turns=2.0)
+    pts = sp.sample_points(101)
+    r = np.linalg.norm(pts - [0.1, 0.2], axis=1)
+    np.testing.assert_allclose([r[0], r[-1]], [0.01, 0.04], rtol=1e-12)
+    assert np.all(np.diff(r) > 0)
+    np.testing.assert_allclose(r[50], np.sqrt(0.01 * 0.04), rtol=1e-12)       # logarithmic: geometric mean
+    ar = SpiralPath((0.0, 0.0), 0.01, 0.04, 2.0, "archimedean").sample_points(101)
+    np.testing.assert_allclose(np.linalg.norm(ar[50]), 0.025, rtol=1e-12)     # Archimedean: arithmetic mean
+    hx = HelixPath(center=(0.0, 0.0, 0.0), radius=0.05, pitch=0.02, turns=3.0)
+    np.testing.assert_allclose(hx.total_length, 3 * np.hypot(2 * np.pi * 0.05, 0.02), rtol=1e-15)
+    assert hx.total_height == pytest.approx(0.06)
+    dense = np.sum(np.linalg.norm(np.diff(hx.sample_points(20000), axis=0), axis=1))
+    np.testing.assert_allclose(dense, hx.total_length, rtol=1e-6)
+    seg = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0]])
+    q = np.array([[0.5, 0.5], [2.0, 0.5], [-1.0, 0.0], [1.0, 2.0]])
+    np.testing.assert_allclose(distance_to_polyline(q, seg), [0.5, 1.0, 1.0, 1.0], rtol=1e-15)
+    d = distance_to_helix(np.array([[0.0, 0.0, 0.03]]), (0.0, 0.0, 0.0), 0.05, 0.02, 3.0)
+    # on the axis: between the helix radius and its chords' distance r cos(dtheta / 2)
+    half = 0.5 * 2 * np.pi * 3.0 / (150 - 1)
+    assert 0.05 * np.cos(half) - 1e-9 <= d[0] <= 0.05
+    for bad in (dict(inner_radius=0.0), dict(outer_radius=0.005), dict(turns=0.0)):
+        with pytest.raises(ValueError):
+            SpiralPath(**{**dict(center=(0, 0), inner_radius=0.01, outer_radius=0.04, turns=1.0), **bad})
+    with pytest.raises(ValueError):
+        distance_to_polyline(q, np.zeros((1, 2)))
+    with pytest.raises(ValueError):
+        sfa.SpiralHorn((0, 0), 0.01, 0.02, 0.01, 0.03, 1.0, 0.05, profile="tractrix")
+    with pytest.raises(ValueError):
+        sfa.HelicalTube((0, 0, 0), 0.05, 0.0, 0.01, 0.02, 1.0)

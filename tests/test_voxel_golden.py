@@ -8,7 +8,8 @@ reference's ``geometry/sdf.py`` (SDFPrimitive.voxelize :99-127),
 scaled-down nonuniform resonator plate, three serving requests, random CSG
 trees whose millimetre-rounded parameters put many cell centres exactly on a
 surface, random material regions, Helmholtz resonators, cones, transformed
-trees (translations, scales, rotations), smooth CSG and horns. The product's
+trees (translations, scales, rotations), smooth CSG, horns, spiral horns and
+helical tubes. The product's
 voxeliser must reproduce every mask and id bit for bit. The device voxeliser
 is held to the same fixture in tests/test_gpu_voxelize.py.
 """
@@ -54,6 +55,7 @@ def product_namespace():
                 Intersection=sfa.Intersection, Difference=sfa.Difference, SmoothUnion=sfa.SmoothUnion,
                 SmoothIntersection=sfa.SmoothIntersection, SmoothDifference=sfa.SmoothDifference,
                 Translate=sfa.Translate, Scale=sfa.Scale, Rotate=sfa.Rotate, Horn=sfa.Horn,
+                SpiralHorn=sfa.SpiralHorn, HelicalTube=sfa.HelicalTube,
                 HelmholtzResonator=sfa.HelmholtzResonator, MaterializedGeometry=sfa.MaterializedGeometry,
                 material=sfa.SimpleMaterial(name="m", _rho=1.2, _c=340.0))
 
