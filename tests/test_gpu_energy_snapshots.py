@@ -265,3 +265,44 @@ def test_run_progress_and_callback_equal_plain_run():
             assert np.array_equal(tr[name], base_tr[name]), name
         for k in FIELDS:
             assert np.array_equal(f[k], base_f[k]), k
+
+
+def test_audio_file_waveform_source_bitexact(oracle_lib, tmp_path):
+    """A point source and an x-plane source driven by AudioFileWaveform
+    (core/waveforms.py; a WAV file resampled to 1/dt) give the oracle's
+    fields and traces bit for bit when the oracle is fed the same samples
+    at the accumulated times."""
+    from scipy.io import wavfile
+    import strata_fdtd_amd as sfa
+    rng = np.random.default_rng(9)
+    wavfile.write(tmp_path / "a.wav", 48000, (rng.uniform(-1, 1, 400) * 32767).astype(np.int16))
+    sc = seeded_scenario(47, shape=(24, 26, 40), pml=4, geometry=True)
+    sc.sources = []
+    n = 60
+    pt = (12, 13, 20)
+    s = make_solver(sc)
+    a = sfa.AudioFileWaveform(tmp_path / "a.wav", amplitude=0.5)
+    b = sfa.AudioFileWaveform(tmp_path / "a.wav", amplitude=0.25, loop=True)
+
+    class _Src:
+        def __init__(self, w, position, source_type):
+            self.w, self.position, self.source_type = w, position, source_type
+
+        def waveform(self, t, dt):
+            return self.w.waveform(t, dt)
+
+    srcs = [_Src(a, pt, "point"), _Src(b, {"axis": 0, "index": 7}, "plane")]
+    for src in srcs:
+        s.add_source(src)
+    s.step_n(n)
+    pb = oracle_problem(sc, n)
+    times = T.accumulated_times(s.dt, n)
+    tabs = [np.array([src.waveform(np.array([t]), s.dt)[0] for t in times], dtype=np.float64) for src in srcs]
+    assert np.abs(tabs[0]).max() > 0
+    pb.sources = [(0, int(np.ravel_multi_index(pt, sc.shape)), 0, tabs[0]), (1, 0, 7, tabs[1])]
+    st, tr = cpu.run(pb, n)
+    got = s.get_probe_data()
+    for q in range(len(sc.probes)):
+        assert np.array_equal(got[f"probe{q}"], tr[:, q]), q
+    for k in FIELDS:
+        assert np.array_equal(getattr(s, k), st[k]), k
