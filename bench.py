@@ -271,7 +271,6 @@ def adjoint_bench(n: int = 192, steps: int = 2000, fd_voxels: int = 0):
         return runs[k // 2]
 
     forward_traces(pb, rho, K)  # warm-up
-    t_fwd, _ = median_of(5, lambda: forward_traces(pb, rho, K))
     t_fwd_cell, _ = timed(lambda: forward_traces(pb, rho, K, impl="cell"))
     plain = FDTDSolver(grid=UniformGrid(pb.shape, pb.resolution), c=pb.c, rho=pb.rho, backend="mi355x")
     plain.add_boundary(PML(depth=pb.pml_depth))
@@ -291,14 +290,22 @@ def adjoint_bench(n: int = 192, steps: int = 2000, fd_voxels: int = 0):
 
     grad("all")  # warm-up (history allocation, code objects)
     torch.cuda.empty_cache()
-    t_grad, (loss, g_r, g_k) = median_of(5, lambda: grad("all"))
+    # forward and gradient timed in alternation, so that both see the same
+    # allocator state and buffer placement; medians of 5 each
+    fwd_runs, grad_runs = [], []
+    for _ in range(5):
+        fwd_runs.append(timed(lambda: forward_traces(pb, rho, K)))
+        grad_runs.append(timed(lambda: grad("all")))
+    t_fwd = sorted(t for t, _ in fwd_runs)[2]
+    grad_runs.sort(key=lambda t: t[0])
+    t_grad, (loss, g_r, g_k) = grad_runs[2]
     torch.cuda.empty_cache()
-    t_grad_ck, (loss_ck, g_r_ck, g_k_ck) = timed(lambda: grad("checkpoint"))
+    t_grad_ck, (loss_ck, g_r_ck, g_k_ck) = median_of(3, lambda: grad("checkpoint"))
     same = bool(torch.equal(g_r, g_r_ck) and torch.equal(g_k, g_k_ck))
     from strata_fdtd_amd.adjoint import history_bytes
     out = {
         "shape": list(pb.shape), "steps": steps, "design_box": list(pb.box_shape), "checkpoint": pb.checkpoint,
-        "timing": "median of 5 runs for forward_s and gradient_s",
+        "timing": "medians of 5 alternating forward / gradient runs; median of 3 checkpointed gradients",
         "forward_s": round(t_fwd, 4), "forward_cell_kernels_s": round(t_fwd_cell, 4),
         "forward_plain_fused_s": round(t_plain, 4),
         "gradient_s": round(t_grad, 4), "gradient_history": "all steps in HBM",
