@@ -26,6 +26,7 @@ from .microphones import POLAR_PATTERNS, Microphone
 from .solver import FDTDSolver, has_gpu_support, has_mi355x_backend, has_native_kernels
 from .sources import GaussianPulse, Probe
 from .waveforms import AudioFileWaveform
+from .material_models import DampingMaterial, PorousMaterial, ViscoelasticSolid, get_material, list_materials
 
 __version__ = "0.1.0"
 
@@ -36,5 +37,6 @@ __all__ = [
     "Transform", "Translate", "Scale", "Rotate", "rotation_matrix", "Horn", "conical_horn", "exponential_horn",
     "hyperbolic_horn", "tractrix_horn", "SpiralHorn", "HelicalTube", "HelmholtzResonator", "MaterializedGeometry", "Pole",
     "PoleType", "SimpleMaterial", "AcousticMaterial", "AIR_20C", "AIR_25C", "AIR_15C", "WATER_20C", "Microphone",
-    "POLAR_PATTERNS", "HDF5ResultReader", "AudioFileWaveform", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
+    "POLAR_PATTERNS", "HDF5ResultReader", "AudioFileWaveform", "PorousMaterial", "ViscoelasticSolid",
+    "DampingMaterial", "get_material", "list_materials", "MembraneSource", "CircularMembraneSource", "RectangularMembraneSource",
 ]

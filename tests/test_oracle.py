@@ -28,7 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 # runs by tests/test_organ_pipes.py, tests/test_simple_pulse.py and their GPU
 # counterparts)
 GOLDEN = sorted(p for p in glob.glob(os.path.join(HERE, "golden", "*.npz"))
-                if not os.path.basename(p).startswith(("voxel_", "full_", "organ_", "simple_pulse", "boundary_")))
+                if not os.path.basename(p).startswith(("voxel_", "full_", "organ_", "simple_pulse", "boundary_", "material_")))
 FIELDS = ("p", "vx", "vy", "vz")
 
 
