@@ -31,6 +31,9 @@ class Scenario:
     sources: list = field(default_factory=list)    # ("point", (i,j,k), f) / ("plane", axis, idx, f)
     probes: list = field(default_factory=list)     # (i,j,k)
     coords: tuple | None = None                 # nonuniform (xc, yc, zc)
+    # id -> material object (rho_inf, K_inf, Debye poles) registered as it is,
+    # in place of a pole-spec list
+    material_objects: dict = field(default_factory=dict)
 
     @property
     def nonuniform(self) -> bool:
@@ -85,16 +88,21 @@ def oracle_problem(sc: Scenario, n_steps: int) -> cpu.Problem:
     poles = []
     rho_inf = np.zeros(256, np.float32)
     K_inf = np.zeros(256, np.float32)
-    for mid, plist in sc.materials.items():
-        rho_m, c_m = sc.rho, sc.c
+    specs = dict(sc.materials)
+    props = {mid: (sc.rho, sc.rho * sc.c**2) for mid in sc.materials}
+    for mid, obj in sc.material_objects.items():
+        specs[mid] = [dict(kind="debye", target=p.target, tau=p.tau, dchi=p.delta_chi) for p in obj.poles]
+        props[mid] = (obj.rho_inf, obj.K_inf)
+    for mid, plist in specs.items():
+        rho_m, K_m = props[mid]
         rho_inf[mid] = rho_m
-        K_inf[mid] = rho_m * c_m**2
+        K_inf[mid] = K_m
         for pl in plist:
             tgt = 0 if pl["target"] == "density" else 1
             if tgt == 0:
                 corr = T.native_density_correction(dt, rho_m, min_dx)
             else:
-                corr = T.native_modulus_correction(dt, rho_m * c_m**2)
+                corr = T.native_modulus_correction(dt, K_m)
             if pl["kind"] == "debye":
                 a, b = T.debye_coefficients(pl["tau"], pl["dchi"], dt)
                 poles.append(cpu.Pole(mid, tgt, 0, a, b, 0.0, corr))
@@ -131,6 +139,9 @@ def make_solver(sc: Scenario, tiling=None):
                                       gamma=pl["gamma"]))
         mat = sfa.SimpleMaterial(name=f"m{mid}", _rho=sc.rho, _c=sc.c, _poles=poles)
         s.register_material(mat, material_id=mid)
+        s.set_material_region(sc.material == mid, mid)
+    for mid, obj in sc.material_objects.items():
+        s.register_material(obj, material_id=mid)
         s.set_material_region(sc.material == mid, mid)
     for src in sc.sources:
         if src[0] == "point":

@@ -306,3 +306,24 @@ def test_audio_file_waveform_source_bitexact(oracle_lib, tmp_path):
         assert np.array_equal(got[f"probe{q}"], tr[:, q]), q
     for k in FIELDS:
         assert np.array_equal(getattr(s, k), st[k]), k
+
+
+def test_library_materials_bitexact(oracle_lib):
+    """Library materials in a solve: a porous absorber (fiberglass, Debye
+    density and modulus poles from the JCA fit) and a damping solid (butyl,
+    modulus poles), each with its own rho_inf and K_inf in the ADE
+    corrections; fields and traces equal the oracle's bit for bit."""
+    import strata_fdtd_amd as sfa
+    sc = seeded_scenario(48, shape=(24, 20, 30), pml=3, geometry=True, ade=True)
+    sc.materials = {}
+    sc.material_objects = {1: sfa.get_material("fiberglass_48kg"), 2: sfa.get_material("butyl_rubber")}
+    n = 24
+    s = make_solver(sc)
+    s.step_n(n)
+    st, tr = cpu.run(oracle_problem(sc, n), n)
+    got = s.get_probe_data()
+    for q in range(len(sc.probes)):
+        assert np.array_equal(got[f"probe{q}"], tr[:, q], equal_nan=True), q
+    for k in FIELDS:
+        assert np.array_equal(getattr(s, k), st[k], equal_nan=True), k
+    assert np.isfinite(tr).any()
